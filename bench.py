@@ -1,0 +1,163 @@
+"""Benchmark of the distance-estimator ray-march pass on one MI355X.
+
+Workload (one "step"): the 1080p Mandelbox batch - 16 seeded cameras on a sphere of radius 4,
+1920x1080 each, amazingbox (-1.5, .5, 1, 1, 15 iterations), G-buffer (depth, normal, trap,
+hit) plus one soft-shadow ray per hit toward the global light (.577, .577, -.577), k = 16,
+at most 128 shadow steps. Outputs stay in HBM (torch tensors); the persistent batched
+schedule renders the 16 frames in one launch per pass, the plain schedule launches one
+kernel per frame. value = primary rays (with their shadow ray) per second of the persistent
+schedule, whole job, wall clock around the timed steps.
+
+Prints one JSON line. --gpus N > 1 exits: the path runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3         # MI355X FP32 vector, packed (published)
+FP32_PEAK_UNPACKED_TFLOPS = 78.6  # without packed FP32 instructions
+METRIC = "Mrays/s persistent-batched vs plain @1080p Mandelbox, %FP32 peak; grad rays/s"
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def flops_table():
+    with open(os.path.join(ROOT, "tests", "golden", "flops_per_de.json")) as f:
+        return json.load(f)
+
+
+def make_jobs(nd, torch, n_cams):
+    jobs = []
+    for i in range(n_cams):
+        u = nd.config_uniforms(2, i)
+        W, H = int(u.resolution[0]), int(u.resolution[1])
+        g = nd.GBuffer(W, H, device="cuda:0", fields=("depth", "normal", "trap", "hit", "shadow"))
+        jobs.append((u, g))
+    return jobs
+
+
+def timed(r, jobs, kernel, steps, warmup, torch, abi, label):
+    stats = None
+    for i in range(warmup):
+        stats = r.render_batch(jobs, shadows=True, shadow_max_steps=128, kernel=kernel)
+        log("%s warmup %d: %.1f ms device" % (label, i, stats.ms_total))
+    torch.cuda.synchronize()
+    r.sync()
+    agg = {"ms_march": 0.0, "ms_total": 0.0, "de_march": 0, "de_total": 0, "rays": 0, "hits": 0}
+    t0 = time.perf_counter()
+    for i in range(steps):
+        stats = r.render_batch(jobs, shadows=True, shadow_max_steps=128, kernel=kernel)
+        agg["ms_march"] += stats.ms_march
+        agg["ms_total"] += stats.ms_total
+        agg["de_march"] += stats.de_evals_march
+        agg["de_total"] += stats.de_evals
+        agg["rays"] += stats.rays
+        agg["hits"] += stats.hits
+        log("%s step %d: %.1f ms device, %d DE evals" % (label, i, stats.ms_total, stats.de_evals))
+    r.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    agg["wall_s"] = wall
+    return agg
+
+
+def cpu_baseline(nd, target_s=15.0):
+    """FP32 CPU restatement of the same per-ray math (the oracle instantiated for float),
+    on host threads, over a band of rows of camera 0 at full 1920x1080."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    u = nd.config_uniforms(2, 0)
+    W, H = int(u.resolution[0]), int(u.resolution[1])
+    rows = 4
+    t0 = time.perf_counter()
+    oracle.render_gbuffer(u, rect=(0, H // 2, W, rows), prec=1, normal_mode=0, shadows=True,
+                          shadow_max_steps=128, threads=threads)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    rows = int(max(4, min(H // 2, rows * target_s / dt)))
+    y0 = max(0, H // 2 - rows // 2)
+    t0 = time.perf_counter()
+    oracle.render_gbuffer(u, rect=(0, y0, W, rows), prec=1, normal_mode=0, shadows=True,
+                          shadow_max_steps=128, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": W * rows / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "camera 0 of the batch, rows %d..%d of 1920x1080 (%d rays), FP32 CPU restatement of "
+                      "the same march + analytic normal + trap + 128-step shadow, %.1f s" % (y0, y0 + rows, W * rows, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cams", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this path runs on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import numericdream_amd as nd
+    from numericdream_amd import abi
+
+    r = nd.Renderer(0)
+    jobs = make_jobs(nd, torch, args.cams)
+    log("jobs ready: %d x 1920x1080" % len(jobs))
+    if args.autotune:
+        t = r.autotune(jobs, shadows=True, shadow_max_steps=128)
+        log("autotune: waves/CU %d, refill_min_idle %d, chunk %d" % (t.waves_per_cu, t.refill_min_idle,
+                                                                     t.chunk_rays))
+    pers = timed(r, jobs, abi.KERNEL_PERSISTENT, args.steps, args.warmup, torch, abi, "persistent")
+    plain = None
+    if not args.no_plain:
+        plain = timed(r, jobs, abi.KERNEL_PLAIN, args.steps, args.warmup, torch, abi, "plain")
+
+    fl = flops_table()["workloads"]["2"]
+    f_march = fl["march"]["flops_per_de"]
+    steps = args.steps
+    rays = pers["rays"]
+    value = rays / pers["wall_s"] / 1e6
+    ms_step = pers["wall_s"] * 1e3 / steps
+    # roofline of the dominant kernel (the persistent march pass), timed with HIP events on
+    # the library's stream
+    march_s = pers["ms_march"] / 1e3
+    achieved = f_march * pers["de_march"] / march_s / 1e12 if march_s > 0 else 0.0
+    res = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded cameras, analytic fractal scene)",
+        "config": {"workload": "c2: Mandelbox (-1.5,.5,1,1,15 it) G-buffer + 128-step soft shadow, %d cameras x "
+                               "1920x1080, persistent batched" % len(jobs),
+                   "cameras": len(jobs), "width": 1920, "height": 1080, "shadow_max_steps": 128},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac_unpacked_peak": round(achieved / FP32_PEAK_UNPACKED_TFLOPS, 4),
+                     "kernel": "k_march<true> (persistent march pass)",
+                     "flops_per_de": round(f_march, 2), "de_evals_per_launch": pers["de_march"] // steps,
+                     "ms_per_launch": round(pers["ms_march"] / steps, 3)},
+        "de_evals_per_s": round(pers["de_total"] / (pers["ms_total"] / 1e3), 1),
+        "hits_per_step": pers["hits"] // steps,
+    }
+    if plain:
+        pv = plain["rays"] / plain["wall_s"] / 1e6
+        res["plain_mrays_per_s"] = round(pv, 3)
+        res["persistent_vs_plain"] = round(value / pv, 3) if pv > 0 else None
+        res["plain_ms_per_step"] = round(plain["wall_s"] * 1e3 / steps, 3)
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(nd)
+    print(json.dumps(res), flush=True)
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,244 @@
+/*
+ * nd_hip.h - C ABI of the MI355X (gfx950) distance-estimator ray-march and path-trace
+ * passes. Plain C99: no C++ types, no HIP types, plain pointers and sizes, so the Zig host
+ * can bind it with translate-c exactly as it binds the webgpu shim (build.zig:364-366,
+ * 394-400).
+ *
+ * What each entry point replaces in the reference (all paths relative to the reference
+ * tree; fr.zig = src/gpu/fractal_renderer.zig):
+ *
+ *   nd_ctx_create / nd_ctx_destroy  Context.init + FractalRenderer.init
+ *                                   (src/gpu/webgpu_context.zig:243-283, fr.zig:1769-)
+ *   nd_last_error                   the uncaptured-error sink, 512-byte message buffer
+ *                                   (src/gpu/webgpu_context.zig:131-146, 203-225)
+ *   nd_bind_formulas                the formula code table that stampFormulaCodes writes
+ *                                   into FractalInstance.code (fr.zig:602-615, 2882-2896);
+ *                                   here a code selects a built-in HIP DE instead of a
+ *                                   spliced WGSL function (fr.zig:490-539)
+ *   nd_render_gbuffer               renderToImage/renderTile -> drawSlot(.simple) ->
+ *                                   fs_simple/simple_geometry (fr.zig:3548, 4292-4298;
+ *                                   template.wgsl:4492-4538) plus the light-0 soft shadow
+ *                                   of shade_local (template.wgsl:2365-2375)
+ *   nd_render_batch                 SampleSet.per_sample + ExportBatch (fr.zig:762-783,
+ *                                   1066-1094): many frames/jobs, here one persistent launch
+ *   nd_path_trace                   encodeWavefront + fs_wf_resolve (fr.zig:4334-4381;
+ *                                   template.wgsl:4898-5407)
+ *   nd_adjoint                      no counterpart in the reference: gradient of hit depth
+ *                                   with respect to formula parameters and camera position
+ *   nd_sync / nd_set_cancel         retireOldest / export cancel flag (fr.zig:1100-1137,
+ *                                   src/app/export_image.zig:46-47)
+ *
+ * Threading: one context owns one HIP stream; calls on one context must be serialised by
+ * the caller (the reference submits from the main thread only). Contexts are independent.
+ * Errors: every call returns an nd_status; the message of the last failure is kept in the
+ * context (nd_last_error). Nothing throws or longjmps across this boundary.
+ */
+#ifndef ND_HIP_H
+#define ND_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "nd_uniforms.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nd_ctx nd_ctx;
+
+typedef enum nd_status {
+    ND_OK = 0,
+    ND_E_INVALID = 1,              /* bad argument: size, null, range */
+    ND_E_UNSUPPORTED_FORMULA = 2,  /* an instance code has no HIP DE bound; caller may fall back */
+    ND_E_OOM = 3,                  /* device allocation failed */
+    ND_E_DEVICE = 4,               /* HIP runtime error or no device */
+    ND_E_TIMEOUT = 5,              /* a persistent kernel hit its round cap */
+    ND_E_CANCELLED = 6,            /* the cancel flag was raised */
+    ND_E_UNSUPPORTED = 7           /* a Uniforms feature outside this library (accel, warps, fog...) */
+} nd_status;
+
+/* Built-in distance estimators (the four formula plug-ins on the path). */
+typedef enum nd_builtin_formula {
+    ND_FORMULA_NONE = -1,
+    ND_FORMULA_MANDELBULB = 0,  /* src/gpu/shaders/mandelbrot.wgsl */
+    ND_FORMULA_MANDELBOX = 1,   /* formulas/amazingbox_box.wgsl */
+    ND_FORMULA_MENGER3 = 2,     /* formulas/menger3_ifs.wgsl */
+    ND_FORMULA_KLEINIAN = 3     /* formulas/kleinian_kleinian.wgsl */
+} nd_builtin_formula;
+
+#define ND_MAX_CODES 16
+
+/* Normal estimators. */
+typedef enum nd_normal_mode {
+    ND_NORMAL_ANALYTIC = 0,  /* forward-mode derivative of the DE: the h->0 limit of the central stencil */
+    ND_NORMAL_TETRA = 1,     /* reference tetrahedral 4-tap, e = eps(t) (template.wgsl:1781-1804) */
+    ND_NORMAL_CENTRAL = 2    /* 6-tap central difference, h = 0.5 * eps(t) */
+} nd_normal_mode;
+
+/* Kernel choice for the geometry passes. */
+typedef enum nd_kernel {
+    ND_KERNEL_AUTO = 0,        /* persistent batched */
+    ND_KERNEL_PLAIN = 1,       /* one launch per frame, one lane per pixel, 8x8 tile per wave64 */
+    ND_KERNEL_PERSISTENT = 2   /* persistent kernels pulling rays from device queues */
+} nd_kernel;
+
+typedef struct nd_march_opts {
+    uint32_t normal_mode;       /* nd_normal_mode */
+    uint32_t shadow_max_steps;  /* 0: the reference rule min(max_steps, 40) (template.wgsl:2353, 2370) */
+    uint32_t shadows;           /* non-zero: march the light-0 soft shadow of every hit */
+    uint32_t kernel;            /* nd_kernel */
+} nd_march_opts;
+
+/* Pixel rectangle inside the render target. The target is resolution[0] x resolution[1]
+ * of the job's Uniforms; w == 0 or h == 0 means the whole target. */
+typedef struct nd_rect {
+    uint32_t x, y, w, h;
+} nd_rect;
+
+/* G-buffer outputs, row-major over the rect (pixel (x, y) of the rect at y * w + x).
+ * Any pointer may be NULL (that output is skipped). With device_ptrs == 0 they are host
+ * pointers; otherwise device pointers on the context's device.
+ *   depth   t at the hit, 0 on a miss (FragOut.depth, template.wgsl:3762-3774)
+ *   normal  3 floats per pixel, unit length at hits, 0 on a miss
+ *   trap    orbit-trap value at the hit, blended across instances by the combine weight
+ *           like the material (template.wgsl:2128-2155); 0 on a miss
+ *   hit     1 at hits, 0 at misses (FragOut.normal.w)
+ *   shadow  light-0 visibility in [0, 1]; 1 on a miss or when shadows are off
+ *   evals   DE evaluations spent on the pixel (march + normal + trap + shadow) */
+typedef struct nd_gbuffer_out {
+    float* depth;
+    float* normal;
+    float* trap;
+    uint8_t* hit;
+    float* shadow;
+    uint32_t* evals;
+    uint32_t device_ptrs;
+    uint32_t _pad;
+} nd_gbuffer_out;
+
+/* One render job of a batch: its own Uniforms (camera, formula, resolution) and outputs. */
+typedef struct nd_job {
+    const nd_uniforms* uniforms;  /* host pointer, 12,016 bytes */
+    nd_rect rect;
+    nd_gbuffer_out out;
+} nd_job;
+
+typedef struct nd_batch_stats {
+    uint64_t rays;          /* primary rays completed */
+    uint64_t hits;
+    uint64_t de_evals;      /* all scene_de evaluations, counted on the device */
+    uint64_t de_evals_march;
+    uint64_t de_evals_surface;
+    uint64_t de_evals_shadow;
+    float ms_total;         /* device time of the whole call (hipEvents) */
+    float ms_march;
+    float ms_surface;
+    float ms_shadow;
+    uint32_t kernels;       /* kernel launches issued */
+    uint32_t _pad;
+} nd_batch_stats;
+
+/* ---- context ------------------------------------------------------------------ */
+nd_status nd_ctx_create(int hip_device, nd_ctx** out);
+void nd_ctx_destroy(nd_ctx* ctx);
+const char* nd_last_error(const nd_ctx* ctx);
+nd_status nd_sync(nd_ctx* ctx);
+nd_status nd_set_cancel(nd_ctx* ctx, const volatile int* flag);
+/* The HIP stream (hipStream_t) the context launches on, as an opaque pointer, so callers
+ * can time or order work against it. */
+void* nd_ctx_stream(nd_ctx* ctx);
+
+/* code_to_builtin[c] is the nd_builtin_formula for FractalInstance.code == c (c < n <= 16).
+ * The default binding is identity: code c -> builtin c for c < 4. */
+nd_status nd_bind_formulas(nd_ctx* ctx, const int32_t* code_to_builtin, uint32_t n);
+
+/* ---- geometry passes ----------------------------------------------------------- */
+/* One frame (or the rect of it) of depth / normal / trap / hit / shadow. */
+nd_status nd_render_gbuffer(nd_ctx* ctx, const void* uniforms, size_t uniforms_size,
+                            const nd_march_opts* opts, nd_rect rect, const nd_gbuffer_out* out,
+                            nd_batch_stats* stats);
+
+/* Many jobs (frames of different cameras, formulas and resolutions). With
+ * ND_KERNEL_PERSISTENT every pass of the whole batch is one launch; with ND_KERNEL_PLAIN
+ * each job is its own launch. stats may be NULL. */
+nd_status nd_render_batch(nd_ctx* ctx, const nd_job* jobs, uint32_t n_jobs,
+                          const nd_march_opts* opts, nd_batch_stats* stats);
+
+/* Per-job device latency of the last nd_render_batch call in milliseconds (the time from
+ * the batch start until the job's last ray retired), for p50/p99 reporting. */
+nd_status nd_batch_job_latency(nd_ctx* ctx, float* ms_out, uint32_t n_jobs);
+
+/* ---- autotuner ----------------------------------------------------------------- */
+typedef struct nd_tuning {
+    uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups */
+    uint32_t refill_min_idle;  /* a wave pulls new rays when at least this many lanes are idle */
+    uint32_t chunk_rays;       /* rays a wave claims from the global queue per atomic */
+    uint32_t _pad;
+} nd_tuning;
+
+nd_status nd_get_tuning(nd_ctx* ctx, nd_tuning* out);
+nd_status nd_set_tuning(nd_ctx* ctx, const nd_tuning* t);
+/* Times candidate tunings on the given batch and keeps the fastest. */
+nd_status nd_autotune(nd_ctx* ctx, const nd_job* jobs, uint32_t n_jobs,
+                      const nd_march_opts* opts, nd_tuning* best);
+
+/* ---- adjoint ------------------------------------------------------------------- */
+/* Differentiable parameters. */
+typedef enum nd_param_id {
+    ND_PARAM_BULB_POWER = 0,     /* Mandelbulb p[0] of the first Mandelbulb instance */
+    ND_PARAM_BOX_SCALE = 1,      /* Mandelbox p[0] of the first Mandelbox instance */
+    ND_PARAM_BOX_MIN_RADIUS = 2, /* Mandelbox p[1] */
+    ND_PARAM_CAM_X = 3,          /* camera_pos translation */
+    ND_PARAM_CAM_Y = 4,
+    ND_PARAM_CAM_Z = 5,
+    ND_PARAM_COUNT = 6
+} nd_param_id;
+
+/* Gradient of the depth loss L = sum over pixels hit in both (depth - target)^2 with
+ * respect to the parameters, through the implicit-function theorem at each hit:
+ * dt/dp = -(dDE/dp) / (grad DE . d - epsilon_coefficient).
+ * target_depth: per-view target depth (w*h floats each, 0 = miss), device or host as
+ * device_ptrs says. depth_out / dtdp_out (ND_PARAM_COUNT floats per pixel) may be NULL.
+ * grad_out: ND_PARAM_COUNT doubles (host), loss_out: one double (host). */
+typedef struct nd_adjoint_io {
+    const float* target_depth;
+    float* depth_out;
+    float* dtdp_out;
+    double* grad_out;
+    double* loss_out;
+    uint64_t* pixels_used;
+    uint32_t device_ptrs;
+    uint32_t _pad;
+} nd_adjoint_io;
+
+nd_status nd_adjoint(nd_ctx* ctx, const nd_uniforms* views, uint32_t n_views,
+                     const nd_adjoint_io* io, nd_batch_stats* stats);
+
+/* ---- wavefront Monte Carlo path trace -------------------------------------------- */
+typedef struct nd_image_out {
+    float* color;      /* RGB float per pixel: the average over spp samples */
+    float* depth;      /* primary hit depth, 0 on a miss */
+    float* normal;     /* 3 floats per pixel */
+    uint32_t device_ptrs;
+    uint32_t _pad;
+} nd_image_out;
+
+typedef struct nd_trace_stats {
+    uint64_t paths;          /* camera paths traced (pixels x spp) */
+    uint64_t segments;       /* extend rays over all bounces */
+    uint64_t shadow_rays;
+    uint64_t de_evals;
+    float ms_total;
+    uint32_t max_depth_reached;
+    uint32_t _pad;
+} nd_trace_stats;
+
+nd_status nd_path_trace(nd_ctx* ctx, const void* uniforms, size_t uniforms_size, uint32_t spp,
+                        nd_rect rect, const nd_image_out* out, nd_trace_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ND_HIP_H */

@@ -1,0 +1,244 @@
+"""numericdream_amd - MI355X (gfx950) distance-estimator ray march and path trace.
+
+Host-side Python binding of the C ABI in include/nd_hip.h (libndhip.so). It mirrors the
+reference's renderer entry points for this path (src/gpu/fractal_renderer.zig:
+renderToImage / drawSlot(.simple) / encodeWavefront) with the same argument meaning:
+a 12,016-byte Uniforms block per sample, a pixel rect, and G-buffer outputs. Device
+memory for outputs may be torch tensors on the GPU (their data pointers are passed with
+device_ptrs = 1) or numpy arrays on the host.
+
+The library must be built (numericdream_amd/build.py); importing fails loudly otherwise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (BatchStats, GBufferOut, Job, MarchOpts, Rect, Tuning, Uniforms, AdjointIO, ImageOut,
+                  TraceStats)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libndhip.so")
+
+
+class NdError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(status, status), message))
+        self.status = status
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("numericdream_amd: %s is missing; run numericdream_amd/build.py "
+                          "(or __graft_entry__.build()) first" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    P = C.POINTER
+    lib.nd_ctx_create.argtypes = [i32, P(vp)]
+    lib.nd_ctx_destroy.argtypes = [vp]
+    lib.nd_ctx_destroy.restype = None
+    lib.nd_last_error.argtypes = [vp]
+    lib.nd_last_error.restype = C.c_char_p
+    lib.nd_sync.argtypes = [vp]
+    lib.nd_ctx_stream.argtypes = [vp]
+    lib.nd_ctx_stream.restype = vp
+    lib.nd_bind_formulas.argtypes = [vp, P(C.c_int32), u32]
+    lib.nd_render_gbuffer.argtypes = [vp, vp, C.c_size_t, P(MarchOpts), Rect, P(GBufferOut), P(BatchStats)]
+    lib.nd_render_batch.argtypes = [vp, P(Job), u32, P(MarchOpts), P(BatchStats)]
+    lib.nd_batch_job_latency.argtypes = [vp, P(C.c_float), u32]
+    lib.nd_get_tuning.argtypes = [vp, P(Tuning)]
+    lib.nd_set_tuning.argtypes = [vp, P(Tuning)]
+    lib.nd_autotune.argtypes = [vp, P(Job), u32, P(MarchOpts), P(Tuning)]
+    lib.nd_adjoint.argtypes = [vp, P(Uniforms), u32, P(AdjointIO), P(BatchStats)]
+    lib.nd_path_trace.argtypes = [vp, vp, C.c_size_t, u32, Rect, P(ImageOut), P(TraceStats)]
+    lib.nd_config_job_count.argtypes = [u32]
+    lib.nd_config_job_count.restype = u32
+    lib.nd_uniforms_defaults.argtypes = [P(Uniforms)]
+    lib.nd_uniforms_defaults.restype = None
+    lib.nd_config_build.argtypes = [u32, u32, C.c_uint64, P(Uniforms)]
+    lib.nd_camera_look_at.argtypes = [P(Uniforms), P(C.c_float), P(C.c_float), C.c_float]
+    lib.nd_camera_look_at.restype = None
+    lib.nd_set_pixel_epsilon.argtypes = [P(Uniforms), C.c_float]
+    lib.nd_set_pixel_epsilon.restype = None
+    return lib
+
+
+lib = _load()
+
+
+# ----------------------------------------------------------------------------- configs
+def config_job_count(config):
+    return int(lib.nd_config_job_count(config))
+
+
+def config_uniforms(config, index=0, seed=0):
+    """Uniforms block `index` of workload `config` (1..5), as include/nd_config.h builds it."""
+    u = Uniforms()
+    st = lib.nd_config_build(config, index, seed, C.byref(u))
+    if st != 0:
+        raise NdError(st, "nd_config_build(%d, %d)" % (config, index))
+    return u
+
+
+def default_uniforms():
+    u = Uniforms()
+    lib.nd_uniforms_defaults(C.byref(u))
+    return u
+
+
+def camera_look_at(u, pos, target, fov_deg=60.0):
+    p = (C.c_float * 3)(*pos)
+    t = (C.c_float * 3)(*target)
+    lib.nd_camera_look_at(C.byref(u), p, t, fov_deg)
+
+
+def set_pixel_epsilon(u, fov_deg=60.0):
+    lib.nd_set_pixel_epsilon(C.byref(u), fov_deg)
+
+
+def resize(u, w, h, fov_deg=60.0):
+    """Same camera at another resolution; the epsilon follows the pixel angle."""
+    u.resolution[0] = float(w)
+    u.resolution[1] = float(h)
+    set_pixel_epsilon(u, fov_deg)
+    return u
+
+
+def uniforms_bytes(u):
+    return bytes(C.string_at(C.addressof(u), abi.UNIFORMS_SIZE))
+
+
+# ----------------------------------------------------------------------------- outputs
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):  # torch tensor
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+class GBuffer:
+    """Host (numpy) or device (torch) G-buffer arrays for a w x h rect."""
+
+    def __init__(self, w, h, device=None, fields=("depth", "normal", "trap", "hit", "shadow", "evals")):
+        self.w, self.h = w, h
+        self.device = device
+        shapes = {"depth": ((h, w), "f4"), "normal": ((h, w, 3), "f4"), "trap": ((h, w), "f4"),
+                  "hit": ((h, w), "u1"), "shadow": ((h, w), "f4"), "evals": ((h, w), "u4")}
+        for k in ("depth", "normal", "trap", "hit", "shadow", "evals"):
+            if k not in fields:
+                setattr(self, k, None)
+                continue
+            shape, dt = shapes[k]
+            if device is None:
+                setattr(self, k, np.zeros(shape, dt))
+            else:
+                import torch
+                tdt = {"f4": torch.float32, "u1": torch.uint8, "u4": torch.int32}[dt]
+                setattr(self, k, torch.zeros(shape, dtype=tdt, device=device))
+
+    def out(self):
+        return GBufferOut(_ptr(self.depth), _ptr(self.normal), _ptr(self.trap), _ptr(self.hit), _ptr(self.shadow),
+                          _ptr(self.evals), 0 if self.device is None else 1, 0)
+
+    def numpy(self):
+        r = {}
+        for k in ("depth", "normal", "trap", "hit", "shadow", "evals"):
+            a = getattr(self, k)
+            if a is None:
+                continue
+            r[k] = a if isinstance(a, np.ndarray) else a.cpu().numpy()
+        if "evals" in r:
+            r["evals"] = r["evals"].view(np.uint32)
+        return r
+
+
+# ----------------------------------------------------------------------------- renderer
+class Renderer:
+    """One HIP context (device + stream), like the reference's FractalRenderer."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        st = lib.nd_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise NdError(st, "nd_ctx_create(%d)" % device)
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib.nd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != 0:
+            raise NdError(st, "%s: %s" % (what, lib.nd_last_error(self._h).decode()))
+
+    @property
+    def stream(self):
+        return lib.nd_ctx_stream(self._h)
+
+    def bind_formulas(self, code_to_builtin):
+        arr = (C.c_int32 * len(code_to_builtin))(*code_to_builtin)
+        self._check(lib.nd_bind_formulas(self._h, arr, len(code_to_builtin)), "nd_bind_formulas")
+
+    def render_gbuffer(self, u, gbuf, rect=None, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0,
+                       kernel=abi.KERNEL_PERSISTENT):
+        opts = MarchOpts(normal_mode, shadow_max_steps, 1 if shadows else 0, kernel)
+        r = Rect(*(rect or (0, 0, 0, 0)))
+        st = BatchStats()
+        out = gbuf.out()
+        self._check(lib.nd_render_gbuffer(self._h, C.byref(u), abi.UNIFORMS_SIZE, C.byref(opts), r, C.byref(out),
+                                          C.byref(st)), "nd_render_gbuffer")
+        return st
+
+    def render_batch(self, jobs, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0,
+                     kernel=abi.KERNEL_PERSISTENT):
+        """jobs: list of (Uniforms, GBuffer[, rect])."""
+        arr = (Job * len(jobs))()
+        keep = []
+        for i, jb in enumerate(jobs):
+            u, g = jb[0], jb[1]
+            rect = jb[2] if len(jb) > 2 else (0, 0, 0, 0)
+            keep.append(u)
+            arr[i].uniforms = C.pointer(u)
+            arr[i].rect = Rect(*rect)
+            arr[i].out = g.out()
+        opts = MarchOpts(normal_mode, shadow_max_steps, 1 if shadows else 0, kernel)
+        st = BatchStats()
+        self._check(lib.nd_render_batch(self._h, arr, len(jobs), C.byref(opts), C.byref(st)), "nd_render_batch")
+        return st
+
+    def job_latency_ms(self, n):
+        a = (C.c_float * n)()
+        self._check(lib.nd_batch_job_latency(self._h, a, n), "nd_batch_job_latency")
+        return np.array(a[:], dtype=np.float32)
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib.nd_get_tuning(self._h, C.byref(t)), "nd_get_tuning")
+        return t
+
+    def set_tuning(self, waves_per_cu=0, refill_min_idle=1, chunk_rays=64):
+        t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, 0)
+        self._check(lib.nd_set_tuning(self._h, C.byref(t)), "nd_set_tuning")
+
+    def autotune(self, jobs, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0):
+        arr = (Job * len(jobs))()
+        for i, jb in enumerate(jobs):
+            arr[i].uniforms = C.pointer(jb[0])
+            arr[i].rect = Rect(*(jb[2] if len(jb) > 2 else (0, 0, 0, 0)))
+            arr[i].out = jb[1].out()
+        opts = MarchOpts(normal_mode, shadow_max_steps, 1 if shadows else 0, abi.KERNEL_PERSISTENT)
+        t = Tuning()
+        self._check(lib.nd_autotune(self._h, arr, len(jobs), C.byref(opts), C.byref(t)), "nd_autotune")
+        return t
+
+    def sync(self):
+        self._check(lib.nd_sync(self._h), "nd_sync")

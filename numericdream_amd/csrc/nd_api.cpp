@@ -1,0 +1,576 @@
+// nd_api.cpp - the C ABI (include/nd_hip.h): context, error sink, Uniforms decoding into
+// device job records, scheduling of the plain and persistent geometry passes, the
+// autotuner and device statistics.
+//
+// Reference counterparts: Context.init / FractalRenderer.init (src/gpu/webgpu_context.zig:
+// 243-283, src/gpu/fractal_renderer.zig:1769-), the error sink (webgpu_context.zig:131-146),
+// renderToImage / renderTile / drawSlot (fractal_renderer.zig:3548-3940, 4284-4298) and
+// stampFormulaCodes (fractal_renderer.zig:2882-2896).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nd_config.h"
+#include "nd_hip.h"
+#include "nd_internal.h"
+#include "nd_ctx.h"
+
+// ----------------------------------------------------------------------------- errors
+nd_status nd_fail(nd_ctx* c, nd_status st, const char* fmt, ...) {
+    if (!c) return st;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+    return st;
+}
+
+nd_status nd_hip_fail(nd_ctx* c, hipError_t e, const char* what) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return nd_fail(c, ND_E_OOM, "%s: %s", what, hipGetErrorString(e));
+    return nd_fail(c, ND_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+nd_status DevBuf::ensure(nd_ctx* c, size_t bytes) {
+    if (bytes <= cap) return ND_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return nd_hip_fail(c, e, "hipMalloc");
+    }
+    cap = want;
+    c->graph_valid = false;  // captured graphs hold the old pointers
+    return ND_OK;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+// ----------------------------------------------------------------------------- packing
+static int i32(float x) { return (int)x; }
+
+static void hoist(DevInst& in) {
+    std::memset(in.h, 0, sizeof(in.h));
+    const float* p = in.p;
+    switch (in.formula) {
+        case ND_FORMULA_MANDELBULB:
+            in.h[0] = p[0];
+            in.h[1] = p[0] - 1.0f;
+            in.bulb_pow8 = p[0] == 8.0f;
+            break;
+        case ND_FORMULA_MANDELBOX: {
+            float m2 = p[1] * p[1], f2 = p[2] * p[2];
+            in.h[0] = p[0];
+            in.h[1] = std::fabs(p[0]);
+            in.h[2] = m2;
+            in.h[3] = f2;
+            in.h[4] = f2 / m2;
+            in.h[5] = p[3];
+            in.h[6] = p[8];
+            in.h[7] = p[9];
+            in.h[8] = p[10];
+            in.h[9] = p[11];
+            in.h[10] = std::fmax(std::fabs(p[8]), 1.0f);
+            break;
+        }
+        case ND_FORMULA_MENGER3:
+            break;
+        case ND_FORMULA_KLEINIAN:
+            in.h[6] = 1.0f / p[5];
+            in.h[7] = p[7];
+            in.h[8] = p[8];
+            in.h[9] = p[9];
+            in.h[10] = p[10];
+            in.h[11] = std::fmax(std::fabs(p[7]), 1.0f);
+            break;
+    }
+    // formula-internal Euler rotation (degrees): amazingbox p[5..7], menger3 p[5..7]
+    in.rot_on = 0;
+    if (in.formula == ND_FORMULA_MANDELBOX || in.formula == ND_FORMULA_MENGER3) {
+        float rx = p[5], ry = p[6], rz = p[7];
+        if (!(rx == 0.0f && ry == 0.0f && rz == 0.0f)) {
+            in.rot_on = 1;
+            const float k = 3.14159265358979323846f / 180.0f;
+            in.rot_cs[0] = std::cos(rx * k); in.rot_cs[1] = std::sin(rx * k);
+            in.rot_cs[2] = std::cos(ry * k); in.rot_cs[3] = std::sin(ry * k);
+            in.rot_cs[4] = std::cos(rz * k); in.rot_cs[5] = std::sin(rz * k);
+        }
+    }
+}
+
+static int formula_iters(int f, const float* p) {
+    switch (f) {
+        case ND_FORMULA_MANDELBULB: return i32(p[1]);  // mandelbrot.wgsl:3-5
+        case ND_FORMULA_MANDELBOX: return i32(p[4]);   // amazingbox_box.wgsl:13-15
+        case ND_FORMULA_MENGER3: return i32(p[0]);     // menger3_ifs.wgsl:9-11
+        case ND_FORMULA_KLEINIAN: return i32(p[6]);    // kleinian_kleinian.wgsl:12-14
+    }
+    return 0;
+}
+
+nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
+    std::memset(&s, 0, sizeof(s));
+    if (u.accel_enabled > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "the empty-space accelerator (accel_enabled) is not implemented");
+    if (u.warp_count > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "domain warps (warp_count) are not implemented");
+    if (u.fft_active > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "the FFT view (fft_active) is not implemented");
+    if (((uint32_t)u.approx_flags & (1u << 8)) != 0u)
+        return nd_fail(c, ND_E_UNSUPPORTED, "the distance LOD approximation is not implemented");
+    s.count = std::max(i32(u.instance_count), 1);
+    if (s.count > ND_MAX_INSTANCES) s.count = ND_MAX_INSTANCES;
+    s.empty_de = u.max_dist * 2.0f + 1.0f;
+    s.carve_count = std::min(std::max(i32(u.carve_count), 0), ND_MAX_CARVES);
+    for (int i = 0; i < ND_MAX_CARVES; ++i)
+        for (int k = 0; k < 4; ++k) s.carves[i][k] = u.carves[i][k];
+    for (int i = 0; i < ND_MAX_INSTANCES; ++i) {
+        const nd_fractal_instance& fi = u.instances[i];
+        DevInst& in = s.inst[i];
+        std::memset(&in, 0, sizeof(in));
+        for (int k = 0; k < 16; ++k) in.dparam[k] = -1;
+        in.hidden = fi.hidden > 0.5f;
+        const int code = i32(fi.code);
+        in.formula = (code >= 0 && code < ND_MAX_CODES) ? c->code_map[code] : -1;
+        if (i < s.count && !in.hidden) {
+            if (code >= 0 && in.formula < 0)
+                return nd_fail(c, ND_E_UNSUPPORTED_FORMULA, "instance %d: formula code %d has no HIP distance estimator bound", i, code);
+            if (fi.mixin_count > 0.5f)
+                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d: hybrid mix-ins are not implemented", i);
+        }
+        in.combine = i32(fi.combine_mode + 0.5f);
+        in.blend_k = fi.blend_k;
+        for (int k = 0; k < 16; ++k) in.p[k] = fi.params[k];
+        in.iters = formula_iters(in.formula, in.p);
+        float mag[3];
+        bool ident = true;
+        for (int k = 0; k < 3; ++k) {
+            mag[k] = std::fmax(std::fabs(fi.scale[k]), 0.001f);
+            float sgn = fi.scale[k] < 0.0f ? -mag[k] : mag[k];
+            in.inv_s[k] = 1.0f / sgn;
+            in.offset[k] = fi.offset[k];
+            if (fi.offset[k] != 0.0f || sgn != 1.0f) ident = false;
+            for (int r = 0; r < 3; ++r) {
+                in.frame[3 * k + r] = fi.frame[k][r];
+                if (fi.frame[k][r] != (k == r ? 1.0f : 0.0f)) ident = false;
+            }
+        }
+        in.identity = ident;
+        in.de_factor = std::fmin(mag[0], std::fmin(mag[1], mag[2])) * std::fmin(std::fmax(fi.step_safety, 0.001f), 1.0f);
+        in.trap_shape = i32(fi.trap_shape + 0.5f);
+        in.trap_mode = i32(fi.trap_mode + 0.5f);
+        for (int k = 0; k < 3; ++k) {
+            in.trap_center[k] = fi.trap_center[k];
+            in.trap_box[k] = fi.trap_box[k];
+        }
+        in.trap_radius = fi.trap_radius;
+        in.trap_tube = fi.trap_tube;
+        hoist(in);
+    }
+    return ND_OK;
+}
+
+nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, nd_rect rect, DevJob& j) {
+    nd_status st = nd_pack_scene(c, u, j.scene);
+    if (st != ND_OK) return st;
+    if (!(u.resolution[0] >= 1.0f && u.resolution[1] >= 1.0f && u.resolution[0] <= 65536.0f && u.resolution[1] <= 65536.0f))
+        return nd_fail(c, ND_E_INVALID, "resolution %gx%g out of range", u.resolution[0], u.resolution[1]);
+    j.target_w = (int)u.resolution[0];
+    j.target_h = (int)u.resolution[1];
+    if (rect.w == 0 || rect.h == 0) {
+        rect.x = rect.y = 0;
+        rect.w = (uint32_t)j.target_w;
+        rect.h = (uint32_t)j.target_h;
+    }
+    if ((uint64_t)rect.x + rect.w > (uint64_t)j.target_w || (uint64_t)rect.y + rect.h > (uint64_t)j.target_h)
+        return nd_fail(c, ND_E_INVALID, "rect %ux%u+%u+%u outside the %dx%d target", rect.w, rect.h, rect.x, rect.y, j.target_w, j.target_h);
+    j.rect_x = (int)rect.x; j.rect_y = (int)rect.y; j.rect_w = (int)rect.w; j.rect_h = (int)rect.h;
+    j.tiles_x = (j.rect_w + 7) / 8;
+    j.tiles_y = (j.rect_h + 7) / 8;
+    j.ray_count = (uint32_t)j.tiles_x * (uint32_t)j.tiles_y * 64u;
+    for (int k = 0; k < 3; ++k) {
+        j.cam[k] = u.camera_pos[k];
+        j.right[k] = u.camera_right[k];
+        j.up[k] = u.camera_up[k];
+        j.fwd[k] = u.camera_forward[k];
+    }
+    j.tile_scale[0] = u.tile_scale[0]; j.tile_scale[1] = u.tile_scale[1];
+    j.tile_bias[0] = u.tile_bias[0]; j.tile_bias[1] = u.tile_bias[1];
+    j.aspect = u.resolution[0] / u.resolution[1];
+    j.inv_tw = 1.0f / (float)j.target_w;
+    j.inv_th = 1.0f / (float)j.target_h;
+    // march_to preamble (template.wgsl:1859-1871), coarse and overstep modes off
+    j.max_steps = std::max(i32(u.max_steps), 0);
+    j.max_dist = u.max_dist;
+    j.eps_c = u.epsilon_coefficient;
+    j.eps_f = u.epsilon_floor;
+    j.hq = u.high_quality > 0.5f;
+    int budget = std::max(j.hq ? i32(u.refine_hq) : i32(u.refine_fast), 0);
+    bool on_hit = u.march_verify < 0.5f;
+    j.refine_budget = on_hit ? 0 : budget;
+    j.bisect_budget = on_hit ? budget : 0;
+    j.footprint = u.hq_footprint_budget_px * 2.0f / std::max(u.resolution[1], 1.0f);
+    // light 0 soft shadow (shadow_factor, template.wgsl:2365-2375)
+    j.shadow_on = 0;
+    const nd_light& L = u.lights[0];
+    if (o.shadows && i32(u.light_count) >= 1 && L.cast_shadows > 0.5f) {
+        j.shadow_on = 1;
+        j.light_type = i32(L.light_type + 0.5f) == 1 ? 1 : 0;
+        if (j.light_type == 1) {
+            double n = std::sqrt((double)L.position_or_direction[0] * L.position_or_direction[0] +
+                                 (double)L.position_or_direction[1] * L.position_or_direction[1] +
+                                 (double)L.position_or_direction[2] * L.position_or_direction[2]);
+            if (!(n > 0)) return nd_fail(c, ND_E_INVALID, "light 0 has a zero direction");
+            for (int k = 0; k < 3; ++k) j.light_v[k] = (float)(L.position_or_direction[k] / n);
+        } else {
+            for (int k = 0; k < 3; ++k) j.light_v[k] = L.position_or_direction[k];
+        }
+        j.light_k = L.shadow_softness + (512.0f - L.shadow_softness) * L.hard_shadows;
+        j.light_far = u.max_dist * 2.0f;
+        j.shadow_steps = o.shadow_max_steps ? (int)o.shadow_max_steps : std::min(i32(u.max_steps), 40);
+    }
+    j.normal_mode = (int)o.normal_mode;
+    return ND_OK;
+}
+
+// ----------------------------------------------------------------------------- context
+extern "C" {
+
+nd_status nd_ctx_create(int hip_device, nd_ctx** out) {
+    if (!out) return ND_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return ND_E_DEVICE;
+    if (hip_device < 0 || hip_device >= n) return ND_E_INVALID;
+    if (hipSetDevice(hip_device) != hipSuccess) return ND_E_DEVICE;
+    nd_ctx* c = new nd_ctx();
+    c->device = hip_device;
+    for (int i = 0; i < ND_MAX_CODES; ++i) c->code_map[i] = i < 4 ? i : -1;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ND_E_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) {
+        c->cu_count = prop.multiProcessorCount;
+        std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    }
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, hip_device) == hipSuccess && khz > 0)
+        c->wall_khz = khz;
+    for (auto& e : c->ev) (void)hipEventCreate(&e);
+    c->occupancy_waves = nd_march_occupancy();
+    c->tuning.waves_per_cu = 0;  // 0 = from occupancy
+    c->tuning.refill_min_idle = 1;
+    c->tuning.chunk_rays = 64;
+    *out = c;
+    return ND_OK;
+}
+
+void nd_ctx_destroy(nd_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    c->jobs.release(); c->job_base.release(); c->ctr.release(); c->hits.release(); c->shared.release();
+    c->staging.release(); c->latency.release(); c->adj.release(); c->mc.release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* nd_last_error(const nd_ctx* c) { return c ? c->err : "null context"; }
+
+nd_status nd_sync(nd_ctx* c) {
+    if (!c) return ND_E_INVALID;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return nd_hip_fail(c, e, "hipStreamSynchronize");
+    return ND_OK;
+}
+
+nd_status nd_set_cancel(nd_ctx* c, const volatile int* flag) {
+    if (!c) return ND_E_INVALID;
+    c->cancel = flag;
+    return ND_OK;
+}
+
+void* nd_ctx_stream(nd_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+nd_status nd_bind_formulas(nd_ctx* c, const int32_t* map, uint32_t n) {
+    if (!c) return ND_E_INVALID;
+    if ((!map && n) || n > ND_MAX_CODES) return nd_fail(c, ND_E_INVALID, "code table of %u entries (max %d)", n, ND_MAX_CODES);
+    for (uint32_t i = 0; i < n; ++i)
+        if (map[i] < -1 || map[i] > ND_FORMULA_KLEINIAN)
+            return nd_fail(c, ND_E_INVALID, "code %u maps to unknown built-in %d", i, map[i]);
+    for (int i = 0; i < ND_MAX_CODES; ++i) c->code_map[i] = (uint32_t)i < n ? map[i] : -1;
+    return ND_OK;
+}
+
+nd_status nd_get_tuning(nd_ctx* c, nd_tuning* out) {
+    if (!c || !out) return ND_E_INVALID;
+    *out = c->tuning;
+    return ND_OK;
+}
+
+nd_status nd_set_tuning(nd_ctx* c, const nd_tuning* t) {
+    if (!c || !t) return ND_E_INVALID;
+    if (t->refill_min_idle < 1 || t->refill_min_idle > 64) return nd_fail(c, ND_E_INVALID, "refill_min_idle must be 1..64");
+    if (t->chunk_rays < 1 || t->chunk_rays > (1u << 20)) return nd_fail(c, ND_E_INVALID, "chunk_rays must be 1..2^20");
+    if (t->waves_per_cu > 64) return nd_fail(c, ND_E_INVALID, "waves_per_cu must be 0..64");
+    c->tuning = *t;
+    return ND_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------- batch
+static size_t out_bytes(const nd_gbuffer_out& o, size_t px) {
+    size_t b = 0;
+    if (o.depth) b += 4 * px;
+    if (o.normal) b += 12 * px;
+    if (o.trap) b += 4 * px;
+    if (o.hit) b += px;
+    if (o.shadow) b += 4 * px;
+    if (o.evals) b += 4 * px;
+    return b;
+}
+
+uint32_t nd_persistent_grid(nd_ctx* c) {
+    uint32_t wpc = c->tuning.waves_per_cu;
+    if (wpc == 0) {
+        int blocks = 0;
+        // k_march is the kernel that bounds occupancy; 1 wave64 per workgroup
+        wpc = c->occupancy_waves ? c->occupancy_waves : 8;
+        (void)blocks;
+    }
+    return (uint32_t)std::max(c->cu_count, 1) * wpc;
+}
+
+nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_opts* opts, nd_batch_stats* stats) {
+    if (!c) return ND_E_INVALID;
+    if (!jobs || n == 0) return nd_fail(c, ND_E_INVALID, "empty batch");
+    if (n > 65535) return nd_fail(c, ND_E_INVALID, "at most 65535 jobs per batch");
+    if (c->cancel && *c->cancel) return nd_fail(c, ND_E_CANCELLED, "cancelled before launch");
+    nd_march_opts o{};
+    if (opts) o = *opts;
+    if (o.normal_mode > ND_NORMAL_CENTRAL) return nd_fail(c, ND_E_INVALID, "normal_mode %u", o.normal_mode);
+    if (o.kernel > ND_KERNEL_PERSISTENT) return nd_fail(c, ND_E_INVALID, "kernel %u", o.kernel);
+    if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
+
+    std::vector<DevJob>& hj = c->host_jobs;
+    hj.assign(n, DevJob{});
+    std::vector<uint32_t> base(n);
+    uint64_t total = 0;
+    size_t staging = 0;
+    bool any_shadow = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!jobs[i].uniforms) return nd_fail(c, ND_E_INVALID, "job %u: null uniforms", i);
+        nd_status st = nd_pack_job(c, *jobs[i].uniforms, o, jobs[i].rect, hj[i]);
+        if (st != ND_OK) return st;
+        hj[i].ray_base = (uint32_t)total;
+        base[i] = (uint32_t)total;
+        total += hj[i].ray_count;
+        any_shadow = any_shadow || hj[i].shadow_on;
+        const nd_gbuffer_out& go = jobs[i].out;
+        size_t px = (size_t)hj[i].rect_w * hj[i].rect_h;
+        if (!go.device_ptrs) staging += out_bytes(go, px) + 6 * 256;
+    }
+    if (total >= (1ull << 31)) return nd_fail(c, ND_E_INVALID, "batch of %llu ray slots exceeds 2^31", (unsigned long long)total);
+
+    // outputs: device pointers as given, host pointers through a staging buffer
+    nd_status st = c->staging.ensure(c, staging);
+    if (st != ND_OK) return st;
+    {
+        char* sp = (char*)c->staging.p;
+        auto carve = [&](size_t bytes) { char* r = sp; sp += (bytes + 255) & ~(size_t)255; return r; };
+        for (uint32_t i = 0; i < n; ++i) {
+            const nd_gbuffer_out& go = jobs[i].out;
+            DevJob& j = hj[i];
+            size_t px = (size_t)j.rect_w * j.rect_h;
+            if (go.device_ptrs) {
+                j.out_depth = go.depth; j.out_normal = go.normal; j.out_trap = go.trap;
+                j.out_hit = go.hit; j.out_shadow = go.shadow; j.out_evals = go.evals;
+            } else {
+                j.out_depth = go.depth ? (float*)carve(4 * px) : nullptr;
+                j.out_normal = go.normal ? (float*)carve(12 * px) : nullptr;
+                j.out_trap = go.trap ? (float*)carve(4 * px) : nullptr;
+                j.out_hit = go.hit ? (uint8_t*)carve(px) : nullptr;
+                j.out_shadow = go.shadow ? (float*)carve(4 * px) : nullptr;
+                j.out_evals = go.evals ? (uint32_t*)carve(4 * px) : nullptr;
+            }
+        }
+    }
+
+    bool shared = true;
+    for (uint32_t i = 1; i < n && shared; ++i) shared = std::memcmp(&hj[i].scene, &hj[0].scene, sizeof(DevScene)) == 0;
+
+    if ((st = c->jobs.ensure(c, sizeof(DevJob) * n)) != ND_OK) return st;
+    if ((st = c->job_base.ensure(c, sizeof(uint32_t) * n)) != ND_OK) return st;
+    if ((st = c->ctr.ensure(c, sizeof(DevCounters))) != ND_OK) return st;
+    if ((st = c->shared.ensure(c, sizeof(DevScene))) != ND_OK) return st;
+    if ((st = c->latency.ensure(c, sizeof(unsigned long long) * n)) != ND_OK) return st;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(c->jobs.p, hj.data(), sizeof(DevJob) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "upload jobs");
+    if ((e = hipMemcpyAsync(c->job_base.p, base.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "upload job table");
+    if (shared && (e = hipMemcpyAsync(c->shared.p, &hj[0].scene, sizeof(DevScene), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "upload scene");
+
+    nd_batch_stats s{};
+    float ms_total = 0, ms_march = 0, ms_surface = 0, ms_shadow = 0;
+    const bool persistent = o.kernel != ND_KERNEL_PLAIN;
+    if (!persistent) {
+        // one launch per frame: the job record is read from device memory by one kernel each
+        (void)hipEventRecord(c->ev[0], c->stream);
+        for (uint32_t i = 0; i < n; ++i) {
+            e = nd_launch_plain((const DevJob*)c->jobs.p + i, hj[i].tiles_x, hj[i].tiles_y, c->stream);
+            if (e != hipSuccess) return nd_hip_fail(c, e, "launch plain march");
+        }
+        (void)hipEventRecord(c->ev[3], c->stream);
+        s.kernels = n;
+    } else {
+        size_t hit_cap = (size_t)total;
+        size_t hb_bytes = hit_cap * (4 + 2 + 4 * 4) + 8 * 256;
+        if ((st = c->hits.ensure(c, hb_bytes)) != ND_OK) return st;
+        char* hp = (char*)c->hits.p;
+        auto carve = [&](size_t bytes) { char* r = hp; hp += (bytes + 255) & ~(size_t)255; return r; };
+        NdPassLaunch L{};
+        L.hb.ray = (uint32_t*)carve(4 * hit_cap);
+        L.hb.job = (uint16_t*)carve(2 * hit_cap);
+        L.hb.t = (float*)carve(4 * hit_cap);
+        L.hb.nx = (float*)carve(4 * hit_cap);
+        L.hb.ny = (float*)carve(4 * hit_cap);
+        L.hb.nz = (float*)carve(4 * hit_cap);
+        L.hb.capacity = (uint32_t)hit_cap;
+        L.jobs = (const DevJob*)c->jobs.p;
+        L.shared = shared ? (const DevScene*)c->shared.p : nullptr;
+        L.job_base = (const uint32_t*)c->job_base.p;
+        L.n_jobs = (int)n;
+        L.total_rays = (uint32_t)total;
+        L.ctr = (DevCounters*)c->ctr.p;
+        L.grid = nd_persistent_grid(c);
+        L.chunk = c->tuning.chunk_rays;
+        L.min_idle = c->tuning.refill_min_idle;
+        L.round_cap = c->round_cap;
+        L.any_shadow = any_shadow;
+        L.job_done_cycles = (unsigned long long*)c->latency.p;
+        for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];
+        e = nd_launch_persistent(L, c->stream);
+        if (e != hipSuccess) return nd_hip_fail(c, e, "launch persistent passes");
+        s.kernels = 3 + (any_shadow ? 1 : 0) + 1;
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return nd_hip_fail(c, e, "batch execution");
+    (void)hipEventElapsedTime(&ms_total, c->ev[0], c->ev[3]);
+    if (persistent) {
+        (void)hipEventElapsedTime(&ms_march, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&ms_surface, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&ms_shadow, c->ev[2], c->ev[3]);
+        DevCounters dc;
+        if ((e = hipMemcpy(&dc, c->ctr.p, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
+            return nd_hip_fail(c, e, "read counters");
+        s.hits = dc.hit_count;
+        s.de_evals_march = dc.de_march;
+        s.de_evals_surface = dc.de_surface;
+        s.de_evals_shadow = dc.de_shadow;
+        s.de_evals = dc.de_march + dc.de_surface + dc.de_shadow;
+        std::vector<unsigned long long> done(n);
+        if ((e = hipMemcpy(done.data(), c->latency.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost)) != hipSuccess)
+            return nd_hip_fail(c, e, "read latency");
+        c->job_latency_ms.assign(n, 0.0f);
+        unsigned long long t0 = dc.clock_start;
+        for (uint32_t i = 0; i < n; ++i)
+            c->job_latency_ms[i] = done[i] > t0 ? (float)((double)(done[i] - t0) / (double)c->wall_khz) : 0.0f;
+        if (dc.rounds_capped) return nd_fail(c, ND_E_TIMEOUT, "%u waves reached the round cap", dc.rounds_capped);
+    } else {
+        c->job_latency_ms.assign(n, ms_total);
+    }
+    for (uint32_t i = 0; i < n; ++i) s.rays += (uint64_t)hj[i].rect_w * hj[i].rect_h;
+    s.ms_total = ms_total;
+    s.ms_march = ms_march;
+    s.ms_surface = ms_surface;
+    s.ms_shadow = ms_shadow;
+
+    // host outputs back
+    for (uint32_t i = 0; i < n; ++i) {
+        const nd_gbuffer_out& go = jobs[i].out;
+        if (go.device_ptrs) continue;
+        const DevJob& j = hj[i];
+        size_t px = (size_t)j.rect_w * j.rect_h;
+        struct { void* dst; const void* src; size_t b; } cp[6] = {
+            {go.depth, j.out_depth, 4 * px}, {go.normal, j.out_normal, 12 * px}, {go.trap, j.out_trap, 4 * px},
+            {go.hit, j.out_hit, px}, {go.shadow, j.out_shadow, 4 * px}, {go.evals, j.out_evals, 4 * px}};
+        for (auto& k : cp) {
+            if (!k.dst) continue;
+            if ((e = hipMemcpy(k.dst, k.src, k.b, hipMemcpyDeviceToHost)) != hipSuccess) return nd_hip_fail(c, e, "read outputs");
+        }
+    }
+    if (stats) *stats = s;
+    if (c->cancel && *c->cancel) return nd_fail(c, ND_E_CANCELLED, "cancelled");
+    return ND_OK;
+}
+
+extern "C" {
+
+nd_status nd_render_batch(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_opts* opts, nd_batch_stats* stats) {
+    return nd_batch_impl(c, jobs, n, opts, stats);
+}
+
+nd_status nd_render_gbuffer(nd_ctx* c, const void* uniforms, size_t size, const nd_march_opts* opts, nd_rect rect,
+                            const nd_gbuffer_out* out, nd_batch_stats* stats) {
+    if (!c) return ND_E_INVALID;
+    if (!uniforms) return nd_fail(c, ND_E_INVALID, "null uniforms");
+    if (size != sizeof(nd_uniforms)) return nd_fail(c, ND_E_INVALID, "uniforms_size %zu != %zu", size, sizeof(nd_uniforms));
+    if (!out) return nd_fail(c, ND_E_INVALID, "null output set");
+    nd_job j;
+    j.uniforms = (const nd_uniforms*)uniforms;
+    j.rect = rect;
+    j.out = *out;
+    return nd_batch_impl(c, &j, 1, opts, stats);
+}
+
+nd_status nd_batch_job_latency(nd_ctx* c, float* ms_out, uint32_t n) {
+    if (!c || !ms_out) return ND_E_INVALID;
+    if (n != c->job_latency_ms.size()) return nd_fail(c, ND_E_INVALID, "last batch had %zu jobs", c->job_latency_ms.size());
+    std::memcpy(ms_out, c->job_latency_ms.data(), sizeof(float) * n);
+    return ND_OK;
+}
+
+nd_status nd_autotune(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_opts* opts, nd_tuning* best) {
+    if (!c) return ND_E_INVALID;
+    nd_march_opts o{};
+    if (opts) o = *opts;
+    o.kernel = ND_KERNEL_PERSISTENT;
+    const nd_tuning saved = c->tuning;
+    const uint32_t occ = c->occupancy_waves ? c->occupancy_waves : 8;
+    const uint32_t wpcs[] = {occ / 2 ? occ / 2 : 1, occ, occ * 2};
+    const uint32_t idles[] = {1, 8, 32};
+    const uint32_t chunks[] = {64, 256};
+    nd_tuning bt = saved;
+    float bms = 1e30f;
+    for (uint32_t w : wpcs)
+        for (uint32_t mi : idles)
+            for (uint32_t ch : chunks) {
+                nd_tuning t{w, mi, ch, 0};
+                c->tuning = t;
+                nd_batch_stats s;
+                nd_status st = nd_batch_impl(c, jobs, n, &o, &s);
+                if (st != ND_OK) { c->tuning = saved; return st; }
+                if (s.ms_total < bms) { bms = s.ms_total; bt = t; }
+            }
+    c->tuning = bt;
+    if (best) *best = bt;
+    return ND_OK;
+}
+
+}  // extern "C"

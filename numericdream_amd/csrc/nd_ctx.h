@@ -1,0 +1,42 @@
+// nd_ctx.h - the context behind the opaque nd_ctx handle of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "nd_dev.h"
+#include "nd_hip.h"
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    nd_status ensure(nd_ctx* c, size_t bytes);
+    void release();
+};
+
+struct nd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char err[512] = {0};  // the error sink (webgpu_context.zig:131-146 keeps 512 bytes too)
+    int code_map[ND_MAX_CODES];
+    nd_tuning tuning{};
+    int cu_count = 256;
+    uint32_t occupancy_waves = 0;  // resident k_march waves per CU at full occupancy
+    char arch[64] = {0};
+    int wall_khz = 100000;
+    uint32_t round_cap = 0xffffffffu;
+    const volatile int* cancel = nullptr;
+    hipEvent_t ev[8] = {};
+    bool graph_valid = false;
+    hipGraphExec_t graph_exec = nullptr;
+    DevBuf jobs, job_base, ctr, hits, shared, staging, latency, adj, mc;
+    std::vector<DevJob> host_jobs;
+    std::vector<float> job_latency_ms;
+};
+
+nd_status nd_fail(nd_ctx* c, nd_status st, const char* fmt, ...);
+nd_status nd_hip_fail(nd_ctx* c, hipError_t e, const char* what);
+nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s);
+nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, nd_rect rect, DevJob& j);
+uint32_t nd_persistent_grid(nd_ctx* c);

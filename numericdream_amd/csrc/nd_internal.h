@@ -1,0 +1,27 @@
+// nd_internal.h - launch records shared between the C-ABI host code and the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nd_dev.h"
+
+struct NdPassLaunch {
+    const DevJob* jobs;          // device array of jobs
+    const DevScene* shared;      // device scene shared by all jobs, or null
+    const uint32_t* job_base;    // device: first ray slot of each job
+    int n_jobs;
+    uint32_t total_rays;
+    DevCounters* ctr;
+    DevHitBuffers hb;
+    uint32_t grid;               // persistent workgroups (one wave64 each)
+    uint32_t chunk;
+    uint32_t min_idle;
+    uint32_t round_cap;
+    int any_shadow;
+    unsigned long long* job_done_cycles;  // per-job completion clock, or null
+    hipEvent_t ev[4];            // optional: before march / surface / shadow / after
+};
+
+hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, hipStream_t s);
+hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s);
+uint32_t nd_march_occupancy();

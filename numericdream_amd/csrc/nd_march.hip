@@ -1,0 +1,530 @@
+// nd_march.hip - geometry passes for gfx950: the primary march with on-hit bisection,
+// the G-buffer surface pass (normal + orbit trap) and the light-0 soft shadow.
+//
+// Two schedules of the same per-ray math:
+//   plain       one launch per frame, one lane per pixel, one wave64 per 8x8 tile; a wave
+//               runs until its slowest ray is done (the reference's fragment dispatch,
+//               fs_simple, template.wgsl:4521-4538).
+//   persistent  three persistent passes over the whole batch of jobs: march (rays pulled
+//               from a device queue, a finished lane is refilled with the next ray at once,
+//               hits appended to a compacted hit list with one atomic per wave), surface
+//               (normal + trap of every hit), shadow (refilled like the march).
+// Reference semantics: march_to (template.wgsl:1859-1963), estimate_normal
+// (:1781-1804), visibility_march / shadow_factor (:2282-2375), frag_out (:3762-3774).
+#include <hip/hip_runtime.h>
+
+#include "nd_de.hpp"
+#include "nd_internal.h"
+
+namespace nd {
+
+// ----------------------------------------------------------------------------- rays
+// Primary ray (cs_wf_generate, template.wgsl:5119-5127).
+NDD V3<float> primary_dir(const DevJob& J, int px, int py) {
+    float nx = ((float)px + 0.5f) * J.inv_tw * 2.f - 1.f;
+    float ny = 1.f - ((float)py + 0.5f) * J.inv_th * 2.f;
+    float pfx = nx * J.tile_scale[0] + J.tile_bias[0];
+    float pfy = ny * J.tile_scale[1] + J.tile_bias[1];
+    float ux = pfx * J.aspect, uy = -pfy;
+    float dx = J.fwd[0] + ux * J.right[0] + uy * J.up[0];
+    float dy = J.fwd[1] + ux * J.right[1] + uy * J.up[1];
+    float dz = J.fwd[2] + ux * J.right[2] + uy * J.up[2];
+    float il = __frsqrt_rn(dx * dx + dy * dy + dz * dz);
+    return mk3(dx * il, dy * il, dz * il);
+}
+
+// Ray slot -> (job, x, y inside the rect). Slots enumerate 8x8 tiles of each job.
+NDD int find_job(const uint32_t* __restrict__ job_base, int n_jobs, uint32_t r) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (job_base[mid] <= r) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+NDD bool slot_pixel(const DevJob& J, uint32_t local, int* x, int* y) {
+    uint32_t tile = local >> 6, l = local & 63u;
+    int tx = (int)(tile % (uint32_t)J.tiles_x), ty = (int)(tile / (uint32_t)J.tiles_x);
+    *x = tx * 8 + (int)(l & 7u);
+    *y = ty * 8 + (int)(l >> 3);
+    return *x < J.rect_w && *y < J.rect_h;
+}
+
+// ----------------------------------------------------------------------------- march state
+enum { MS_CONT = 0, MS_HIT = 1, MS_MISS = 2 };
+struct March {
+    float t, lo, hi;
+    int i, refines, n, evals;
+    int probing, bisecting, resolved;
+};
+NDD void march_init(March& m) {
+    m.t = 0.f; m.lo = 0.f; m.hi = 0.f;
+    m.i = 0; m.refines = 0; m.n = 0; m.evals = 0;
+    m.probing = 1; m.bisecting = 0; m.resolved = 0;
+}
+NDD float march_sample_t(const March& m) { return m.probing ? m.t : 0.5f * (m.lo + m.hi); }
+// One step of march_to's loop after the DE d at sample_t s. *dist receives the result.
+NDD int march_update(March& m, const DevJob& J, float s, float d, float* dist) {
+    const int max_samples = J.max_steps * (1 + J.refine_budget) + J.bisect_budget;
+    m.n++;
+    m.evals++;
+    const float eps = J.eps_c * s + J.eps_f;
+    if (m.bisecting) {
+        if (d < eps) m.hi = s; else m.lo = s;
+        m.refines++;
+        if (m.hi - m.lo <= eps || m.refines >= J.bisect_budget) { *dist = m.hi; return MS_HIT; }
+        if (m.n >= max_samples) { *dist = m.t; return MS_MISS; }
+        return MS_CONT;
+    }
+    if (d < eps) {
+        if (J.refine_budget == 0 && J.bisect_budget > 0 && s - m.lo > eps) {
+            m.bisecting = 1; m.probing = 0; m.hi = s; m.refines = 0;
+            if (m.n >= max_samples) { *dist = m.t; return MS_MISS; }
+            return MS_CONT;
+        }
+        *dist = s;
+        return MS_HIT;
+    }
+    if (m.probing) {
+        m.lo = m.t; m.hi = m.t + d; m.refines = 0; m.resolved = 0; m.probing = 0;
+    } else {
+        if (d >= m.hi - s) m.resolved = 1; else m.hi = s;
+        m.refines++;
+    }
+    if (m.resolved || m.refines >= J.refine_budget) {
+        if (J.hq && !m.resolved && J.refine_budget > 0)
+            m.hi = fminf(m.hi, m.lo + fmaxf(m.t, 1.f) * J.footprint);
+        m.t = m.hi;
+        m.i++;
+        if (m.t > J.max_dist || m.i >= J.max_steps) { *dist = m.t; return MS_MISS; }
+        m.probing = 1;
+    }
+    if (m.n >= max_samples) { *dist = m.t; return MS_MISS; }
+    return MS_CONT;
+}
+
+// ----------------------------------------------------------------------------- surface
+NDD V3<float> vnorm(const V3<float>& g) {
+    float il = __frsqrt_rn(dot3(g, g));
+    return mk3(g.x * il, g.y * il, g.z * il);
+}
+NDD V3<float> surface_normal(const DevScene& sc, int mode, const V3<float>& p, float eps, int* evals) {
+    if (mode == 1) {  // tetrahedral, e = eps(t)
+        const float K[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+        V3<float> g = mk3(0.f, 0.f, 0.f);
+        for (int i = 0; i < 4; ++i) {
+            V3<float> k = mk3(K[i][0], K[i][1], K[i][2]);
+            float d = scene_de<false>(sc, mk3(p.x + k.x * eps, p.y + k.y * eps, p.z + k.z * eps), nullptr);
+            g = mk3(g.x + k.x * d, g.y + k.y * d, g.z + k.z * d);
+        }
+        *evals += 4;
+        return vnorm(g);
+    }
+    if (mode == 2) {  // central, h = eps / 2
+        float h = 0.5f * eps;
+        float gx = scene_de<false>(sc, mk3(p.x + h, p.y, p.z), nullptr) - scene_de<false>(sc, mk3(p.x - h, p.y, p.z), nullptr);
+        float gy = scene_de<false>(sc, mk3(p.x, p.y + h, p.z), nullptr) - scene_de<false>(sc, mk3(p.x, p.y - h, p.z), nullptr);
+        float gz = scene_de<false>(sc, mk3(p.x, p.y, p.z + h), nullptr) - scene_de<false>(sc, mk3(p.x, p.y, p.z - h), nullptr);
+        *evals += 6;
+        return vnorm(mk3(gx, gy, gz));
+    }
+    // analytic: forward-mode derivative of the DE
+    using D3 = DualN<3>;
+    V3<D3> q = mk3(dvar<3>(p.x, 0), dvar<3>(p.y, 1), dvar<3>(p.z, 2));
+    D3 d = scene_de_d<D3>(sc, q);
+    *evals += 1;
+    return vnorm(mk3(d.d[0], d.d[1], d.d[2]));
+}
+
+// Light 0 direction and distance at a surface point (light_sample, template.wgsl:2178-2190).
+NDD void light_at(const DevJob& J, const V3<float>& p, V3<float>* dir, float* dist) {
+    if (J.light_type == 1) {
+        *dir = mk3(J.light_v[0], J.light_v[1], J.light_v[2]);
+        *dist = J.light_far;
+    } else {
+        V3<float> to = mk3(J.light_v[0] - p.x, J.light_v[1] - p.y, J.light_v[2] - p.z);
+        float dd = len3(to);
+        float inv = frcp(fmaxf(dd, 1e-4f));
+        *dir = mk3(to.x * inv, to.y * inv, to.z * inv);
+        *dist = dd;
+    }
+}
+
+// With SHARED every job of the batch has the same scene: it is read through a
+// kernel-argument pointer, so its fields are wave-uniform scalar loads.
+template <bool SHARED>
+NDD const DevScene& scene_of(const DevJob* jobs, const DevScene* shared, int job) {
+    if (SHARED) return *shared;
+    return jobs[job].scene;
+}
+
+// ============================================================================ plain kernel
+// One lane per pixel of one job; blockDim 64 = one 8x8 tile per wave64.
+__global__ void __launch_bounds__(64) k_plain(const DevJob* __restrict__ jobp) {
+    const DevJob& J = *jobp;
+    const DevScene& sc = J.scene;
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7);
+    const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (x >= J.rect_w || y >= J.rect_h) return;
+    const V3<float> dir = primary_dir(J, J.rect_x + x, J.rect_y + y);
+    const V3<float> org = mk3(J.cam[0], J.cam[1], J.cam[2]);
+    March m;
+    march_init(m);
+    float dist = 0.f;
+    int st;
+    do {
+        float s = march_sample_t(m);
+        float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
+        st = march_update(m, J, s, d, &dist);
+    } while (st == MS_CONT);
+    const size_t o = (size_t)y * J.rect_w + x;
+    int evals = m.evals;
+    if (st != MS_HIT) {
+        if (J.out_depth) J.out_depth[o] = 0.f;
+        if (J.out_normal) { J.out_normal[3 * o] = 0.f; J.out_normal[3 * o + 1] = 0.f; J.out_normal[3 * o + 2] = 0.f; }
+        if (J.out_trap) J.out_trap[o] = 0.f;
+        if (J.out_hit) J.out_hit[o] = 0;
+        if (J.out_shadow) J.out_shadow[o] = 1.f;
+        if (J.out_evals) J.out_evals[o] = (uint32_t)evals;
+        return;
+    }
+    const V3<float> hp = mk3(org.x + dir.x * dist, org.y + dir.y * dist, org.z + dir.z * dist);
+    int sev = 0;
+    const V3<float> n = surface_normal(sc, J.normal_mode, hp, J.eps_c * dist + J.eps_f, &sev);
+    float trap = 0.f;
+    (void)scene_de<true>(sc, hp, &trap);
+    float shadow = 1.f;
+    if (J.shadow_on) {
+        V3<float> ld; float ldist;
+        light_at(J, hp, &ld, &ldist);
+        if (dot3(n, ld) <= 0.f) {
+            shadow = 0.f;
+        } else {
+            const V3<float> so = mk3(hp.x + n.x * 0.004f, hp.y + n.y * 0.004f, hp.z + n.z * 0.004f);
+            float t = 0.004f, res = 1.f;
+            for (int i = 0; i < J.shadow_steps; ++i) {
+                if (t >= ldist) break;
+                float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
+                evals++;
+                if (d < 0.0005f) { res = 0.f; break; }
+                res = fminf(res, J.light_k * d * frcp(t));
+                t += fmaxf(d, 0.001f);
+            }
+            shadow = fclamp(res, 0.f, 1.f);
+        }
+    }
+    if (J.out_depth) J.out_depth[o] = dist;
+    if (J.out_normal) { J.out_normal[3 * o] = n.x; J.out_normal[3 * o + 1] = n.y; J.out_normal[3 * o + 2] = n.z; }
+    if (J.out_trap) J.out_trap[o] = trap;
+    if (J.out_hit) J.out_hit[o] = 1;
+    if (J.out_shadow) J.out_shadow[o] = shadow;
+    if (J.out_evals) J.out_evals[o] = (uint32_t)evals;
+}
+
+// ============================================================================ persistent passes
+__global__ void k_zero_counters(DevCounters* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->ray_next = 0; c->hit_count = 0; c->surf_next = 0; c->shadow_next = 0;
+        c->de_march = 0; c->de_surface = 0; c->de_shadow = 0; c->rays_done = 0;
+        c->rounds_capped = 0;
+        c->clock_start = wall_clock64();
+    }
+}
+
+NDD uint64_t lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Claims work items for the idle lanes of the wave from a global queue in chunks of
+// `chunk` items (one atomic per chunk). Returns per lane the item index or ~0u.
+// cnext/cend are wave-uniform; work_left is cleared when the queue is empty.
+NDD uint32_t claim(bool idle, uint32_t& cnext, uint32_t& cend, bool& work_left, uint32_t* head, uint32_t total,
+                   uint32_t chunk) {
+    uint32_t item = ~0u;
+    uint64_t need = __builtin_amdgcn_ballot_w64(idle);
+    while (need != 0ull && work_left) {
+        if (cnext >= cend) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(head, chunk);
+            base = __shfl(base, 0);
+            if (base >= total) { work_left = false; break; }
+            cnext = base;
+            cend = min(base + chunk, total);
+        }
+        const uint32_t avail = cend - cnext;
+        const uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
+        const bool mine = ((need >> __lane_id()) & 1ull) && rank < avail;
+        if (mine) item = cnext + rank;
+        const uint32_t taken = min((uint32_t)__popcll(need), avail);
+        cnext += taken;
+        need = __builtin_amdgcn_ballot_w64(((need >> __lane_id()) & 1ull) && !mine);
+    }
+    return item;
+}
+
+struct PassArgs {
+    const DevJob* jobs;
+    const DevScene* shared;  // non-null when every job has the same scene
+    const uint32_t* job_base;
+    int n_jobs;
+    uint32_t total_rays;
+    DevCounters* ctr;
+    DevHitBuffers hb;
+    uint32_t chunk;
+    uint32_t min_idle;
+    uint32_t round_cap;
+    float* job_done_ms;      // unused on device; latency is tracked with job_done_cycles
+    unsigned long long* job_done_cycles;  // per job: wall clock (s_memrealtime) of its last retired ray
+};
+
+NDD void write_miss(const DevJob& J, size_t o, int evals) {
+    if (J.out_depth) J.out_depth[o] = 0.f;
+    if (J.out_normal) { J.out_normal[3 * o] = 0.f; J.out_normal[3 * o + 1] = 0.f; J.out_normal[3 * o + 2] = 0.f; }
+    if (J.out_trap) J.out_trap[o] = 0.f;
+    if (J.out_hit) J.out_hit[o] = 0;
+    if (J.out_shadow) J.out_shadow[o] = 1.f;
+    if (J.out_evals) J.out_evals[o] = (uint32_t)evals;
+}
+
+// --- pass 1: march. One DE per active lane per round; finished lanes refill at once.
+template <bool SHARED>
+__global__ void __launch_bounds__(64) k_march(PassArgs a) {
+    uint32_t cnext = 0, cend = 0;
+    bool work_left = true;
+    bool active = false;
+    uint32_t ray = 0;
+    int job = 0, px = 0, py = 0;
+    V3<float> org = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 1.f);
+    March m;
+    march_init(m);
+    unsigned long long de = 0;
+    for (uint32_t round = 0;; ++round) {
+        if (work_left) {
+            const uint64_t idle = __builtin_amdgcn_ballot_w64(!active);
+            if ((uint32_t)__popcll(idle) >= a.min_idle || idle == ~0ull) {
+                uint32_t r = claim(!active, cnext, cend, work_left, &a.ctr->ray_next, a.total_rays, a.chunk);
+                if (r != ~0u) {
+                    ray = r;
+                    job = find_job(a.job_base, a.n_jobs, r);
+                    const DevJob& J = a.jobs[job];
+                    if (slot_pixel(J, r - J.ray_base, &px, &py)) {
+                        dir = primary_dir(J, J.rect_x + px, J.rect_y + py);
+                        org = mk3(J.cam[0], J.cam[1], J.cam[2]);
+                        march_init(m);
+                        active = true;
+                    }
+                }
+            }
+        }
+        const uint64_t act = __builtin_amdgcn_ballot_w64(active);
+        if (act == 0ull) {
+            if (!work_left) break;
+            continue;
+        }
+        de += (unsigned long long)__popcll(act);
+        bool hit_now = false;
+        float dist = 0.f;
+        if (active) {
+            const DevJob& J = a.jobs[job];
+            const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
+            const float s = march_sample_t(m);
+            const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
+            const int st = march_update(m, J, s, d, &dist);
+            if (st != MS_CONT) {
+                const size_t o = (size_t)py * J.rect_w + px;
+                if (st == MS_HIT) {
+                    hit_now = true;
+                    if (J.out_depth) J.out_depth[o] = dist;
+                    if (J.out_hit) J.out_hit[o] = 1;
+                    if (J.out_evals) J.out_evals[o] = (uint32_t)m.evals;
+                } else {
+                    write_miss(J, o, m.evals);
+                }
+                active = false;
+                if (a.job_done_cycles) atomicMax(&a.job_done_cycles[job], (unsigned long long)wall_clock64());
+            }
+        }
+        const uint64_t hm = __builtin_amdgcn_ballot_w64(hit_now);
+        if (hm) {
+            const int leader = __builtin_ctzll(hm);
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&a.ctr->hit_count, (uint32_t)__popcll(hm));
+            base = __shfl(base, leader);
+            if (hit_now) {
+                const uint32_t idx = base + (uint32_t)__popcll(hm & lanemask_lt());
+                a.hb.ray[idx] = ray;
+                a.hb.job[idx] = (uint16_t)job;
+                a.hb.t[idx] = dist;
+            }
+        }
+        if (round >= a.round_cap) {
+            if (__lane_id() == 0) atomicAdd(&a.ctr->rounds_capped, 1u);
+            break;
+        }
+    }
+    if (__lane_id() == 0) atomicAdd(&a.ctr->de_march, de);
+}
+
+// Decodes hit i: job, output index, hit point, view direction.
+NDD void hit_geometry(const PassArgs& a, uint32_t i, int* job, size_t* o, V3<float>* hp) {
+    const uint32_t r = a.hb.ray[i];
+    *job = a.hb.job[i];
+    const DevJob& J = a.jobs[*job];
+    int x, y;
+    slot_pixel(J, r - J.ray_base, &x, &y);
+    *o = (size_t)y * J.rect_w + x;
+    const V3<float> dir = primary_dir(J, J.rect_x + x, J.rect_y + y);
+    const float t = a.hb.t[i];
+    *hp = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
+}
+
+// --- pass 2: surface. Normal + trap of every hit (fixed cost per hit, no refill needed).
+template <bool SHARED>
+__global__ void __launch_bounds__(64) k_surface(PassArgs a) {
+    const uint32_t n_hits = a.ctr->hit_count;
+    unsigned long long de = 0;
+    for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i - threadIdx.x < n_hits; i += gridDim.x * 64) {
+        if (i < n_hits) {
+            int job; size_t o; V3<float> hp;
+            hit_geometry(a, i, &job, &o, &hp);
+            const DevJob& J = a.jobs[job];
+            const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
+            const float t = a.hb.t[i];
+            int ev = 0;
+            const V3<float> n = surface_normal(sc, J.normal_mode, hp, J.eps_c * t + J.eps_f, &ev);
+            float trap = 0.f;
+            (void)scene_de<true>(sc, hp, &trap);
+            ev += 1;
+            de += (unsigned long long)ev;
+            a.hb.nx[i] = n.x; a.hb.ny[i] = n.y; a.hb.nz[i] = n.z;
+            if (J.out_normal) { J.out_normal[3 * o] = n.x; J.out_normal[3 * o + 1] = n.y; J.out_normal[3 * o + 2] = n.z; }
+            if (J.out_trap) J.out_trap[o] = trap;
+            if (J.out_shadow && !J.shadow_on) J.out_shadow[o] = 1.f;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off);
+    if (threadIdx.x == 0) atomicAdd(&a.ctr->de_surface, de);
+}
+
+// --- pass 3: light-0 soft shadow of every hit, refilled like the march.
+template <bool SHARED>
+__global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
+    const uint32_t n_hits = a.ctr->hit_count;
+    uint32_t cnext = 0, cend = 0;
+    bool work_left = true, active = false;
+    int job = 0, steps = 0, step = 0;
+    size_t o = 0;
+    V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
+    float t = 0.f, res = 1.f, ldist = 0.f;
+    unsigned long long de = 0;
+    for (uint32_t round = 0;; ++round) {
+        if (work_left) {
+            const uint64_t idle = __builtin_amdgcn_ballot_w64(!active);
+            if ((uint32_t)__popcll(idle) >= a.min_idle || idle == ~0ull) {
+                uint32_t i = claim(!active, cnext, cend, work_left, &a.ctr->shadow_next, n_hits, a.chunk);
+                if (i != ~0u) {
+                    V3<float> hp;
+                    hit_geometry(a, i, &job, &o, &hp);
+                    const DevJob& J = a.jobs[job];
+                    if (J.shadow_on) {
+                        const V3<float> n = mk3(a.hb.nx[i], a.hb.ny[i], a.hb.nz[i]);
+                        light_at(J, hp, &ld, &ldist);
+                        if (dot3(n, ld) <= 0.f) {
+                            if (J.out_shadow) J.out_shadow[o] = 0.f;
+                        } else {
+                            so = mk3(hp.x + n.x * 0.004f, hp.y + n.y * 0.004f, hp.z + n.z * 0.004f);
+                            t = 0.004f; res = 1.f; step = 0; steps = J.shadow_steps;
+                            if (steps > 0 && t < ldist) active = true;
+                            else if (J.out_shadow) J.out_shadow[o] = 1.f;
+                        }
+                    }
+                }
+            }
+        }
+        const uint64_t act = __builtin_amdgcn_ballot_w64(active);
+        if (act == 0ull) {
+            if (!work_left) break;
+            continue;
+        }
+        de += (unsigned long long)__popcll(act);
+        if (active) {
+            const DevJob& J = a.jobs[job];
+            const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
+            const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
+            step++;
+            bool done = false;
+            if (d < 0.0005f) { res = 0.f; done = true; }
+            else {
+                res = fminf(res, J.light_k * d * frcp(t));
+                t += fmaxf(d, 0.001f);
+                if (step >= steps || t >= ldist) done = true;
+            }
+            if (done) {
+                if (J.out_shadow) J.out_shadow[o] = fclamp(res, 0.f, 1.f);
+                if (J.out_evals) J.out_evals[o] += (uint32_t)step;
+                active = false;
+                if (a.job_done_cycles) atomicMax(&a.job_done_cycles[job], (unsigned long long)wall_clock64());
+            }
+        }
+        if (round >= a.round_cap) {
+            if (__lane_id() == 0) atomicAdd(&a.ctr->rounds_capped, 1u);
+            break;
+        }
+    }
+    if (__lane_id() == 0) atomicAdd(&a.ctr->de_shadow, de);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+}  // namespace nd
+
+// ----------------------------------------------------------------------------- launchers
+hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, hipStream_t s) {
+    hipLaunchKernelGGL(nd::k_plain, dim3(tiles_x, tiles_y), dim3(64), 0, s, d_job);
+    return hipGetLastError();
+}
+
+hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
+    nd::PassArgs a;
+    a.jobs = L.jobs;
+    a.shared = L.shared;
+    a.job_base = L.job_base;
+    a.n_jobs = L.n_jobs;
+    a.total_rays = L.total_rays;
+    a.ctr = L.ctr;
+    a.hb = L.hb;
+    a.chunk = L.chunk;
+    a.min_idle = L.min_idle;
+    a.round_cap = L.round_cap;
+    a.job_done_ms = nullptr;
+    a.job_done_cycles = L.job_done_cycles;
+    hipLaunchKernelGGL(nd::k_zero_counters, dim3(1), dim3(64), 0, s, L.ctr);
+    if (L.job_done_cycles) {
+        hipLaunchKernelGGL(nd::k_zero_u64, dim3((L.n_jobs + 255) / 256), dim3(256), 0, s, L.job_done_cycles, L.n_jobs);
+    }
+    if (L.ev[0]) hipEventRecord(L.ev[0], s);
+    if (L.shared) hipLaunchKernelGGL(nd::k_march<true>, dim3(L.grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(nd::k_march<false>, dim3(L.grid), dim3(64), 0, s, a);
+    if (L.ev[1]) hipEventRecord(L.ev[1], s);
+    if (L.shared) hipLaunchKernelGGL(nd::k_surface<true>, dim3(L.grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(nd::k_surface<false>, dim3(L.grid), dim3(64), 0, s, a);
+    if (L.ev[2]) hipEventRecord(L.ev[2], s);
+    if (L.any_shadow) {
+        if (L.shared) hipLaunchKernelGGL(nd::k_shadow<true>, dim3(L.grid), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL(nd::k_shadow<false>, dim3(L.grid), dim3(64), 0, s, a);
+    }
+    if (L.ev[3]) hipEventRecord(L.ev[3], s);
+    return hipGetLastError();
+}
+
+// Resident wave64 workgroups of the march kernel per CU at full occupancy.
+uint32_t nd_march_occupancy() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, nd::k_march<true>, 64, 0) != hipSuccess || blocks <= 0)
+        return 8;
+    return (uint32_t)blocks;
+}

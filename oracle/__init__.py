@@ -1,0 +1,147 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the scalar CPU oracle (oracle/libndoracle.so).
+
+The oracle restates the reference's WGSL march, G-buffer and soft shadow in C++ templated on
+the scalar type (FP64 oracle, FP32 CPU restatement, FLOP counter, dual numbers). Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it, as the checker,
+never as the thing measured or shipped.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libndoracle.so")
+
+
+def build():
+    r = subprocess.run(["make", "-C", _HERE, "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, i32, dp = C.c_void_p, C.c_int, C.POINTER(C.c_double)
+        L.ndo_abi_size.restype = i32
+        L.ndo_scene_de.argtypes = [vp, vp, i32, vp, i32, vp, vp]
+        L.ndo_march.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp, vp, vp]
+        L.ndo_normal.argtypes = [vp, vp, i32, vp, vp, i32, vp]
+        L.ndo_shadow.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]
+        L.ndo_primary_dir.argtypes = [vp, i32, i32, vp]
+        L.ndo_render_gbuffer.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp,
+                                         i32, vp]
+        L.ndo_flops_per_de.argtypes = [vp, vp, vp, i32, vp]
+        L.ndo_flops_per_de.restype = C.c_double
+        L.ndo_helper.argtypes = [i32, vp, vp]
+        L.ndo_formula_step.argtypes = [i32, vp, C.c_double, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _cm(code_map):
+    if code_map is None:
+        return None, None
+    a = np.asarray(list(code_map) + [-1] * (16 - len(code_map)), np.int32)
+    return a, a.ctypes.data
+
+
+def _u(u):
+    return C.addressof(u)
+
+
+def scene_de(u, pts, prec=0, trap=False, code_map=None):
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    d = np.zeros(len(pts))
+    t = np.zeros(len(pts)) if trap else None
+    cm, cmp_ = _cm(code_map)
+    lib().ndo_scene_de(_u(u), cmp_, prec, _p(pts), len(pts), _p(d), _p(t))
+    return (d, t) if trap else d
+
+
+def march(u, origins, dirs, prec=0, code_map=None):
+    o = np.ascontiguousarray(origins, np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    n = len(o)
+    hit = np.zeros(n, np.int32); t = np.zeros(n); steps = np.zeros(n, np.int32); ev = np.zeros(n, np.int32)
+    cm, cmp_ = _cm(code_map)
+    lib().ndo_march(_u(u), cmp_, prec, _p(o), _p(d), n, _p(hit), _p(t), _p(steps), _p(ev))
+    return hit.astype(bool), t, steps, ev
+
+
+def normal(u, pts, mode, e=None, code_map=None):
+    p = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    e = np.zeros(len(p)) if e is None else np.ascontiguousarray(np.broadcast_to(e, (len(p),)), np.float64)
+    out = np.zeros((len(p), 3))
+    cm, cmp_ = _cm(code_map)
+    lib().ndo_normal(_u(u), cmp_, mode, _p(p), _p(e), len(p), _p(out))
+    return out
+
+
+def shadow(u, origin, direction, light_dist, k, steps, code_map=None):
+    o = np.asarray(origin, np.float64); d = np.asarray(direction, np.float64)
+    vis = np.zeros(1); ev = np.zeros(1, np.int32)
+    cm, cmp_ = _cm(code_map)
+    lib().ndo_shadow(_u(u), cmp_, _p(o), _p(d), float(light_dist), float(k), int(steps), _p(vis), _p(ev))
+    return float(vis[0]), int(ev[0])
+
+
+def primary_dir(u, px, py):
+    out = np.zeros(3)
+    lib().ndo_primary_dir(_u(u), px, py, _p(out))
+    return out
+
+
+def render_gbuffer(u, rect=None, prec=0, normal_mode=0, shadows=True, shadow_max_steps=0, threads=None,
+                   code_map=None, cam_offset=None):
+    """G-buffer of a rect (x, y, w, h) as numpy arrays (same layout as the HIP outputs)."""
+    W, H = int(u.resolution[0]), int(u.resolution[1])
+    x, y, w, h = rect if rect else (0, 0, W, H)
+    out = {"depth": np.zeros((h, w), np.float32), "normal": np.zeros((h, w, 3), np.float32),
+           "trap": np.zeros((h, w), np.float32), "hit": np.zeros((h, w), np.uint8),
+           "shadow": np.zeros((h, w), np.float32), "evals": np.zeros((h, w), np.uint32)}
+    cm, cmp_ = _cm(code_map)
+    co = None if cam_offset is None else np.asarray(cam_offset, np.float64)
+    threads = threads or min(16, os.cpu_count() or 1)
+    lib().ndo_render_gbuffer(_u(u), cmp_, prec, normal_mode, shadow_max_steps, 1 if shadows else 0, x, y, w, h,
+                             _p(out["depth"]), _p(out["normal"]), _p(out["trap"]), _p(out["hit"]),
+                             _p(out["shadow"]), _p(out["evals"]), threads, _p(co))
+    return out
+
+
+def flops_per_de(u, pts, code_map=None):
+    p = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    tops = np.zeros(1)
+    cm, cmp_ = _cm(code_map)
+    f = lib().ndo_flops_per_de(_u(u), cmp_, _p(p), len(p), _p(tops))
+    return f, float(tops[0])
+
+
+def helper(op, args, nout=4):
+    a = np.zeros(64); a[:len(args)] = args
+    out = np.zeros(nout)
+    r = lib().ndo_helper(op, _p(a), _p(out))
+    if r != 0:
+        raise ValueError("unknown helper op %d" % op)
+    return out
+
+
+def formula_step(f, z, dr, pos, params):
+    z = np.asarray(z, np.float64); pos = np.asarray(pos, np.float64)
+    p = np.zeros(16); p[:len(params)] = params
+    out = np.zeros(4)
+    lib().ndo_formula_step(f, _p(z), float(dr), _p(pos), _p(p), _p(out))
+    return out[:3], out[3]

@@ -1,0 +1,325 @@
+// nd_oracle.cpp - TEST INFRASTRUCTURE ONLY: C entry points over the scalar restatement in
+// nd_oracle.hpp, loaded by tests/ (ctypes), the smoke check and the bench's cpu_baseline
+// leg. Never linked into, or called by, the product library.
+//
+// prec = 0 evaluates in FP64 (the oracle), prec = 1 in FP32 (the CPU restatement).
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "nd_oracle.hpp"
+
+using namespace ndo;
+
+namespace {
+
+CodeMap code_map(const int32_t* cm) {
+    CodeMap m;
+    if (cm)
+        for (int i = 0; i < 16; ++i) m.map[i] = cm[i];
+    return m;
+}
+
+struct GbufOpts {
+    int normal_mode;       // 0 analytic, 1 tetra, 2 central (h = 0.5 eps)
+    int shadow_max_steps;  // 0: min(max_steps, 40)
+    int shadows;
+};
+
+template <class T>
+void pixel_gbuffer(const nd_uniforms& u, const Scene<T>& s, const MarchCfg& mc, const GbufOpts& o, int px,
+                   int py, float* depth, float* normal, float* trap, uint8_t* hit, float* shadow,
+                   uint32_t* evals) {
+    V3<T> dir = primary_dir<T>(u, px, py, (int)u.resolution[0], (int)u.resolution[1]);
+    V3<T> org = V3<T>::from(u.camera_pos) + V3<T>::from(s.cam_offset);
+    MarchResult<T> m = march_to(s, mc, org, dir, T(u.max_dist));
+    int ev = m.evals;
+    if (!m.hit) {
+        *depth = 0; normal[0] = normal[1] = normal[2] = 0; *trap = 0; *hit = 0; *shadow = 1;
+        *evals = (uint32_t)ev;
+        return;
+    }
+    V3<T> hp = org + dir * m.dist;
+    T eps = T(mc.eps_c) * m.dist + T(mc.eps_f);
+    V3<T> n;
+    if (o.normal_mode == 1) n = normal_tetra(s, hp, eps);
+    else if (o.normal_mode == 2) n = normal_central(s, hp, T(0.5) * eps);
+    else {
+        // analytic: forward-mode derivative evaluated in FP64 at the (T-precision) hit point
+        Scene<double> sd;
+        sd.count = s.count; sd.max_dist = s.max_dist; sd.carve_count = s.carve_count;
+        std::memcpy(sd.carves, s.carves, sizeof(s.carves));
+        for (int i = 0; i < ND_MAX_INSTANCES; ++i) {
+            const auto& a = s.inst[i]; auto& b = sd.inst[i];
+            b.formula = a.formula; b.hidden = a.hidden; b.combine = a.combine; b.blend_k = a.blend_k;
+            std::memcpy(b.offset, a.offset, sizeof(a.offset)); std::memcpy(b.scale, a.scale, sizeof(a.scale));
+            std::memcpy(b.frame, a.frame, sizeof(a.frame)); b.step_safety = a.step_safety; b.trap = a.trap;
+            for (int k = 0; k < 16; ++k) b.p.p[k] = (double)val(a.p.p[k]);
+        }
+        V3<double> nd = normal_analytic(sd, V3<double>((double)hp.x, (double)hp.y, (double)hp.z));
+        n = V3<T>(T(nd.x), T(nd.y), T(nd.z));
+    }
+    T tr = T(0);
+    scene_de(s, hp, &tr);
+    float sh = 1.0f;
+    if (o.shadows && (int)u.light_count >= 1) {
+        const nd_light& L = u.lights[0];
+        if (L.cast_shadows > 0.5f) {
+            V3<T> ld; T dist;
+            light_dir_dist(L, u.max_dist, hp, &ld, &dist);
+            if (dot(n, ld) <= T(0)) {
+                sh = 0.0f;  // shade_local skips a light with n.l <= 0 (template.wgsl:3276-3278)
+            } else {
+                double k = L.shadow_softness + (512.0 - L.shadow_softness) * L.hard_shadows;
+                int steps = o.shadow_max_steps > 0 ? o.shadow_max_steps : std::min(i32(u.max_steps), 40);
+                int sev = 0;
+                T vis = visibility_march_opaque(s, hp + n * T(0.004), ld, dist, T(k), steps, &sev);
+                ev += sev;
+                sh = (float)val(vis);
+            }
+        }
+    }
+    *depth = (float)val(m.dist);
+    normal[0] = (float)val(n.x); normal[1] = (float)val(n.y); normal[2] = (float)val(n.z);
+    *trap = (float)val(tr);
+    *hit = 1;
+    *shadow = sh;
+    *evals = (uint32_t)ev;
+}
+
+template <class T>
+int render_gbuffer_t(const nd_uniforms* u, const int32_t* cm, const GbufOpts& o, int rx, int ry, int rw, int rh,
+                     float* depth, float* normal, float* trap, uint8_t* hit, float* shadow, uint32_t* evals,
+                     int nthreads, const double* cam_offset) {
+    Scene<T> s = decode_scene<T>(*u, code_map(cm));
+    if (cam_offset) for (int k = 0; k < 3; ++k) s.cam_offset[k] = cam_offset[k];
+    MarchCfg mc = march_cfg(*u);
+    std::atomic<int> next_row{0};
+    auto worker = [&]() {
+        for (;;) {
+            int y = next_row.fetch_add(1);
+            if (y >= rh) break;
+            for (int x = 0; x < rw; ++x) {
+                size_t i = (size_t)y * rw + x;
+                float dd, nn[3], tt, ss; uint8_t hh; uint32_t ee;
+                pixel_gbuffer<T>(*u, s, mc, o, rx + x, ry + y, &dd, nn, &tt, &hh, &ss, &ee);
+                if (depth) depth[i] = dd;
+                if (normal) { normal[3 * i] = nn[0]; normal[3 * i + 1] = nn[1]; normal[3 * i + 2] = nn[2]; }
+                if (trap) trap[i] = tt;
+                if (hit) hit[i] = hh;
+                if (shadow) shadow[i] = ss;
+                if (evals) evals[i] = ee;
+            }
+        }
+    };
+    nthreads = std::max(1, nthreads);
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthreads; ++i) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ndo_abi_size(void) { return (int)sizeof(nd_uniforms); }
+
+int ndo_scene_de(const nd_uniforms* u, const int32_t* cm, int prec, const double* p, int n, double* d_out,
+                 double* trap_out) {
+    if (prec == 0) {
+        Scene<double> s = decode_scene<double>(*u, code_map(cm));
+        for (int i = 0; i < n; ++i) {
+            double tr = 0;
+            d_out[i] = scene_de(s, V3<double>(p[3 * i], p[3 * i + 1], p[3 * i + 2]), trap_out ? &tr : nullptr);
+            if (trap_out) trap_out[i] = tr;
+        }
+    } else {
+        Scene<float> s = decode_scene<float>(*u, code_map(cm));
+        for (int i = 0; i < n; ++i) {
+            float tr = 0;
+            d_out[i] = scene_de(s, V3<float>((float)p[3 * i], (float)p[3 * i + 1], (float)p[3 * i + 2]),
+                                trap_out ? &tr : nullptr);
+            if (trap_out) trap_out[i] = tr;
+        }
+    }
+    return 0;
+}
+
+int ndo_march(const nd_uniforms* u, const int32_t* cm, int prec, const double* o, const double* d, int n,
+              int32_t* hit, double* t, int32_t* steps, int32_t* evals) {
+    MarchCfg mc = march_cfg(*u);
+    if (prec == 0) {
+        Scene<double> s = decode_scene<double>(*u, code_map(cm));
+        for (int i = 0; i < n; ++i) {
+            auto m = march_to(s, mc, V3<double>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                              V3<double>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), (double)u->max_dist);
+            hit[i] = m.hit; t[i] = m.dist; steps[i] = m.steps; evals[i] = m.evals;
+        }
+    } else {
+        Scene<float> s = decode_scene<float>(*u, code_map(cm));
+        for (int i = 0; i < n; ++i) {
+            auto m = march_to(s, mc, V3<float>((float)o[3 * i], (float)o[3 * i + 1], (float)o[3 * i + 2]),
+                              V3<float>((float)d[3 * i], (float)d[3 * i + 1], (float)d[3 * i + 2]), u->max_dist);
+            hit[i] = m.hit; t[i] = m.dist; steps[i] = m.steps; evals[i] = m.evals;
+        }
+    }
+    return 0;
+}
+
+// mode: 0 analytic (FP64 dual numbers), 1 tetra e = eps, 2 central h = eps / 2; e is given.
+int ndo_normal(const nd_uniforms* u, const int32_t* cm, int mode, const double* pos, const double* e, int n,
+               double* out) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    for (int i = 0; i < n; ++i) {
+        V3<double> p(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        V3<double> r = mode == 1 ? normal_tetra(s, p, e[i]) : mode == 2 ? normal_central(s, p, e[i]) : normal_analytic(s, p);
+        out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+    }
+    return 0;
+}
+
+int ndo_shadow(const nd_uniforms* u, const int32_t* cm, const double* origin, const double* dir, double light_dist,
+               double k, int steps, double* vis, int32_t* evals) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    int ev = 0;
+    *vis = visibility_march_opaque(s, V3<double>(origin[0], origin[1], origin[2]), V3<double>(dir[0], dir[1], dir[2]),
+                                   light_dist, k, steps, &ev);
+    if (evals) *evals = ev;
+    return 0;
+}
+
+int ndo_primary_dir(const nd_uniforms* u, int px, int py, double* out) {
+    V3<double> d = primary_dir<double>(*u, px, py, (int)u->resolution[0], (int)u->resolution[1]);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+    return 0;
+}
+
+int ndo_render_gbuffer(const nd_uniforms* u, const int32_t* cm, int prec, int normal_mode, int shadow_max_steps,
+                       int shadows, int rx, int ry, int rw, int rh, float* depth, float* normal, float* trap,
+                       uint8_t* hit, float* shadow, uint32_t* evals, int nthreads, const double* cam_offset) {
+    GbufOpts o{normal_mode, shadow_max_steps, shadows};
+    if (prec == 0)
+        return render_gbuffer_t<double>(u, cm, o, rx, ry, rw, rh, depth, normal, trap, hit, shadow, evals, nthreads,
+                                        cam_offset);
+    return render_gbuffer_t<float>(u, cm, o, rx, ry, rw, rh, depth, normal, trap, hit, shadow, evals, nthreads,
+                                   cam_offset);
+}
+
+// Mean ND-FLOP v1 per scene_de over the given points (uniform-only subexpressions
+// hoisted); *tops_out receives the mean transcendental count.
+double ndo_flops_per_de(const nd_uniforms* u, const int32_t* cm, const double* p, int n, double* tops_out) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    g_flops = FlopCounter{};
+    for (int i = 0; i < n; ++i) {
+        V3<Flop> q(Flop(p[3 * i]), Flop(p[3 * i + 1]), Flop(p[3 * i + 2]));
+        (void)scene_de(s, q);
+    }
+    if (tops_out) *tops_out = n ? (double)g_flops.tops / n : 0.0;
+    return n ? (double)g_flops.flops / n : 0.0;
+}
+
+// FP64 G-buffer render of a rect with the FLOP probe on: mean ND-FLOP v1 and T-ops per
+// scene_de over the march (which = 0) or shadow (which = 1) evaluation points.
+int ndo_flops_render(const nd_uniforms* u, const int32_t* cm, int shadow_max_steps, int rx, int ry, int rw, int rh,
+                     int step, double* out) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    MarchCfg mc = march_cfg(*u);
+    uint64_t ev[2] = {0, 0}, fl[2] = {0, 0}, tp[2] = {0, 0};
+    for (int y = 0; y < rh; y += step)
+        for (int x = 0; x < rw; x += step) {
+            // march evaluations
+            g_probe = FlopProbe{};
+            g_probe.on = true;
+            V3<double> dir = primary_dir<double>(*u, rx + x, ry + y, (int)u->resolution[0], (int)u->resolution[1]);
+            V3<double> org = V3<double>::from(u->camera_pos);
+            MarchResult<double> m = march_to(s, mc, org, dir, (double)u->max_dist);
+            ev[0] += g_probe.evals; fl[0] += g_probe.flops; tp[0] += g_probe.tops;
+            g_probe = FlopProbe{};
+            if (!m.hit || (int)u->light_count < 1 || u->lights[0].cast_shadows < 0.5f) continue;
+            V3<double> hp = org + dir * m.dist;
+            V3<double> n = normal_analytic(s, hp);
+            V3<double> ld; double dist;
+            light_dir_dist(u->lights[0], u->max_dist, hp, &ld, &dist);
+            if (dot(n, ld) <= 0) continue;
+            const nd_light& L = u->lights[0];
+            double k = L.shadow_softness + (512.0 - L.shadow_softness) * L.hard_shadows;
+            int steps = shadow_max_steps > 0 ? shadow_max_steps : std::min(i32(u->max_steps), 40);
+            g_probe.on = true;
+            (void)visibility_march_opaque(s, hp + n * 0.004, ld, dist, k, steps, (int*)nullptr);
+            ev[1] += g_probe.evals; fl[1] += g_probe.flops; tp[1] += g_probe.tops;
+            g_probe = FlopProbe{};
+        }
+    for (int w = 0; w < 2; ++w) {
+        out[3 * w] = (double)ev[w];
+        out[3 * w + 1] = ev[w] ? (double)fl[w] / ev[w] : 0.0;
+        out[3 * w + 2] = ev[w] ? (double)tp[w] / ev[w] : 0.0;
+    }
+    return 0;
+}
+
+// Individual helpers for closed-form and invariant checks.
+//   0 box_fold(z3, L) -> z3                    1 sphere_fold(z3, dr, m2, F2) -> z3, dr
+//   2 sphere_invert_offset(z3, dr, c3, R2)     3 lattice_wrap(p3, cell) -> p3
+//   4 sort_desc_abs(z3) -> z3                  5 rotate_xyz_deg(z3, deg3) -> z3
+//   6 bulb_power_step(z3, dr, add3, n) -> z3, dr
+//   7..12 smin, smax, ssub, smin_lin, smin_nlin, smix (a, b, k) -> r
+//   13 combine_de(mode, d, di, k) -> d, h      14 trap_distance(shape, z3, c3, box3, r, tube)
+//   16 formula_finalize(f, z3, dr) -> d  (formula steps: ndo_formula_step)
+//   17 formula_iterations(f, p16) -> n         18 trap run: mode, shape, radius, iters, zs(3*iters)
+int ndo_helper(int op, const double* a, double* out) {
+    using D3 = V3<double>;
+    switch (op) {
+        case 0: { D3 r = box_fold(D3(a[0], a[1], a[2]), a[3]); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 1: { auto r = sphere_fold(D3(a[0], a[1], a[2]), a[3], a[4], a[5]);
+                  out[0] = r.z.x; out[1] = r.z.y; out[2] = r.z.z; out[3] = r.dr; return 0; }
+        case 2: { auto r = sphere_invert_offset(D3(a[0], a[1], a[2]), a[3], D3(a[4], a[5], a[6]), a[7]);
+                  out[0] = r.z.x; out[1] = r.z.y; out[2] = r.z.z; out[3] = r.dr; return 0; }
+        case 3: { D3 r = lattice_wrap(D3(a[0], a[1], a[2]), a[3]); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 4: { D3 r = sort_desc_abs(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 5: { D3 r = rotate_xyz_deg(D3(a[0], a[1], a[2]), a[3], a[4], a[5]); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 6: { Carry<double> c{D3(a[0], a[1], a[2]), a[3]};
+                  auto r = bulb_power_step(c, D3(a[4], a[5], a[6]), a[7]);
+                  out[0] = r.z.x; out[1] = r.z.y; out[2] = r.z.z; out[3] = r.dr; return 0; }
+        case 7: out[0] = smin(a[0], a[1], a[2]); return 0;
+        case 8: out[0] = smax(a[0], a[1], a[2]); return 0;
+        case 9: out[0] = ssub(a[0], a[1], a[2]); return 0;
+        case 10: out[0] = smin_lin(a[0], a[1], a[2]); return 0;
+        case 11: out[0] = smin_nlin(a[0], a[1], a[2]); return 0;
+        case 12: out[0] = smix(a[0], a[1], a[2]); return 0;
+        case 13: { double d, h; combine_de((int)a[0], a[1], a[2], a[3], &d, &h); out[0] = d; out[1] = h; return 0; }
+        case 14: { TrapSpec t{}; t.shape = (int)a[0];
+                   for (int k = 0; k < 3; ++k) { t.center[k] = a[4 + k]; t.box[k] = a[7 + k]; }
+                   t.radius = a[10]; t.tube = a[11];
+                   out[0] = trap_distance(t, D3(a[1], a[2], a[3])); return 0; }
+        case 16: { Carry<double> c{D3(a[1], a[2], a[3]), a[4]}; out[0] = formula_finalize((int)a[0], c); return 0; }
+        case 17: { Params<double> p; for (int k = 0; k < 16; ++k) p.p[k] = a[1 + k];
+                   out[0] = formula_iterations((int)a[0], p); return 0; }
+        case 18: { TrapSpec t{}; t.mode = (int)a[0]; t.shape = (int)a[1]; t.radius = a[2]; t.tube = 0;
+                   t.box[0] = t.box[1] = t.box[2] = 1;
+                   int iters = (int)a[3];
+                   TrapAcc<double> acc = trap_begin<double>(t);
+                   D3 prev(a[4], a[5], a[6]);
+                   for (int it = 0; it < iters; ++it) {
+                       D3 z(a[7 + 3 * it], a[8 + 3 * it], a[9 + 3 * it]);
+                       acc = trap_update(acc, t, z, prev, it);
+                       prev = z;
+                   }
+                   out[0] = trap_finish(acc, t, iters); return 0; }
+    }
+    return -1;
+}
+
+// formula step with an explicit layout: a = [f, z3, dr, pos3, p16]
+int ndo_formula_step(int f, const double* z, double dr, const double* pos, const double* p16, double* out) {
+    Params<double> p;
+    for (int k = 0; k < 16; ++k) p.p[k] = p16[k];
+    Carry<double> c{V3<double>(z[0], z[1], z[2]), dr};
+    auto r = formula_step(f, c, V3<double>(pos[0], pos[1], pos[2]), p);
+    out[0] = r.z.x; out[1] = r.z.y; out[2] = r.z.z; out[3] = r.dr;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+"""Comparison metrics between a HIP G-buffer and the oracle's, with the tolerances of the
+workloads: hit mask agreement >= 99.9 % away from silhouettes, depth within 1e-4 relative,
+normals within 0.5 degrees, all stated as the fraction of pixels that must meet them."""
+import numpy as np
+
+
+def silhouette_mask(hit):
+    """Pixels whose 3x3 neighbourhood mixes hits and misses (in the reference mask)."""
+    h = hit.astype(bool)
+    pad = np.pad(h, 1, mode="edge")
+    H, W = h.shape
+    anyh = np.zeros_like(h)
+    allh = np.ones_like(h)
+    for dy in (0, 1, 2):
+        for dx in (0, 1, 2):
+            win = pad[dy:dy + H, dx:dx + W]
+            anyh |= win
+            allh &= win
+    return anyh & ~allh
+
+
+def compare(gpu, ref):
+    """Returns a dict of agreement statistics (fractions in [0, 1], errors in their units)."""
+    gh = gpu["hit"].astype(bool)
+    rh = ref["hit"].astype(bool)
+    sil = silhouette_mask(rh) | silhouette_mask(gh)
+    inner = ~sil
+    both = gh & rh
+    st = {"pixels": int(gh.size), "hits_ref": int(rh.sum()), "hits_gpu": int(gh.sum()),
+          "hit_agree_all": float((gh == rh).mean()),
+          "hit_agree_inner": float((gh == rh)[inner].mean()) if inner.any() else 1.0}
+    if both.any():
+        gd = gpu["depth"][both].astype(np.float64)
+        rd = ref["depth"][both].astype(np.float64)
+        rel = np.abs(gd - rd) / np.maximum(np.abs(rd), 1e-12)
+        st["depth_rel_median"] = float(np.median(rel))
+        st["depth_rel_max"] = float(rel.max())
+        st["depth_rel_le_1e-4"] = float((rel <= 1e-4).mean())
+        if "normal" in gpu and "normal" in ref:
+            gn = gpu["normal"][both].astype(np.float64)
+            rn = ref["normal"][both].astype(np.float64)
+            cosang = np.clip(np.sum(gn * rn, axis=1) / np.maximum(
+                np.linalg.norm(gn, axis=1) * np.linalg.norm(rn, axis=1), 1e-30), -1, 1)
+            ang = np.degrees(np.arccos(cosang))
+            st["normal_deg_median"] = float(np.median(ang))
+            st["normal_deg_le_0.5"] = float((ang <= 0.5).mean())
+        if "trap" in gpu and "trap" in ref:
+            st["trap_equal"] = float((np.abs(gpu["trap"][both] - ref["trap"][both]) <= 1e-5).mean())
+        if "shadow" in gpu and "shadow" in ref:
+            ds = np.abs(gpu["shadow"][both].astype(np.float64) - ref["shadow"][both])
+            st["shadow_abs_le_0.02"] = float((ds <= 0.02).mean())
+            st["shadow_abs_median"] = float(np.median(ds))
+    return st

@@ -1,0 +1,100 @@
+"""GPU parity of the geometry passes against the FP64 oracle, through the C ABI.
+
+Sizes are reduced (the same cameras at lower resolution, epsilon following the pixel
+angle) so the scalar oracle finishes in seconds. Tolerances follow the workloads: hit mask
+agreement >= 99.9 % away from silhouettes, depth within 1e-4 relative, normals within
+0.5 degrees; each is asserted as the fraction of pixels that must meet it.
+"""
+import numpy as np
+import pytest
+
+import numericdream_amd as nd
+from numericdream_amd import abi
+import oracle
+from tests.parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_render(r, u, kernel, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0):
+    W, H = int(u.resolution[0]), int(u.resolution[1])
+    g = nd.GBuffer(W, H)
+    r.render_gbuffer(u, g, normal_mode=normal_mode, shadows=shadows, shadow_max_steps=shadow_max_steps,
+                     kernel=kernel)
+    return g.numpy()
+
+
+def report(name, st):
+    print("\n[parity] %s %s" % (name, " ".join("%s=%.6g" % (k, v) for k, v in st.items())), flush=True)
+
+
+def check(st, normal_frac=0.97, depth_frac=0.99, trap_frac=0.98, shadow_frac=0.95):
+    assert st["hit_agree_inner"] >= 0.999, st
+    assert st["depth_rel_le_1e-4"] >= depth_frac, st
+    assert st["normal_deg_le_0.5"] >= normal_frac, st
+    assert st["trap_equal"] >= trap_frac, st
+    if "shadow_abs_le_0.02" in st:
+        assert st["shadow_abs_le_0.02"] >= shadow_frac, st
+
+
+def test_mandelbulb_c1_quarter(renderer):
+    u = nd.resize(nd.config_uniforms(1, 0), 160, 90)
+    print("\n[case] mandelbulb 160x90", flush=True)
+    ref = oracle.render_gbuffer(u, normal_mode=0, shadows=False)
+    for kernel in (abi.KERNEL_PERSISTENT, abi.KERNEL_PLAIN):
+        g = gpu_render(renderer, u, kernel, shadows=False)
+        st = compare(g, ref)
+        report("c1 kernel=%d" % kernel, st)
+        check(st)
+
+
+def test_mandelbox_c2_shadow(renderer):
+    u = nd.resize(nd.config_uniforms(2, 0), 240, 135)
+    print("\n[case] mandelbox 240x135 shadow 128", flush=True)
+    ref = oracle.render_gbuffer(u, normal_mode=0, shadows=True, shadow_max_steps=128)
+    for kernel in (abi.KERNEL_PERSISTENT, abi.KERNEL_PLAIN):
+        g = gpu_render(renderer, u, kernel, shadows=True, shadow_max_steps=128)
+        st = compare(g, ref)
+        report("c2 kernel=%d" % kernel, st)
+        check(st)
+
+
+def test_plain_and_persistent_identical(renderer):
+    """Both schedules run the same per-ray math, so their outputs are bit-identical."""
+    u = nd.resize(nd.config_uniforms(2, 3), 200, 120)
+    a = gpu_render(renderer, u, abi.KERNEL_PLAIN, shadows=True, shadow_max_steps=128)
+    b = gpu_render(renderer, u, abi.KERNEL_PERSISTENT, shadows=True, shadow_max_steps=128)
+    for k in ("depth", "normal", "trap", "hit", "shadow", "evals"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_reference_shadow_rule_and_tetra(renderer):
+    """Reference settings: tetrahedral normal (e = eps) and min(max_steps, 40) shadow steps."""
+    u = nd.resize(nd.config_uniforms(2, 1), 160, 90)
+    ref = oracle.render_gbuffer(u, normal_mode=1, shadows=True, shadow_max_steps=0)
+    g = gpu_render(renderer, u, abi.KERNEL_PERSISTENT, normal_mode=abi.NORMAL_TETRA, shadows=True,
+                   shadow_max_steps=0)
+    st = compare(g, ref)
+    report("c2 tetra shadow40", st)
+    assert st["hit_agree_inner"] >= 0.999
+    assert st["depth_rel_le_1e-4"] >= 0.99
+
+
+def test_batch_mixed_jobs(renderer):
+    """Mixed formulas, iteration counts and resolutions in one persistent launch."""
+    jobs, refs = [], []
+    for i in range(6):
+        u = nd.config_uniforms(4, i)
+        side = int(u.resolution[0])
+        s = max(16, side // 8)
+        nd.resize(u, s, s)
+        g = nd.GBuffer(s, s)
+        jobs.append((u, g))
+        refs.append(oracle.render_gbuffer(u, normal_mode=0, shadows=False))
+    renderer.render_batch(jobs, shadows=False)
+    for i, ((u, g), ref) in enumerate(zip(jobs, refs)):
+        st = compare(g.numpy(), ref)
+        report("c4 job %d %dx%d" % (i, g.w, g.h), st)
+        assert st["hit_agree_inner"] >= 0.995, st
+        if st["hits_ref"] > 50:
+            assert st["depth_rel_le_1e-4"] >= 0.97, st
