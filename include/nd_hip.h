@@ -101,6 +101,7 @@ typedef struct nd_rect {
  * pointers; otherwise device pointers on the context's device.
  *   depth   t at the hit, 0 on a miss (FragOut.depth, template.wgsl:3762-3774)
  *   normal  3 floats per pixel, unit length at hits, 0 on a miss
+ *   position  3 floats per pixel: the hit point o + t d as the kernel computed it, 0 on a miss
  *   trap    orbit-trap value at the hit, blended across instances by the combine weight
  *           like the material (template.wgsl:2128-2155); 0 on a miss
  *   hit     1 at hits, 0 at misses (FragOut.normal.w)
@@ -109,6 +110,7 @@ typedef struct nd_rect {
 typedef struct nd_gbuffer_out {
     float* depth;
     float* normal;
+    float* position;
     float* trap;
     uint8_t* hit;
     float* shadow;
@@ -170,11 +172,13 @@ nd_status nd_render_batch(nd_ctx* ctx, const nd_job* jobs, uint32_t n_jobs,
 nd_status nd_batch_job_latency(nd_ctx* ctx, float* ms_out, uint32_t n_jobs);
 
 /* ---- autotuner ----------------------------------------------------------------- */
+#define ND_TUNE_TRACK_LATENCY 1u  /* record per-job completion times (nd_batch_job_latency) */
+
 typedef struct nd_tuning {
-    uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups */
+    uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups; 0 = occupancy */
     uint32_t refill_min_idle;  /* a wave pulls new rays when at least this many lanes are idle */
     uint32_t chunk_rays;       /* rays a wave claims from the global queue per atomic */
-    uint32_t _pad;
+    uint32_t flags;            /* ND_TUNE_* */
 } nd_tuning;
 
 nd_status nd_get_tuning(nd_ctx* ctx, nd_tuning* out);

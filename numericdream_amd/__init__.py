@@ -121,12 +121,14 @@ def _ptr(a):
 class GBuffer:
     """Host (numpy) or device (torch) G-buffer arrays for a w x h rect."""
 
-    def __init__(self, w, h, device=None, fields=("depth", "normal", "trap", "hit", "shadow", "evals")):
+    FIELDS = ("depth", "normal", "position", "trap", "hit", "shadow", "evals")
+
+    def __init__(self, w, h, device=None, fields=("depth", "normal", "position", "trap", "hit", "shadow", "evals")):
         self.w, self.h = w, h
         self.device = device
-        shapes = {"depth": ((h, w), "f4"), "normal": ((h, w, 3), "f4"), "trap": ((h, w), "f4"),
-                  "hit": ((h, w), "u1"), "shadow": ((h, w), "f4"), "evals": ((h, w), "u4")}
-        for k in ("depth", "normal", "trap", "hit", "shadow", "evals"):
+        shapes = {"depth": ((h, w), "f4"), "normal": ((h, w, 3), "f4"), "position": ((h, w, 3), "f4"),
+                  "trap": ((h, w), "f4"), "hit": ((h, w), "u1"), "shadow": ((h, w), "f4"), "evals": ((h, w), "u4")}
+        for k in self.FIELDS:
             if k not in fields:
                 setattr(self, k, None)
                 continue
@@ -139,12 +141,12 @@ class GBuffer:
                 setattr(self, k, torch.zeros(shape, dtype=tdt, device=device))
 
     def out(self):
-        return GBufferOut(_ptr(self.depth), _ptr(self.normal), _ptr(self.trap), _ptr(self.hit), _ptr(self.shadow),
-                          _ptr(self.evals), 0 if self.device is None else 1, 0)
+        return GBufferOut(_ptr(self.depth), _ptr(self.normal), _ptr(self.position), _ptr(self.trap), _ptr(self.hit),
+                          _ptr(self.shadow), _ptr(self.evals), 0 if self.device is None else 1, 0)
 
     def numpy(self):
         r = {}
-        for k in ("depth", "normal", "trap", "hit", "shadow", "evals"):
+        for k in self.FIELDS:
             a = getattr(self, k)
             if a is None:
                 continue
@@ -225,8 +227,8 @@ class Renderer:
         self._check(lib.nd_get_tuning(self._h, C.byref(t)), "nd_get_tuning")
         return t
 
-    def set_tuning(self, waves_per_cu=0, refill_min_idle=1, chunk_rays=64):
-        t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, 0)
+    def set_tuning(self, waves_per_cu=0, refill_min_idle=1, chunk_rays=64, track_latency=False):
+        t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, abi.TUNE_TRACK_LATENCY if track_latency else 0)
         self._check(lib.nd_set_tuning(self._h, C.byref(t)), "nd_set_tuning")
 
     def autotune(self, jobs, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0):

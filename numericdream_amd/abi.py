@@ -116,7 +116,8 @@ class Rect(C.Structure):
 
 
 class GBufferOut(C.Structure):
-    _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("trap", C.c_void_p), ("hit", C.c_void_p),
+    _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("position", C.c_void_p), ("trap", C.c_void_p),
+                ("hit", C.c_void_p),
                 ("shadow", C.c_void_p), ("evals", C.c_void_p), ("device_ptrs", U32), ("_pad", U32)]
 
 
@@ -132,7 +133,10 @@ class BatchStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("waves_per_cu", U32), ("refill_min_idle", U32), ("chunk_rays", U32), ("_pad", U32)]
+    _fields_ = [("waves_per_cu", U32), ("refill_min_idle", U32), ("chunk_rays", U32), ("flags", U32)]
+
+
+TUNE_TRACK_LATENCY = 1
 
 
 class AdjointIO(C.Structure):

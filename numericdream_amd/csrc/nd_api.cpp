@@ -272,6 +272,7 @@ nd_status nd_ctx_create(int hip_device, nd_ctx** out) {
     c->tuning.waves_per_cu = 0;  // 0 = from occupancy
     c->tuning.refill_min_idle = 1;
     c->tuning.chunk_rays = 64;
+    c->tuning.flags = 0;
     *out = c;
     return ND_OK;
 }
@@ -337,6 +338,7 @@ static size_t out_bytes(const nd_gbuffer_out& o, size_t px) {
     size_t b = 0;
     if (o.depth) b += 4 * px;
     if (o.normal) b += 12 * px;
+    if (o.position) b += 12 * px;
     if (o.trap) b += 4 * px;
     if (o.hit) b += px;
     if (o.shadow) b += 4 * px;
@@ -355,8 +357,23 @@ uint32_t nd_persistent_grid(nd_ctx* c) {
     return (uint32_t)std::max(c->cu_count, 1) * wpc;
 }
 
-nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_opts* opts, nd_batch_stats* stats) {
+// Tangent slots of the adjoint: 0..2 spatial, 3 Mandelbulb power, 4 Mandelbox scale,
+// 5 Mandelbox min radius, on the first visible instance of each formula.
+static void attach_adjoint_slots(DevScene& s) {
+    bool bulb = false, box = false;
+    for (int i = 0; i < s.count; ++i) {
+        DevInst& in = s.inst[i];
+        if (in.hidden) continue;
+        if (in.formula == ND_FORMULA_MANDELBULB && !bulb) { in.dparam[0] = 3; bulb = true; }
+        if (in.formula == ND_FORMULA_MANDELBOX && !box) { in.dparam[0] = 4; in.dparam[1] = 5; box = true; }
+    }
+}
+
+nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_opts* opts, nd_batch_stats* stats,
+                        const NdBatchExtra* extra) {
     if (!c) return ND_E_INVALID;
+    NdBatchExtra ex;
+    if (extra) ex = *extra;
     if (!jobs || n == 0) return nd_fail(c, ND_E_INVALID, "empty batch");
     if (n > 65535) return nd_fail(c, ND_E_INVALID, "at most 65535 jobs per batch");
     if (c->cancel && *c->cancel) return nd_fail(c, ND_E_CANCELLED, "cancelled before launch");
@@ -376,13 +393,14 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         if (!jobs[i].uniforms) return nd_fail(c, ND_E_INVALID, "job %u: null uniforms", i);
         nd_status st = nd_pack_job(c, *jobs[i].uniforms, o, jobs[i].rect, hj[i]);
         if (st != ND_OK) return st;
+        if (ex.adjoint_slots) attach_adjoint_slots(hj[i].scene);
         hj[i].ray_base = (uint32_t)total;
         base[i] = (uint32_t)total;
         total += hj[i].ray_count;
         any_shadow = any_shadow || hj[i].shadow_on;
         const nd_gbuffer_out& go = jobs[i].out;
         size_t px = (size_t)hj[i].rect_w * hj[i].rect_h;
-        if (!go.device_ptrs) staging += out_bytes(go, px) + 6 * 256;
+        if (!go.device_ptrs) staging += out_bytes(go, px) + 7 * 256;
     }
     if (total >= (1ull << 31)) return nd_fail(c, ND_E_INVALID, "batch of %llu ray slots exceeds 2^31", (unsigned long long)total);
 
@@ -397,11 +415,12 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
             DevJob& j = hj[i];
             size_t px = (size_t)j.rect_w * j.rect_h;
             if (go.device_ptrs) {
-                j.out_depth = go.depth; j.out_normal = go.normal; j.out_trap = go.trap;
+                j.out_depth = go.depth; j.out_normal = go.normal; j.out_pos = go.position; j.out_trap = go.trap;
                 j.out_hit = go.hit; j.out_shadow = go.shadow; j.out_evals = go.evals;
             } else {
                 j.out_depth = go.depth ? (float*)carve(4 * px) : nullptr;
                 j.out_normal = go.normal ? (float*)carve(12 * px) : nullptr;
+                j.out_pos = go.position ? (float*)carve(12 * px) : nullptr;
                 j.out_trap = go.trap ? (float*)carve(4 * px) : nullptr;
                 j.out_hit = go.hit ? (uint8_t*)carve(px) : nullptr;
                 j.out_shadow = go.shadow ? (float*)carve(4 * px) : nullptr;
@@ -412,6 +431,7 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
 
     bool shared = true;
     for (uint32_t i = 1; i < n && shared; ++i) shared = std::memcmp(&hj[i].scene, &hj[0].scene, sizeof(DevScene)) == 0;
+    c->last_shared = shared;
 
     if ((st = c->jobs.ensure(c, sizeof(DevJob) * n)) != ND_OK) return st;
     if ((st = c->job_base.ensure(c, sizeof(uint32_t) * n)) != ND_OK) return st;
@@ -431,16 +451,20 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     const bool persistent = o.kernel != ND_KERNEL_PLAIN;
     if (!persistent) {
         // one launch per frame: the job record is read from device memory by one kernel each
+        if ((e = nd_launch_zero_counters((DevCounters*)c->ctr.p, c->stream)) != hipSuccess)
+            return nd_hip_fail(c, e, "zero counters");
         (void)hipEventRecord(c->ev[0], c->stream);
         for (uint32_t i = 0; i < n; ++i) {
-            e = nd_launch_plain((const DevJob*)c->jobs.p + i, hj[i].tiles_x, hj[i].tiles_y, c->stream);
+            e = nd_launch_plain((const DevJob*)c->jobs.p + i, hj[i].tiles_x, hj[i].tiles_y, (DevCounters*)c->ctr.p,
+                                c->stream);
             if (e != hipSuccess) return nd_hip_fail(c, e, "launch plain march");
         }
         (void)hipEventRecord(c->ev[3], c->stream);
         s.kernels = n;
     } else {
-        size_t hit_cap = (size_t)total;
-        size_t hb_bytes = hit_cap * (4 + 2 + 4 * 4) + 8 * 256;
+        // hit slots: every ray may hit, plus at most one partly used 64-slot chunk per wave
+        size_t hit_cap = (size_t)total + (size_t)nd_persistent_grid(c) * 64;
+        size_t hb_bytes = hit_cap * (4 + 2 + 4 + 4 + 2 + 7 * 4) + 16 * 256;
         if ((st = c->hits.ensure(c, hb_bytes)) != ND_OK) return st;
         char* hp = (char*)c->hits.p;
         auto carve = [&](size_t bytes) { char* r = hp; hp += (bytes + 255) & ~(size_t)255; return r; };
@@ -448,9 +472,15 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.hb.ray = (uint32_t*)carve(4 * hit_cap);
         L.hb.job = (uint16_t*)carve(2 * hit_cap);
         L.hb.t = (float*)carve(4 * hit_cap);
-        L.hb.nx = (float*)carve(4 * hit_cap);
-        L.hb.ny = (float*)carve(4 * hit_cap);
-        L.hb.nz = (float*)carve(4 * hit_cap);
+        L.hb.opix = (uint32_t*)carve(4 * hit_cap);
+        L.hb.steps = (uint16_t*)carve(2 * hit_cap);
+        L.hb.sox = (float*)carve(4 * hit_cap);
+        L.hb.soy = (float*)carve(4 * hit_cap);
+        L.hb.soz = (float*)carve(4 * hit_cap);
+        L.hb.sdx = (float*)carve(4 * hit_cap);
+        L.hb.sdy = (float*)carve(4 * hit_cap);
+        L.hb.sdz = (float*)carve(4 * hit_cap);
+        L.hb.sdist = (float*)carve(4 * hit_cap);
         L.hb.capacity = (uint32_t)hit_cap;
         L.jobs = (const DevJob*)c->jobs.p;
         L.shared = shared ? (const DevScene*)c->shared.p : nullptr;
@@ -459,11 +489,14 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.total_rays = (uint32_t)total;
         L.ctr = (DevCounters*)c->ctr.p;
         L.grid = nd_persistent_grid(c);
+        c->last_grid = L.grid;
+        c->last_hb = L.hb;
         L.chunk = c->tuning.chunk_rays;
         L.min_idle = c->tuning.refill_min_idle;
         L.round_cap = c->round_cap;
         L.any_shadow = any_shadow;
-        L.job_done_cycles = (unsigned long long*)c->latency.p;
+        L.surface = !ex.skip_surface;
+        L.job_done_cycles = (c->tuning.flags & ND_TUNE_TRACK_LATENCY) ? (unsigned long long*)c->latency.p : nullptr;
         for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];
         e = nd_launch_persistent(L, c->stream);
         if (e != hipSuccess) return nd_hip_fail(c, e, "launch persistent passes");
@@ -479,7 +512,7 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         DevCounters dc;
         if ((e = hipMemcpy(&dc, c->ctr.p, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
             return nd_hip_fail(c, e, "read counters");
-        s.hits = dc.hit_count;
+        s.hits = dc.hits;
         s.de_evals_march = dc.de_march;
         s.de_evals_surface = dc.de_surface;
         s.de_evals_shadow = dc.de_shadow;
@@ -493,6 +526,11 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
             c->job_latency_ms[i] = done[i] > t0 ? (float)((double)(done[i] - t0) / (double)c->wall_khz) : 0.0f;
         if (dc.rounds_capped) return nd_fail(c, ND_E_TIMEOUT, "%u waves reached the round cap", dc.rounds_capped);
     } else {
+        DevCounters dc;
+        if ((e = hipMemcpy(&dc, c->ctr.p, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
+            return nd_hip_fail(c, e, "read counters");
+        s.de_evals = dc.de_march;
+        s.de_evals_march = dc.de_march;
         c->job_latency_ms.assign(n, ms_total);
     }
     for (uint32_t i = 0; i < n; ++i) s.rays += (uint64_t)hj[i].rect_w * hj[i].rect_h;
@@ -507,8 +545,9 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         if (go.device_ptrs) continue;
         const DevJob& j = hj[i];
         size_t px = (size_t)j.rect_w * j.rect_h;
-        struct { void* dst; const void* src; size_t b; } cp[6] = {
-            {go.depth, j.out_depth, 4 * px}, {go.normal, j.out_normal, 12 * px}, {go.trap, j.out_trap, 4 * px},
+        struct { void* dst; const void* src; size_t b; } cp[7] = {
+            {go.depth, j.out_depth, 4 * px}, {go.normal, j.out_normal, 12 * px}, {go.position, j.out_pos, 12 * px},
+            {go.trap, j.out_trap, 4 * px},
             {go.hit, j.out_hit, px}, {go.shadow, j.out_shadow, 4 * px}, {go.evals, j.out_evals, 4 * px}};
         for (auto& k : cp) {
             if (!k.dst) continue;
@@ -561,7 +600,7 @@ nd_status nd_autotune(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_
     for (uint32_t w : wpcs)
         for (uint32_t mi : idles)
             for (uint32_t ch : chunks) {
-                nd_tuning t{w, mi, ch, 0};
+                nd_tuning t{w, mi, ch, saved.flags};
                 c->tuning = t;
                 nd_batch_stats s;
                 nd_status st = nd_batch_impl(c, jobs, n, &o, &s);

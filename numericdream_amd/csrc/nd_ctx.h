@@ -31,10 +31,21 @@ struct nd_ctx {
     bool graph_valid = false;
     hipGraphExec_t graph_exec = nullptr;
     DevBuf jobs, job_base, ctr, hits, shared, staging, latency, adj, mc;
+    bool last_shared = false;       // scene of the last batch was shared by all jobs
+    uint32_t last_grid = 0;
+    DevHitBuffers last_hb{};        // hit list of the last persistent batch
     std::vector<DevJob> host_jobs;
     std::vector<float> job_latency_ms;
 };
 
+// Internal options of a batch beyond the public nd_march_opts.
+struct NdBatchExtra {
+    bool skip_surface = false;   // no normal / trap pass (the adjoint needs only t)
+    bool adjoint_slots = false;  // attach adjoint tangents to the differentiable params
+};
+
+nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_opts* opts, nd_batch_stats* stats,
+                        const NdBatchExtra* extra = nullptr);
 nd_status nd_fail(nd_ctx* c, nd_status st, const char* fmt, ...);
 nd_status nd_hip_fail(nd_ctx* c, hipError_t e, const char* what);
 nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s);

@@ -76,6 +76,7 @@ struct DevJob {
     float _pad2;
     float* out_depth;
     float* out_normal;
+    float* out_pos;
     float* out_trap;
     uint8_t* out_hit;
     float* out_shadow;
@@ -92,6 +93,7 @@ struct DevCounters {
     unsigned long long de_surface;
     unsigned long long de_shadow;
     unsigned long long rays_done;
+    unsigned long long hits;         // hits appended (the hit list also holds empty slots)
     unsigned long long clock_start;  // wall clock (s_memrealtime) when the batch started
     uint32_t rounds_capped;  // waves that stopped at the round cap
     uint32_t _pad;
@@ -99,12 +101,18 @@ struct DevCounters {
 
 // Per-hit SoA state passed from the march pass to the surface and shadow passes.
 struct DevHitBuffers {
-    uint32_t* ray;     // global ray slot of the hit
-    uint16_t* job;     // job index
-    float* t;          // hit distance
-    float* nx;         // normal (surface pass output), SoA
-    float* ny;
-    float* nz;
+    uint32_t* ray;     // global ray slot of the hit (march pass)
+    uint16_t* job;     // job index (march pass)
+    float* t;          // hit distance (march pass)
+    uint32_t* opix;    // output pixel index inside the job's rect (surface pass)
+    uint16_t* steps;   // shadow step budget, 0 = no shadow march (surface pass)
+    float* sox;        // shadow ray origin (surface pass)
+    float* soy;
+    float* soz;
+    float* sdx;        // shadow ray direction
+    float* sdy;
+    float* sdz;
+    float* sdist;      // light distance
     uint32_t capacity;
     uint32_t _pad;
 };

@@ -18,10 +18,12 @@ struct NdPassLaunch {
     uint32_t min_idle;
     uint32_t round_cap;
     int any_shadow;
+    int surface;                 // run the normal / trap pass
     unsigned long long* job_done_cycles;  // per-job completion clock, or null
     hipEvent_t ev[4];            // optional: before march / surface / shadow / after
 };
 
-hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, hipStream_t s);
+hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, hipStream_t s);
+hipError_t nd_launch_zero_counters(DevCounters* c, hipStream_t s);
 hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s);
 uint32_t nd_march_occupancy();

@@ -15,40 +15,9 @@
 
 #include "nd_de.hpp"
 #include "nd_internal.h"
+#include "nd_rays.hpp"
 
 namespace nd {
-
-// ----------------------------------------------------------------------------- rays
-// Primary ray (cs_wf_generate, template.wgsl:5119-5127).
-NDD V3<float> primary_dir(const DevJob& J, int px, int py) {
-    float nx = ((float)px + 0.5f) * J.inv_tw * 2.f - 1.f;
-    float ny = 1.f - ((float)py + 0.5f) * J.inv_th * 2.f;
-    float pfx = nx * J.tile_scale[0] + J.tile_bias[0];
-    float pfy = ny * J.tile_scale[1] + J.tile_bias[1];
-    float ux = pfx * J.aspect, uy = -pfy;
-    float dx = J.fwd[0] + ux * J.right[0] + uy * J.up[0];
-    float dy = J.fwd[1] + ux * J.right[1] + uy * J.up[1];
-    float dz = J.fwd[2] + ux * J.right[2] + uy * J.up[2];
-    float il = __frsqrt_rn(dx * dx + dy * dy + dz * dz);
-    return mk3(dx * il, dy * il, dz * il);
-}
-
-// Ray slot -> (job, x, y inside the rect). Slots enumerate 8x8 tiles of each job.
-NDD int find_job(const uint32_t* __restrict__ job_base, int n_jobs, uint32_t r) {
-    int lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (job_base[mid] <= r) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-NDD bool slot_pixel(const DevJob& J, uint32_t local, int* x, int* y) {
-    uint32_t tile = local >> 6, l = local & 63u;
-    int tx = (int)(tile % (uint32_t)J.tiles_x), ty = (int)(tile / (uint32_t)J.tiles_x);
-    *x = tx * 8 + (int)(l & 7u);
-    *y = ty * 8 + (int)(l >> 3);
-    return *x < J.rect_w && *y < J.rect_h;
-}
 
 // ----------------------------------------------------------------------------- march state
 enum { MS_CONT = 0, MS_HIT = 1, MS_MISS = 2 };
@@ -63,8 +32,21 @@ NDD void march_init(March& m) {
     m.probing = 1; m.bisecting = 0; m.resolved = 0;
 }
 NDD float march_sample_t(const March& m) { return m.probing ? m.t : 0.5f * (m.lo + m.hi); }
+// March precision of one ray, held in registers by the persistent passes.
+struct MarchParams {
+    float eps_c, eps_f, max_dist, footprint;
+    int max_steps, bisect_budget, refine_budget, hq;
+};
+NDD MarchParams march_params(const DevJob& J) {
+    MarchParams p;
+    p.eps_c = J.eps_c; p.eps_f = J.eps_f; p.max_dist = J.max_dist; p.footprint = J.footprint;
+    p.max_steps = J.max_steps; p.bisect_budget = J.bisect_budget; p.refine_budget = J.refine_budget; p.hq = J.hq;
+    return p;
+}
 // One step of march_to's loop after the DE d at sample_t s. *dist receives the result.
-NDD int march_update(March& m, const DevJob& J, float s, float d, float* dist) {
+// J is a DevJob (plain kernel, wave-uniform) or MarchParams (persistent, per lane).
+template <class MP>
+NDD int march_update(March& m, const MP& J, float s, float d, float* dist) {
     const int max_samples = J.max_steps * (1 + J.refine_budget) + J.bisect_budget;
     m.n++;
     m.evals++;
@@ -158,14 +140,8 @@ NDD const DevScene& scene_of(const DevJob* jobs, const DevScene* shared, int job
     return jobs[job].scene;
 }
 
-// ============================================================================ plain kernel
-// One lane per pixel of one job; blockDim 64 = one 8x8 tile per wave64.
-__global__ void __launch_bounds__(64) k_plain(const DevJob* __restrict__ jobp) {
-    const DevJob& J = *jobp;
-    const DevScene& sc = J.scene;
-    const int x = blockIdx.x * 8 + (threadIdx.x & 7);
-    const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
-    if (x >= J.rect_w || y >= J.rect_h) return;
+// The whole per-pixel pipeline of the plain schedule; returns its DE evaluations.
+NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
     const V3<float> dir = primary_dir(J, J.rect_x + x, J.rect_y + y);
     const V3<float> org = mk3(J.cam[0], J.cam[1], J.cam[2]);
     March m;
@@ -186,7 +162,8 @@ __global__ void __launch_bounds__(64) k_plain(const DevJob* __restrict__ jobp) {
         if (J.out_hit) J.out_hit[o] = 0;
         if (J.out_shadow) J.out_shadow[o] = 1.f;
         if (J.out_evals) J.out_evals[o] = (uint32_t)evals;
-        return;
+        if (J.out_pos) { J.out_pos[3 * o] = 0.f; J.out_pos[3 * o + 1] = 0.f; J.out_pos[3 * o + 2] = 0.f; }
+        return evals;
     }
     const V3<float> hp = mk3(org.x + dir.x * dist, org.y + dir.y * dist, org.z + dir.z * dist);
     int sev = 0;
@@ -219,13 +196,29 @@ __global__ void __launch_bounds__(64) k_plain(const DevJob* __restrict__ jobp) {
     if (J.out_hit) J.out_hit[o] = 1;
     if (J.out_shadow) J.out_shadow[o] = shadow;
     if (J.out_evals) J.out_evals[o] = (uint32_t)evals;
+    if (J.out_pos) { J.out_pos[3 * o] = hp.x; J.out_pos[3 * o + 1] = hp.y; J.out_pos[3 * o + 2] = hp.z; }
+    return evals + sev + 1;
+}
+
+// ============================================================================ plain kernel
+// One lane per pixel of one job; blockDim 64 = one 8x8 tile per wave64.
+__global__ void __launch_bounds__(64) k_plain(const DevJob* __restrict__ jobp, DevCounters* ctr) {
+    const DevJob& J = *jobp;
+    const DevScene& sc = J.scene;
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7);
+    const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const bool live = x < J.rect_w && y < J.rect_h;
+    const int ev = live ? plain_pixel(J, sc, x, y) : 0;
+    unsigned long long v = (unsigned long long)ev;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (threadIdx.x == 0 && ctr) atomicAdd(&ctr->de_march, v);
 }
 
 // ============================================================================ persistent passes
 __global__ void k_zero_counters(DevCounters* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         c->ray_next = 0; c->hit_count = 0; c->surf_next = 0; c->shadow_next = 0;
-        c->de_march = 0; c->de_surface = 0; c->de_shadow = 0; c->rays_done = 0;
+        c->de_march = 0; c->de_surface = 0; c->de_shadow = 0; c->rays_done = 0; c->hits = 0;
         c->rounds_capped = 0;
         c->clock_start = wall_clock64();
     }
@@ -235,33 +228,8 @@ NDD uint64_t lanemask_lt() {
     const unsigned lane = __lane_id();
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
-
-// Claims work items for the idle lanes of the wave from a global queue in chunks of
-// `chunk` items (one atomic per chunk). Returns per lane the item index or ~0u.
-// cnext/cend are wave-uniform; work_left is cleared when the queue is empty.
-NDD uint32_t claim(bool idle, uint32_t& cnext, uint32_t& cend, bool& work_left, uint32_t* head, uint32_t total,
-                   uint32_t chunk) {
-    uint32_t item = ~0u;
-    uint64_t need = __builtin_amdgcn_ballot_w64(idle);
-    while (need != 0ull && work_left) {
-        if (cnext >= cend) {
-            uint32_t base = 0;
-            if (__lane_id() == 0) base = atomicAdd(head, chunk);
-            base = __shfl(base, 0);
-            if (base >= total) { work_left = false; break; }
-            cnext = base;
-            cend = min(base + chunk, total);
-        }
-        const uint32_t avail = cend - cnext;
-        const uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
-        const bool mine = ((need >> __lane_id()) & 1ull) && rank < avail;
-        if (mine) item = cnext + rank;
-        const uint32_t taken = min((uint32_t)__popcll(need), avail);
-        cnext += taken;
-        need = __builtin_amdgcn_ballot_w64(((need >> __lane_id()) & 1ull) && !mine);
-    }
-    return item;
-}
+NDD bool lane_bit(uint64_t m) { return (m >> __lane_id()) & 1ull; }
+NDD uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 struct PassArgs {
     const DevJob* jobs;
@@ -274,46 +242,75 @@ struct PassArgs {
     uint32_t chunk;
     uint32_t min_idle;
     uint32_t round_cap;
-    float* job_done_ms;      // unused on device; latency is tracked with job_done_cycles
     unsigned long long* job_done_cycles;  // per job: wall clock (s_memrealtime) of its last retired ray
 };
 
 NDD void write_miss(const DevJob& J, size_t o, int evals) {
     if (J.out_depth) J.out_depth[o] = 0.f;
     if (J.out_normal) { J.out_normal[3 * o] = 0.f; J.out_normal[3 * o + 1] = 0.f; J.out_normal[3 * o + 2] = 0.f; }
+    if (J.out_pos) { J.out_pos[3 * o] = 0.f; J.out_pos[3 * o + 1] = 0.f; J.out_pos[3 * o + 2] = 0.f; }
     if (J.out_trap) J.out_trap[o] = 0.f;
     if (J.out_hit) J.out_hit[o] = 0;
     if (J.out_shadow) J.out_shadow[o] = 1.f;
     if (J.out_evals) J.out_evals[o] = (uint32_t)evals;
 }
 
-// --- pass 1: march. One DE per active lane per round; finished lanes refill at once.
+// --- pass 1: march. Every round each active lane takes one DE sample; a lane whose ray
+// finished is refilled at once from the wave's current chunk of ray slots. The chunk and
+// its job are wave-uniform, so a refill reads the job's camera and precision with scalar
+// loads and the hot loop touches no per-lane memory.
 template <bool SHARED>
 __global__ void __launch_bounds__(64) k_march(PassArgs a) {
-    uint32_t cnext = 0, cend = 0;
+    uint32_t cnext = 0, cend = 0;          // wave-uniform chunk of ray slots [cnext, cend)
+    int cjob = 0;                          // wave-uniform job of the chunk position
+    uint32_t cjstart = 1, cjend = 0;       // ray slots of cjob
+    uint32_t hnext = 0, hend = 0;          // wave-uniform chunk of hit-list slots
     bool work_left = true;
     bool active = false;
     uint32_t ray = 0;
     int job = 0, px = 0, py = 0;
+    int lat_job = -1;                      // per-lane latency record: last job retired and when
+    unsigned long long lat_clock = 0;
     V3<float> org = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 1.f);
     March m;
     march_init(m);
-    unsigned long long de = 0;
+    MarchParams mp = {};
+    unsigned long long de = 0, nhits = 0;
     for (uint32_t round = 0;; ++round) {
         if (work_left) {
-            const uint64_t idle = __builtin_amdgcn_ballot_w64(!active);
-            if ((uint32_t)__popcll(idle) >= a.min_idle || idle == ~0ull) {
-                uint32_t r = claim(!active, cnext, cend, work_left, &a.ctr->ray_next, a.total_rays, a.chunk);
-                if (r != ~0u) {
-                    ray = r;
-                    job = find_job(a.job_base, a.n_jobs, r);
-                    const DevJob& J = a.jobs[job];
-                    if (slot_pixel(J, r - J.ray_base, &px, &py)) {
-                        dir = primary_dir(J, J.rect_x + px, J.rect_y + py);
-                        org = mk3(J.cam[0], J.cam[1], J.cam[2]);
-                        march_init(m);
-                        active = true;
+            uint64_t need = __builtin_amdgcn_ballot_w64(!active);
+            if ((uint32_t)__popcll(need) >= a.min_idle || need == ~0ull) {
+                while (need != 0ull) {
+                    if (cnext >= cend) {
+                        uint32_t base = 0;
+                        if (__lane_id() == 0) base = atomicAdd(&a.ctr->ray_next, a.chunk);
+                        base = uniform_u32(base);
+                        if (base >= a.total_rays) { work_left = false; break; }
+                        cnext = base;
+                        cend = uniform_u32(min(base + a.chunk, a.total_rays));
                     }
+                    if (cnext < cjstart || cnext >= cjend) {
+                        cjob = (int)uniform_u32((uint32_t)find_job(a.job_base, a.n_jobs, cnext));
+                        cjstart = a.jobs[cjob].ray_base;
+                        cjend = cjstart + a.jobs[cjob].ray_count;
+                    }
+                    const DevJob& CJ = a.jobs[cjob];
+                    const uint32_t avail = min(cend, cjend) - cnext;
+                    const uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
+                    const bool mine = lane_bit(need) && rank < avail;
+                    if (mine) {
+                        ray = cnext + rank;
+                        if (slot_pixel(CJ, ray - cjstart, &px, &py)) {
+                            job = cjob;
+                            dir = primary_dir(CJ, CJ.rect_x + px, CJ.rect_y + py);
+                            org = mk3(CJ.cam[0], CJ.cam[1], CJ.cam[2]);
+                            mp = march_params(CJ);
+                            march_init(m);
+                            active = true;
+                        }
+                    }
+                    cnext += min((uint32_t)__popcll(need), avail);
+                    need = __builtin_amdgcn_ballot_w64(lane_bit(need) && !mine);
                 }
             }
         }
@@ -326,12 +323,12 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a) {
         bool hit_now = false;
         float dist = 0.f;
         if (active) {
-            const DevJob& J = a.jobs[job];
             const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
             const float s = march_sample_t(m);
             const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
-            const int st = march_update(m, J, s, d, &dist);
+            const int st = march_update(m, mp, s, d, &dist);
             if (st != MS_CONT) {
+                const DevJob& J = a.jobs[job];
                 const size_t o = (size_t)py * J.rect_w + px;
                 if (st == MS_HIT) {
                     hit_now = true;
@@ -342,103 +339,149 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a) {
                     write_miss(J, o, m.evals);
                 }
                 active = false;
-                if (a.job_done_cycles) atomicMax(&a.job_done_cycles[job], (unsigned long long)wall_clock64());
+                if (a.job_done_cycles) {
+                    // jobs reach a lane in increasing order: flush a job's clock when the lane moves on
+                    if (lat_job >= 0 && lat_job != job) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
+                    lat_job = job;
+                    lat_clock = (unsigned long long)wall_clock64();
+                }
             }
         }
+        // append this round's hits to the compacted hit list through the wave's slot chunk
         const uint64_t hm = __builtin_amdgcn_ballot_w64(hit_now);
         if (hm) {
-            const int leader = __builtin_ctzll(hm);
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(&a.ctr->hit_count, (uint32_t)__popcll(hm));
-            base = __shfl(base, leader);
+            const uint32_t nh = (uint32_t)__popcll(hm);
+            nhits += nh;
+            const uint32_t room = hend - hnext;
+            uint32_t nbase = 0;
+            if (nh > room) {
+                if (__lane_id() == 0) nbase = atomicAdd(&a.ctr->hit_count, 64u);
+                nbase = uniform_u32(nbase);
+            }
             if (hit_now) {
-                const uint32_t idx = base + (uint32_t)__popcll(hm & lanemask_lt());
+                const uint32_t rank = (uint32_t)__popcll(hm & lanemask_lt());
+                const uint32_t idx = rank < room ? hnext + rank : nbase + (rank - room);
                 a.hb.ray[idx] = ray;
                 a.hb.job[idx] = (uint16_t)job;
                 a.hb.t[idx] = dist;
             }
+            if (nh > room) { hnext = nbase + (nh - room); hend = nbase + 64u; }
+            else hnext += nh;
         }
         if (round >= a.round_cap) {
             if (__lane_id() == 0) atomicAdd(&a.ctr->rounds_capped, 1u);
             break;
         }
     }
-    if (__lane_id() == 0) atomicAdd(&a.ctr->de_march, de);
+    // unused slots of the last hit chunk are marked empty for the later passes
+    for (uint32_t i = hnext + __lane_id(); i < hend; i += 64) a.hb.ray[i] = ~0u;
+    if (a.job_done_cycles && lat_job >= 0) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
+    if (__lane_id() == 0) {
+        atomicAdd(&a.ctr->de_march, de);
+        atomicAdd(&a.ctr->hits, nhits);
+    }
 }
 
-// Decodes hit i: job, output index, hit point, view direction.
-NDD void hit_geometry(const PassArgs& a, uint32_t i, int* job, size_t* o, V3<float>* hp) {
-    const uint32_t r = a.hb.ray[i];
-    *job = a.hb.job[i];
-    const DevJob& J = a.jobs[*job];
-    int x, y;
-    slot_pixel(J, r - J.ray_base, &x, &y);
-    *o = (size_t)y * J.rect_w + x;
-    const V3<float> dir = primary_dir(J, J.rect_x + x, J.rect_y + y);
-    const float t = a.hb.t[i];
-    *hp = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
-}
-
-// --- pass 2: surface. Normal + trap of every hit (fixed cost per hit, no refill needed).
+// --- pass 2: surface. Normal + trap of every hit (fixed cost per hit), and the shadow
+// ray of the hit, stored SoA for pass 3.
 template <bool SHARED>
 __global__ void __launch_bounds__(64) k_surface(PassArgs a) {
     const uint32_t n_hits = a.ctr->hit_count;
     unsigned long long de = 0;
     for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i - threadIdx.x < n_hits; i += gridDim.x * 64) {
-        if (i < n_hits) {
-            int job; size_t o; V3<float> hp;
-            hit_geometry(a, i, &job, &o, &hp);
+        const uint32_t r = i < n_hits ? a.hb.ray[i] : ~0u;
+        if (i < n_hits && r == ~0u) a.hb.steps[i] = 0;  // empty slot of a partly used chunk
+        if (r != ~0u) {
+            const int job = a.hb.job[i];
             const DevJob& J = a.jobs[job];
             const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
+            int x, y;
+            slot_pixel(J, r - J.ray_base, &x, &y);
+            const size_t o = (size_t)y * J.rect_w + x;
+            const V3<float> dir = primary_dir(J, J.rect_x + x, J.rect_y + y);
             const float t = a.hb.t[i];
+            const V3<float> hp = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
             int ev = 0;
             const V3<float> n = surface_normal(sc, J.normal_mode, hp, J.eps_c * t + J.eps_f, &ev);
             float trap = 0.f;
             (void)scene_de<true>(sc, hp, &trap);
             ev += 1;
             de += (unsigned long long)ev;
-            a.hb.nx[i] = n.x; a.hb.ny[i] = n.y; a.hb.nz[i] = n.z;
             if (J.out_normal) { J.out_normal[3 * o] = n.x; J.out_normal[3 * o + 1] = n.y; J.out_normal[3 * o + 2] = n.z; }
+            if (J.out_pos) { J.out_pos[3 * o] = hp.x; J.out_pos[3 * o + 1] = hp.y; J.out_pos[3 * o + 2] = hp.z; }
             if (J.out_trap) J.out_trap[o] = trap;
-            if (J.out_shadow && !J.shadow_on) J.out_shadow[o] = 1.f;
+            // shadow ray of light 0 (shadow_factor, template.wgsl:2365-2375); steps = 0 marks
+            // a hit that needs no march (shadows off, or the light is behind the surface)
+            int steps = 0;
+            float shadow_now = 1.f;
+            V3<float> ld = mk3(0.f, 0.f, 0.f);
+            float ldist = 0.f;
+            if (J.shadow_on) {
+                light_at(J, hp, &ld, &ldist);
+                if (dot3(n, ld) <= 0.f) shadow_now = 0.f;
+                else if (J.shadow_steps > 0 && 0.004f < ldist) steps = J.shadow_steps;
+            }
+            if (steps == 0 && J.out_shadow) J.out_shadow[o] = shadow_now;
+            a.hb.opix[i] = (uint32_t)o;
+            a.hb.steps[i] = (uint16_t)steps;
+            a.hb.sox[i] = hp.x + n.x * 0.004f;
+            a.hb.soy[i] = hp.y + n.y * 0.004f;
+            a.hb.soz[i] = hp.z + n.z * 0.004f;
+            a.hb.sdx[i] = ld.x;
+            a.hb.sdy[i] = ld.y;
+            a.hb.sdz[i] = ld.z;
+            a.hb.sdist[i] = ldist;
         }
     }
     for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off);
     if (threadIdx.x == 0) atomicAdd(&a.ctr->de_surface, de);
 }
 
-// --- pass 3: light-0 soft shadow of every hit, refilled like the march.
+// --- pass 3: light-0 soft shadow of every hit (visibility_march on an opaque scene,
+// template.wgsl:2282-2342), refilled like the march from the SoA shadow rays.
 template <bool SHARED>
 __global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
     const uint32_t n_hits = a.ctr->hit_count;
     uint32_t cnext = 0, cend = 0;
     bool work_left = true, active = false;
+    uint32_t hit = 0;
     int job = 0, steps = 0, step = 0;
-    size_t o = 0;
     V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
-    float t = 0.f, res = 1.f, ldist = 0.f;
+    float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
+    int lat_job = -1;
+    unsigned long long lat_clock = 0;
     unsigned long long de = 0;
     for (uint32_t round = 0;; ++round) {
         if (work_left) {
-            const uint64_t idle = __builtin_amdgcn_ballot_w64(!active);
-            if ((uint32_t)__popcll(idle) >= a.min_idle || idle == ~0ull) {
-                uint32_t i = claim(!active, cnext, cend, work_left, &a.ctr->shadow_next, n_hits, a.chunk);
-                if (i != ~0u) {
-                    V3<float> hp;
-                    hit_geometry(a, i, &job, &o, &hp);
-                    const DevJob& J = a.jobs[job];
-                    if (J.shadow_on) {
-                        const V3<float> n = mk3(a.hb.nx[i], a.hb.ny[i], a.hb.nz[i]);
-                        light_at(J, hp, &ld, &ldist);
-                        if (dot3(n, ld) <= 0.f) {
-                            if (J.out_shadow) J.out_shadow[o] = 0.f;
-                        } else {
-                            so = mk3(hp.x + n.x * 0.004f, hp.y + n.y * 0.004f, hp.z + n.z * 0.004f);
-                            t = 0.004f; res = 1.f; step = 0; steps = J.shadow_steps;
-                            if (steps > 0 && t < ldist) active = true;
-                            else if (J.out_shadow) J.out_shadow[o] = 1.f;
+            uint64_t need = __builtin_amdgcn_ballot_w64(!active);
+            if ((uint32_t)__popcll(need) >= a.min_idle || need == ~0ull) {
+                while (need != 0ull) {
+                    if (cnext >= cend) {
+                        uint32_t base = 0;
+                        if (__lane_id() == 0) base = atomicAdd(&a.ctr->shadow_next, a.chunk);
+                        base = uniform_u32(base);
+                        if (base >= n_hits) { work_left = false; break; }
+                        cnext = base;
+                        cend = uniform_u32(min(base + a.chunk, n_hits));
+                    }
+                    const uint32_t avail = cend - cnext;
+                    const uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
+                    const bool mine = lane_bit(need) && rank < avail;
+                    if (mine) {
+                        hit = cnext + rank;
+                        steps = a.hb.steps[hit];
+                        if (steps > 0) {
+                            job = a.hb.job[hit];
+                            so = mk3(a.hb.sox[hit], a.hb.soy[hit], a.hb.soz[hit]);
+                            ld = mk3(a.hb.sdx[hit], a.hb.sdy[hit], a.hb.sdz[hit]);
+                            ldist = a.hb.sdist[hit];
+                            k = a.jobs[job].light_k;
+                            t = 0.004f; res = 1.f; step = 0;
+                            active = true;
                         }
                     }
+                    cnext += min((uint32_t)__popcll(need), avail);
+                    need = __builtin_amdgcn_ballot_w64(lane_bit(need) && !mine);
                 }
             }
         }
@@ -449,22 +492,27 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
         }
         de += (unsigned long long)__popcll(act);
         if (active) {
-            const DevJob& J = a.jobs[job];
             const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
             const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
             step++;
             bool done = false;
             if (d < 0.0005f) { res = 0.f; done = true; }
             else {
-                res = fminf(res, J.light_k * d * frcp(t));
+                res = fminf(res, k * d * frcp(t));
                 t += fmaxf(d, 0.001f);
                 if (step >= steps || t >= ldist) done = true;
             }
             if (done) {
+                const DevJob& J = a.jobs[job];
+                const uint32_t o = a.hb.opix[hit];
                 if (J.out_shadow) J.out_shadow[o] = fclamp(res, 0.f, 1.f);
                 if (J.out_evals) J.out_evals[o] += (uint32_t)step;
                 active = false;
-                if (a.job_done_cycles) atomicMax(&a.job_done_cycles[job], (unsigned long long)wall_clock64());
+                if (a.job_done_cycles) {
+                    if (lat_job >= 0 && lat_job != job) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
+                    lat_job = job;
+                    lat_clock = (unsigned long long)wall_clock64();
+                }
             }
         }
         if (round >= a.round_cap) {
@@ -472,6 +520,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
             break;
         }
     }
+    if (a.job_done_cycles && lat_job >= 0) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
     if (__lane_id() == 0) atomicAdd(&a.ctr->de_shadow, de);
 }
 
@@ -483,8 +532,13 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 }  // namespace nd
 
 // ----------------------------------------------------------------------------- launchers
-hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, hipStream_t s) {
-    hipLaunchKernelGGL(nd::k_plain, dim3(tiles_x, tiles_y), dim3(64), 0, s, d_job);
+hipError_t nd_launch_zero_counters(DevCounters* c, hipStream_t s) {
+    hipLaunchKernelGGL(nd::k_zero_counters, dim3(1), dim3(64), 0, s, c);
+    return hipGetLastError();
+}
+
+hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, hipStream_t s) {
+    hipLaunchKernelGGL(nd::k_plain, dim3(tiles_x, tiles_y), dim3(64), 0, s, d_job, ctr);
     return hipGetLastError();
 }
 
@@ -500,7 +554,6 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
     a.chunk = L.chunk;
     a.min_idle = L.min_idle;
     a.round_cap = L.round_cap;
-    a.job_done_ms = nullptr;
     a.job_done_cycles = L.job_done_cycles;
     hipLaunchKernelGGL(nd::k_zero_counters, dim3(1), dim3(64), 0, s, L.ctr);
     if (L.job_done_cycles) {
@@ -510,8 +563,10 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
     if (L.shared) hipLaunchKernelGGL(nd::k_march<true>, dim3(L.grid), dim3(64), 0, s, a);
     else hipLaunchKernelGGL(nd::k_march<false>, dim3(L.grid), dim3(64), 0, s, a);
     if (L.ev[1]) hipEventRecord(L.ev[1], s);
-    if (L.shared) hipLaunchKernelGGL(nd::k_surface<true>, dim3(L.grid), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(nd::k_surface<false>, dim3(L.grid), dim3(64), 0, s, a);
+    if (L.surface) {
+        if (L.shared) hipLaunchKernelGGL(nd::k_surface<true>, dim3(L.grid), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL(nd::k_surface<false>, dim3(L.grid), dim3(64), 0, s, a);
+    }
     if (L.ev[2]) hipEventRecord(L.ev[2], s);
     if (L.any_shadow) {
         if (L.shared) hipLaunchKernelGGL(nd::k_shadow<true>, dim3(L.grid), dim3(64), 0, s, a);

@@ -51,3 +51,30 @@ def compare(gpu, ref):
             st["shadow_abs_le_0.02"] = float((ds <= 0.02).mean())
             st["shadow_abs_median"] = float(np.median(ds))
     return st
+
+
+def pointwise(gpu, u, normal_mode, code_map=None):
+    """Normal and trap of the oracle (FP64) evaluated at the GPU's own hit points.
+
+    At the workloads' epsilon (1e-3 of a pixel) the DE gradient of a fractal changes
+    direction over distances comparable to FP32 rounding of the hit point, so end-to-end
+    normals mix march rounding with normal arithmetic. Evaluating the oracle at the exact
+    FP32 hit point (the position output) isolates the normal and trap computation."""
+    import oracle
+    h = gpu["hit"].astype(bool)
+    pos = gpu["position"][h].astype(np.float64)
+    t = gpu["depth"][h].astype(np.float64)
+    eps = float(u.epsilon_coefficient) * t + float(u.epsilon_floor)
+    e = eps if normal_mode == 1 else 0.5 * eps
+    ref_n = oracle.normal(u, pos, normal_mode, e, code_map=code_map)
+    gn = gpu["normal"][h].astype(np.float64)
+    cosang = np.clip(np.sum(gn * ref_n, axis=1) / np.maximum(
+        np.linalg.norm(gn, axis=1) * np.linalg.norm(ref_n, axis=1), 1e-30), -1, 1)
+    ang = np.degrees(np.arccos(cosang))
+    _, ref_trap = oracle.scene_de(u, pos, trap=True, code_map=code_map)
+    st = {"pw_hits": int(h.sum())}
+    if h.any():
+        st["pw_normal_deg_median"] = float(np.median(ang))
+        st["pw_normal_deg_le_0.5"] = float((ang <= 0.5).mean())
+        st["pw_trap_equal"] = float((np.abs(gpu["trap"][h] - ref_trap) <= 1e-5).mean())
+    return st

@@ -11,7 +11,7 @@ import pytest
 import numericdream_amd as nd
 from numericdream_amd import abi
 import oracle
-from tests.parity import compare
+from tests.parity import compare, pointwise
 
 pytestmark = pytest.mark.gpu
 
@@ -28,11 +28,17 @@ def report(name, st):
     print("\n[parity] %s %s" % (name, " ".join("%s=%.6g" % (k, v) for k, v in st.items())), flush=True)
 
 
-def check(st, normal_frac=0.97, depth_frac=0.99, trap_frac=0.98, shadow_frac=0.95):
+def check(st, normal_frac, depth_frac=0.99, trap_frac=0.995, shadow_frac=0.95):
+    """End-to-end: hit mask, depth, shadow. Pointwise (oracle at the GPU hit point): normal
+    within 0.5 degrees and trap equality. The normal fractions are what FP32 reaches: at an
+    epsilon of 1e-3 pixel a late fold or bail-out branch of the DE iteration can flip under
+    FP32 rounding of z_k, which changes the gradient though not the distance."""
+    assert st["pw_normal_deg_median"] <= 0.1, st
     assert st["hit_agree_inner"] >= 0.999, st
     assert st["depth_rel_le_1e-4"] >= depth_frac, st
-    assert st["normal_deg_le_0.5"] >= normal_frac, st
-    assert st["trap_equal"] >= trap_frac, st
+    assert st["depth_rel_median"] <= 1e-5, st
+    assert st["pw_normal_deg_le_0.5"] >= normal_frac, st
+    assert st["pw_trap_equal"] >= trap_frac, st
     if "shadow_abs_le_0.02" in st:
         assert st["shadow_abs_le_0.02"] >= shadow_frac, st
 
@@ -44,8 +50,9 @@ def test_mandelbulb_c1_quarter(renderer):
     for kernel in (abi.KERNEL_PERSISTENT, abi.KERNEL_PLAIN):
         g = gpu_render(renderer, u, kernel, shadows=False)
         st = compare(g, ref)
+        st.update(pointwise(g, u, 0))
         report("c1 kernel=%d" % kernel, st)
-        check(st)
+        check(st, normal_frac=0.90)
 
 
 def test_mandelbox_c2_shadow(renderer):
@@ -55,8 +62,9 @@ def test_mandelbox_c2_shadow(renderer):
     for kernel in (abi.KERNEL_PERSISTENT, abi.KERNEL_PLAIN):
         g = gpu_render(renderer, u, kernel, shadows=True, shadow_max_steps=128)
         st = compare(g, ref)
+        st.update(pointwise(g, u, 0))
         report("c2 kernel=%d" % kernel, st)
-        check(st)
+        check(st, normal_frac=0.97)
 
 
 def test_plain_and_persistent_identical(renderer):
@@ -64,7 +72,7 @@ def test_plain_and_persistent_identical(renderer):
     u = nd.resize(nd.config_uniforms(2, 3), 200, 120)
     a = gpu_render(renderer, u, abi.KERNEL_PLAIN, shadows=True, shadow_max_steps=128)
     b = gpu_render(renderer, u, abi.KERNEL_PERSISTENT, shadows=True, shadow_max_steps=128)
-    for k in ("depth", "normal", "trap", "hit", "shadow", "evals"):
+    for k in ("depth", "normal", "position", "trap", "hit", "shadow", "evals"):
         assert np.array_equal(a[k], b[k]), k
 
 
@@ -75,9 +83,11 @@ def test_reference_shadow_rule_and_tetra(renderer):
     g = gpu_render(renderer, u, abi.KERNEL_PERSISTENT, normal_mode=abi.NORMAL_TETRA, shadows=True,
                    shadow_max_steps=0)
     st = compare(g, ref)
+    st.update(pointwise(g, u, 1))
     report("c2 tetra shadow40", st)
     assert st["hit_agree_inner"] >= 0.999
     assert st["depth_rel_le_1e-4"] >= 0.99
+    assert st["pw_trap_equal"] >= 0.995
 
 
 def test_batch_mixed_jobs(renderer):
@@ -93,7 +103,9 @@ def test_batch_mixed_jobs(renderer):
         refs.append(oracle.render_gbuffer(u, normal_mode=0, shadows=False))
     renderer.render_batch(jobs, shadows=False)
     for i, ((u, g), ref) in enumerate(zip(jobs, refs)):
-        st = compare(g.numpy(), ref)
+        gn = g.numpy()
+        st = compare(gn, ref)
+        st.update(pointwise(gn, u, 0))
         report("c4 job %d %dx%d" % (i, g.w, g.h), st)
         assert st["hit_agree_inner"] >= 0.995, st
         if st["hits_ref"] > 50:
