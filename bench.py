@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -68,7 +70,63 @@ def timed(r, jobs, kernel, steps, warmup, torch, abi, label):
     return agg
 
 
-def cpu_baseline(nd, target_s=15.0):
+def batch_jobs_leg(r, nd, torch, abi, reps=3):
+    """256 uneven G-buffer jobs (Mandelbulb / Mandelbox, 64..1024 px, 8..20 iterations) in one
+    persistent launch per pass, against one plain launch per job."""
+    jobs = []
+    for i in range(nd.config_job_count(4)):
+        u = nd.config_uniforms(4, i)
+        w, h = int(u.resolution[0]), int(u.resolution[1])
+        jobs.append((u, nd.GBuffer(w, h, device="cuda:0", fields=("depth", "normal", "trap", "hit"))))
+    rays = sum(j[1].w * j[1].h for j in jobs)
+    t = r.tuning()
+    r.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays, track_latency=True)
+    res = {}
+    for name, kernel in (("persistent", abi.KERNEL_PERSISTENT), ("plain", abi.KERNEL_PLAIN)):
+        r.render_batch(jobs, shadows=False, kernel=kernel)  # warmup
+        r.sync()
+        t0 = time.perf_counter()
+        ms = []
+        for _ in range(reps):
+            st = r.render_batch(jobs, shadows=False, kernel=kernel)
+            ms.append(st.ms_total)
+        wall = (time.perf_counter() - t0) / reps
+        res[name] = {"ms_per_batch": round(wall * 1e3, 3), "device_ms": round(min(ms), 3),
+                     "jobs_per_s": round(len(jobs) / wall, 1), "mrays_per_s": round(rays / wall / 1e6, 2)}
+        if kernel == abi.KERNEL_PERSISTENT:
+            lat = np.sort(r.job_latency_ms(len(jobs)))
+            res[name]["latency_ms_p50"] = round(float(np.percentile(lat, 50)), 3)
+            res[name]["latency_ms_p99"] = round(float(np.percentile(lat, 99)), 3)
+            res[name]["de_evals"] = int(st.de_evals)
+        log("c4 %s: %.1f ms per batch of %d jobs" % (name, wall * 1e3, len(jobs)))
+    r.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays, track_latency=False)
+    res["jobs"] = len(jobs)
+    res["rays"] = rays
+    res["persistent_vs_plain"] = round(res["persistent"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 3)
+    return res
+
+
+def gradient_leg(r, nd, torch, reps=3):
+    """Differentiable render: 8 views of 512x512, depth loss against a target, gradient of the
+    loss with respect to (bulb power, box scale, box min radius, camera x/y/z)."""
+    views = [nd.config_uniforms(5, i) for i in range(nd.config_job_count(5))]
+    total = sum(int(v.resolution[0]) * int(v.resolution[1]) for v in views)
+    target = torch.zeros(total, dtype=torch.float32, device="cuda:0")
+    r.adjoint(views, depth_out=target)
+    for v in views:  # perturbed parameters: the loss is not zero
+        v.instances[0].params[0] = 8.2
+    r.adjoint(views, target=target)
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, grad, used, st = r.adjoint(views, target=target)
+    wall = (time.perf_counter() - t0) / reps
+    return {"grad_rays_per_s": round(total / wall, 1), "ms_per_call": round(wall * 1e3, 3),
+            "device_ms": round(st.ms_total, 3), "adjoint_pass_ms": round(st.ms_surface, 3),
+            "views": len(views), "pixels": total, "pixels_in_loss": used}
+
+
+def cpu_baseline(nd, target_s=20.0):
     """FP32 CPU restatement of the same per-ray math (the oracle instantiated for float),
     on host threads, over a band of rows of camera 0 at full 1920x1080."""
     import oracle
@@ -100,6 +158,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="skip the 256-job and gradient legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
@@ -152,6 +211,11 @@ def main():
         res["plain_mrays_per_s"] = round(pv, 3)
         res["persistent_vs_plain"] = round(value / pv, 3) if pv > 0 else None
         res["plain_ms_per_step"] = round(plain["wall_s"] * 1e3 / steps, 3)
+    if not args.no_legs:
+        res["c4_batch"] = batch_jobs_leg(r, nd, torch, abi)
+        g = gradient_leg(r, nd, torch)
+        res["grad_rays_per_s"] = g["grad_rays_per_s"]
+        res["c5_gradient"] = g
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(nd)

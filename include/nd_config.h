@@ -44,8 +44,11 @@ nd_status nd_config_build(uint32_t config, uint32_t index, uint64_t seed, nd_uni
  * both scaled by tan(fov/2) so that 2 atan(|up|) = fov. */
 void nd_camera_look_at(nd_uniforms* u, const float pos[3], const float target[3], float fov_deg);
 
-/* Epsilon of the workloads: eps(t) = 1e-3 * pixel_angle * t, pixel_angle = 2 tan(fov/2) / H. */
-void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg);
+/* Epsilon of the workloads: eps(t) = fraction * pixel_angle * t with pixel_angle =
+ * 2 tan(fov/2) / H and epsilon_floor = 0. Workloads 1-4 use fraction 1e-3; the
+ * differentiable workload 5 uses a whole pixel footprint (fraction 1), where the hit depth
+ * is a smooth function of the parameters on the scale of the finite-difference step. */
+void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg, float fraction);
 
 #ifdef __cplusplus
 }

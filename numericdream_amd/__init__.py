@@ -29,6 +29,13 @@ class NdError(RuntimeError):
 
 
 def _load():
+    # PyTorch-ROCm ships its own libamdhip64.so.7 under the same soname as the system one;
+    # the first one loaded serves the whole process. Loading torch first makes both this
+    # library and torch's tensors use one HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError("numericdream_amd: %s is missing; run numericdream_amd/build.py "
                           "(or __graft_entry__.build()) first" % LIB_PATH)
@@ -59,7 +66,7 @@ def _load():
     lib.nd_config_build.argtypes = [u32, u32, C.c_uint64, P(Uniforms)]
     lib.nd_camera_look_at.argtypes = [P(Uniforms), P(C.c_float), P(C.c_float), C.c_float]
     lib.nd_camera_look_at.restype = None
-    lib.nd_set_pixel_epsilon.argtypes = [P(Uniforms), C.c_float]
+    lib.nd_set_pixel_epsilon.argtypes = [P(Uniforms), C.c_float, C.c_float]
     lib.nd_set_pixel_epsilon.restype = None
     return lib
 
@@ -93,15 +100,23 @@ def camera_look_at(u, pos, target, fov_deg=60.0):
     lib.nd_camera_look_at(C.byref(u), p, t, fov_deg)
 
 
-def set_pixel_epsilon(u, fov_deg=60.0):
-    lib.nd_set_pixel_epsilon(C.byref(u), fov_deg)
+def set_pixel_epsilon(u, fov_deg=60.0, fraction=1e-3):
+    lib.nd_set_pixel_epsilon(C.byref(u), fov_deg, fraction)
+
+
+def pixel_fraction(u, fov_deg=60.0):
+    """The epsilon of u as a fraction of its pixel footprint."""
+    import math
+    theta = 2.0 * math.tan(math.radians(fov_deg) / 2) / float(u.resolution[1])
+    return float(u.epsilon_coefficient) / theta
 
 
 def resize(u, w, h, fov_deg=60.0):
-    """Same camera at another resolution; the epsilon follows the pixel angle."""
+    """Same camera at another resolution; the epsilon keeps its fraction of a pixel."""
+    frac = pixel_fraction(u, fov_deg)
     u.resolution[0] = float(w)
     u.resolution[1] = float(h)
-    set_pixel_epsilon(u, fov_deg)
+    set_pixel_epsilon(u, fov_deg, frac)
     return u
 
 
@@ -244,3 +259,22 @@ class Renderer:
 
     def sync(self):
         self._check(lib.nd_sync(self._h), "nd_sync")
+
+    def adjoint(self, views, target=None, depth_out=None, dtdp_out=None):
+        """Depth-loss gradient over views (list of Uniforms with the current parameters).
+
+        target / depth_out / dtdp_out: per-view depth maps concatenated in view order
+        (W*H floats each; dtdp has 6 floats per pixel), all numpy (host) or all torch
+        tensors on the device. Returns (loss, grad[6], pixels_used, stats).
+        Parameter order: bulb power, box scale, box min radius, camera x, y, z."""
+        n = len(views)
+        arr = (Uniforms * n)(*views)
+        dev = any(x is not None and hasattr(x, "data_ptr") for x in (target, depth_out, dtdp_out))
+        grad = (C.c_double * 6)()
+        loss = C.c_double()
+        used = C.c_uint64()
+        io = AdjointIO(_ptr(target), _ptr(depth_out), _ptr(dtdp_out), C.addressof(grad), C.addressof(loss),
+                       C.addressof(used), 1 if dev else 0, 0)
+        st = BatchStats()
+        self._check(lib.nd_adjoint(self._h, arr, n, C.byref(io), C.byref(st)), "nd_adjoint")
+        return loss.value, np.array(grad[:]), int(used.value), st

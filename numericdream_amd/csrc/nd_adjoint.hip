@@ -36,7 +36,8 @@ struct AdjArgs {
 };
 
 template <bool SHARED>
-__global__ void __launch_bounds__(64) k_adjoint(AdjArgs a) {
+__global__ void __launch_bounds__(64) k_adjoint(AdjArgs a, const DevJob* __restrict__ jobs,
+                                                 const DevScene* __restrict__ shared) {
     using D6 = DualN<6>;
     const uint32_t n_hits = a.ctr->hit_count;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -45,8 +46,8 @@ __global__ void __launch_bounds__(64) k_adjoint(AdjArgs a) {
         const uint32_t r = a.hb.ray[i];
         if (r == ~0u) continue;  // empty slot of a partly used hit chunk
         const int job = a.hb.job[i];
-        const DevJob& J = a.jobs[job];
-        const DevScene& sc = SHARED ? *a.shared : J.scene;
+        const DevJob& J = jobs[job];
+        const DevScene& sc = SHARED ? *shared : J.scene;
         int x, y;
         slot_pixel(J, r - J.ray_base, &x, &y);
         const size_t o = (size_t)y * J.rect_w + x;
@@ -171,8 +172,8 @@ extern "C" nd_status nd_adjoint(nd_ctx* c, const nd_uniforms* views, uint32_t n,
     a.pix_base = d_base;
     a.acc = acc;
     (void)hipEventRecord(c->ev[4], c->stream);
-    if (a.shared) hipLaunchKernelGGL(nd::k_adjoint<true>, dim3(c->last_grid), dim3(64), 0, c->stream, a);
-    else hipLaunchKernelGGL(nd::k_adjoint<false>, dim3(c->last_grid), dim3(64), 0, c->stream, a);
+    if (a.shared) hipLaunchKernelGGL(nd::k_adjoint<true>, dim3(c->last_grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
+    else hipLaunchKernelGGL(nd::k_adjoint<false>, dim3(c->last_grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
     if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch adjoint");
     (void)hipEventRecord(c->ev[5], c->stream);
     double h_acc[8];

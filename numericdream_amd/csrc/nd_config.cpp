@@ -230,9 +230,9 @@ void nd_camera_look_at(nd_uniforms* u, const float pos[3], const float target[3]
     }
 }
 
-void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg) {
+void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg, float fraction) {
     double theta_px = 2.0 * std::tan(0.5 * (double)fov_deg * M_PI / 180.0) / (double)u->resolution[1];
-    u->epsilon_coefficient = (float)(1e-3 * theta_px);
+    u->epsilon_coefficient = (float)((double)fraction * theta_px);
     u->epsilon_floor = 0.0f;
 }
 
@@ -255,7 +255,7 @@ nd_status nd_config_build(uint32_t config, uint32_t index, uint64_t seed, nd_uni
         black_white_strip(in);
         const float cam[3] = {0, 0, -2.5f};
         nd_camera_look_at(u, cam, origin, fov);
-        nd_set_pixel_epsilon(u, fov);
+        nd_set_pixel_epsilon(u, fov, 1e-3f);
         return ND_OK;
     }
     if (config == 2) {
@@ -269,7 +269,7 @@ nd_status nd_config_build(uint32_t config, uint32_t index, uint64_t seed, nd_uni
         float cam[3];
         sphere_point(g, 4.0, cam);
         nd_camera_look_at(u, cam, origin, fov);
-        nd_set_pixel_epsilon(u, fov);
+        nd_set_pixel_epsilon(u, fov, 1e-3f);
         u->light_count = 1;
         nd_light* L = &u->lights[0];
         set3(L->color, 1, 1, 1);
@@ -324,7 +324,7 @@ nd_status nd_config_build(uint32_t config, uint32_t index, uint64_t seed, nd_uni
         k->params[7] = 1;                                   // CScale
         const float cam[3] = {0.0f, 1.5f, -9.0f};
         nd_camera_look_at(u, cam, origin, fov);
-        nd_set_pixel_epsilon(u, fov);
+        nd_set_pixel_epsilon(u, fov, 1e-3f);
         u->light_count = 1;
         nd_light* L = &u->lights[0];
         set3(L->color, 1, 1, 1);
@@ -359,7 +359,7 @@ nd_status nd_config_build(uint32_t config, uint32_t index, uint64_t seed, nd_uni
         float cam[3];
         sphere_point(g, g.uniform(2.5, 5.0), cam);
         nd_camera_look_at(u, cam, origin, fov);
-        nd_set_pixel_epsilon(u, fov);
+        nd_set_pixel_epsilon(u, fov, 1e-3f);
         return ND_OK;
     }
     if (config == 5) {
@@ -381,7 +381,7 @@ nd_status nd_config_build(uint32_t config, uint32_t index, uint64_t seed, nd_uni
         float cam[3];
         sphere_point(g, 3.0, cam);
         nd_camera_look_at(u, cam, origin, fov);
-        nd_set_pixel_epsilon(u, fov);
+        nd_set_pixel_epsilon(u, fov, 1.0f);  // one pixel footprint (see nd_config.h)
         return ND_OK;
     }
     return ND_E_INVALID;

@@ -24,13 +24,13 @@ namespace nd {
 // ----------------------------------------------------------------------------- float math
 NDD float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 NDD float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-NDD float fsqrt(float x) { return __builtin_sqrtf(x); }
+NDD float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }  // v_sqrt_f32, 1 ulp
 NDD float flog2(float x) { return __builtin_amdgcn_logf(x); }
 NDD float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 NDD float fln(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
 NDD float fmin_(float a, float b) { return fminf(a, b); }
 NDD float fmax_(float a, float b) { return fmaxf(a, b); }
-NDD float fclamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+NDD float fclamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }  // v_med3_f32
 NDD float fabs_(float x) { return fabsf(x); }
 
 // ----------------------------------------------------------------------------- dual numbers
@@ -252,19 +252,35 @@ struct BoxF {
     float S, absS, m2, F2, kF, L, dr_add;
 };
 template <class S> struct BoxT { S Sc_, m2, F2, L; };
-NDD Carry<float> box_step(const Carry<float>& c, const V3<float>& posc, const BoxF& b, const DevInst& in) {
+// Branch-free: the fold factor is selected, the reciprocal always computed (a zero r2 is
+// covered by the r2 < m2 arm), the rotation is a compile-time choice and the fold factor
+// is folded into the scale: z' = z (t S) + posc, dr' = dr (t |S|) + max(|cs|, 1).
+template <bool ROT, bool CLAMP>
+NDD Carry<float> box_step(const Carry<float>& c, const V3<float>& posc, const BoxF& b, const float* rot_cs) {
     V3<float> z;
     z.x = fclamp(c.z.x, -b.L, b.L) * 2.f - c.z.x;
     z.y = fclamp(c.z.y, -b.L, b.L) * 2.f - c.z.y;
     z.z = fclamp(c.z.z, -b.L, b.L) * 2.f - c.z.z;
-    float r2 = dot3(z, z);
-    float t = r2 < b.m2 ? b.kF : (r2 < b.F2 ? b.F2 * frcp(r2) : 1.f);
-    z = z * t;
-    float dr = c.dr * t;
-    if (in.rot_on) z = rotate_cs(z, in.rot_cs);
+    const float r2 = dot3(z, z);
+    // sphere_fold's three arms (template.wgsl:959-969) as one clamp: with m2 <= F2,
+    // F2 / r2 > F2 / m2 exactly when r2 < m2 and F2 / r2 <= 1 exactly when r2 >= F2, so
+    // t = clamp(F2 / r2, 1, F2 / m2). The host keeps m2 > F2 off this path (kF < 1).
+    float t = b.F2 * frcp(r2);
+    if (CLAMP) {
+        t = __builtin_amdgcn_fmed3f(t, 1.f, b.kF);
+    } else {
+        t = r2 < b.F2 ? t : 1.f;
+        t = r2 < b.m2 ? b.kF : t;
+    }
     Carry<float> o;
-    o.z = mk3(z.x * b.S + posc.x, z.y * b.S + posc.y, z.z * b.S + posc.z);
-    o.dr = dr * b.absS + b.dr_add;
+    if (ROT) {
+        z = rotate_cs(mk3(z.x * t, z.y * t, z.z * t), rot_cs);
+        o.z = mk3(z.x * b.S + posc.x, z.y * b.S + posc.y, z.z * b.S + posc.z);
+    } else {
+        const float ts = t * b.S;
+        o.z = mk3(z.x * ts + posc.x, z.y * ts + posc.y, z.z * ts + posc.z);
+    }
+    o.dr = c.dr * (t * b.absS) + b.dr_add;
     return o;
 }
 template <class S> NDD Carry<S> box_step_t(const Carry<S>& c, const V3<S>& posc, const S& scale, const S& m2,
@@ -386,6 +402,19 @@ NDD float trap_finish(const TrapAcc& a, const DevInst& in, int iters) {
     return a.x;
 }
 
+// The Mandelbox iteration with the uniform choices (rotation, clamp form) taken outside
+// the loop.
+template <bool ROT, bool CLAMP, bool TRAP>
+NDD Carry<float> box_loop(Carry<float> c, const V3<float>& posc, const BoxF& b, const DevInst& in, int iters,
+                          TrapAcc& acc) {
+    for (int it = 0; it < iters; ++it) {
+        V3<float> zp = c.z;
+        c = box_step<ROT, CLAMP>(c, posc, b, in.rot_cs);
+        if (TRAP) trap_update(acc, in, c.z, zp, it);
+    }
+    return c;
+}
+
 // ----------------------------------------------------------------------------- formula wrapper
 // formula_N (fr.zig:490-513): iterate de_step de_iterations times from (pos, 1), update the
 // trap after every step, finalize. The float path exits early once every lane of the wave
@@ -424,10 +453,13 @@ NDD float formula_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
             b.S = in.h[0]; b.absS = in.h[1]; b.m2 = in.h[2]; b.F2 = in.h[3]; b.kF = in.h[4]; b.L = in.h[5];
             b.dr_add = in.h[10];
             V3<float> posc = mk3(pos.x * in.h[6] + in.h[7], pos.y * in.h[6] + in.h[8], pos.z * in.h[6] + in.h[9]);
-            for (int it = 0; it < iters; ++it) {
-                V3<float> zp = c.z;
-                c = box_step(c, posc, b, in);
-                if (TRAP) trap_update(acc, in, c.z, zp, it);
+            const bool clamp = b.kF >= 1.f;
+            if (in.rot_on) {
+                if (clamp) c = box_loop<true, true, TRAP>(c, posc, b, in, iters, acc);
+                else c = box_loop<true, false, TRAP>(c, posc, b, in, iters, acc);
+            } else {
+                if (clamp) c = box_loop<false, true, TRAP>(c, posc, b, in, iters, acc);
+                else c = box_loop<false, false, TRAP>(c, posc, b, in, iters, acc);
             }
             if (TRAP) *trap = trap_finish(acc, in, iters);
             return fdiv(len3(c.z), fabsf(c.dr));

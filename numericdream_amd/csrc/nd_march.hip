@@ -135,7 +135,7 @@ NDD void light_at(const DevJob& J, const V3<float>& p, V3<float>* dir, float* di
 // With SHARED every job of the batch has the same scene: it is read through a
 // kernel-argument pointer, so its fields are wave-uniform scalar loads.
 template <bool SHARED>
-NDD const DevScene& scene_of(const DevJob* jobs, const DevScene* shared, int job) {
+NDD const DevScene& scene_of(const DevJob* __restrict__ jobs, const DevScene* __restrict__ shared, int job) {
     if (SHARED) return *shared;
     return jobs[job].scene;
 }
@@ -260,7 +260,8 @@ NDD void write_miss(const DevJob& J, size_t o, int evals) {
 // its job are wave-uniform, so a refill reads the job's camera and precision with scalar
 // loads and the hot loop touches no per-lane memory.
 template <bool SHARED>
-__global__ void __launch_bounds__(64) k_march(PassArgs a) {
+__global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restrict__ jobs,
+                                                 const DevScene* __restrict__ shared) {
     uint32_t cnext = 0, cend = 0;          // wave-uniform chunk of ray slots [cnext, cend)
     int cjob = 0;                          // wave-uniform job of the chunk position
     uint32_t cjstart = 1, cjend = 0;       // ray slots of cjob
@@ -291,10 +292,10 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a) {
                     }
                     if (cnext < cjstart || cnext >= cjend) {
                         cjob = (int)uniform_u32((uint32_t)find_job(a.job_base, a.n_jobs, cnext));
-                        cjstart = a.jobs[cjob].ray_base;
-                        cjend = cjstart + a.jobs[cjob].ray_count;
+                        cjstart = jobs[cjob].ray_base;
+                        cjend = cjstart + jobs[cjob].ray_count;
                     }
-                    const DevJob& CJ = a.jobs[cjob];
+                    const DevJob& CJ = jobs[cjob];
                     const uint32_t avail = min(cend, cjend) - cnext;
                     const uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
                     const bool mine = lane_bit(need) && rank < avail;
@@ -323,12 +324,12 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a) {
         bool hit_now = false;
         float dist = 0.f;
         if (active) {
-            const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
+            const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             const float s = march_sample_t(m);
             const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
             const int st = march_update(m, mp, s, d, &dist);
             if (st != MS_CONT) {
-                const DevJob& J = a.jobs[job];
+                const DevJob& J = jobs[job];
                 const size_t o = (size_t)py * J.rect_w + px;
                 if (st == MS_HIT) {
                     hit_now = true;
@@ -385,7 +386,8 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a) {
 // --- pass 2: surface. Normal + trap of every hit (fixed cost per hit), and the shadow
 // ray of the hit, stored SoA for pass 3.
 template <bool SHARED>
-__global__ void __launch_bounds__(64) k_surface(PassArgs a) {
+__global__ void __launch_bounds__(64) k_surface(PassArgs a, const DevJob* __restrict__ jobs,
+                                                 const DevScene* __restrict__ shared) {
     const uint32_t n_hits = a.ctr->hit_count;
     unsigned long long de = 0;
     for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i - threadIdx.x < n_hits; i += gridDim.x * 64) {
@@ -393,8 +395,8 @@ __global__ void __launch_bounds__(64) k_surface(PassArgs a) {
         if (i < n_hits && r == ~0u) a.hb.steps[i] = 0;  // empty slot of a partly used chunk
         if (r != ~0u) {
             const int job = a.hb.job[i];
-            const DevJob& J = a.jobs[job];
-            const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
+            const DevJob& J = jobs[job];
+            const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             int x, y;
             slot_pixel(J, r - J.ray_base, &x, &y);
             const size_t o = (size_t)y * J.rect_w + x;
@@ -440,7 +442,8 @@ __global__ void __launch_bounds__(64) k_surface(PassArgs a) {
 // --- pass 3: light-0 soft shadow of every hit (visibility_march on an opaque scene,
 // template.wgsl:2282-2342), refilled like the march from the SoA shadow rays.
 template <bool SHARED>
-__global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
+__global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restrict__ jobs,
+                                                 const DevScene* __restrict__ shared) {
     const uint32_t n_hits = a.ctr->hit_count;
     uint32_t cnext = 0, cend = 0;
     bool work_left = true, active = false;
@@ -475,7 +478,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
                             so = mk3(a.hb.sox[hit], a.hb.soy[hit], a.hb.soz[hit]);
                             ld = mk3(a.hb.sdx[hit], a.hb.sdy[hit], a.hb.sdz[hit]);
                             ldist = a.hb.sdist[hit];
-                            k = a.jobs[job].light_k;
+                            k = jobs[job].light_k;
                             t = 0.004f; res = 1.f; step = 0;
                             active = true;
                         }
@@ -492,7 +495,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
         }
         de += (unsigned long long)__popcll(act);
         if (active) {
-            const DevScene& sc = scene_of<SHARED>(a.jobs, a.shared, job);
+            const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
             step++;
             bool done = false;
@@ -503,7 +506,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a) {
                 if (step >= steps || t >= ldist) done = true;
             }
             if (done) {
-                const DevJob& J = a.jobs[job];
+                const DevJob& J = jobs[job];
                 const uint32_t o = a.hb.opix[hit];
                 if (J.out_shadow) J.out_shadow[o] = fclamp(res, 0.f, 1.f);
                 if (J.out_evals) J.out_evals[o] += (uint32_t)step;
@@ -560,17 +563,17 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
         hipLaunchKernelGGL(nd::k_zero_u64, dim3((L.n_jobs + 255) / 256), dim3(256), 0, s, L.job_done_cycles, L.n_jobs);
     }
     if (L.ev[0]) hipEventRecord(L.ev[0], s);
-    if (L.shared) hipLaunchKernelGGL(nd::k_march<true>, dim3(L.grid), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(nd::k_march<false>, dim3(L.grid), dim3(64), 0, s, a);
+    if (L.shared) hipLaunchKernelGGL(nd::k_march<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+    else hipLaunchKernelGGL(nd::k_march<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
     if (L.ev[1]) hipEventRecord(L.ev[1], s);
     if (L.surface) {
-        if (L.shared) hipLaunchKernelGGL(nd::k_surface<true>, dim3(L.grid), dim3(64), 0, s, a);
-        else hipLaunchKernelGGL(nd::k_surface<false>, dim3(L.grid), dim3(64), 0, s, a);
+        if (L.shared) hipLaunchKernelGGL(nd::k_surface<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL(nd::k_surface<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
     }
     if (L.ev[2]) hipEventRecord(L.ev[2], s);
     if (L.any_shadow) {
-        if (L.shared) hipLaunchKernelGGL(nd::k_shadow<true>, dim3(L.grid), dim3(64), 0, s, a);
-        else hipLaunchKernelGGL(nd::k_shadow<false>, dim3(L.grid), dim3(64), 0, s, a);
+        if (L.shared) hipLaunchKernelGGL(nd::k_shadow<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL(nd::k_shadow<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
     }
     if (L.ev[3]) hipEventRecord(L.ev[3], s);
     return hipGetLastError();

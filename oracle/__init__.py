@@ -145,3 +145,30 @@ def formula_step(f, z, dr, pos, params):
     out = np.zeros(4)
     lib().ndo_formula_step(f, _p(z), float(dr), _p(pos), _p(p), _p(out))
     return out[:3], out[3]
+
+
+def adjoint_at(u, pixels, t, code_map=None):
+    """FP64 implicit-function gradient (6 parameters) at p = o + d t for the given depths."""
+    L = lib()
+    L.ndo_adjoint_at.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    pix = np.ascontiguousarray(pixels, np.int32).reshape(-1, 2)
+    tt = np.ascontiguousarray(t, np.float64).reshape(-1)
+    g = np.zeros((len(pix), 6))
+    cm, cmp_ = _cm(code_map)
+    L.ndo_adjoint_at(_u(u), cmp_, _p(pix), _p(tt), len(pix), _p(g))
+    return g
+
+
+def adjoint_pixels(u, pixels, h=1e-3, code_map=None):
+    """FP64 reference gradients of hit depth for pixels [(x, y), ...]: polished root depth,
+    implicit-function-theorem gradient and central finite differences (6 parameters:
+    bulb power, box scale, box min radius, camera x/y/z) and a per-parameter FD-valid mask."""
+    L = lib()
+    L.ndo_adjoint_pixels.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]
+    pix = np.ascontiguousarray(pixels, np.int32).reshape(-1, 2)
+    n = len(pix)
+    t = np.zeros(n); gi = np.zeros((n, 6)); gf = np.zeros((n, 6)); ok = np.zeros(n, np.int32)
+    cm, cmp_ = _cm(code_map)
+    L.ndo_adjoint_pixels(_u(u), cmp_, _p(pix), n, float(h), _p(t), _p(gi), _p(gf), _p(ok))
+    return t, gi, gf, ok

@@ -260,6 +260,109 @@ int ndo_flops_render(const nd_uniforms* u, const int32_t* cm, int shadow_max_ste
     return 0;
 }
 
+// FP64 reference gradients of the hit depth for a list of pixels.
+// Parameters: 0 Mandelbulb power, 1 Mandelbox scale, 2 Mandelbox min radius (first visible
+// instance of each formula), 3..5 camera translation. For each pixel:
+//   t_root   the polished hit depth (0 = miss)
+//   g_ift    dt/dp from the implicit-function theorem with FP64 dual numbers at t_root:
+//            -(dDE/dp) / (grad DE . d - c), camera: -(grad DE) / (grad DE . d - c)
+//   g_fd     central differences (t(p + h) - t(p - h)) / 2h of polished roots
+//   ok       bit k set when both FD roots exist for parameter k
+int ndo_adjoint_pixels(const nd_uniforms* u, const int32_t* cm, const int32_t* pix, int n, double h, double* t_root,
+                       double* g_ift, double* g_fd, int32_t* ok) {
+    using D6 = Dual<6>;
+    const CodeMap map = code_map(cm);
+    Scene<double> s = decode_scene<double>(*u, map);
+    MarchCfg mc = march_cfg(*u);
+    int bulb = -1, box = -1;
+    for (int i = 0; i < s.count; ++i) {
+        if (s.inst[i].hidden) continue;
+        if (s.inst[i].formula == F_MANDELBULB && bulb < 0) bulb = i;
+        if (s.inst[i].formula == F_MANDELBOX && box < 0) box = i;
+    }
+    Scene<D6> sd = decode_scene<D6>(*u, map);
+    if (bulb >= 0) sd.inst[bulb].p.p[0] = D6::var(s.inst[bulb].p.p[0], 3);
+    if (box >= 0) {
+        sd.inst[box].p.p[0] = D6::var(s.inst[box].p.p[0], 4);
+        sd.inst[box].p.p[1] = D6::var(s.inst[box].p.p[1], 5);
+    }
+    V3<double> o0 = V3<double>::from(u->camera_pos);
+    auto perturbed = [&](int k, double delta, V3<double>* o) {
+        Scene<double> q = s;
+        *o = o0;
+        if (k == 0 && bulb >= 0) q.inst[bulb].p.p[0] += delta;
+        if (k == 1 && box >= 0) q.inst[box].p.p[0] += delta;
+        if (k == 2 && box >= 0) q.inst[box].p.p[1] += delta;
+        if (k == 3) o->x += delta;
+        if (k == 4) o->y += delta;
+        if (k == 5) o->z += delta;
+        return q;
+    };
+    for (int i = 0; i < n; ++i) {
+        int px = pix[2 * i], py = pix[2 * i + 1];
+        V3<double> d = primary_dir<double>(*u, px, py, (int)u->resolution[0], (int)u->resolution[1]);
+        double t;
+        ok[i] = 0;
+        for (int k = 0; k < 6; ++k) g_ift[6 * i + k] = g_fd[6 * i + k] = 0;
+        if (!polished_root(s, mc, o0, d, &t)) { t_root[i] = 0; continue; }
+        t_root[i] = t;
+        V3<double> p = o0 + d * t;
+        V3<D6> q(D6::var(p.x, 0), D6::var(p.y, 1), D6::var(p.z, 2));
+        D6 de = scene_de(sd, q);
+        double den = de.d[0] * d.x + de.d[1] * d.y + de.d[2] * d.z - mc.eps_c;
+        if (den != 0) {
+            g_ift[6 * i + 0] = -de.d[3] / den;
+            g_ift[6 * i + 1] = -de.d[4] / den;
+            g_ift[6 * i + 2] = -de.d[5] / den;
+            g_ift[6 * i + 3] = -de.d[0] / den;
+            g_ift[6 * i + 4] = -de.d[1] / den;
+            g_ift[6 * i + 5] = -de.d[2] / den;
+        }
+        for (int k = 0; k < 6; ++k) {
+            V3<double> op, om;
+            Scene<double> sp = perturbed(k, h, &op), sm = perturbed(k, -h, &om);
+            double tp, tm;
+            if (polished_root(sp, mc, op, d, &tp) && polished_root(sm, mc, om, d, &tm)) {
+                g_fd[6 * i + k] = (tp - tm) / (2 * h);
+                ok[i] |= 1 << k;
+            }
+        }
+    }
+    return 0;
+}
+
+// FP64 implicit-function gradient at given hit points p = o + d t (t per pixel, e.g. the
+// GPU's own hit depths), so the FP32 gradient can be compared at the point it was taken.
+int ndo_adjoint_at(const nd_uniforms* u, const int32_t* cm, const int32_t* pix, const double* t_in, int n,
+                   double* g_out) {
+    using D6 = Dual<6>;
+    const CodeMap map = code_map(cm);
+    Scene<double> s = decode_scene<double>(*u, map);
+    MarchCfg mc = march_cfg(*u);
+    Scene<D6> sd = decode_scene<D6>(*u, map);
+    bool bulb = false, box = false;
+    for (int i = 0; i < sd.count; ++i) {
+        if (sd.inst[i].hidden) continue;
+        if (sd.inst[i].formula == F_MANDELBULB && !bulb) { sd.inst[i].p.p[0] = D6::var(s.inst[i].p.p[0], 3); bulb = true; }
+        if (sd.inst[i].formula == F_MANDELBOX && !box) {
+            sd.inst[i].p.p[0] = D6::var(s.inst[i].p.p[0], 4);
+            sd.inst[i].p.p[1] = D6::var(s.inst[i].p.p[1], 5);
+            box = true;
+        }
+    }
+    V3<double> o = V3<double>::from(u->camera_pos);
+    for (int i = 0; i < n; ++i) {
+        V3<double> d = primary_dir<double>(*u, pix[2 * i], pix[2 * i + 1], (int)u->resolution[0], (int)u->resolution[1]);
+        V3<double> p = o + d * t_in[i];
+        V3<D6> q(D6::var(p.x, 0), D6::var(p.y, 1), D6::var(p.z, 2));
+        D6 de = scene_de(sd, q);
+        double den = de.d[0] * d.x + de.d[1] * d.y + de.d[2] * d.z - mc.eps_c;
+        const int slot[6] = {3, 4, 5, 0, 1, 2};
+        for (int k = 0; k < 6; ++k) g_out[6 * i + k] = den != 0 ? -de.d[slot[k]] / den : 0.0;
+    }
+    return 0;
+}
+
 // Individual helpers for closed-form and invariant checks.
 //   0 box_fold(z3, L) -> z3                    1 sphere_fold(z3, dr, m2, F2) -> z3, dr
 //   2 sphere_invert_offset(z3, dr, c3, R2)     3 lattice_wrap(p3, cell) -> p3

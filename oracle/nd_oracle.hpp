@@ -794,7 +794,34 @@ T visibility_march_opaque(const Scene<P>& s, V3<T> origin, V3<T> dir, T light_di
     return tclamp(res, T(0), T(1));
 }
 
-// ----------------------------------------------------------------------------- camera
+// ----------------------------------------------------------------------------- adjoint
+// Hit depth as a smooth function of the parameters: the march's hit is refined to the
+// root of F(t) = DE(o + t d) - (c t + f) (the level set the on-hit bisection converges to,
+// template.wgsl:1924-1931) by FP64 bisection on a bracket [lo, hi] with F(lo) >= 0 > F(hi).
+template <class P>
+bool polished_root(const Scene<P>& s, const MarchCfg& m, V3<double> o, V3<double> d, double* t_out) {
+    MarchResult<double> r = march_to(s, m, o, d, m.max_dist);
+    if (!r.hit) return false;
+    auto F = [&](double t) { return val(scene_de(s, o + d * t)) - (m.eps_c * t + m.eps_f); };
+    double hi = r.dist;
+    if (F(hi) >= 0) { *t_out = hi; return true; }
+    double step = std::max(m.eps_c * hi + m.eps_f, 1e-9), lo = hi;
+    int k = 0;
+    for (; k < 60; ++k) {
+        lo = std::max(hi - step, 0.0);
+        if (F(lo) >= 0 || lo == 0.0) break;
+        step *= 2;
+    }
+    if (F(lo) < 0) return false;
+    for (int i = 0; i < 200 && hi - lo > 1e-15 * hi; ++i) {
+        double mid = 0.5 * (lo + hi);
+        if (F(mid) >= 0) lo = mid; else hi = mid;
+    }
+    *t_out = 0.5 * (lo + hi);
+    return true;
+}
+
+// ---------------------------------------------------------------------------- camera
 // Primary ray of pixel (px, py) of a target of size (tw, th): cs_wf_generate
 // (template.wgsl:5119-5127), identical to the fragment path of vs_main + fs_simple.
 template <class T> V3<T> primary_dir(const nd_uniforms& u, int px, int py, int tw, int th) {
