@@ -126,6 +126,27 @@ def gradient_leg(r, nd, torch, reps=3):
             "views": len(views), "pixels": total, "pixels_in_loss": used}
 
 
+def mc_leg(r, nd, spp=64):
+    """Wavefront Monte Carlo path trace of the four-fractal smooth-union scene at 1920x1080 and
+    64 samples per pixel (one frame)."""
+    u = nd.config_uniforms(3, 0)
+    W, H = int(u.resolution[0]), int(u.resolution[1])
+    import torch
+    col = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+    r.path_trace(u, 1, color=col)  # warmup
+    r.sync()
+    t0 = time.perf_counter()
+    _, _, _, st = r.path_trace(u, spp, color=col)
+    r.sync()
+    wall = time.perf_counter() - t0
+    log("c3 mc: %.1f ms for %d spp, %d segments" % (wall * 1e3, spp, st.segments))
+    return {"spp": spp, "width": W, "height": H, "ms_per_frame": round(wall * 1e3, 2),
+            "device_ms": round(st.ms_total, 2), "samples_per_s": round(st.paths / wall, 1),
+            "msamples_per_s": round(st.paths / wall / 1e6, 3), "segments_per_s": round(st.segments / wall, 1),
+            "segments_per_path": round(st.segments / st.paths, 3), "shadow_rays": int(st.shadow_rays),
+            "de_evals_per_s": round(st.de_evals / wall, 1), "de_evals_per_path": round(st.de_evals / st.paths, 1)}
+
+
 def cpu_baseline(nd, target_s=20.0):
     """FP32 CPU restatement of the same per-ray math (the oracle instantiated for float),
     on host threads, over a band of rows of camera 0 at full 1920x1080."""
@@ -216,6 +237,7 @@ def main():
         g = gradient_leg(r, nd, torch)
         res["grad_rays_per_s"] = g["grad_rays_per_s"]
         res["c5_gradient"] = g
+        res["c3_path_trace"] = mc_leg(r, nd)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(nd)

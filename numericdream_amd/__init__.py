@@ -278,3 +278,20 @@ class Renderer:
         st = BatchStats()
         self._check(lib.nd_adjoint(self._h, arr, n, C.byref(io), C.byref(st)), "nd_adjoint")
         return loss.value, np.array(grad[:]), int(used.value), st
+
+    def path_trace(self, u, spp, rect=None, color=None, depth=None, normal=None):
+        """Monte Carlo path trace of rect (default: the whole frame) with spp samples per pixel
+        (sample i uses mc_sample = u.mc_sample + i). color (h, w, 3), depth (h, w) and normal
+        (h, w, 3) are float32 numpy arrays or torch tensors on the device; color is allocated
+        on the host when omitted. Returns (color, depth, normal, stats)."""
+        W, H = int(u.resolution[0]), int(u.resolution[1])
+        x, y, w, h = rect if rect is not None else (0, 0, W, H)
+        if color is None:
+            color = np.zeros((h, w, 3), np.float32)
+        dev = hasattr(color, "data_ptr")
+        out = ImageOut(_ptr(color), _ptr(depth), _ptr(normal), 1 if dev else 0, 0)
+        st = TraceStats()
+        ub = Uniforms.from_buffer_copy(bytes(u))
+        self._check(lib.nd_path_trace(self._h, C.byref(ub), C.sizeof(Uniforms), spp, Rect(x, y, w, h), C.byref(out),
+                                      C.byref(st)), "nd_path_trace")
+        return color, depth, normal, st

@@ -15,75 +15,10 @@
 
 #include "nd_de.hpp"
 #include "nd_internal.h"
+#include "nd_march_state.hpp"
 #include "nd_rays.hpp"
 
 namespace nd {
-
-// ----------------------------------------------------------------------------- march state
-enum { MS_CONT = 0, MS_HIT = 1, MS_MISS = 2 };
-struct March {
-    float t, lo, hi;
-    int i, refines, n, evals;
-    int probing, bisecting, resolved;
-};
-NDD void march_init(March& m) {
-    m.t = 0.f; m.lo = 0.f; m.hi = 0.f;
-    m.i = 0; m.refines = 0; m.n = 0; m.evals = 0;
-    m.probing = 1; m.bisecting = 0; m.resolved = 0;
-}
-NDD float march_sample_t(const March& m) { return m.probing ? m.t : 0.5f * (m.lo + m.hi); }
-// March precision of one ray, held in registers by the persistent passes.
-struct MarchParams {
-    float eps_c, eps_f, max_dist, footprint;
-    int max_steps, bisect_budget, refine_budget, hq;
-};
-NDD MarchParams march_params(const DevJob& J) {
-    MarchParams p;
-    p.eps_c = J.eps_c; p.eps_f = J.eps_f; p.max_dist = J.max_dist; p.footprint = J.footprint;
-    p.max_steps = J.max_steps; p.bisect_budget = J.bisect_budget; p.refine_budget = J.refine_budget; p.hq = J.hq;
-    return p;
-}
-// One step of march_to's loop after the DE d at sample_t s. *dist receives the result.
-// J is a DevJob (plain kernel, wave-uniform) or MarchParams (persistent, per lane).
-template <class MP>
-NDD int march_update(March& m, const MP& J, float s, float d, float* dist) {
-    const int max_samples = J.max_steps * (1 + J.refine_budget) + J.bisect_budget;
-    m.n++;
-    m.evals++;
-    const float eps = J.eps_c * s + J.eps_f;
-    if (m.bisecting) {
-        if (d < eps) m.hi = s; else m.lo = s;
-        m.refines++;
-        if (m.hi - m.lo <= eps || m.refines >= J.bisect_budget) { *dist = m.hi; return MS_HIT; }
-        if (m.n >= max_samples) { *dist = m.t; return MS_MISS; }
-        return MS_CONT;
-    }
-    if (d < eps) {
-        if (J.refine_budget == 0 && J.bisect_budget > 0 && s - m.lo > eps) {
-            m.bisecting = 1; m.probing = 0; m.hi = s; m.refines = 0;
-            if (m.n >= max_samples) { *dist = m.t; return MS_MISS; }
-            return MS_CONT;
-        }
-        *dist = s;
-        return MS_HIT;
-    }
-    if (m.probing) {
-        m.lo = m.t; m.hi = m.t + d; m.refines = 0; m.resolved = 0; m.probing = 0;
-    } else {
-        if (d >= m.hi - s) m.resolved = 1; else m.hi = s;
-        m.refines++;
-    }
-    if (m.resolved || m.refines >= J.refine_budget) {
-        if (J.hq && !m.resolved && J.refine_budget > 0)
-            m.hi = fminf(m.hi, m.lo + fmaxf(m.t, 1.f) * J.footprint);
-        m.t = m.hi;
-        m.i++;
-        if (m.t > J.max_dist || m.i >= J.max_steps) { *dist = m.t; return MS_MISS; }
-        m.probing = 1;
-    }
-    if (m.n >= max_samples) { *dist = m.t; return MS_MISS; }
-    return MS_CONT;
-}
 
 // ----------------------------------------------------------------------------- surface
 NDD V3<float> vnorm(const V3<float>& g) {

@@ -1,0 +1,984 @@
+// nd_mc.hip - wavefront Monte Carlo path trace for gfx950 (opaque scenes).
+//
+// Transport of the reference's trace_path (template.wgsl:3819-4137): 8 hero-stratified
+// wavelengths, the procedural sky, point / global lights with soft shadows, Blinn-Phong
+// shading, Schlick reflection off roughness-perturbed normals, one cosine-sampled indirect
+// bounce at the primary hit, Russian roulette from depth 4.
+//
+// Wavefront schedule (the reference's cs_wf_* sequence, template.wgsl:4898-5407, as a
+// device-side loop): per bounce
+//   extend    persistent march of every live path (rays refilled per lane)
+//   shade     normal, material, scatter; the light-0..n shadow rays and the primary's
+//             indirect ray are queued with their spectral weights, the continuation is
+//             compacted into the next live queue (one atomic per wave and queue)
+//   shadow    persistent visibility march of the shadow queue
+//   indirect  (bounce 0) persistent march of the indirect queue, then its shading
+// then resolve: per pixel, the average over samples of gamut_fit(spectrum -> RGB).
+// Path state is SoA in HBM. Random numbers hash the FP32 bits of the same seeds as the
+// reference (hash12w), computed without fused multiply-adds.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "nd_ctx.h"
+#include "nd_de.hpp"
+#include "nd_internal.h"
+#include "nd_march_state.hpp"
+#include "nd_rays.hpp"
+
+namespace nd {
+
+// ----------------------------------------------------------------------------- scene
+struct McStop {
+    float color[3], position, glossiness, transparency, reflectiveness, ior, roughness;
+};
+struct McInst {
+    int color_count, trap_repeat;
+    float trap_span, trap_offset;
+    McStop stops[16];
+};
+struct McLight {
+    int type;  // 0 point, 1 global
+    int cast;
+    float color[3];  // colour * brightness
+    float v[3];      // position or normalised direction
+    float k;         // penumbra
+};
+struct McScene {
+    McInst inst[ND_DEV_MAX_INST];
+    McLight lights[8];
+    int light_count;
+    int shadows_on, specular_on, ao_on, reflections_on, indirect_on, sky_on;
+    int max_depth;
+    int shadow_steps;
+    float sky_zenith[3], sky_horizon[3], sky_ground[3], sky_sun_color[3], sky_sun_dir[3];
+    float sky_yaw_c, sky_yaw_s, sky_inv_falloff, sky_sun_cos, sky_sun_intensity, sky_intensity;
+    int sky_sun;
+    float max_dist;
+};
+
+// ----------------------------------------------------------------------------- random numbers
+NDD uint32_t hash_u32(float px, float py) {
+    uint32_t x = __float_as_uint(px), y = __float_as_uint(py);
+    const uint32_t M = 1664525u, A = 1013904223u;
+    x = x * M + A;
+    y = y * M + A;
+    x += y * M;
+    y += x * M;
+    x ^= x >> 16;
+    y ^= y >> 16;
+    x += y * M;
+    y += x * M;
+    x ^= x >> 16;
+    y ^= y >> 16;
+    return x ^ y;
+}
+NDD float hash12w(float px, float py) { return (float)hash_u32(px, py) * 2.3283064365386963e-10f; }
+// seed arithmetic without contraction, as the shader's separate multiply and add
+NDD float mad_rn(float a, float b, float c) { return __fadd_rn(__fmul_rn(a, b), c); }
+
+// ----------------------------------------------------------------------------- spectra
+struct Sp8 { float v[8]; };
+NDD Sp8 sp_splat(float a) { Sp8 s; for (int k = 0; k < 8; ++k) s.v[k] = a; return s; }
+NDD float sp_max(const Sp8& a) { float m = a.v[0]; for (int k = 1; k < 8; ++k) m = fmaxf(m, a.v[k]); return m; }
+
+// RGB -> spectrum basis of a path's 8 wavelengths (spectral_setup, template.wgsl:816-850)
+struct Basis { float r[8], g[8], b[8]; };
+NDD void lambdas_of(float hero, float* l) {
+    const float base = hero - floorf(hero);
+    for (int k = 0; k < 8; ++k) {
+        float f = base + 0.125f * (float)k;
+        f = f - floorf(f);
+        l[k] = 380.0f + 350.0f * f;
+    }
+}
+NDD Basis basis_of(float hero) {
+    float l[8];
+    lambdas_of(hero, l);
+    Basis B;
+    for (int k = 0; k < 8; ++k) {
+        float blue = 1.0f / (1.0f + expf((l[k] - 498.09f) / 13.50f));
+        float red = 1.0f / (1.0f + expf((600.31f - l[k]) / 8.00f));
+        float t = (l[k] - 530.0f) / 25.0f;
+        float green = 2.49f * expf(-0.5f * t * t);
+        float is = 1.0f / (red + green + blue);
+        float a = red * is, b = green * is, c = blue * is;
+        B.r[k] = a * 0.99575300f + b * -0.00755455f + c * 0.02732810f;
+        B.g[k] = a * -0.00656071f + b * 1.00830220f + c * -0.00674214f;
+        B.b[k] = a * 0.01080772f + b * -0.00074766f + c * 0.97941404f;
+    }
+    return B;
+}
+NDD Sp8 from_rgb(const Basis& B, float r, float g, float b) {
+    Sp8 s;
+    for (int k = 0; k < 8; ++k) s.v[k] = fmaxf(r * B.r[k] + g * B.g[k] + b * B.b[k], 0.f);
+    return s;
+}
+NDD float cie_lobe(float l, float mu, float slo, float shi) {
+    float s = l < mu ? slo : shi;
+    float u = (l - mu) / s;
+    return expf(-0.5f * u * u);
+}
+// spectrum -> RGB with the CIE-fit weights, then gamut_fit (template.wgsl:880-908)
+NDD void to_rgb_fit(float hero, const Sp8& s, float* rgb) {
+    float l[8];
+    lambdas_of(hero, l);
+    float wr[8], wg[8], wb[8], sr = 0.f, sg = 0.f, sb = 0.f;
+    for (int k = 0; k < 8; ++k) {
+        float x = 1.056f * cie_lobe(l[k], 599.8f, 37.9f, 31.0f) + 0.362f * cie_lobe(l[k], 442.0f, 16.0f, 26.7f) -
+                  0.065f * cie_lobe(l[k], 501.1f, 20.4f, 26.2f);
+        float y = 0.821f * cie_lobe(l[k], 568.8f, 46.9f, 40.5f) + 0.286f * cie_lobe(l[k], 530.9f, 16.3f, 31.1f);
+        float z = 1.217f * cie_lobe(l[k], 437.0f, 11.8f, 36.0f) + 0.681f * cie_lobe(l[k], 459.0f, 26.0f, 13.8f);
+        wr[k] = 3.2404542f * x - 1.5371385f * y - 0.4985314f * z;
+        wg[k] = -0.9692660f * x + 1.8760108f * y + 0.0415560f * z;
+        wb[k] = 0.0556434f * x - 0.2040259f * y + 1.0572252f * z;
+        sr += wr[k]; sg += wg[k]; sb += wb[k];
+    }
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    for (int k = 0; k < 8; ++k) { c0 += s.v[k] * wr[k]; c1 += s.v[k] * wg[k]; c2 += s.v[k] * wb[k]; }
+    c0 /= sr; c1 /= sg; c2 /= sb;
+    const float y = c0 * 0.2126f + c1 * 0.7152f + c2 * 0.0722f;
+    if (y <= 0.f) { rgb[0] = rgb[1] = rgb[2] = 0.f; return; }
+    const float m = fminf(c0, fminf(c1, c2));
+    if (m < 0.f) {
+        const float f = y / (y - m);
+        c0 = y + (c0 - y) * f; c1 = y + (c1 - y) * f; c2 = y + (c2 - y) * f;
+    }
+    rgb[0] = c0; rgb[1] = c1; rgb[2] = c2;
+}
+
+// ----------------------------------------------------------------------------- sky & materials
+NDD V3<float> sky_color(const McScene& S, const V3<float>& dir) {
+    if (!S.sky_on) return mk3(0.f, 0.f, 0.f);
+    V3<float> d = mk3(S.sky_yaw_c * dir.x - S.sky_yaw_s * dir.z, dir.y, S.sky_yaw_s * dir.x + S.sky_yaw_c * dir.z);
+    V3<float> c;
+    if (d.y >= 0.f) {
+        float t = powf(fclamp(d.y, 0.f, 1.f), S.sky_inv_falloff);
+        c = mk3(S.sky_horizon[0] + (S.sky_zenith[0] - S.sky_horizon[0]) * t,
+                S.sky_horizon[1] + (S.sky_zenith[1] - S.sky_horizon[1]) * t,
+                S.sky_horizon[2] + (S.sky_zenith[2] - S.sky_horizon[2]) * t);
+    } else {
+        float t = fclamp(-d.y, 0.f, 1.f);
+        c = mk3(S.sky_horizon[0] + (S.sky_ground[0] - S.sky_horizon[0]) * t,
+                S.sky_horizon[1] + (S.sky_ground[1] - S.sky_horizon[1]) * t,
+                S.sky_horizon[2] + (S.sky_ground[2] - S.sky_horizon[2]) * t);
+    }
+    if (S.sky_sun) {
+        float dd = d.x * S.sky_sun_dir[0] + d.y * S.sky_sun_dir[1] + d.z * S.sky_sun_dir[2];
+        float e0 = S.sky_sun_cos, e1 = S.sky_sun_cos + (1.f - S.sky_sun_cos) * 0.15f;
+        float x = fclamp((dd - e0) / (e1 - e0), 0.f, 1.f);
+        float w = S.sky_sun_intensity * x * x * (3.f - 2.f * x);
+        c = mk3(c.x + S.sky_sun_color[0] * w, c.y + S.sky_sun_color[1] * w, c.z + S.sky_sun_color[2] * w);
+    }
+    return mk3(fmaxf(c.x * S.sky_intensity, 0.f), fmaxf(c.y * S.sky_intensity, 0.f), fmaxf(c.z * S.sky_intensity, 0.f));
+}
+
+struct Mat { float color[3], glossiness, reflectiveness, ior, roughness; };
+NDD Mat mat_default() { Mat m; m.color[0] = 0.8f; m.color[1] = 0.8f; m.color[2] = 0.85f; m.glossiness = 0.3f; m.reflectiveness = 0.f; m.ior = 1.5f; m.roughness = 0.f; return m; }
+NDD Mat mat_stop(const McStop& s) { Mat m; for (int k = 0; k < 3; ++k) m.color[k] = s.color[k]; m.glossiness = s.glossiness; m.reflectiveness = s.reflectiveness; m.ior = s.ior; m.roughness = s.roughness; return m; }
+// mix_material(a, b, h) = mix(b, a, h) (template.wgsl:1980-2001)
+NDD Mat mat_mix(const Mat& a, const Mat& b, float h) {
+    Mat r;
+    for (int k = 0; k < 3; ++k) r.color[k] = b.color[k] * (1.f - h) + a.color[k] * h;
+    r.glossiness = b.glossiness * (1.f - h) + a.glossiness * h;
+    r.reflectiveness = b.reflectiveness * (1.f - h) + a.reflectiveness * h;
+    r.ior = b.ior * (1.f - h) + a.ior * h;
+    r.roughness = b.roughness * (1.f - h) + a.roughness * h;
+    return r;
+}
+// sample_gradient + orbit_trap_to_strip (template.wgsl:2003-2035)
+NDD Mat strip_mat(const McInst& I, float trap) {
+    float x = trap / fmaxf(I.trap_span, 1e-4f) + I.trap_offset;
+    float t;
+    if (I.trap_repeat == 1) t = x - floorf(x);
+    else if (I.trap_repeat == 2) { float f = x * 0.5f - floorf(x * 0.5f); t = 1.f - fabsf(f * 2.f - 1.f); }
+    else t = fclamp(x, 0.f, 1.f);
+    const int count = min(I.color_count, 16);
+    if (count <= 0) return mat_default();
+    if (count == 1 || t <= I.stops[0].position) return mat_stop(I.stops[0]);
+    if (t >= I.stops[count - 1].position) return mat_stop(I.stops[count - 1]);
+    for (int i = 0; i < count - 1; ++i) {
+        const McStop &a = I.stops[i], &b = I.stops[i + 1];
+        if (t >= a.position && t <= b.position) {
+            float f = (t - a.position) / fmaxf(b.position - a.position, 1e-4f);
+            return mat_mix(mat_stop(a), mat_stop(b), 1.f - f);
+        }
+    }
+    return mat_stop(I.stops[count - 1]);
+}
+// hit_material (template.wgsl:2128-2155)
+NDD Mat hit_material(const DevScene& sc, const McScene& S, const V3<float>& p) {
+    bool first = true;
+    float d = sc.empty_de;
+    Mat m = mat_default();
+    for (int i = 0; i < sc.count; ++i) {
+        const DevInst& in = sc.inst[i];
+        if (in.hidden) continue;
+        float ti = 0.f;
+        float di = formula_de_f<true>(in, instance_local<float>(in, p), &ti) * in.de_factor;
+        Mat mi = strip_mat(S.inst[i], ti);
+        if (first) { d = di; m = mi; first = false; continue; }
+        float nd, h;
+        combine_de(in.combine, d, di, in.blend_k, &nd, &h);
+        m = mat_mix(m, mi, h);
+        d = nd;
+    }
+    return m;
+}
+
+NDD V3<float> vnorm3(const V3<float>& g) {
+    float il = __frsqrt_rn(dot3(g, g));
+    return mk3(g.x * il, g.y * il, g.z * il);
+}
+// tetra_gradient / estimate_normal (template.wgsl:1781-1804), e = eps(t)
+NDD V3<float> normal_tetra(const DevScene& sc, const V3<float>& p, float e) {
+    const float K[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+    V3<float> g = mk3(0.f, 0.f, 0.f);
+    for (int i = 0; i < 4; ++i) {
+        float d = scene_de<false>(sc, mk3(p.x + K[i][0] * e, p.y + K[i][1] * e, p.z + K[i][2] * e), nullptr);
+        g = mk3(g.x + K[i][0] * d, g.y + K[i][1] * d, g.z + K[i][2] * d);
+    }
+    return vnorm3(g);
+}
+// cosine_hemisphere_from (template.wgsl:3341-3350)
+NDD V3<float> cosine_sample(const V3<float>& n, float sx, float sy) {
+    const float u1 = hash12w(sx, sy);
+    const float u2 = hash12w(__fadd_rn(sx, 91.73f), __fadd_rn(sy, 31.13f));
+    const float r = sqrtf(u1);
+    float sp, cp;
+    sincosf(u2 * 6.28318530718f, &sp, &cp);
+    const V3<float> local = mk3(r * cp, r * sp, sqrtf(fmaxf(1.f - u1, 0.f)));
+    const V3<float> up = fabsf(n.z) > 0.99f ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 0.f, 1.f);
+    const V3<float> tg = vnorm3(mk3(up.y * n.z - up.z * n.y, up.z * n.x - up.x * n.z, up.x * n.y - up.y * n.x));
+    const V3<float> bt = mk3(n.y * tg.z - n.z * tg.y, n.z * tg.x - n.x * tg.z, n.x * tg.y - n.y * tg.x);
+    return vnorm3(mk3(tg.x * local.x + bt.x * local.y + n.x * local.z, tg.y * local.x + bt.y * local.y + n.y * local.z,
+                      tg.z * local.x + bt.z * local.y + n.z * local.z));
+}
+
+NDD void light_of(const McLight& L, float max_dist, const V3<float>& p, V3<float>* dir, V3<float>* col, float* dist) {
+    if (L.type == 1) {
+        *dir = mk3(L.v[0], L.v[1], L.v[2]);
+        *col = mk3(L.color[0], L.color[1], L.color[2]);
+        *dist = max_dist * 2.f;
+    } else {
+        V3<float> to = mk3(L.v[0] - p.x, L.v[1] - p.y, L.v[2] - p.z);
+        float d = len3(to);
+        float inv = 1.f / fmaxf(d, 1e-4f);
+        *dir = mk3(to.x * inv, to.y * inv, to.z * inv);
+        float a = 1.f / fmaxf(d * d, 0.01f);
+        *col = mk3(L.color[0] * a, L.color[1] * a, L.color[2] * a);
+        *dist = d;
+    }
+}
+
+// ----------------------------------------------------------------------------- path state
+struct McPaths {
+    // SoA path state
+    float *px, *py, *pz, *dx, *dy, *dz;
+    float* thr;   // 8 per path
+    float* rad;   // 8 per path
+    float *hero, *seedx, *seedy;
+    float* hit_t;
+    uint8_t *depth, *flags;  // flags: 1 reflected, 2 hit
+    // queues (path indices) and counters
+    uint32_t *live_a, *live_b;
+    uint32_t* shadow_path;
+    float *sox, *soy, *soz, *sdx, *sdy, *sdz, *sdist, *sk;
+    float* sw;  // 8 per shadow ray
+    uint32_t* ind_path;
+    float *iox, *ioy, *ioz, *idx, *idy, *idz, *it;
+    uint8_t* ihit;
+    float* iw;  // 8 per indirect ray
+    uint32_t* ctr;  // [0] live in, [1] live out, [2] shadows, [3] indirect, [4..7] queue heads, [8..9] de counters (u64)
+};
+enum { C_LIVE_IN = 0, C_LIVE_OUT = 1, C_SHADOW = 2, C_IND = 3, C_HEAD_EXT = 4, C_HEAD_SH = 5, C_HEAD_IND = 6,
+       C_SEG = 10, C_SHTOT = 11, C_DE = 12 };
+
+struct McArgs {
+    const DevJob* job;
+    const DevScene* scene;
+    const McScene* S;
+    McPaths P;
+    uint32_t npix;
+    uint32_t sample_base;
+    uint32_t spp_chunk;
+    float mc_sample0;
+    uint32_t chunk;
+    float* out_rgb;
+    float* out_depth;
+    float* out_normal;
+    uint32_t spp_total;
+};
+
+NDD unsigned long long* de_ctr(const McArgs& a) { return (unsigned long long*)(a.P.ctr + C_DE); }
+
+NDD uint32_t append(uint32_t* ctr, bool want, uint32_t* slot) {
+    const uint64_t m = __builtin_amdgcn_ballot_w64(want);
+    if (!m) return 0;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    const unsigned lane = __lane_id();
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    *slot = base + (uint32_t)__popcll(m & lt);
+    return (uint32_t)__popcll(m);
+}
+
+// --- generate: camera paths of samples [sample_base, sample_base + spp_chunk)
+__global__ void k_mc_generate(McArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = a.npix * a.spp_chunk;
+    if (p >= n) return;
+    const DevJob& J = *a.job;
+    const uint32_t pix = p % a.npix, s = a.sample_base + p / a.npix;
+    const int x = (int)(pix % (uint32_t)J.rect_w), y = (int)(pix / (uint32_t)J.rect_w);
+    const int gx = J.rect_x + x, gy = J.rect_y + y;
+    const V3<float> d = primary_dir(J, gx, gy);
+    const float ms = __fadd_rn(a.mc_sample0, (float)s);
+    // base seed = frag + mc_sample * (91.73, 57.31) (fs_main, template.wgsl:4818)
+    const float bx = mad_rn(ms, 91.73f, (float)gx + 0.5f);
+    const float by = mad_rn(ms, 57.31f, (float)gy + 0.5f);
+    a.P.px[p] = J.cam[0]; a.P.py[p] = J.cam[1]; a.P.pz[p] = J.cam[2];
+    a.P.dx[p] = d.x; a.P.dy[p] = d.y; a.P.dz[p] = d.z;
+    for (int k = 0; k < 8; ++k) { a.P.thr[8 * p + k] = 1.f; a.P.rad[8 * p + k] = 0.f; }
+    // path_begin (template.wgsl:3819-3839)
+    a.P.seedx[p] = mad_rn(ms, 27.31f, bx);
+    a.P.seedy[p] = mad_rn(ms, 45.17f, by);
+    a.P.hero[p] = mad_rn(ms, 0.61803399f, hash12w(__fadd_rn(bx, 19.31f), __fadd_rn(by, 47.11f)));
+    a.P.depth[p] = 0;
+    a.P.flags[p] = 0;
+    a.P.live_a[p] = p;
+}
+
+// --- persistent march over a queue of rays. RAY provides setup(i, org, dir) and
+// finish(i, hit, t). All rays share the job's march precision.
+template <class RAY>
+NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
+    const DevScene& sc = *a.scene;
+    const MarchParams mp = march_params(*a.job);
+    uint32_t cnext = 0, cend = 0;
+    bool work_left = true, active = false;
+    uint32_t item = 0;
+    V3<float> org = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 1.f);
+    March m;
+    march_init(m);
+    unsigned long long de = 0;
+    for (;;) {
+        if (work_left) {
+            uint64_t need = __builtin_amdgcn_ballot_w64(!active);
+            if (__popcll(need) >= 8 || need == ~0ull) {
+                while (need != 0ull) {
+                    if (cnext >= cend) {
+                        uint32_t base = 0;
+                        if (__lane_id() == 0) base = atomicAdd(head, a.chunk);
+                        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+                        if (base >= n) { work_left = false; break; }
+                        cnext = base;
+                        cend = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(base + a.chunk, n));
+                    }
+                    const uint32_t avail = cend - cnext;
+                    const unsigned lane = __lane_id();
+                    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+                    const uint32_t rank = (uint32_t)__popcll(need & lt);
+                    const bool mine = ((need >> lane) & 1ull) && rank < avail;
+                    if (mine) {
+                        item = cnext + rank;
+                        R.setup(item, &org, &dir);
+                        march_init(m);
+                        active = true;
+                    }
+                    cnext += min((uint32_t)__popcll(need), avail);
+                    need = __builtin_amdgcn_ballot_w64(((need >> lane) & 1ull) && !mine);
+                }
+            }
+        }
+        const uint64_t act = __builtin_amdgcn_ballot_w64(active);
+        if (act == 0ull) {
+            if (!work_left) break;
+            continue;
+        }
+        de += (unsigned long long)__popcll(act);
+        if (active) {
+            const float s = march_sample_t(m);
+            const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
+            float dist = 0.f;
+            const int st = march_update(m, mp, s, d, &dist);
+            if (st != MS_CONT) {
+                R.finish(item, st == MS_HIT, dist);
+                active = false;
+            }
+        }
+    }
+    if (__lane_id() == 0) atomicAdd(de_ctr(a), de);
+}
+
+struct ExtendRays {
+    const McArgs& a;
+    const uint32_t* q;
+    NDD void setup(uint32_t i, V3<float>* o, V3<float>* d) const {
+        const uint32_t p = q[i];
+        *o = mk3(a.P.px[p], a.P.py[p], a.P.pz[p]);
+        *d = mk3(a.P.dx[p], a.P.dy[p], a.P.dz[p]);
+    }
+    NDD void finish(uint32_t i, bool hit, float t) const {
+        const uint32_t p = q[i];
+        a.P.hit_t[p] = t;
+        a.P.flags[p] = (uint8_t)((a.P.flags[p] & 1u) | (hit ? 2u : 0u));
+    }
+};
+struct IndirectRays {
+    const McArgs& a;
+    NDD void setup(uint32_t i, V3<float>* o, V3<float>* d) const {
+        *o = mk3(a.P.iox[i], a.P.ioy[i], a.P.ioz[i]);
+        *d = mk3(a.P.idx[i], a.P.idy[i], a.P.idz[i]);
+    }
+    NDD void finish(uint32_t i, bool hit, float t) const {
+        a.P.it[i] = t;
+        a.P.ihit[i] = hit ? 1 : 0;
+    }
+};
+
+__global__ void __launch_bounds__(64) k_mc_extend(McArgs a, int ping) {
+    const uint32_t* q = ping ? a.P.live_b : a.P.live_a;
+    const uint32_t n = a.P.ctr[C_LIVE_IN];
+    ExtendRays R{a, q};
+    persistent_march(a, R, n, a.P.ctr + C_HEAD_EXT);
+}
+
+__global__ void __launch_bounds__(64) k_mc_indirect_march(McArgs a) {
+    IndirectRays R{a};
+    persistent_march(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
+}
+
+// shade_local (template.wgsl:3222-3331) for opaque, film- and subsurface-free materials.
+// With defer, each shadowed light's contribution is queued as a shadow ray with weight
+// w_scale * light * (albedo n.l + specular); otherwise it is added unshadowed.
+NDD Sp8 shade_lights(const McArgs& a, const Basis& B, const V3<float>& pos, const V3<float>& dir, const V3<float>& n,
+                     const Mat& m, const Sp8& ambient, bool shadows, const Sp8& w_scale, uint32_t path) {
+    const McScene& S = *a.S;
+    const V3<float> view = mk3(-dir.x, -dir.y, -dir.z);
+    const float spec_power = 4.f + 124.f * fclamp(m.glossiness, 0.f, 1.f);
+    const Sp8 albedo = from_rgb(B, m.color[0], m.color[1], m.color[2]);
+    Sp8 acc = sp_splat(0.f);
+    for (int i = 0; i < S.light_count; ++i) {
+        const McLight& L = S.lights[i];
+        V3<float> ld, col;
+        float dist;
+        light_of(L, S.max_dist, pos, &ld, &col, &dist);
+        if (fmaxf(col.x, fmaxf(col.y, col.z)) < 1e-5f) continue;
+        const float ndotl = fmaxf(dot3(n, ld), 0.f);
+        const V3<float> hd = vnorm3(mk3(ld.x + view.x, ld.y + view.y, ld.z + view.z));
+        const float spec = S.specular_on ? powf(fmaxf(dot3(n, hd), 0.f), spec_power) * m.glossiness : 0.f;
+        if (ndotl <= 0.f) continue;
+        const Sp8 ls = from_rgb(B, col.x, col.y, col.z);
+        Sp8 c;
+        for (int k = 0; k < 8; ++k) c.v[k] = ls.v[k] * (albedo.v[k] * ndotl + spec);
+        if (shadows && L.cast && S.shadow_steps > 0) {
+            uint32_t slot;
+            append(a.P.ctr + C_SHADOW, true, &slot);
+            a.P.shadow_path[slot] = path;
+            a.P.sox[slot] = pos.x + n.x * 0.004f;
+            a.P.soy[slot] = pos.y + n.y * 0.004f;
+            a.P.soz[slot] = pos.z + n.z * 0.004f;
+            a.P.sdx[slot] = ld.x; a.P.sdy[slot] = ld.y; a.P.sdz[slot] = ld.z;
+            a.P.sdist[slot] = dist;
+            a.P.sk[slot] = L.k;
+            for (int k = 0; k < 8; ++k) a.P.sw[8 * slot + k] = c.v[k] * w_scale.v[k];
+        } else {
+            for (int k = 0; k < 8; ++k) acc.v[k] += c.v[k];
+        }
+    }
+    Sp8 r;
+    for (int k = 0; k < 8; ++k) r.v[k] = albedo.v[k] * ambient.v[k] + acc.v[k];
+    return r;
+}
+
+// --- shade (path_shade, template.wgsl:3902-4124). One path per lane.
+__global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
+    const uint32_t* qin = ping ? a.P.live_b : a.P.live_a;
+    uint32_t* qout = ping ? a.P.live_a : a.P.live_b;
+    const uint32_t n = a.P.ctr[C_LIVE_IN];
+    const DevJob& J = *a.job;
+    const DevScene& sc = *a.scene;
+    const McScene& S = *a.S;
+    unsigned long long de = 0;
+    for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i - threadIdx.x < n; i += gridDim.x * 64) {
+        bool cont = false;
+        uint32_t p = 0;
+        if (i < n) {
+            p = qin[i];
+            const int depth = a.P.depth[p];
+            const uint8_t fl = a.P.flags[p];
+            const bool reflected = fl & 1u;
+            const V3<float> pos = mk3(a.P.px[p], a.P.py[p], a.P.pz[p]);
+            const V3<float> dir = mk3(a.P.dx[p], a.P.dy[p], a.P.dz[p]);
+            const float hero = a.P.hero[p];
+            const Basis B = basis_of(hero);
+            Sp8 thr, rad;
+            for (int k = 0; k < 8; ++k) { thr.v[k] = a.P.thr[8 * p + k]; rad.v[k] = a.P.rad[8 * p + k]; }
+            // path_seed (template.wgsl:3845-3847)
+            const float sx = __fadd_rn(a.P.seedx[p], mad_rn((float)depth, 17.77f, 3.1f));
+            const float sy = __fadd_rn(a.P.seedy[p], mad_rn((float)depth, 41.13f, 7.7f));
+            const uint32_t pix = p % a.npix;
+            if (!(fl & 2u)) {
+                const V3<float> sky = sky_color(S, dir);
+                const Sp8 sk = from_rgb(B, sky.x, sky.y, sky.z);
+                for (int k = 0; k < 8; ++k) rad.v[k] += thr.v[k] * sk.v[k];
+                if (depth == 0 && a.sample_base + p / a.npix == 0) {
+                    if (a.out_depth) a.out_depth[pix] = 0.f;
+                    if (a.out_normal) { a.out_normal[3 * pix] = 0.f; a.out_normal[3 * pix + 1] = 0.f; a.out_normal[3 * pix + 2] = 0.f; }
+                }
+            } else if (sp_max(thr) >= 0.002f) {
+                const float t = a.P.hit_t[p];
+                const float eps = J.eps_c * t + J.eps_f;
+                const V3<float> hp = mk3(pos.x + dir.x * t, pos.y + dir.y * t, pos.z + dir.z * t);
+                const V3<float> nrm = normal_tetra(sc, hp, eps);
+                const Mat mat = hit_material(sc, S, hp);
+                de += 5;
+                if (depth == 0 && a.sample_base + p / a.npix == 0) {
+                    if (a.out_depth) a.out_depth[pix] = t;
+                    if (a.out_normal) { a.out_normal[3 * pix] = nrm.x; a.out_normal[3 * pix + 1] = nrm.y; a.out_normal[3 * pix + 2] = nrm.z; }
+                }
+                // scatter_at with allow_diffuse = false (template.wgsl:3552-3616), opaque
+                const V3<float> nf = dot3(dir, nrm) > 0.f ? mk3(-nrm.x, -nrm.y, -nrm.z) : nrm;
+                const float rough = fclamp(mat.roughness, 0.f, 1.f);
+                const float alpha = rough * rough;
+                V3<float> nr = nf;
+                if (alpha > 1e-4f) {
+                    const V3<float> hs = cosine_sample(nf, __fadd_rn(sx, 5.19f), __fadd_rn(sy, 12.73f));
+                    nr = vnorm3(mk3(nf.x + (hs.x - nf.x) * alpha, nf.y + (hs.y - nf.y) * alpha, nf.z + (hs.z - nf.z) * alpha));
+                }
+                const float ior = fmaxf(mat.ior, 1.f);
+                const float r0r = (1.f - ior) / (1.f + ior);
+                const float r0 = r0r * r0r;
+                const float cth = fclamp(1.f - fabsf(dot3(dir, nr)), 0.f, 1.f);
+                const float c2 = cth * cth;
+                const float f = r0 + (1.f - r0) * c2 * c2 * cth;
+                const float refl = fclamp(mat.reflectiveness, 0.f, 1.f);
+                const float p_reflect = fclamp(refl + (1.f - refl) * f, 0.f, 1.f);
+                const float local_share = 1.f - p_reflect;
+                if (local_share > 0.001f) {
+                    Sp8 ambient = sp_splat(0.15f);
+                    Sp8 wscale;
+                    for (int k = 0; k < 8; ++k) wscale.v[k] = thr.v[k] * local_share;
+                    if (depth == 0 && S.indirect_on) {
+                        // deferred indirect_light: weight throughput * local_share * albedo
+                        const Sp8 albedo = from_rgb(B, mat.color[0], mat.color[1], mat.color[2]);
+                        const V3<float> idir = cosine_sample(nrm, __fadd_rn(sx, 13.13f), __fadd_rn(sy, 71.71f));
+                        uint32_t slot;
+                        append(a.P.ctr + C_IND, true, &slot);
+                        a.P.ind_path[slot] = p;
+                        a.P.iox[slot] = hp.x + nrm.x * 0.004f;
+                        a.P.ioy[slot] = hp.y + nrm.y * 0.004f;
+                        a.P.ioz[slot] = hp.z + nrm.z * 0.004f;
+                        a.P.idx[slot] = idir.x; a.P.idy[slot] = idir.y; a.P.idz[slot] = idir.z;
+                        for (int k = 0; k < 8; ++k) a.P.iw[8 * slot + k] = wscale.v[k] * albedo.v[k];
+                        ambient = sp_splat(0.f);
+                    } else if (!reflected && S.ao_on) {
+                        // calc_ao (template.wgsl:1825-1841)
+                        const float radius = fclamp(t * 0.04f, 0.003f, 0.25f);
+                        float occ = 0.f, w = 1.f;
+                        for (int k = 0; k < 5; ++k) {
+                            const float step = radius * (0.2f + 0.2f * (float)k);
+                            const float dd = scene_de<false>(sc, mk3(hp.x + nrm.x * step, hp.y + nrm.y * step, hp.z + nrm.z * step), nullptr);
+                            occ += fmaxf(step - dd, 0.f) * w;
+                            w *= 0.6f;
+                        }
+                        de += 5;
+                        ambient = sp_splat(0.15f * fclamp(1.f - 2.f * occ / radius, 0.f, 1.f));
+                    }
+                    const Sp8 lit = shade_lights(a, B, hp, dir, nrm, mat, ambient, !reflected && S.shadows_on, wscale, p);
+                    for (int k = 0; k < 8; ++k) rad.v[k] += wscale.v[k] * lit.v[k];
+                }
+                if (p_reflect > 0.f && S.reflections_on) {
+                    // reflect: weight mix(color, 1, f) * p_reflect
+                    const Sp8 w = from_rgb(B, mat.color[0] + (1.f - mat.color[0]) * f, mat.color[1] + (1.f - mat.color[1]) * f,
+                                           mat.color[2] + (1.f - mat.color[2]) * f);
+                    for (int k = 0; k < 8; ++k) thr.v[k] *= w.v[k] * p_reflect;
+                    const float dn = 2.f * dot3(nr, dir);
+                    const V3<float> nd = mk3(dir.x - nr.x * dn, dir.y - nr.y * dn, dir.z - nr.z * dn);
+                    const float bias = fmaxf(0.003f, 2.f * eps);
+                    cont = true;
+                    if (depth >= 4) {
+                        const float survive = fclamp(sp_max(thr), 0.05f, 1.f);
+                        if (hash12w(__fadd_rn(sx, 59.3f), __fadd_rn(sy, 23.9f)) > survive) cont = false;
+                        else for (int k = 0; k < 8; ++k) thr.v[k] *= 1.f / survive;
+                    }
+                    if (cont && depth + 1 < S.max_depth) {
+                        a.P.px[p] = hp.x + nd.x * bias; a.P.py[p] = hp.y + nd.y * bias; a.P.pz[p] = hp.z + nd.z * bias;
+                        a.P.dx[p] = nd.x; a.P.dy[p] = nd.y; a.P.dz[p] = nd.z;
+                        a.P.depth[p] = (uint8_t)(depth + 1);
+                        a.P.flags[p] = 1u;
+                    } else {
+                        cont = false;
+                    }
+                    for (int k = 0; k < 8; ++k) a.P.thr[8 * p + k] = thr.v[k];
+                }
+            }
+            for (int k = 0; k < 8; ++k) a.P.rad[8 * p + k] = rad.v[k];
+        }
+        uint32_t slot = 0;
+        append(a.P.ctr + C_LIVE_OUT, cont, &slot);
+        if (cont) qout[slot] = p;
+    }
+    for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off);
+    if (threadIdx.x == 0) atomicAdd(de_ctr(a), de);
+}
+
+// --- indirect shading: shade_local at the indirect hit, no shadows, ambient 0.15
+// (indirect_light_traced, template.wgsl:3359-3377)
+__global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
+    const uint32_t n = a.P.ctr[C_IND];
+    const DevJob& J = *a.job;
+    const DevScene& sc = *a.scene;
+    const McScene& S = *a.S;
+    unsigned long long de = 0;
+    for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+        const uint32_t p = a.P.ind_path[i];
+        const Basis B = basis_of(a.P.hero[p]);
+        const V3<float> dir = mk3(a.P.idx[i], a.P.idy[i], a.P.idz[i]);
+        Sp8 L;
+        if (!a.P.ihit[i]) {
+            const V3<float> sky = sky_color(S, dir);
+            L = from_rgb(B, sky.x, sky.y, sky.z);
+        } else {
+            const float t = a.P.it[i];
+            const V3<float> bp = mk3(a.P.iox[i] + dir.x * t, a.P.ioy[i] + dir.y * t, a.P.ioz[i] + dir.z * t);
+            const V3<float> bn = normal_tetra(sc, bp, J.eps_c * t + J.eps_f);
+            const Mat bm = hit_material(sc, S, bp);
+            de += 5;
+            L = shade_lights(a, B, bp, dir, bn, bm, sp_splat(0.15f), false, sp_splat(0.f), p);
+        }
+        for (int k = 0; k < 8; ++k) atomicAdd(&a.P.rad[8 * p + k], a.P.iw[8 * i + k] * L.v[k]);
+    }
+    for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off);
+    if (threadIdx.x == 0) atomicAdd(de_ctr(a), de);
+}
+
+// --- shadow: visibility_march on an opaque scene (template.wgsl:2282-2342), refilled
+__global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
+    const DevScene& sc = *a.scene;
+    const uint32_t n = a.P.ctr[C_SHADOW];
+    const int max_steps = a.S->shadow_steps;
+    uint32_t cnext = 0, cend = 0;
+    bool work_left = true, active = false;
+    uint32_t item = 0;
+    V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
+    float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
+    int step = 0;
+    unsigned long long de = 0;
+    for (;;) {
+        if (work_left) {
+            uint64_t need = __builtin_amdgcn_ballot_w64(!active);
+            if (__popcll(need) >= 8 || need == ~0ull) {
+                while (need != 0ull) {
+                    if (cnext >= cend) {
+                        uint32_t base = 0;
+                        if (__lane_id() == 0) base = atomicAdd(a.P.ctr + C_HEAD_SH, a.chunk);
+                        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+                        if (base >= n) { work_left = false; break; }
+                        cnext = base;
+                        cend = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(base + a.chunk, n));
+                    }
+                    const uint32_t avail = cend - cnext;
+                    const unsigned lane = __lane_id();
+                    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+                    const uint32_t rank = (uint32_t)__popcll(need & lt);
+                    const bool mine = ((need >> lane) & 1ull) && rank < avail;
+                    if (mine) {
+                        item = cnext + rank;
+                        so = mk3(a.P.sox[item], a.P.soy[item], a.P.soz[item]);
+                        ld = mk3(a.P.sdx[item], a.P.sdy[item], a.P.sdz[item]);
+                        ldist = a.P.sdist[item];
+                        k = a.P.sk[item];
+                        t = 0.004f; res = 1.f; step = 0;
+                        if (t < ldist) active = true;
+                        else {
+                            const uint32_t p = a.P.shadow_path[item];
+                            for (int q = 0; q < 8; ++q) atomicAdd(&a.P.rad[8 * p + q], a.P.sw[8 * item + q]);
+                        }
+                    }
+                    cnext += min((uint32_t)__popcll(need), avail);
+                    need = __builtin_amdgcn_ballot_w64(((need >> lane) & 1ull) && !mine);
+                }
+            }
+        }
+        const uint64_t act = __builtin_amdgcn_ballot_w64(active);
+        if (act == 0ull) {
+            if (!work_left) break;
+            continue;
+        }
+        de += (unsigned long long)__popcll(act);
+        if (active) {
+            const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
+            step++;
+            bool done = false;
+            if (d < 0.0005f) { res = 0.f; done = true; }
+            else {
+                res = fminf(res, k * d / t);
+                t += fmaxf(d, 0.001f);
+                if (step >= max_steps || t >= ldist) done = true;
+            }
+            if (done) {
+                const float vis = fclamp(res, 0.f, 1.f);
+                if (vis > 0.f) {
+                    const uint32_t p = a.P.shadow_path[item];
+                    for (int q = 0; q < 8; ++q) atomicAdd(&a.P.rad[8 * p + q], a.P.sw[8 * item + q] * vis);
+                }
+                active = false;
+            }
+        }
+    }
+    if (__lane_id() == 0) atomicAdd(de_ctr(a), de);
+}
+
+// --- resolve: per pixel, the average over this chunk's samples is added into out_rgb
+// (scaled by 1 / spp_total)
+__global__ void k_mc_resolve(McArgs a) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= a.npix) return;
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (uint32_t s = 0; s < a.spp_chunk; ++s) {
+        const uint32_t p = s * a.npix + pix;
+        Sp8 r;
+        for (int k = 0; k < 8; ++k) r.v[k] = a.P.rad[8 * p + k];
+        float rgb[3];
+        to_rgb_fit(a.P.hero[p], r, rgb);
+        acc[0] += rgb[0]; acc[1] += rgb[1]; acc[2] += rgb[2];
+    }
+    const float inv = 1.f / (float)a.spp_total;
+    for (int k = 0; k < 3; ++k) a.out_rgb[3 * pix + k] += acc[k] * inv;
+}
+
+// counters between passes (no host round trip)
+__global__ void k_mc_counters(uint32_t* ctr, int op) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (op == 0) {  // new chunk: all queues empty, the initial live count is set by the host
+        for (int i = 0; i < 16; ++i) ctr[i] = 0;
+    } else if (op == 1) {  // before a bounce: clear the out queue and the side queues
+        ctr[C_LIVE_OUT] = 0; ctr[C_SHADOW] = 0; ctr[C_IND] = 0;
+        ctr[C_HEAD_EXT] = 0; ctr[C_HEAD_SH] = 0; ctr[C_HEAD_IND] = 0;
+    } else if (op == 2) {  // after a bounce: the out queue becomes the in queue
+        ctr[C_SEG] += ctr[C_LIVE_IN];
+        ctr[C_SHTOT] += ctr[C_SHADOW];
+        ctr[C_LIVE_IN] = ctr[C_LIVE_OUT];
+    }
+}
+__global__ void k_mc_set_live(uint32_t* ctr, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[C_LIVE_IN] = n;
+}
+__global__ void k_mc_zero(float* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+}  // namespace nd
+
+// ----------------------------------------------------------------------------- host
+static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
+    std::memset(&S, 0, sizeof(S));
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    for (int i = 0; i < ND_MAX_INSTANCES; ++i) {
+        const nd_fractal_instance& fi = u.instances[i];
+        nd::McInst& I = S.inst[i];
+        I.color_count = (int)fi.color_count;
+        I.trap_repeat = (int)(fi.trap_repeat + 0.5f);
+        I.trap_span = fi.trap_span;
+        I.trap_offset = fi.trap_offset;
+        for (int s = 0; s < 16; ++s) {
+            const nd_color_stop& c = fi.colors[s];
+            nd::McStop& d = I.stops[s];
+            for (int k = 0; k < 3; ++k) d.color[k] = c.color[k];
+            d.position = c.position; d.glossiness = c.glossiness; d.transparency = c.transparency;
+            d.reflectiveness = c.reflectiveness; d.ior = c.ior; d.roughness = c.roughness;
+        }
+    }
+    S.light_count = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min((int)u.light_count, ND_MAX_LIGHTS);
+    for (int i = 0; i < S.light_count; ++i) {
+        const nd_light& L = u.lights[i];
+        nd::McLight& d = S.lights[i];
+        d.type = (int)(L.light_type + 0.5f) == 1 ? 1 : 0;
+        d.cast = L.cast_shadows > 0.5f;
+        for (int k = 0; k < 3; ++k) d.color[k] = L.color[k] * L.brightness;
+        if (d.type == 1) {
+            double n = std::sqrt((double)L.position_or_direction[0] * L.position_or_direction[0] +
+                                 (double)L.position_or_direction[1] * L.position_or_direction[1] +
+                                 (double)L.position_or_direction[2] * L.position_or_direction[2]);
+            for (int k = 0; k < 3; ++k) d.v[k] = (float)(L.position_or_direction[k] / n);
+        } else {
+            for (int k = 0; k < 3; ++k) d.v[k] = L.position_or_direction[k];
+        }
+        d.k = L.shadow_softness + (512.0f - L.shadow_softness) * L.hard_shadows;
+    }
+    S.shadows_on = !(parts & ND_PART_SHADOWS);
+    S.specular_on = !(parts & ND_PART_SPECULAR);
+    S.ao_on = !(parts & ND_PART_AMBIENT_OCCLUSION);
+    S.reflections_on = !(parts & ND_PART_REFLECTIONS);
+    S.indirect_on = u.mc_enabled > 0.5f && !(parts & ND_PART_MC_INDIRECT);
+    S.sky_on = !(parts & ND_PART_SKY);
+    S.max_depth = std::min(std::max((int)u.max_reflection_bounces + 8, 1), 16);
+    S.shadow_steps = std::min((int)u.max_steps, 40);
+    for (int k = 0; k < 3; ++k) {
+        S.sky_zenith[k] = u.sky.zenith[k]; S.sky_horizon[k] = u.sky.horizon[k]; S.sky_ground[k] = u.sky.ground[k];
+        S.sky_sun_color[k] = u.sky.sun_color[k];
+    }
+    double sn = std::sqrt((double)u.sky.sun_dir[0] * u.sky.sun_dir[0] + (double)u.sky.sun_dir[1] * u.sky.sun_dir[1] +
+                          (double)u.sky.sun_dir[2] * u.sky.sun_dir[2]);
+    for (int k = 0; k < 3; ++k) S.sky_sun_dir[k] = sn > 0 ? (float)(u.sky.sun_dir[k] / sn) : 0.f;
+    S.sky_yaw_c = std::cos(u.sky.yaw);
+    S.sky_yaw_s = std::sin(u.sky.yaw);
+    S.sky_inv_falloff = 1.0f / std::max(u.sky.falloff, 0.01f);
+    S.sky_sun = u.sky.sun_cos > -1.0f;
+    S.sky_sun_cos = u.sky.sun_cos;
+    S.sky_sun_intensity = u.sky.sun_intensity;
+    S.sky_intensity = u.sky.intensity;
+    S.max_dist = u.max_dist;
+}
+
+extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size, uint32_t spp, nd_rect rect,
+                                   const nd_image_out* out, nd_trace_stats* stats) {
+    if (!c) return ND_E_INVALID;
+    if (!uniforms || size != sizeof(nd_uniforms)) return nd_fail(c, ND_E_INVALID, "uniforms_size %zu != %zu", size, sizeof(nd_uniforms));
+    if (!out || !out->color) return nd_fail(c, ND_E_INVALID, "path trace needs a colour output");
+    if (spp == 0 || spp > 65536) return nd_fail(c, ND_E_INVALID, "spp %u", spp);
+    const nd_uniforms& u = *(const nd_uniforms*)uniforms;
+    if (u.has_transparency > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "transmissive materials are not implemented in the path trace");
+    if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.dof_enabled > 0.5f || u.adapt_enabled > 0.5f ||
+        u.approx_flags != 0.0f || u.primary_cache_key > 0.5f)
+        return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field, adaptive sampling, approximations and the hit cache are not implemented");
+    for (int i = 0; i < std::min((int)u.light_count, 8); ++i)
+        if ((int)(u.lights[i].light_type + 0.5f) == 2) return nd_fail(c, ND_E_UNSUPPORTED, "ray lights are not implemented");
+    for (int i = 0; i < ND_MAX_INSTANCES; ++i)
+        for (int s = 0; s < (int)u.instances[i].color_count && s < 16; ++s) {
+            const nd_color_stop& cs = u.instances[i].colors[s];
+            if (cs.transparency > 0.f || cs.subsurface > 0.001f || cs.film_strength > 0.001f || cs.abbe > 0.5f)
+                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d stop %d: transmission, subsurface, film and dispersion are not implemented", i, s);
+        }
+    if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
+    nd_march_opts o{ND_NORMAL_TETRA, 0, 0, ND_KERNEL_PERSISTENT};
+    DevJob job;
+    std::memset(&job, 0, sizeof(job));
+    nd_status st = nd_pack_job(c, u, o, rect, job);
+    if (st != ND_OK) return st;
+    nd::McScene S;
+    pack_mc_scene(u, S);
+    const uint32_t npix = (uint32_t)job.rect_w * (uint32_t)job.rect_h;
+    // sample chunk: bound the path pool to about 2^24 paths
+    uint32_t chunk = std::max(1u, std::min(spp, (uint32_t)((1u << 24) / std::max(npix, 1u))));
+    const size_t P = (size_t)npix * chunk;
+    // device layout
+    size_t bytes = 0;
+    auto sz = [&](size_t b) { bytes += (b + 255) & ~(size_t)255; };
+    sz(sizeof(DevJob)); sz(sizeof(DevScene)); sz(sizeof(nd::McScene));
+    for (int i = 0; i < 6; ++i) sz(4 * P);  // pos, dir
+    sz(32 * P); sz(32 * P);                 // thr, rad
+    for (int i = 0; i < 4; ++i) sz(4 * P);  // hero, seeds, hit_t
+    sz(P); sz(P);                           // depth, flags
+    sz(4 * P); sz(4 * P);                   // live queues
+    const size_t SQ = P * (size_t)std::max(S.light_count, 1);
+    sz(4 * SQ); for (int i = 0; i < 8; ++i) sz(4 * SQ); sz(32 * SQ);
+    sz(4 * P); for (int i = 0; i < 7; ++i) sz(4 * P); sz(P); sz(32 * P);
+    sz(256);
+    const bool dev_out = out->device_ptrs != 0;
+    if (!dev_out) { sz(12 * (size_t)npix); sz(4 * (size_t)npix); sz(12 * (size_t)npix); }
+    if ((st = c->mc.ensure(c, bytes)) != ND_OK) return st;
+    char* base = (char*)c->mc.p;
+    auto carve = [&](size_t b) { char* r = base; base += (b + 255) & ~(size_t)255; return r; };
+    DevJob* d_job = (DevJob*)carve(sizeof(DevJob));
+    DevScene* d_scene = (DevScene*)carve(sizeof(DevScene));
+    nd::McScene* d_S = (nd::McScene*)carve(sizeof(nd::McScene));
+    nd::McArgs a;
+    std::memset(&a, 0, sizeof(a));
+    nd::McPaths& Pp = a.P;
+    Pp.px = (float*)carve(4 * P); Pp.py = (float*)carve(4 * P); Pp.pz = (float*)carve(4 * P);
+    Pp.dx = (float*)carve(4 * P); Pp.dy = (float*)carve(4 * P); Pp.dz = (float*)carve(4 * P);
+    Pp.thr = (float*)carve(32 * P); Pp.rad = (float*)carve(32 * P);
+    Pp.hero = (float*)carve(4 * P); Pp.seedx = (float*)carve(4 * P); Pp.seedy = (float*)carve(4 * P);
+    Pp.hit_t = (float*)carve(4 * P);
+    Pp.depth = (uint8_t*)carve(P); Pp.flags = (uint8_t*)carve(P);
+    Pp.live_a = (uint32_t*)carve(4 * P); Pp.live_b = (uint32_t*)carve(4 * P);
+    Pp.shadow_path = (uint32_t*)carve(4 * SQ);
+    Pp.sox = (float*)carve(4 * SQ); Pp.soy = (float*)carve(4 * SQ); Pp.soz = (float*)carve(4 * SQ);
+    Pp.sdx = (float*)carve(4 * SQ); Pp.sdy = (float*)carve(4 * SQ); Pp.sdz = (float*)carve(4 * SQ);
+    Pp.sdist = (float*)carve(4 * SQ); Pp.sk = (float*)carve(4 * SQ);
+    Pp.sw = (float*)carve(32 * SQ);
+    Pp.ind_path = (uint32_t*)carve(4 * P);
+    Pp.iox = (float*)carve(4 * P); Pp.ioy = (float*)carve(4 * P); Pp.ioz = (float*)carve(4 * P);
+    Pp.idx = (float*)carve(4 * P); Pp.idy = (float*)carve(4 * P); Pp.idz = (float*)carve(4 * P);
+    Pp.it = (float*)carve(4 * P);
+    Pp.ihit = (uint8_t*)carve(P);
+    Pp.iw = (float*)carve(32 * P);
+    Pp.ctr = (uint32_t*)carve(256);
+    float* d_rgb = dev_out ? out->color : (float*)carve(12 * (size_t)npix);
+    float* d_depth = dev_out ? out->depth : (out->depth ? (float*)carve(4 * (size_t)npix) : nullptr);
+    float* d_normal = dev_out ? out->normal : (out->normal ? (float*)carve(12 * (size_t)npix) : nullptr);
+    hipStream_t s = c->stream;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload job");
+    if ((e = hipMemcpyAsync(d_scene, &job.scene, sizeof(DevScene), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload scene");
+    if ((e = hipMemcpyAsync(d_S, &S, sizeof(S), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload materials");
+    a.job = d_job; a.scene = d_scene; a.S = d_S;
+    a.npix = npix; a.mc_sample0 = u.mc_sample; a.chunk = 64;
+    a.out_rgb = d_rgb; a.out_depth = d_depth; a.out_normal = d_normal; a.spp_total = spp;
+    const uint32_t grid = nd_persistent_grid(c);
+    hipLaunchKernelGGL(nd::k_mc_zero, dim3(1024), dim3(256), 0, s, d_rgb, (uint64_t)3 * npix);
+    (void)hipEventRecord(c->ev[6], s);
+    int launches = 1;
+    unsigned long long segs_total = 0, de_total = 0, shadows_total = 0;
+    for (uint32_t sb = 0; sb < spp; sb += chunk) {
+        const uint32_t cs = std::min(chunk, spp - sb);
+        a.sample_base = sb;
+        a.spp_chunk = cs;
+        const uint32_t n0 = npix * cs;
+        hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 0);
+        hipLaunchKernelGGL(nd::k_mc_set_live, dim3(1), dim3(64), 0, s, Pp.ctr, n0);
+        hipLaunchKernelGGL(nd::k_mc_generate, dim3((n0 + 255) / 256), dim3(256), 0, s, a);
+        launches += 3;
+        int ping = 0;
+        for (int depth = 0; depth < S.max_depth; ++depth) {
+            hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 1);
+            hipLaunchKernelGGL(nd::k_mc_extend, dim3(grid), dim3(64), 0, s, a, ping);
+            hipLaunchKernelGGL(nd::k_mc_shade, dim3(grid), dim3(64), 0, s, a, ping);
+            hipLaunchKernelGGL(nd::k_mc_shadow, dim3(grid), dim3(64), 0, s, a);
+            launches += 4;
+            if (depth == 0 && S.indirect_on) {
+                hipLaunchKernelGGL(nd::k_mc_indirect_march, dim3(grid), dim3(64), 0, s, a);
+                hipLaunchKernelGGL(nd::k_mc_indirect_shade, dim3(grid), dim3(64), 0, s, a);
+                launches += 2;
+            }
+            hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 2);
+            launches += 1;
+            ping ^= 1;
+        }
+        hipLaunchKernelGGL(nd::k_mc_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, a);
+        launches += 1;
+        if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch path trace");
+        uint32_t h_ctr[16];
+        if ((e = hipMemcpyAsync(h_ctr, Pp.ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, s)) != hipSuccess) return nd_hip_fail(c, e, "read counters");
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return nd_hip_fail(c, e, "path trace execution");
+        segs_total += h_ctr[nd::C_SEG];
+        de_total += *(unsigned long long*)(h_ctr + nd::C_DE);
+        shadows_total += h_ctr[nd::C_SHTOT];
+        if (c->cancel && *c->cancel) return nd_fail(c, ND_E_CANCELLED, "cancelled");
+    }
+    (void)hipEventRecord(c->ev[7], s);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return nd_hip_fail(c, e, "path trace execution");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    if (!dev_out) {
+        if ((e = hipMemcpy(out->color, d_rgb, 12 * (size_t)npix, hipMemcpyDeviceToHost)) != hipSuccess) return nd_hip_fail(c, e, "read colour");
+        if (out->depth && (e = hipMemcpy(out->depth, d_depth, 4 * (size_t)npix, hipMemcpyDeviceToHost)) != hipSuccess) return nd_hip_fail(c, e, "read depth");
+        if (out->normal && (e = hipMemcpy(out->normal, d_normal, 12 * (size_t)npix, hipMemcpyDeviceToHost)) != hipSuccess) return nd_hip_fail(c, e, "read normal");
+    }
+    if (stats) {
+        stats->paths = (uint64_t)npix * spp;
+        stats->segments = segs_total;
+        stats->shadow_rays = shadows_total;
+        stats->de_evals = de_total;
+        stats->ms_total = ms;
+        stats->max_depth_reached = (uint32_t)S.max_depth;
+    }
+    (void)launches;
+    return ND_OK;
+}

@@ -172,3 +172,24 @@ def adjoint_pixels(u, pixels, h=1e-3, code_map=None):
     cm, cmp_ = _cm(code_map)
     L.ndo_adjoint_pixels(_u(u), cmp_, _p(pix), n, float(h), _p(t), _p(gi), _p(gf), _p(ok))
     return t, gi, gf, ok
+
+
+def render_mc(u, spp=1, rect=None, prec=0, threads=None, code_map=None, per_sample=False):
+    """Monte Carlo path trace (opaque scenes) of a rect: mean colour over spp samples
+    (sample i uses mc_sample = u.mc_sample + i), depth and normal of sample 0, DE evaluations,
+    and optionally the per-sample colours (spp, h, w, 3)."""
+    L = lib()
+    vp, i32 = C.c_void_p, C.c_int
+    L.ndo_render_mc.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
+    W, H = int(u.resolution[0]), int(u.resolution[1])
+    x, y, w, h = rect if rect else (0, 0, W, H)
+    out = {"rgb": np.zeros((h, w, 3), np.float32), "depth": np.zeros((h, w), np.float32),
+           "normal": np.zeros((h, w, 3), np.float32), "evals": np.zeros((h, w), np.uint32)}
+    samp = np.zeros((spp, h, w, 3), np.float32) if per_sample else None
+    cm, cmp_ = _cm(code_map)
+    threads = threads or min(16, os.cpu_count() or 1)
+    L.ndo_render_mc(_u(u), cmp_, prec, x, y, w, h, spp, _p(out["rgb"]), _p(out["depth"]), _p(out["normal"]),
+                    _p(samp), _p(out["evals"]), threads)
+    if per_sample:
+        out["samples"] = samp
+    return out

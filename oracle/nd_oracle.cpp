@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "nd_oracle.hpp"
+#include "nd_oracle_mc.hpp"
 
 using namespace ndo;
 
@@ -331,6 +332,56 @@ int ndo_adjoint_pixels(const nd_uniforms* u, const int32_t* cm, const int32_t* p
     return 0;
 }
 
+// Monte Carlo path trace of a rect, spp samples per pixel with mc_sample = u.mc_sample + s
+// (the host's per-sample stamp, fractal_renderer.zig:3761-3775), averaged. Per-sample colors
+// (spp x h x w x 3) are written too when sample_rgb is non-null.
+int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, int ry, int rw, int rh, int spp,
+                  float* rgb, float* depth, float* normal, float* sample_rgb, uint32_t* evals, int nthreads) {
+    std::atomic<int> next_row{0};
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        nd_uniforms u = *u0;
+        Scene<T> s = decode_scene<T>(u, code_map(cm));
+        MarchCfg mc = march_cfg(u);
+        for (;;) {
+            int y = next_row.fetch_add(1);
+            if (y >= rh) break;
+            for (int x = 0; x < rw; ++x) {
+                size_t i = (size_t)y * rw + x;
+                double acc[3] = {0, 0, 0};
+                uint32_t ev = 0;
+                int px = rx + x, py = ry + y;
+                V3<T> dir = primary_dir<T>(u, px, py, (int)u.resolution[0], (int)u.resolution[1]);
+                V3<T> org = V3<T>::from(u.camera_pos);
+                for (int sidx = 0; sidx < spp; ++sidx) {
+                    u.mc_sample = u0->mc_sample + (float)sidx;
+                    const float ms = u.mc_sample;
+                    Seed base{((float)px + 0.5f) + ms * 91.73f, ((float)py + 0.5f) + ms * 57.31f};
+                    PathResult r = trace_path<T>(u, s, mc, org, dir, base);
+                    for (int k = 0; k < 3; ++k) acc[k] += r.rgb[k];
+                    if (sample_rgb)
+                        for (int k = 0; k < 3; ++k) sample_rgb[((size_t)sidx * rw * rh + i) * 3 + k] = (float)r.rgb[k];
+                    if (sidx == 0) {
+                        if (depth) depth[i] = (float)r.depth;
+                        if (normal) for (int k = 0; k < 3; ++k) normal[3 * i + k] = (float)r.normal[k];
+                    }
+                    ev += (uint32_t)r.evals;
+                }
+                for (int k = 0; k < 3; ++k) rgb[3 * i + k] = (float)(acc[k] / spp);
+                if (evals) evals[i] = ev;
+            }
+        }
+    };
+    nthreads = std::max(1, nthreads);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t) {
+        if (prec == 0) pool.emplace_back([&] { run(0.0); });
+        else pool.emplace_back([&] { run(0.0f); });
+    }
+    for (auto& th : pool) th.join();
+    return 0;
+}
+
 // FP64 implicit-function gradient at given hit points p = o + d t (t per pixel, e.g. the
 // GPU's own hit depths), so the FP32 gradient can be compared at the point it was taken.
 int ndo_adjoint_at(const nd_uniforms* u, const int32_t* cm, const int32_t* pix, const double* t_in, int n,
@@ -372,6 +423,9 @@ int ndo_adjoint_at(const nd_uniforms* u, const int32_t* cm, const int32_t* pix, 
 //   13 combine_de(mode, d, di, k) -> d, h      14 trap_distance(shape, z3, c3, box3, r, tube)
 //   16 formula_finalize(f, z3, dr) -> d  (formula steps: ndo_formula_step)
 //   17 formula_iterations(f, p16) -> n         18 trap run: mode, shape, radius, iters, zs(3*iters)
+//   20 hash12w(sx, sy) -> u                    21 cosine_sample(n3, sx, sy) -> dir3
+//   22 spectral round trip (hero, rgb3) -> rgb3 23 gamut_fit(rgb3) -> rgb3
+//   24 fresnel_schlick(cos, ior) -> f          25 hero wavelengths(hero) -> lambda8
 int ndo_helper(int op, const double* a, double* out) {
     using D3 = V3<double>;
     switch (op) {
@@ -411,6 +465,14 @@ int ndo_helper(int op, const double* a, double* out) {
                        prev = z;
                    }
                    out[0] = trap_finish(acc, t, iters); return 0; }
+        case 20: out[0] = hash12w((float)a[0], (float)a[1]); return 0;
+        case 21: { D3 r = cosine_sample(normalize(D3(a[0], a[1], a[2])), Seed{(float)a[3], (float)a[4]});
+                   out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 22: { Spectral<double> sp(a[0]);
+                   sp.to_rgb(sp.from_rgb(a[1], a[2], a[3]), out); return 0; }
+        case 23: { out[0] = a[0]; out[1] = a[1]; out[2] = a[2]; gamut_fit(out); return 0; }
+        case 24: out[0] = fresnel_schlick(a[0], a[1]); return 0;
+        case 25: { Spectral<double> sp(a[0]); for (int k = 0; k < 8; ++k) out[k] = sp.lambda[k]; return 0; }
     }
     return -1;
 }

@@ -1,0 +1,403 @@
+// nd_oracle_mc.hpp - TEST INFRASTRUCTURE ONLY. Scalar restatement of the reference's Monte
+// Carlo path trace (trace_path and what it calls) for opaque scenes: spectral hero
+// wavelengths, the procedural sky, point / global lights with soft shadows, Blinn-Phong
+// shading, Schlick reflection with roughness-perturbed normals, the single cosine-sampled
+// indirect bounce at the primary hit, Russian roulette. Citations: template.wgsl lines.
+//
+// Random numbers: the reference hashes the raw FP32 bits of screen-space seeds (hash12w,
+// template.wgsl:2420-2434), so every seed here is computed in float exactly as the shader
+// computes it. The oracle (T = double) then draws the same random numbers as an FP32
+// renderer, and per-sample differences come from geometry rounding only.
+#pragma once
+
+#include "nd_oracle.hpp"
+
+namespace ndo {
+
+// ----------------------------------------------------------------------------- hashing
+// hash_u32 / hash12w (template.wgsl:2420-2434)
+inline uint32_t hash_u32(float px, float py) {
+    uint32_t x, y;
+    std::memcpy(&x, &px, 4);
+    std::memcpy(&y, &py, 4);
+    const uint32_t M = 1664525u, A = 1013904223u;
+    x = x * M + A;
+    y = y * M + A;
+    x += y * M;
+    y += x * M;
+    x ^= x >> 16;
+    y ^= y >> 16;
+    x += y * M;
+    y += x * M;
+    x ^= x >> 16;
+    y ^= y >> 16;
+    return x ^ y;
+}
+inline float hash12w(float px, float py) { return (float)hash_u32(px, py) * 2.3283064365386963e-10f; }
+struct Seed { float x, y; };
+inline Seed seed_add(Seed s, float ax, float ay) { return {s.x + ax, s.y + ay}; }
+
+// ----------------------------------------------------------------------------- spectra
+// 8 hero-stratified wavelengths and the RGB <-> spectrum bases (template.wgsl:713-883).
+template <class T> struct Spec { T v[8]; };
+template <class T> Spec<T> splat(T a) { Spec<T> s; for (auto& x : s.v) x = a; return s; }
+template <class T> Spec<T> operator+(const Spec<T>& a, const Spec<T>& b) { Spec<T> r; for (int k = 0; k < 8; ++k) r.v[k] = a.v[k] + b.v[k]; return r; }
+template <class T> Spec<T> operator*(const Spec<T>& a, const Spec<T>& b) { Spec<T> r; for (int k = 0; k < 8; ++k) r.v[k] = a.v[k] * b.v[k]; return r; }
+template <class T> Spec<T> operator*(const Spec<T>& a, T s) { Spec<T> r; for (int k = 0; k < 8; ++k) r.v[k] = a.v[k] * s; return r; }
+template <class T> T smax(const Spec<T>& a) { T m = a.v[0]; for (int k = 1; k < 8; ++k) m = std::max(m, a.v[k]); return m; }
+
+template <class T> struct Spectral {
+    T lambda[8];
+    Spec<T> br, bg, bb;  // basis of R, G, B
+    Spec<T> wr, wg, wb;  // projection weights back to R, G, B
+
+    explicit Spectral(T hero) {
+        const T lo = T(380.0), span = T(730.0 - 380.0);
+        const double strata[8] = {0.0, 0.125, 0.25, 0.375, 0.5, 0.625, 0.75, 0.875};
+        T base = tfract(hero);
+        for (int k = 0; k < 8; ++k) lambda[k] = lo + span * tfract(base + T(strata[k]));
+        const double FR[3] = {0.99575300, -0.00656071, 0.01080772};
+        const double FG[3] = {-0.00755455, 1.00830220, -0.00074766};
+        const double FB[3] = {0.02732810, -0.00674214, 0.97941404};
+        T sr = T(0), sg = T(0), sb = T(0);
+        for (int k = 0; k < 8; ++k) {
+            T l = lambda[k];
+            // spectral_prebasis (template.wgsl:789-797)
+            T blue = T(1) / (T(1) + exp((l - T(498.09)) / T(13.50)));
+            T red = T(1) / (T(1) + exp((T(600.31) - l) / T(8.00)));
+            T t = (l - T(530.0)) / T(25.0);
+            T green = T(2.49) * exp(T(-0.5) * t * t);
+            T sum = red + green + blue;
+            T a = red / sum, b = green / sum, c = blue / sum;
+            br.v[k] = a * T(FR[0]) + b * T(FG[0]) + c * T(FB[0]);
+            bg.v[k] = a * T(FR[1]) + b * T(FG[1]) + c * T(FB[1]);
+            bb.v[k] = a * T(FR[2]) + b * T(FG[2]) + c * T(FB[2]);
+            // spectral_cie (template.wgsl:805-814): piecewise Gaussian fits of x, y, z
+            auto lobe = [&](double mu, double slo, double shi) {
+                T s = l < T(mu) ? T(slo) : T(shi);
+                T u = (l - T(mu)) / s;
+                return exp(T(-0.5) * u * u);
+            };
+            T x = T(1.056) * lobe(599.8, 37.9, 31.0) + T(0.362) * lobe(442.0, 16.0, 26.7) - T(0.065) * lobe(501.1, 20.4, 26.2);
+            T y = T(0.821) * lobe(568.8, 46.9, 40.5) + T(0.286) * lobe(530.9, 16.3, 31.1);
+            T z = T(1.217) * lobe(437.0, 11.8, 36.0) + T(0.681) * lobe(459.0, 26.0, 13.8);
+            wr.v[k] = T(3.2404542) * x + T(-1.5371385) * y + T(-0.4985314) * z;
+            wg.v[k] = T(-0.9692660) * x + T(1.8760108) * y + T(0.0415560) * z;
+            wb.v[k] = T(0.0556434) * x + T(-0.2040259) * y + T(1.0572252) * z;
+            sr = sr + wr.v[k]; sg = sg + wg.v[k]; sb = sb + wb.v[k];
+        }
+        for (int k = 0; k < 8; ++k) { wr.v[k] = wr.v[k] / sr; wg.v[k] = wg.v[k] / sg; wb.v[k] = wb.v[k] / sb; }
+    }
+    // spec_from_rgb (template.wgsl:873-877): clamped at 0 per wavelength
+    Spec<T> from_rgb(T r, T g, T b) const {
+        Spec<T> s;
+        for (int k = 0; k < 8; ++k) s.v[k] = tmax(r * br.v[k] + g * bg.v[k] + b * bb.v[k], T(0));
+        return s;
+    }
+    void to_rgb(const Spec<T>& s, T* rgb) const {
+        rgb[0] = rgb[1] = rgb[2] = T(0);
+        for (int k = 0; k < 8; ++k) {
+            rgb[0] = rgb[0] + s.v[k] * wr.v[k];
+            rgb[1] = rgb[1] + s.v[k] * wg.v[k];
+            rgb[2] = rgb[2] + s.v[k] * wb.v[k];
+        }
+    }
+};
+
+// gamut_fit (template.wgsl:898-908)
+template <class T> void gamut_fit(T* c) {
+    T y = c[0] * T(0.2126) + c[1] * T(0.7152) + c[2] * T(0.0722);
+    if (y <= T(0)) { c[0] = c[1] = c[2] = T(0); return; }
+    T m = tmin(c[0], tmin(c[1], c[2]));
+    if (m >= T(0)) return;
+    T f = y / (y - m);
+    for (int k = 0; k < 3; ++k) c[k] = y + (c[k] - y) * f;
+}
+
+// ----------------------------------------------------------------------------- sky
+// sky_color for the procedural sky (template.wgsl:2050-2094)
+template <class T> V3<T> sky_color(const nd_uniforms& u, V3<T> dir) {
+    if (((uint32_t)u.debug_parts & ND_PART_SKY) != 0u) return V3<T>();
+    T c = T(std::cos((double)u.sky.yaw)), s = T(std::sin((double)u.sky.yaw));
+    V3<T> d(c * dir.x - s * dir.z, dir.y, s * dir.x + c * dir.z);
+    V3<T> base;
+    V3<T> hz = V3<T>::from(u.sky.horizon), ze = V3<T>::from(u.sky.zenith), gr = V3<T>::from(u.sky.ground);
+    if (d.y >= T(0)) {
+        T t = pow(tclamp(d.y, T(0), T(1)), T(1.0 / std::max((double)u.sky.falloff, 0.01)));
+        base = hz + (ze - hz) * t;
+    } else {
+        base = hz + (gr - hz) * tclamp(-d.y, T(0), T(1));
+    }
+    if (u.sky.sun_cos > -1.0f) {
+        V3<T> sd = normalize(V3<T>::from(u.sky.sun_dir));
+        T dd = dot(d, sd);
+        T e0 = T(u.sky.sun_cos), e1 = T(u.sky.sun_cos) + (T(1) - T(u.sky.sun_cos)) * T(0.15);
+        T x = tclamp((dd - e0) / (e1 - e0), T(0), T(1));
+        T ss = x * x * (T(3) - T(2) * x);
+        base = base + V3<T>::from(u.sky.sun_color) * (T(u.sky.sun_intensity) * ss);
+    }
+    T k = T(u.sky.intensity);
+    return V3<T>(tmax(base.x * k, T(0)), tmax(base.y * k, T(0)), tmax(base.z * k, T(0)));
+}
+
+// ----------------------------------------------------------------------------- materials
+struct Mat {
+    double color[3];
+    double glossiness, transparency, reflectiveness, ior, roughness;
+};
+inline Mat stop_mat(const nd_color_stop& c) {
+    return Mat{{c.color[0], c.color[1], c.color[2]}, c.glossiness, c.transparency, c.reflectiveness, c.ior, c.roughness};
+}
+inline Mat default_mat() { return Mat{{0.8, 0.8, 0.85}, 0.3, 0.0, 0.0, 1.5, 0.0}; }  // template.wgsl:1976-1978
+// mix_material(a, b, h) = mix(b, a, h) per field (template.wgsl:1980-2001)
+inline Mat mix_mat(const Mat& a, const Mat& b, double h) {
+    Mat r;
+    for (int k = 0; k < 3; ++k) r.color[k] = b.color[k] * (1 - h) + a.color[k] * h;
+    r.glossiness = b.glossiness * (1 - h) + a.glossiness * h;
+    r.transparency = b.transparency * (1 - h) + a.transparency * h;
+    r.reflectiveness = b.reflectiveness * (1 - h) + a.reflectiveness * h;
+    r.ior = b.ior * (1 - h) + a.ior * h;
+    r.roughness = b.roughness * (1 - h) + a.roughness * h;
+    return r;
+}
+// sample_gradient (template.wgsl:2003-2023)
+inline Mat sample_gradient(const nd_fractal_instance& fi, double t) {
+    int count = std::min(i32(fi.color_count), ND_MAX_COLOR_STOPS);
+    if (count <= 0) return default_mat();
+    if (count == 1 || t <= fi.colors[0].position) return stop_mat(fi.colors[0]);
+    if (t >= fi.colors[count - 1].position) return stop_mat(fi.colors[count - 1]);
+    for (int i = 0; i < count - 1; ++i) {
+        const nd_color_stop &a = fi.colors[i], &b = fi.colors[i + 1];
+        if (t >= a.position && t <= b.position) {
+            double f = (t - a.position) / std::max((double)(b.position - a.position), 1e-4);
+            return mix_mat(stop_mat(a), stop_mat(b), 1.0 - f);
+        }
+    }
+    return stop_mat(fi.colors[count - 1]);
+}
+// orbit_trap_to_strip (template.wgsl:2025-2035)
+inline double trap_to_strip(const nd_fractal_instance& fi, double v) {
+    double x = v / std::max((double)fi.trap_span, 1e-4) + fi.trap_offset;
+    int rep = i32(fi.trap_repeat + 0.5f);
+    if (rep == 1) return x - std::floor(x);
+    if (rep == 2) { double f = x * 0.5 - std::floor(x * 0.5); return 1.0 - std::fabs(f * 2.0 - 1.0); }
+    return std::min(std::max(x, 0.0), 1.0);
+}
+// hit_material (template.wgsl:2128-2155): every visible instance's strip material, blended
+// by the combine weight.
+template <class T, class P> Mat hit_material(const nd_uniforms& u, const Scene<P>& s, V3<T> pos) {
+    bool first = true;
+    T d = T(s.max_dist * 2.0 + 1.0);
+    Mat mat = default_mat();
+    for (int i = 0; i < s.count; ++i) {
+        const Instance<P>& in = s.inst[i];
+        if (in.hidden) continue;
+        T di, ti;
+        instance_de(in, pos, &di, &ti);
+        Mat mi = sample_gradient(u.instances[i], trap_to_strip(u.instances[i], (double)val(ti)));
+        if (first) { d = di; mat = mi; first = false; continue; }
+        T nd, h;
+        combine_de(in.combine, d, di, T(in.blend_k), &nd, &h);
+        mat = mix_mat(mat, mi, (double)val(h));
+        d = nd;
+    }
+    return mat;
+}
+
+// ----------------------------------------------------------------------------- shading
+template <class T> V3<T> reflect3(V3<T> i, V3<T> n) { return i - n * (T(2) * dot(n, i)); }
+
+// cosine_hemisphere_from (template.wgsl:3341-3350)
+template <class T> V3<T> cosine_hemisphere(V3<T> n, float u1, float u2) {
+    T r = sqrt(T(u1));
+    T phi = T(u2) * T(6.28318530718);
+    V3<T> local(r * cos(phi), r * sin(phi), sqrt(tmax(T(1) - T(u1), T(0))));
+    V3<T> up = abs(n.z) > T(0.99) ? V3<T>(T(1), T(0), T(0)) : V3<T>(T(0), T(0), T(1));
+    V3<T> tg = normalize(cross(up, n));
+    V3<T> bt = cross(n, tg);
+    return normalize(tg * local.x + bt * local.y + n * local.z);
+}
+template <class T> V3<T> cosine_sample(V3<T> n, Seed s) {
+    return cosine_hemisphere(n, hash12w(s.x, s.y), hash12w(s.x + 91.73f, s.y + 31.13f));
+}
+
+struct LightSampleD { double dir[3], color[3], dist; };
+// light_sample for point and global lights (template.wgsl:2178-2190)
+template <class T> void light_sample(const nd_light& L, double max_dist, V3<T> pos, V3<T>* dir, V3<T>* color, T* dist) {
+    int type = i32(L.light_type + 0.5f);
+    V3<T> lc = V3<T>::from(L.color) * T(L.brightness);
+    if (type == 1) {
+        *dir = normalize(V3<T>::from(L.position_or_direction));
+        *color = lc;
+        *dist = T(max_dist * 2.0);
+    } else {
+        V3<T> to = V3<T>::from(L.position_or_direction) - pos;
+        T d = length(to);
+        *dir = to / tmax(d, T(1e-4));
+        *color = lc / tmax(d * d, T(0.01));
+        *dist = d;
+    }
+}
+
+// shade_local for opaque, film-free, subsurface-free materials (template.wgsl:3222-3331)
+template <class T, class P>
+Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& sp, V3<T> pos, V3<T> dir, V3<T> n,
+                    const Mat& m, const Spec<T>& ambient, bool use_shadows, int* evals) {
+    V3<T> view = -dir;
+    T spec_power = T(4.0) + (T(128.0) - T(4.0)) * tclamp(T(m.glossiness), T(0), T(1));
+    Spec<T> albedo = sp.from_rgb(T(m.color[0]), T(m.color[1]), T(m.color[2]));
+    Spec<T> diffuse = splat(T(0)), specular = splat(T(0));
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    int lc = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min(i32(u.light_count), ND_MAX_LIGHTS);
+    bool shadows_on = use_shadows && !(parts & ND_PART_SHADOWS);
+    for (int i = 0; i < lc; ++i) {
+        const nd_light& L = u.lights[i];
+        V3<T> ld, col; T dist;
+        light_sample(L, u.max_dist, pos, &ld, &col, &dist);
+        if (tmax(col.x, tmax(col.y, col.z)) < T(1e-5)) continue;
+        Spec<T> ls = sp.from_rgb(col.x, col.y, col.z);
+        T ndotl = tmax(dot(n, ld), T(0));
+        V3<T> hd = normalize(ld + view);
+        T spec = (parts & ND_PART_SPECULAR) ? T(0) : pow(tmax(dot(n, hd), T(0)), spec_power) * T(m.glossiness);
+        if (ndotl <= T(0)) continue;
+        T shadow = T(1);
+        if (shadows_on && L.cast_shadows > 0.5f) {
+            double k = L.shadow_softness + (512.0 - L.shadow_softness) * L.hard_shadows;
+            int steps = std::min(i32(u.max_steps), 40);
+            shadow = visibility_march_opaque(s, pos + n * T(0.004), ld, dist, T(k), steps, evals);
+            if (shadow <= T(0)) continue;
+        }
+        Spec<T> reaching = ls * shadow;
+        diffuse = diffuse + reaching * ndotl;
+        specular = specular + reaching * spec;
+    }
+    return albedo * (ambient + diffuse) + specular;
+}
+
+// fresnel_schlick (template.wgsl:3424-3430)
+template <class T> T fresnel_schlick(T cos_theta, T ior) {
+    T r0root = (T(1) - ior) / (T(1) + ior);
+    T r0 = r0root * r0root;
+    T c = tclamp(T(1) - abs(cos_theta), T(0), T(1));
+    T c2 = c * c;
+    return r0 + (T(1) - r0) * c2 * c2 * c;
+}
+
+// ----------------------------------------------------------------------------- trace_path
+struct PathResult { double rgb[3]; double depth; double normal[3]; int evals; int vertices; };
+
+// trace_path (template.wgsl:4126-4137) with path_begin / path_extend / path_shade
+// (:3819-4124) for an opaque scene: no fog, particles, photons, dispersion, film,
+// subsurface, transmission or hit cache.
+template <class T, class P>
+PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, Seed seed) {
+    PathResult out{};
+    const float ms = u.mc_sample;
+    Seed st_seed = seed_add(seed, ms * 27.31f, ms * 45.17f);
+    float hero_f = hash12w(seed.x + 19.31f, seed.y + 47.11f) + ms * 0.61803399f;
+    Spectral<T> sp{T(hero_f)};
+    Spec<T> radiance = splat(T(0)), throughput = splat(T(1));
+    V3<T> pos = org, d = dir;
+    bool reflected = false;
+    const int max_depth = std::min(std::max(i32(u.max_reflection_bounces) + 8, 1), 16);
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    for (int depth = 0; depth < max_depth; ++depth) {
+        out.vertices = depth + 1;
+        Seed sd = seed_add(st_seed, (float)depth * 17.77f + 3.1f, (float)depth * 41.13f + 7.7f);
+        MarchResult<T> m = march_to(s, mc, pos, d, T(u.max_dist));
+        out.evals += m.evals;
+        if (!m.hit) {
+            V3<T> sky = sky_color(u, d);
+            radiance = radiance + throughput * sp.from_rgb(sky.x, sky.y, sky.z);
+            break;
+        }
+        if (smax(throughput) < T(0.002)) break;
+        V3<T> hp = pos + d * m.dist;
+        V3<T> n = normal_tetra(s, hp, T(mc.eps_c) * m.dist + T(mc.eps_f));
+        out.evals += 4;
+        Mat mat = hit_material(u, s, hp);
+        out.evals += 1;
+        if (depth == 0) {
+            out.depth = (double)val(m.dist);
+            out.normal[0] = (double)val(n.x); out.normal[1] = (double)val(n.y); out.normal[2] = (double)val(n.z);
+        }
+        // scatter_at, allow_diffuse = false (template.wgsl:3552-3616), opaque material
+        V3<T> nf = dot(d, n) > T(0) ? -n : n;
+        T rough = tclamp(T(mat.roughness), T(0), T(1));
+        T alpha = rough * rough;
+        V3<T> nr = nf;
+        if (alpha > T(1e-4)) {
+            V3<T> h = cosine_sample(nf, seed_add(sd, 5.19f, 12.73f));
+            nr = normalize(nf + (h - nf) * alpha);
+        }
+        T ior = tmax(T(mat.ior), T(1));
+        T f = fresnel_schlick(dot(d, nr), ior);
+        T refl = tclamp(T(mat.reflectiveness), T(0), T(1));
+        T p_reflect = tclamp(refl + (T(1) - refl) * f, T(0), T(1));
+        T p_diffuse = T(1) - p_reflect;  // transparency 0: p_transmit = 0
+        T local_share = p_diffuse;
+        if (local_share > T(0.001)) {
+            Spec<T> ambient = splat(T(0.15));
+            bool mc_indirect = u.mc_enabled > 0.5f && !(parts & ND_PART_MC_INDIRECT);
+            if (depth == 0 && mc_indirect) {
+                // indirect_light_traced (template.wgsl:3359-3377)
+                Seed is = seed_add(sd, 13.13f, 71.71f);
+                V3<T> idir = cosine_sample(n, is);
+                V3<T> io = hp + n * T(0.004);
+                MarchResult<T> b = march_to(s, mc, io, idir, T(u.max_dist));
+                out.evals += b.evals;
+                if (!b.hit) {
+                    V3<T> sky = sky_color(u, idir);
+                    ambient = sp.from_rgb(sky.x, sky.y, sky.z);
+                } else {
+                    V3<T> bp = io + idir * b.dist;
+                    V3<T> bn = normal_tetra(s, bp, T(mc.eps_c) * b.dist + T(mc.eps_f));
+                    Mat bm = hit_material(u, s, bp);
+                    out.evals += 5;
+                    ambient = shade_local(u, s, sp, bp, idir, bn, bm, splat(T(0.15)), false, &out.evals);
+                }
+            } else if (!reflected) {
+                // calc_ao (template.wgsl:1825-1841)
+                if (!(parts & ND_PART_AMBIENT_OCCLUSION)) {
+                    T radius = tclamp(m.dist * T(0.04), T(0.003), T(0.25));
+                    T occ = T(0), w = T(1);
+                    for (int i = 0; i < 5; ++i) {
+                        T step = radius * (T(0.2) + T(0.2) * T((double)i));
+                        T dd = scene_de(s, hp + n * step);
+                        occ = occ + tmax(step - dd, T(0)) * w;
+                        w = w * T(0.6);
+                    }
+                    out.evals += 5;
+                    ambient = ambient * tclamp(T(1) - T(2) * occ / radius, T(0), T(1));
+                }
+            }
+            Spec<T> lit = shade_local(u, s, sp, hp, d, n, mat, ambient, !reflected, &out.evals);
+            radiance = radiance + throughput * lit * local_share;
+        }
+        if (!(p_reflect > T(0))) break;  // r = hash * p_reflect < p_reflect fails: absorbed
+        if (parts & ND_PART_REFLECTIONS) break;
+        // reflect: weight = mix(color, 1, f) * p_cont (= p_reflect)
+        T w0 = T(mat.color[0]) + (T(1) - T(mat.color[0])) * f;
+        T w1 = T(mat.color[1]) + (T(1) - T(mat.color[1])) * f;
+        T w2 = T(mat.color[2]) + (T(1) - T(mat.color[2])) * f;
+        throughput = throughput * (sp.from_rgb(w0, w1, w2) * p_reflect);
+        reflected = true;
+        V3<T> nd = reflect3(d, nr);
+        T bias = tmax(T(0.003), T(2) * (T(mc.eps_c) * m.dist + T(mc.eps_f)));
+        pos = hp + nd * bias;
+        d = nd;
+        if (depth >= 4) {
+            T survive = tclamp(smax(throughput), T(0.05), T(1));
+            Seed rs = seed_add(sd, 59.3f, 23.9f);
+            if (T(hash12w(rs.x, rs.y)) > survive) break;
+            throughput = throughput * (T(1) / survive);
+        }
+    }
+    T rgb[3];
+    sp.to_rgb(radiance, rgb);
+    gamut_fit(rgb);
+    for (int k = 0; k < 3; ++k) out.rgb[k] = (double)val(rgb[k]);
+    return out;
+}
+
+}  // namespace ndo
