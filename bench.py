@@ -147,7 +147,7 @@ def mc_leg(r, nd, spp=64):
             "de_evals_per_s": round(st.de_evals / wall, 1), "de_evals_per_path": round(st.de_evals / st.paths, 1)}
 
 
-def cpu_baseline(nd, target_s=20.0):
+def cpu_baseline(nd, target_s=40.0):
     """FP32 CPU restatement of the same per-ray math (the oracle instantiated for float),
     on host threads, over a band of rows of camera 0 at full 1920x1080."""
     import oracle
@@ -159,7 +159,7 @@ def cpu_baseline(nd, target_s=20.0):
     oracle.render_gbuffer(u, rect=(0, H // 2, W, rows), prec=1, normal_mode=0, shadows=True,
                           shadow_max_steps=128, threads=threads)
     dt = max(time.perf_counter() - t0, 1e-3)
-    rows = int(max(4, min(H // 2, rows * target_s / dt)))
+    rows = int(max(4, min(H, rows * target_s / dt)))
     y0 = max(0, H // 2 - rows // 2)
     t0 = time.perf_counter()
     oracle.render_gbuffer(u, rect=(0, y0, W, rows), prec=1, normal_mode=0, shadows=True,
@@ -179,7 +179,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--no-legs", action="store_true", help="skip the 256-job and gradient legs")
+    ap.add_argument("--no-legs", action="store_true", help="skip the path-trace, 256-job and gradient legs")
+    ap.add_argument("--legs", default="c3,c4,c5", help="comma list of extra legs: c3 path trace, c4 256 jobs, "
+                                                      "c5 gradient")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the timed steps, run one elementwise copy of MIB MiB (reads MIB, writes MIB): "
+                         "the reference kernel for scaling hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
@@ -190,6 +195,14 @@ def main():
     from numericdream_amd import abi
 
     r = nd.Renderer(0)
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        log("copy calibration: %d MiB read, %d MiB written" % (args.copy_calibration, args.copy_calibration))
+        del a, b
     jobs = make_jobs(nd, torch, args.cams)
     log("jobs ready: %d x 1920x1080" % len(jobs))
     if args.autotune:
@@ -232,11 +245,14 @@ def main():
         res["plain_mrays_per_s"] = round(pv, 3)
         res["persistent_vs_plain"] = round(value / pv, 3) if pv > 0 else None
         res["plain_ms_per_step"] = round(plain["wall_s"] * 1e3 / steps, 3)
-    if not args.no_legs:
+    legs = set() if args.no_legs else set(x.strip() for x in args.legs.split(",") if x.strip())
+    if "c4" in legs:
         res["c4_batch"] = batch_jobs_leg(r, nd, torch, abi)
+    if "c5" in legs:
         g = gradient_leg(r, nd, torch)
         res["grad_rays_per_s"] = g["grad_rays_per_s"]
         res["c5_gradient"] = g
+    if "c3" in legs:
         res["c3_path_trace"] = mc_leg(r, nd)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

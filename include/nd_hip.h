@@ -175,9 +175,10 @@ nd_status nd_batch_job_latency(nd_ctx* ctx, float* ms_out, uint32_t n_jobs);
 #define ND_TUNE_TRACK_LATENCY 1u  /* record per-job completion times (nd_batch_job_latency) */
 
 typedef struct nd_tuning {
-    uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups; 0 = occupancy */
-    uint32_t refill_min_idle;  /* a wave pulls new rays when at least this many lanes are idle */
-    uint32_t chunk_rays;       /* rays a wave claims from the global queue per atomic */
+    uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups; 0 = twice the
+                                  resident waves per CU (the second half starts as the first drains) */
+    uint32_t refill_min_idle;  /* a wave pulls new rays when at least this many lanes are idle (default 8) */
+    uint32_t chunk_rays;       /* rays a wave claims from the global queue per atomic (default 256) */
     uint32_t flags;            /* ND_TUNE_* */
 } nd_tuning;
 

@@ -269,9 +269,9 @@ nd_status nd_ctx_create(int hip_device, nd_ctx** out) {
         c->wall_khz = khz;
     for (auto& e : c->ev) (void)hipEventCreate(&e);
     c->occupancy_waves = nd_march_occupancy();
-    c->tuning.waves_per_cu = 0;  // 0 = from occupancy
-    c->tuning.refill_min_idle = 1;
-    c->tuning.chunk_rays = 64;
+    c->tuning.waves_per_cu = 0;  // 0 = twice the resident waves per CU
+    c->tuning.refill_min_idle = 8;
+    c->tuning.chunk_rays = 256;
     c->tuning.flags = 0;
     *out = c;
     return ND_OK;
@@ -349,10 +349,10 @@ static size_t out_bytes(const nd_gbuffer_out& o, size_t px) {
 uint32_t nd_persistent_grid(nd_ctx* c) {
     uint32_t wpc = c->tuning.waves_per_cu;
     if (wpc == 0) {
-        int blocks = 0;
-        // k_march is the kernel that bounds occupancy; 1 wave64 per workgroup
-        wpc = c->occupancy_waves ? c->occupancy_waves : 8;
-        (void)blocks;
+        // k_march bounds occupancy (one wave64 per workgroup). Twice the resident count: the
+        // second half starts on a CU as soon as waves of the first drain, which evens out the
+        // tail of the persistent passes.
+        wpc = 2 * (c->occupancy_waves ? c->occupancy_waves : 8);
     }
     return (uint32_t)std::max(c->cu_count, 1) * wpc;
 }
@@ -437,7 +437,10 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     if ((st = c->job_base.ensure(c, sizeof(uint32_t) * n)) != ND_OK) return st;
     if ((st = c->ctr.ensure(c, sizeof(DevCounters))) != ND_OK) return st;
     if ((st = c->shared.ensure(c, sizeof(DevScene))) != ND_OK) return st;
-    if ((st = c->latency.ensure(c, sizeof(unsigned long long) * n)) != ND_OK) return st;
+    const bool track = (c->tuning.flags & ND_TUNE_TRACK_LATENCY) != 0;
+    // latency: per-job completion clocks, then (when tracking) one completion tick per ray slot
+    const size_t lat_jobs = (sizeof(unsigned long long) * n + 255) & ~(size_t)255;
+    if ((st = c->latency.ensure(c, lat_jobs + (track ? sizeof(uint32_t) * (size_t)total : 0))) != ND_OK) return st;
     hipError_t e;
     if ((e = hipMemcpyAsync(c->jobs.p, hj.data(), sizeof(DevJob) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return nd_hip_fail(c, e, "upload jobs");
@@ -496,7 +499,8 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.round_cap = c->round_cap;
         L.any_shadow = any_shadow;
         L.surface = !ex.skip_surface;
-        L.job_done_cycles = (c->tuning.flags & ND_TUNE_TRACK_LATENCY) ? (unsigned long long*)c->latency.p : nullptr;
+        L.job_done_cycles = track ? (unsigned long long*)c->latency.p : nullptr;
+        L.ray_done = track ? (uint32_t*)((char*)c->latency.p + lat_jobs) : nullptr;
         for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];
         e = nd_launch_persistent(L, c->stream);
         if (e != hipSuccess) return nd_hip_fail(c, e, "launch persistent passes");
@@ -594,7 +598,7 @@ nd_status nd_autotune(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_
     const uint32_t occ = c->occupancy_waves ? c->occupancy_waves : 8;
     const uint32_t wpcs[] = {occ / 2 ? occ / 2 : 1, occ, occ * 2};
     const uint32_t idles[] = {1, 8, 32};
-    const uint32_t chunks[] = {64, 256};
+    const uint32_t chunks[] = {64, 256, 1024};
     nd_tuning bt = saved;
     float bms = 1e30f;
     for (uint32_t w : wpcs)

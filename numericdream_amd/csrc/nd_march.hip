@@ -177,7 +177,7 @@ struct PassArgs {
     uint32_t chunk;
     uint32_t min_idle;
     uint32_t round_cap;
-    unsigned long long* job_done_cycles;  // per job: wall clock (s_memrealtime) of its last retired ray
+    uint32_t* ray_done;  // per ray slot: completion time in wall-clock ticks after clock_start, or null
 };
 
 NDD void write_miss(const DevJob& J, size_t o, int evals) {
@@ -205,8 +205,9 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
     bool active = false;
     uint32_t ray = 0;
     int job = 0, px = 0, py = 0;
-    int lat_job = -1;                      // per-lane latency record: last job retired and when
-    unsigned long long lat_clock = 0;
+    uint32_t lat_ray = ~0u;                // retired ray slot whose completion time is written at the
+                                           // lane's next refill (latency tracking)
+    const unsigned long long c0 = a.ray_done ? a.ctr->clock_start : 0ull;
     V3<float> org = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 1.f);
     March m;
     march_init(m);
@@ -216,6 +217,12 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
         if (work_left) {
             uint64_t need = __builtin_amdgcn_ballot_w64(!active);
             if ((uint32_t)__popcll(need) >= a.min_idle || need == ~0ull) {
+                // latency tracking reads the clock once per refill: a ray that retired since the
+                // last refill finished no later than now
+                if (a.ray_done && lat_ray != ~0u) {
+                    a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
+                    lat_ray = ~0u;
+                }
                 while (need != 0ull) {
                     if (cnext >= cend) {
                         uint32_t base = 0;
@@ -275,11 +282,11 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
                     write_miss(J, o, m.evals);
                 }
                 active = false;
-                if (a.job_done_cycles) {
-                    // jobs reach a lane in increasing order: flush a job's clock when the lane moves on
-                    if (lat_job >= 0 && lat_job != job) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
-                    lat_job = job;
-                    lat_clock = (unsigned long long)wall_clock64();
+                if (a.ray_done) {
+                    // timed at this lane's next refill; after the queue drained there is none,
+                    // so the clock is read now (the tail of the pass)
+                    if (work_left) lat_ray = ray;
+                    else a.ray_done[ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
                 }
             }
         }
@@ -311,7 +318,7 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
     }
     // unused slots of the last hit chunk are marked empty for the later passes
     for (uint32_t i = hnext + __lane_id(); i < hend; i += 64) a.hb.ray[i] = ~0u;
-    if (a.job_done_cycles && lat_job >= 0) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
+    if (a.ray_done && lat_ray != ~0u) a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
     if (__lane_id() == 0) {
         atomicAdd(&a.ctr->de_march, de);
         atomicAdd(&a.ctr->hits, nhits);
@@ -386,13 +393,18 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
     int job = 0, steps = 0, step = 0;
     V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
     float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
-    int lat_job = -1;
-    unsigned long long lat_clock = 0;
+    uint32_t sray = 0;           // ray slot of the current shadow ray
+    uint32_t lat_ray = ~0u;      // retired ray slot timed at the next refill
+    const unsigned long long c0 = a.ray_done ? a.ctr->clock_start : 0ull;
     unsigned long long de = 0;
     for (uint32_t round = 0;; ++round) {
         if (work_left) {
             uint64_t need = __builtin_amdgcn_ballot_w64(!active);
             if ((uint32_t)__popcll(need) >= a.min_idle || need == ~0ull) {
+                if (a.ray_done && lat_ray != ~0u) {
+                    a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
+                    lat_ray = ~0u;
+                }
                 while (need != 0ull) {
                     if (cnext >= cend) {
                         uint32_t base = 0;
@@ -410,6 +422,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                         steps = a.hb.steps[hit];
                         if (steps > 0) {
                             job = a.hb.job[hit];
+                            if (a.ray_done) sray = a.hb.ray[hit];
                             so = mk3(a.hb.sox[hit], a.hb.soy[hit], a.hb.soz[hit]);
                             ld = mk3(a.hb.sdx[hit], a.hb.sdy[hit], a.hb.sdz[hit]);
                             ldist = a.hb.sdist[hit];
@@ -446,10 +459,9 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                 if (J.out_shadow) J.out_shadow[o] = fclamp(res, 0.f, 1.f);
                 if (J.out_evals) J.out_evals[o] += (uint32_t)step;
                 active = false;
-                if (a.job_done_cycles) {
-                    if (lat_job >= 0 && lat_job != job) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
-                    lat_job = job;
-                    lat_clock = (unsigned long long)wall_clock64();
+                if (a.ray_done) {
+                    if (work_left) lat_ray = sray;
+                    else a.ray_done[sray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
                 }
             }
         }
@@ -458,13 +470,29 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
             break;
         }
     }
-    if (a.job_done_cycles && lat_job >= 0) atomicMax(&a.job_done_cycles[lat_job], lat_clock);
+    if (a.ray_done && lat_ray != ~0u) a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
     if (__lane_id() == 0) atomicAdd(&a.ctr->de_shadow, de);
 }
 
-__global__ void k_zero_u64(unsigned long long* p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Per-job completion clock = clock_start + the latest completion tick over the job's ray
+// slots (slots outside the rect keep 0). One 256-thread workgroup per job.
+__global__ void __launch_bounds__(256) k_job_done(const DevJob* __restrict__ jobs, const uint32_t* __restrict__ ray_done,
+                                                  const DevCounters* ctr, unsigned long long* job_done) {
+    const DevJob& J = jobs[blockIdx.x];
+    uint32_t m = 0;
+    for (uint32_t i = threadIdx.x; i < J.ray_count; i += 256) m = max(m, ray_done[J.ray_base + i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_down((int)m, off));
+    __shared__ uint32_t part[4];
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = max(max(part[0], part[1]), max(part[2], part[3]));
+        job_done[blockIdx.x] = m ? ctr->clock_start + m : 0ull;
+    }
 }
 
 }  // namespace nd
@@ -492,11 +520,9 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
     a.chunk = L.chunk;
     a.min_idle = L.min_idle;
     a.round_cap = L.round_cap;
-    a.job_done_cycles = L.job_done_cycles;
+    a.ray_done = L.job_done_cycles ? L.ray_done : nullptr;
     hipLaunchKernelGGL(nd::k_zero_counters, dim3(1), dim3(64), 0, s, L.ctr);
-    if (L.job_done_cycles) {
-        hipLaunchKernelGGL(nd::k_zero_u64, dim3((L.n_jobs + 255) / 256), dim3(256), 0, s, L.job_done_cycles, L.n_jobs);
-    }
+    if (a.ray_done) hipLaunchKernelGGL(nd::k_zero_u32, dim3(1024), dim3(256), 0, s, a.ray_done, L.total_rays);
     if (L.ev[0]) hipEventRecord(L.ev[0], s);
     if (L.shared) hipLaunchKernelGGL(nd::k_march<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
     else hipLaunchKernelGGL(nd::k_march<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
@@ -510,6 +536,8 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
         if (L.shared) hipLaunchKernelGGL(nd::k_shadow<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
         else hipLaunchKernelGGL(nd::k_shadow<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
     }
+    if (a.ray_done)
+        hipLaunchKernelGGL(nd::k_job_done, dim3(L.n_jobs), dim3(256), 0, s, L.jobs, a.ray_done, L.ctr, L.job_done_cycles);
     if (L.ev[3]) hipEventRecord(L.ev[3], s);
     return hipGetLastError();
 }

@@ -919,7 +919,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     if ((e = hipMemcpyAsync(d_scene, &job.scene, sizeof(DevScene), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload scene");
     if ((e = hipMemcpyAsync(d_S, &S, sizeof(S), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload materials");
     a.job = d_job; a.scene = d_scene; a.S = d_S;
-    a.npix = npix; a.mc_sample0 = u.mc_sample; a.chunk = 64;
+    a.npix = npix; a.mc_sample0 = u.mc_sample; a.chunk = 64;  // bounce queues are short: small claims balance better
     a.out_rgb = d_rgb; a.out_depth = d_depth; a.out_normal = d_normal; a.spp_total = spp;
     const uint32_t grid = nd_persistent_grid(c);
     hipLaunchKernelGGL(nd::k_mc_zero, dim3(1024), dim3(256), 0, s, d_rgb, (uint64_t)3 * npix);

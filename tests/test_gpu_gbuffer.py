@@ -154,3 +154,25 @@ def test_batch_mixed_jobs(renderer):
         assert st["hit_agree_inner"] >= 0.995, st
         if st["hits_ref"] > 50:
             assert st["depth_rel_le_1e-4"] >= 0.97, st
+
+
+def test_job_latency_tracking(renderer):
+    """Per-job completion times of a persistent batch: every job gets one, none after the
+    batch ended, and the last one lands near the end of the march and shadow passes."""
+    jobs = []
+    for i in range(12):
+        u = nd.config_uniforms(4, i)
+        s = max(16, int(u.resolution[0]) // 4)
+        nd.resize(u, s, s)
+        jobs.append((u, nd.GBuffer(s, s)))
+    t = renderer.tuning()
+    renderer.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays, track_latency=True)
+    try:
+        st = renderer.render_batch(jobs, shadows=True)
+        lat = renderer.job_latency_ms(len(jobs))
+    finally:
+        renderer.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays, track_latency=False)
+    print("\n[latency] ms_total %.3f latencies %s" % (st.ms_total, np.round(lat, 3)), flush=True)
+    assert np.all(np.isfinite(lat)) and np.all(lat > 0)
+    assert lat.max() <= st.ms_total * 1.05 + 0.05
+    assert lat.max() >= 0.3 * (st.ms_march + st.ms_shadow)
