@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3         # MI355X FP32 vector, packed (published)
 FP32_PEAK_UNPACKED_TFLOPS = 78.6  # without packed FP32 instructions
 METRIC = "Mrays/s persistent-batched vs plain @1080p Mandelbox, %FP32 peak; grad rays/s"
+# HBM bytes per k_march launch on the 16-camera batch, from two rocprofv3 --pmc runs
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy; profiles/k_march_c2_traffic.md).
+# Hardware counters cannot be read inside the timed run, so the measured figure is carried here.
+K_MARCH_TRAFFIC_BYTES_16CAM = 1.100e9
 
 
 def log(msg):
@@ -140,7 +144,10 @@ def mc_leg(r, nd, spp=64):
     r.sync()
     wall = time.perf_counter() - t0
     log("c3 mc: %.1f ms for %d spp, %d segments" % (wall * 1e3, spp, st.segments))
-    return {"spp": spp, "width": W, "height": H, "ms_per_frame": round(wall * 1e3, 2),
+    f_de = flops_table()["workloads"]["3"]["march"]["flops_per_de"]
+    tflops = f_de * st.de_evals / (st.ms_total / 1e3) / 1e12 if st.ms_total > 0 else 0.0
+    return {"tflops": round(tflops, 3), "frac_fp32_peak": round(tflops / FP32_PEAK_TFLOPS, 4),
+            "flops_per_de": round(f_de, 2),"spp": spp, "width": W, "height": H, "ms_per_frame": round(wall * 1e3, 2),
             "device_ms": round(st.ms_total, 2), "samples_per_s": round(st.paths / wall, 1),
             "msamples_per_s": round(st.paths / wall / 1e6, 3), "segments_per_s": round(st.segments / wall, 1),
             "segments_per_path": round(st.segments / st.paths, 3), "shadow_rays": int(st.shadow_rays),
@@ -232,7 +239,8 @@ def main():
                                "1920x1080, persistent batched" % len(jobs),
                    "cameras": len(jobs), "width": 1920, "height": 1080, "shadow_max_steps": 128},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": K_MARCH_TRAFFIC_BYTES_16CAM if len(jobs) == 16 else None,
                      "frac_unpacked_peak": round(achieved / FP32_PEAK_UNPACKED_TFLOPS, 4),
                      "kernel": "k_march<true> (persistent march pass)",
                      "flops_per_de": round(f_march, 2), "de_evals_per_launch": pers["de_march"] // steps,

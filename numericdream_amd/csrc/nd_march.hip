@@ -479,20 +479,28 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 }
 
 // Per-job completion clock = clock_start + the latest completion tick over the job's ray
-// slots (slots outside the rect keep 0). One 256-thread workgroup per job.
+// slots (slots outside the rect keep 0). Grid (jobs, JOB_DONE_SLICES): each 256-thread
+// workgroup reduces one slice of a job and merges it with one atomicMax into the zeroed
+// job_done.
+constexpr int JOB_DONE_SLICES = 16;
 __global__ void __launch_bounds__(256) k_job_done(const DevJob* __restrict__ jobs, const uint32_t* __restrict__ ray_done,
                                                   const DevCounters* ctr, unsigned long long* job_done) {
     const DevJob& J = jobs[blockIdx.x];
     uint32_t m = 0;
-    for (uint32_t i = threadIdx.x; i < J.ray_count; i += 256) m = max(m, ray_done[J.ray_base + i]);
+    for (uint32_t i = blockIdx.y * 256 + threadIdx.x; i < J.ray_count; i += 256 * JOB_DONE_SLICES)
+        m = max(m, ray_done[J.ray_base + i]);
     for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_down((int)m, off));
     __shared__ uint32_t part[4];
     if (__lane_id() == 0) part[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         m = max(max(part[0], part[1]), max(part[2], part[3]));
-        job_done[blockIdx.x] = m ? ctr->clock_start + m : 0ull;
+        if (m) atomicMax(&job_done[blockIdx.x], ctr->clock_start + m);
     }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
 }  // namespace nd
@@ -522,7 +530,10 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
     a.round_cap = L.round_cap;
     a.ray_done = L.job_done_cycles ? L.ray_done : nullptr;
     hipLaunchKernelGGL(nd::k_zero_counters, dim3(1), dim3(64), 0, s, L.ctr);
-    if (a.ray_done) hipLaunchKernelGGL(nd::k_zero_u32, dim3(1024), dim3(256), 0, s, a.ray_done, L.total_rays);
+    if (a.ray_done) {
+        hipLaunchKernelGGL(nd::k_zero_u32, dim3(1024), dim3(256), 0, s, a.ray_done, L.total_rays);
+        hipLaunchKernelGGL(nd::k_zero_u64, dim3((L.n_jobs + 255) / 256), dim3(256), 0, s, L.job_done_cycles, L.n_jobs);
+    }
     if (L.ev[0]) hipEventRecord(L.ev[0], s);
     if (L.shared) hipLaunchKernelGGL(nd::k_march<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
     else hipLaunchKernelGGL(nd::k_march<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
@@ -537,7 +548,8 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
         else hipLaunchKernelGGL(nd::k_shadow<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
     }
     if (a.ray_done)
-        hipLaunchKernelGGL(nd::k_job_done, dim3(L.n_jobs), dim3(256), 0, s, L.jobs, a.ray_done, L.ctr, L.job_done_cycles);
+        hipLaunchKernelGGL(nd::k_job_done, dim3(L.n_jobs, nd::JOB_DONE_SLICES), dim3(256), 0, s, L.jobs, a.ray_done,
+                           L.ctr, L.job_done_cycles);
     if (L.ev[3]) hipEventRecord(L.ev[3], s);
     return hipGetLastError();
 }

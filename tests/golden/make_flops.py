@@ -26,7 +26,8 @@ def main():
     L.ndo_flops_render.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_void_p]
     res = {"convention": "ND-FLOP v1", "sample_step_px": 16, "workloads": {}}
-    for cfg, jobs, shadow_steps in ((1, [0], 0), (2, [0, 1, 2, 3], 128), (4, list(range(8)), 0), (5, [0, 1], 0)):
+    for cfg, jobs, shadow_steps in ((1, [0], 0), (2, [0, 1, 2, 3], 128), (3, [0], 40), (4, list(range(8)), 0),
+                                    (5, [0, 1], 0)):
         acc = np.zeros(6)
         for j in jobs:
             u = nd.config_uniforms(cfg, j)
