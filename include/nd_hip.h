@@ -235,10 +235,17 @@ typedef struct nd_trace_stats {
     uint64_t shadow_rays;
     uint64_t de_evals;
     float ms_total;
-    uint32_t max_depth_reached;
+    uint32_t max_depth_reached;  /* the depth limit clamp(max_reflection_bounces + 8, 1, 16) */
     uint32_t _pad;
 } nd_trace_stats;
 
+/* Path-traces rect with spp samples per pixel: sample i uses mc_sample = u.mc_sample + i,
+ * the host's per-sample stamp (fr.zig:3761-3775), and trace_path's transport
+ * (template.wgsl:3819-4137) for opaque scenes. out->color (required) receives the mean
+ * over the samples after gamut_fit; depth and normal come from sample 0 (tetrahedral normal).
+ * Transmissive, subsurface, film and dispersive materials, fog, photons, depth of field,
+ * adaptive sampling, the hit cache and ray lights return ND_E_UNSUPPORTED. The cancel flag
+ * is checked between chunks of samples. */
 nd_status nd_path_trace(nd_ctx* ctx, const void* uniforms, size_t uniforms_size, uint32_t spp,
                         nd_rect rect, const nd_image_out* out, nd_trace_stats* stats);
 
