@@ -327,6 +327,7 @@ nd_status nd_set_tuning(nd_ctx* c, const nd_tuning* t) {
     if (t->refill_min_idle < 1 || t->refill_min_idle > 64) return nd_fail(c, ND_E_INVALID, "refill_min_idle must be 1..64");
     if (t->chunk_rays < 1 || t->chunk_rays > (1u << 20)) return nd_fail(c, ND_E_INVALID, "chunk_rays must be 1..2^20");
     if (t->waves_per_cu > 64) return nd_fail(c, ND_E_INVALID, "waves_per_cu must be 0..64");
+    if (t->flags & ~ND_TUNE_TRACK_LATENCY) return nd_fail(c, ND_E_INVALID, "unknown tuning flags 0x%x", t->flags);
     c->tuning = *t;
     return ND_OK;
 }
