@@ -56,7 +56,7 @@ def timed(r, jobs, kernel, steps, warmup, torch, abi, label):
         log("%s warmup %d: %.1f ms device" % (label, i, stats.ms_total))
     torch.cuda.synchronize()
     r.sync()
-    agg = {"ms_march": 0.0, "ms_total": 0.0, "de_march": 0, "de_total": 0, "rays": 0, "hits": 0}
+    agg = {"ms_march": 0.0, "ms_total": 0.0, "de_march": 0, "de_total": 0, "rays": 0, "hits": 0, "rounds": 0}
     t0 = time.perf_counter()
     for i in range(steps):
         stats = r.render_batch(jobs, shadows=True, shadow_max_steps=128, kernel=kernel)
@@ -66,6 +66,7 @@ def timed(r, jobs, kernel, steps, warmup, torch, abi, label):
         agg["de_total"] += stats.de_evals
         agg["rays"] += stats.rays
         agg["hits"] += stats.hits
+        agg["rounds"] += stats.march_wave_rounds
         log("%s step %d: %.1f ms device, %d DE evals" % (label, i, stats.ms_total, stats.de_evals))
     r.sync()
     torch.cuda.synchronize()
@@ -244,7 +245,8 @@ def main():
                      "frac_unpacked_peak": round(achieved / FP32_PEAK_UNPACKED_TFLOPS, 4),
                      "kernel": "k_march<true> (persistent march pass)",
                      "flops_per_de": round(f_march, 2), "de_evals_per_launch": pers["de_march"] // steps,
-                     "ms_per_launch": round(pers["ms_march"] / steps, 3)},
+                     "ms_per_launch": round(pers["ms_march"] / steps, 3),
+                     "lane_occupancy": round(pers["de_march"] / (64.0 * pers["rounds"]), 4) if pers["rounds"] else None},
         "de_evals_per_s": round(pers["de_total"] / (pers["ms_total"] / 1e3), 1),
         "hits_per_step": pers["hits"] // steps,
     }

@@ -139,6 +139,8 @@ typedef struct nd_batch_stats {
     float ms_shadow;
     uint32_t kernels;       /* kernel launches issued */
     uint32_t _pad;
+    uint64_t march_wave_rounds;  /* persistent march: wave rounds, each offering one DE slot to each
+                                    of 64 lanes; de_evals_march / (64 x this) is the lane occupancy */
 } nd_batch_stats;
 
 /* ---- context ------------------------------------------------------------------ */

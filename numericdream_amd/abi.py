@@ -129,7 +129,7 @@ class BatchStats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("hits", C.c_uint64), ("de_evals", C.c_uint64),
                 ("de_evals_march", C.c_uint64), ("de_evals_surface", C.c_uint64),
                 ("de_evals_shadow", C.c_uint64), ("ms_total", F), ("ms_march", F), ("ms_surface", F),
-                ("ms_shadow", F), ("kernels", U32), ("_pad", U32)]
+                ("ms_shadow", F), ("kernels", U32), ("_pad", U32), ("march_wave_rounds", C.c_uint64)]
 
 
 class Tuning(C.Structure):

@@ -519,6 +519,7 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
             return nd_hip_fail(c, e, "read counters");
         s.hits = dc.hits;
         s.de_evals_march = dc.de_march;
+        s.march_wave_rounds = dc.march_rounds;
         s.de_evals_surface = dc.de_surface;
         s.de_evals_shadow = dc.de_shadow;
         s.de_evals = dc.de_march + dc.de_surface + dc.de_shadow;

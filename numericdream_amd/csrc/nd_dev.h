@@ -92,7 +92,7 @@ struct DevCounters {
     unsigned long long de_march;
     unsigned long long de_surface;
     unsigned long long de_shadow;
-    unsigned long long rays_done;
+    unsigned long long march_rounds;  // wave rounds of the persistent march (one DE slot per lane each)
     unsigned long long hits;         // hits appended (the hit list also holds empty slots)
     unsigned long long clock_start;  // wall clock (s_memrealtime) when the batch started
     uint32_t rounds_capped;  // waves that stopped at the round cap

@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(64) k_plain(const DevJob* __restrict__ jobp, D
 __global__ void k_zero_counters(DevCounters* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         c->ray_next = 0; c->hit_count = 0; c->surf_next = 0; c->shadow_next = 0;
-        c->de_march = 0; c->de_surface = 0; c->de_shadow = 0; c->rays_done = 0; c->hits = 0;
+        c->de_march = 0; c->de_surface = 0; c->de_shadow = 0; c->march_rounds = 0; c->hits = 0;
         c->rounds_capped = 0;
         c->clock_start = wall_clock64();
     }
@@ -165,6 +165,7 @@ NDD uint64_t lanemask_lt() {
 }
 NDD bool lane_bit(uint64_t m) { return (m >> __lane_id()) & 1ull; }
 NDD uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
 
 struct PassArgs {
     const DevJob* jobs;
@@ -213,6 +214,7 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
     march_init(m);
     MarchParams mp = {};
     unsigned long long de = 0, nhits = 0;
+    uint32_t rounds_used = 0;  // rounds that evaluated at least one DE
     for (uint32_t round = 0;; ++round) {
         if (work_left) {
             uint64_t need = __builtin_amdgcn_ballot_w64(!active);
@@ -263,6 +265,7 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
             continue;
         }
         de += (unsigned long long)__popcll(act);
+        ++rounds_used;
         bool hit_now = false;
         float dist = 0.f;
         if (active) {
@@ -320,6 +323,7 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
     for (uint32_t i = hnext + __lane_id(); i < hend; i += 64) a.hb.ray[i] = ~0u;
     if (a.ray_done && lat_ray != ~0u) a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
     if (__lane_id() == 0) {
+        atomicAdd(&a.ctr->march_rounds, (unsigned long long)rounds_used);
         atomicAdd(&a.ctr->de_march, de);
         atomicAdd(&a.ctr->hits, nhits);
     }

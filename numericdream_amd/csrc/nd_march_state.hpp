@@ -29,10 +29,43 @@ NDD MarchParams march_params(const DevJob& J) {
     p.max_steps = J.max_steps; p.bisect_budget = J.bisect_budget; p.refine_budget = J.refine_budget; p.hq = J.hq;
     return p;
 }
+// march_update without probe refinement (refine_budget == 0: the export settings
+// high_quality = 1, refine_hq = 20 give plain sphere tracing, then bisection of
+// [t_prev, t] on the first d < eps). Same results as the general update below for that
+// case, with less state: a probe's hi is t + d and becomes the next t; the sample budget
+// max_steps + bisect_budget cannot be reached before max_steps probes or the bisection
+// budget end the ray, so n is not tracked.
+template <class MP>
+NDD int march_update_norefine(March& m, const MP& J, float s, float d, float* dist) {
+    m.evals++;
+    const float eps = J.eps_c * s + J.eps_f;
+    const bool below = d < eps;
+    if (m.bisecting) {
+        if (below) m.hi = s; else m.lo = s;
+        m.refines++;
+        if (m.hi - m.lo <= eps || m.refines >= J.bisect_budget) { *dist = m.hi; return MS_HIT; }
+        return MS_CONT;
+    }
+    if (below) {
+        if (J.bisect_budget > 0 && s - m.lo > eps) {
+            m.bisecting = 1; m.probing = 0; m.hi = s; m.refines = 0;
+            return MS_CONT;
+        }
+        *dist = s;
+        return MS_HIT;
+    }
+    m.lo = m.t;
+    m.t = m.t + d;
+    m.i++;
+    if (m.t > J.max_dist || m.i >= J.max_steps) { *dist = m.t; return MS_MISS; }
+    return MS_CONT;
+}
+
 // One step of march_to's loop after the DE d at sample_t s. *dist receives the result.
 // J is a DevJob (plain kernel, wave-uniform) or MarchParams (persistent, per lane).
 template <class MP>
 NDD int march_update(March& m, const MP& J, float s, float d, float* dist) {
+    if (J.refine_budget == 0) return march_update_norefine(m, J, s, d, dist);
     const int max_samples = J.max_steps * (1 + J.refine_budget) + J.bisect_budget;
     m.n++;
     m.evals++;
