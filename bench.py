@@ -138,13 +138,29 @@ def mc_leg(r, nd, spp=64):
     W, H = int(u.resolution[0]), int(u.resolution[1])
     import torch
     col = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
-    r.path_trace(u, 1, color=col)  # warmup
+    r.path_trace(u, 1, color=col)  # warmup (captures the per-chunk graph)
     r.sync()
     t0 = time.perf_counter()
     _, _, _, st = r.path_trace(u, spp, color=col)
     r.sync()
     wall = time.perf_counter() - t0
     log("c3 mc: %.1f ms for %d spp, %d segments" % (wall * 1e3, spp, st.segments))
+    # progressive rendering: one call per sample, as an interactive host issues them; the
+    # chunk's passes come from the captured hipGraph or are launched one by one
+    per_sample = {}
+    t = r.tuning()
+    for name, graphs in (("graph", True), ("launches", False)):
+        r.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays, graphs=graphs)
+        r.path_trace(u, 1, color=col)
+        r.sync()
+        t0 = time.perf_counter()
+        for i in range(8):
+            u.mc_sample = float(i)
+            r.path_trace(u, 1, color=col)
+        r.sync()
+        per_sample[name] = round((time.perf_counter() - t0) / 8 * 1e3, 3)
+    r.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays)
+    u.mc_sample = 0.0
     f_de = flops_table()["workloads"]["3"]["march"]["flops_per_de"]
     tflops = f_de * st.de_evals / (st.ms_total / 1e3) / 1e12 if st.ms_total > 0 else 0.0
     return {"tflops": round(tflops, 3), "frac_fp32_peak": round(tflops / FP32_PEAK_TFLOPS, 4),
@@ -152,7 +168,8 @@ def mc_leg(r, nd, spp=64):
             "device_ms": round(st.ms_total, 2), "samples_per_s": round(st.paths / wall, 1),
             "msamples_per_s": round(st.paths / wall / 1e6, 3), "segments_per_s": round(st.segments / wall, 1),
             "segments_per_path": round(st.segments / st.paths, 3), "shadow_rays": int(st.shadow_rays),
-            "de_evals_per_s": round(st.de_evals / wall, 1), "de_evals_per_path": round(st.de_evals / st.paths, 1)}
+            "de_evals_per_s": round(st.de_evals / wall, 1), "de_evals_per_path": round(st.de_evals / st.paths, 1),
+            "ms_per_1spp_call": per_sample}
 
 
 def cpu_baseline(nd, target_s=40.0):

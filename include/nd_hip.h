@@ -175,6 +175,8 @@ nd_status nd_batch_job_latency(nd_ctx* ctx, float* ms_out, uint32_t n_jobs);
 
 /* ---- autotuner ----------------------------------------------------------------- */
 #define ND_TUNE_TRACK_LATENCY 1u  /* record per-job completion times (nd_batch_job_latency) */
+#define ND_TUNE_NO_GRAPH 2u       /* nd_path_trace: launch each chunk's passes one by one instead of
+                                     replaying the captured hipGraph */
 
 typedef struct nd_tuning {
     uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups; 0 = twice the

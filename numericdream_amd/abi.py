@@ -137,6 +137,7 @@ class Tuning(C.Structure):
 
 
 TUNE_TRACK_LATENCY = 1
+TUNE_NO_GRAPH = 2
 
 
 class AdjointIO(C.Structure):

@@ -282,6 +282,7 @@ void nd_ctx_destroy(nd_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->mc_graph_exec) (void)hipGraphExecDestroy(c->mc_graph_exec);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     c->jobs.release(); c->job_base.release(); c->ctr.release(); c->hits.release(); c->shared.release();
     c->staging.release(); c->latency.release(); c->adj.release(); c->mc.release();
@@ -327,7 +328,7 @@ nd_status nd_set_tuning(nd_ctx* c, const nd_tuning* t) {
     if (t->refill_min_idle < 1 || t->refill_min_idle > 64) return nd_fail(c, ND_E_INVALID, "refill_min_idle must be 1..64");
     if (t->chunk_rays < 1 || t->chunk_rays > (1u << 20)) return nd_fail(c, ND_E_INVALID, "chunk_rays must be 1..2^20");
     if (t->waves_per_cu > 64) return nd_fail(c, ND_E_INVALID, "waves_per_cu must be 0..64");
-    if (t->flags & ~ND_TUNE_TRACK_LATENCY) return nd_fail(c, ND_E_INVALID, "unknown tuning flags 0x%x", t->flags);
+    if (t->flags & ~(ND_TUNE_TRACK_LATENCY | ND_TUNE_NO_GRAPH)) return nd_fail(c, ND_E_INVALID, "unknown tuning flags 0x%x", t->flags);
     c->tuning = *t;
     return ND_OK;
 }

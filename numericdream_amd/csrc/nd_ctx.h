@@ -28,8 +28,10 @@ struct nd_ctx {
     uint32_t round_cap = 0xffffffffu;
     const volatile int* cancel = nullptr;
     hipEvent_t ev[8] = {};
-    bool graph_valid = false;
+    bool graph_valid = false;          // false after any device buffer was reallocated
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t mc_graph_exec = nullptr;         // the path trace's per-chunk passes
+    alignas(8) unsigned char mc_graph_key[1024] = {};  // what that graph baked in
     DevBuf jobs, job_base, ctr, hits, shared, staging, latency, adj, mc;
     bool last_shared = false;       // scene of the last batch was shared by all jobs
     uint32_t last_grid = 0;

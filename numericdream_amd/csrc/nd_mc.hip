@@ -295,20 +295,27 @@ struct McPaths {
 enum { C_LIVE_IN = 0, C_LIVE_OUT = 1, C_SHADOW = 2, C_IND = 3, C_HEAD_EXT = 4, C_HEAD_SH = 5, C_HEAD_IND = 6,
        C_SEG = 10, C_SHTOT = 11, C_DE = 12 };
 
+// Per-chunk values, read from device memory so that one captured graph of a chunk's
+// passes serves every chunk and every call of the same shape.
+constexpr uint32_t MC_MAX_CHUNKS = 64;  // chunk records uploaded per batch of chunks
+struct McChunk {
+    uint32_t sample_base;  // first sample index of the chunk
+    uint32_t spp_chunk;    // samples in the chunk
+    uint32_t spp_total;    // samples of the call (resolve scales by 1 / spp_total)
+    float mc_sample0;      // mc_sample of sample 0 (Uniforms.mc_sample)
+};
+
 struct McArgs {
     const DevJob* job;
     const DevScene* scene;
     const McScene* S;
+    const McChunk* ch;
     McPaths P;
     uint32_t npix;
-    uint32_t sample_base;
-    uint32_t spp_chunk;
-    float mc_sample0;
-    uint32_t chunk;
+    uint32_t chunk;  // claim size of the persistent passes
     float* out_rgb;
     float* out_depth;
     float* out_normal;
-    uint32_t spp_total;
 };
 
 NDD unsigned long long* de_ctr(const McArgs& a) { return (unsigned long long*)(a.P.ctr + C_DE); }
@@ -329,14 +336,14 @@ NDD uint32_t append(uint32_t* ctr, bool want, uint32_t* slot) {
 // --- generate: camera paths of samples [sample_base, sample_base + spp_chunk)
 __global__ void k_mc_generate(McArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = a.npix * a.spp_chunk;
+    const uint32_t n = a.npix * a.ch->spp_chunk;
     if (p >= n) return;
     const DevJob& J = *a.job;
-    const uint32_t pix = p % a.npix, s = a.sample_base + p / a.npix;
+    const uint32_t pix = p % a.npix, s = a.ch->sample_base + p / a.npix;
     const int x = (int)(pix % (uint32_t)J.rect_w), y = (int)(pix / (uint32_t)J.rect_w);
     const int gx = J.rect_x + x, gy = J.rect_y + y;
     const V3<float> d = primary_dir(J, gx, gy);
-    const float ms = __fadd_rn(a.mc_sample0, (float)s);
+    const float ms = __fadd_rn(a.ch->mc_sample0, (float)s);
     // base seed = frag + mc_sample * (91.73, 57.31) (fs_main, template.wgsl:4818)
     const float bx = mad_rn(ms, 91.73f, (float)gx + 0.5f);
     const float by = mad_rn(ms, 57.31f, (float)gy + 0.5f);
@@ -526,7 +533,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                 const V3<float> sky = sky_color(S, dir);
                 const Sp8 sk = from_rgb(B, sky.x, sky.y, sky.z);
                 for (int k = 0; k < 8; ++k) rad.v[k] += thr.v[k] * sk.v[k];
-                if (depth == 0 && a.sample_base + p / a.npix == 0) {
+                if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
                     if (a.out_depth) a.out_depth[pix] = 0.f;
                     if (a.out_normal) { a.out_normal[3 * pix] = 0.f; a.out_normal[3 * pix + 1] = 0.f; a.out_normal[3 * pix + 2] = 0.f; }
                 }
@@ -537,7 +544,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                 const V3<float> nrm = normal_tetra(sc, hp, eps);
                 const Mat mat = hit_material(sc, S, hp);
                 de += 5;
-                if (depth == 0 && a.sample_base + p / a.npix == 0) {
+                if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
                     if (a.out_depth) a.out_depth[pix] = t;
                     if (a.out_normal) { a.out_normal[3 * pix] = nrm.x; a.out_normal[3 * pix + 1] = nrm.y; a.out_normal[3 * pix + 2] = nrm.z; }
                 }
@@ -740,7 +747,8 @@ __global__ void k_mc_resolve(McArgs a) {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= a.npix) return;
     float acc[3] = {0.f, 0.f, 0.f};
-    for (uint32_t s = 0; s < a.spp_chunk; ++s) {
+    const uint32_t cs = a.ch->spp_chunk;
+    for (uint32_t s = 0; s < cs; ++s) {
         const uint32_t p = s * a.npix + pix;
         Sp8 r;
         for (int k = 0; k < 8; ++k) r.v[k] = a.P.rad[8 * p + k];
@@ -748,15 +756,17 @@ __global__ void k_mc_resolve(McArgs a) {
         to_rgb_fit(a.P.hero[p], r, rgb);
         acc[0] += rgb[0]; acc[1] += rgb[1]; acc[2] += rgb[2];
     }
-    const float inv = 1.f / (float)a.spp_total;
+    const float inv = 1.f / (float)a.ch->spp_total;
     for (int k = 0; k < 3; ++k) a.out_rgb[3 * pix + k] += acc[k] * inv;
 }
 
 // counters between passes (no host round trip)
 __global__ void k_mc_counters(uint32_t* ctr, int op) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (op == 0) {  // new chunk: all queues empty, the initial live count is set by the host
+    if (op == 3) {  // new call: totals too
         for (int i = 0; i < 16; ++i) ctr[i] = 0;
+    } else if (op == 0) {  // new chunk: all queues empty (the totals C_SEG.. are kept)
+        for (int i = 0; i < C_SEG; ++i) ctr[i] = 0;
     } else if (op == 1) {  // before a bounce: clear the out queue and the side queues
         ctr[C_LIVE_OUT] = 0; ctr[C_SHADOW] = 0; ctr[C_IND] = 0;
         ctr[C_HEAD_EXT] = 0; ctr[C_HEAD_SH] = 0; ctr[C_HEAD_IND] = 0;
@@ -766,8 +776,12 @@ __global__ void k_mc_counters(uint32_t* ctr, int op) {
         ctr[C_LIVE_IN] = ctr[C_LIVE_OUT];
     }
 }
-__global__ void k_mc_set_live(uint32_t* ctr, uint32_t n) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[C_LIVE_IN] = n;
+__global__ void k_mc_set_live(uint32_t* ctr, const McChunk* ch, uint32_t npix) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[C_LIVE_IN] = npix * ch->spp_chunk;
+}
+// Copies chunk record k into the current-chunk slot that the captured passes read.
+__global__ void k_mc_select_chunk(McChunk* recs, uint32_t k) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) recs[MC_MAX_CHUNKS] = recs[k];
 }
 __global__ void k_mc_zero(float* p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0.f;
@@ -880,6 +894,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     sz(4 * SQ); for (int i = 0; i < 8; ++i) sz(4 * SQ); sz(32 * SQ);
     sz(4 * P); for (int i = 0; i < 7; ++i) sz(4 * P); sz(P); sz(32 * P);
     sz(256);
+    sz(sizeof(nd::McChunk) * (nd::MC_MAX_CHUNKS + 1));
     const bool dev_out = out->device_ptrs != 0;
     if (!dev_out) { sz(12 * (size_t)npix); sz(4 * (size_t)npix); sz(12 * (size_t)npix); }
     if ((st = c->mc.ensure(c, bytes)) != ND_OK) return st;
@@ -910,6 +925,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     Pp.ihit = (uint8_t*)carve(P);
     Pp.iw = (float*)carve(32 * P);
     Pp.ctr = (uint32_t*)carve(256);
+    nd::McChunk* d_chunks = (nd::McChunk*)carve(sizeof(nd::McChunk) * (nd::MC_MAX_CHUNKS + 1));
     float* d_rgb = dev_out ? out->color : (float*)carve(12 * (size_t)npix);
     float* d_depth = dev_out ? out->depth : (out->depth ? (float*)carve(4 * (size_t)npix) : nullptr);
     float* d_normal = dev_out ? out->normal : (out->normal ? (float*)carve(12 * (size_t)npix) : nullptr);
@@ -918,49 +934,102 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     if ((e = hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload job");
     if ((e = hipMemcpyAsync(d_scene, &job.scene, sizeof(DevScene), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload scene");
     if ((e = hipMemcpyAsync(d_S, &S, sizeof(S), hipMemcpyHostToDevice, s)) != hipSuccess) return nd_hip_fail(c, e, "upload materials");
-    a.job = d_job; a.scene = d_scene; a.S = d_S;
-    a.npix = npix; a.mc_sample0 = u.mc_sample; a.chunk = 64;  // bounce queues are short: small claims balance better
-    a.out_rgb = d_rgb; a.out_depth = d_depth; a.out_normal = d_normal; a.spp_total = spp;
+    a.job = d_job; a.scene = d_scene; a.S = d_S; a.ch = d_chunks + nd::MC_MAX_CHUNKS;
+    a.npix = npix;
+    a.chunk = 64;  // bounce queues are short: small claims balance better
+    a.out_rgb = d_rgb; a.out_depth = d_depth; a.out_normal = d_normal;
     const uint32_t grid = nd_persistent_grid(c);
-    hipLaunchKernelGGL(nd::k_mc_zero, dim3(1024), dim3(256), 0, s, d_rgb, (uint64_t)3 * npix);
-    (void)hipEventRecord(c->ev[6], s);
-    int launches = 1;
-    unsigned long long segs_total = 0, de_total = 0, shadows_total = 0;
-    for (uint32_t sb = 0; sb < spp; sb += chunk) {
-        const uint32_t cs = std::min(chunk, spp - sb);
-        a.sample_base = sb;
-        a.spp_chunk = cs;
-        const uint32_t n0 = npix * cs;
-        hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 0);
-        hipLaunchKernelGGL(nd::k_mc_set_live, dim3(1), dim3(64), 0, s, Pp.ctr, n0);
-        hipLaunchKernelGGL(nd::k_mc_generate, dim3((n0 + 255) / 256), dim3(256), 0, s, a);
-        launches += 3;
+    const int max_depth = S.max_depth;
+    const bool indirect = S.indirect_on != 0;
+    // The passes of one chunk, in stream order. Captured once into a graph per shape and
+    // replayed for every chunk (the chunk's sample range is read from a.ch).
+    auto enqueue_chunk = [&](hipStream_t st) {
+        hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 0);
+        hipLaunchKernelGGL(nd::k_mc_set_live, dim3(1), dim3(64), 0, st, Pp.ctr, (const nd::McChunk*)a.ch, npix);
+        hipLaunchKernelGGL(nd::k_mc_generate, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, st, a);
         int ping = 0;
-        for (int depth = 0; depth < S.max_depth; ++depth) {
-            hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 1);
-            hipLaunchKernelGGL(nd::k_mc_extend, dim3(grid), dim3(64), 0, s, a, ping);
-            hipLaunchKernelGGL(nd::k_mc_shade, dim3(grid), dim3(64), 0, s, a, ping);
-            hipLaunchKernelGGL(nd::k_mc_shadow, dim3(grid), dim3(64), 0, s, a);
-            launches += 4;
-            if (depth == 0 && S.indirect_on) {
-                hipLaunchKernelGGL(nd::k_mc_indirect_march, dim3(grid), dim3(64), 0, s, a);
-                hipLaunchKernelGGL(nd::k_mc_indirect_shade, dim3(grid), dim3(64), 0, s, a);
-                launches += 2;
+        for (int depth = 0; depth < max_depth; ++depth) {
+            hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 1);
+            hipLaunchKernelGGL(nd::k_mc_extend, dim3(grid), dim3(64), 0, st, a, ping);
+            hipLaunchKernelGGL(nd::k_mc_shade, dim3(grid), dim3(64), 0, st, a, ping);
+            hipLaunchKernelGGL(nd::k_mc_shadow, dim3(grid), dim3(64), 0, st, a);
+            if (depth == 0 && indirect) {
+                hipLaunchKernelGGL(nd::k_mc_indirect_march, dim3(grid), dim3(64), 0, st, a);
+                hipLaunchKernelGGL(nd::k_mc_indirect_shade, dim3(grid), dim3(64), 0, st, a);
             }
-            hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 2);
-            launches += 1;
+            hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 2);
             ping ^= 1;
         }
-        hipLaunchKernelGGL(nd::k_mc_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, a);
-        launches += 1;
-        if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch path trace");
+        hipLaunchKernelGGL(nd::k_mc_resolve, dim3((npix + 255) / 256), dim3(256), 0, st, a);
+    };
+    const bool use_graph = !(c->tuning.flags & ND_TUNE_NO_GRAPH);
+    if (use_graph) {
+        // everything the captured launches bake in: kernel arguments and launch shapes
+        struct McGraphKey {
+            nd::McArgs args;
+            uint32_t grid;
+            int max_depth, indirect;
+            uint64_t paths;
+        } key;
+        static_assert(sizeof(McGraphKey) <= sizeof(c->mc_graph_key), "graph key storage");
+        std::memset(&key, 0, sizeof(key));
+        key.args = a;
+        key.grid = grid; key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
+        if (!c->mc_graph_exec || !c->graph_valid || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
+            if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }
+            hipGraph_t g = nullptr;
+            if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return nd_hip_fail(c, e, "begin graph capture");
+            enqueue_chunk(s);
+            hipError_t le = hipGetLastError();
+            if ((e = hipStreamEndCapture(s, &g)) != hipSuccess) return nd_hip_fail(c, e, "end graph capture");
+            if (le != hipSuccess) { (void)hipGraphDestroy(g); return nd_hip_fail(c, le, "capture path-trace passes"); }
+            e = hipGraphInstantiate(&c->mc_graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { c->mc_graph_exec = nullptr; return nd_hip_fail(c, e, "instantiate path-trace graph"); }
+            std::memcpy(c->mc_graph_key, &key, sizeof(key));
+            c->graph_valid = true;
+        }
+    }
+    // chunk records, uploaded MC_MAX_CHUNKS at a time
+    const uint32_t n_chunks = (spp + chunk - 1) / chunk;
+    std::vector<nd::McChunk> hc(n_chunks);
+    for (uint32_t k = 0; k < n_chunks; ++k) {
+        hc[k].sample_base = k * chunk;
+        hc[k].spp_chunk = std::min(chunk, spp - k * chunk);
+        hc[k].spp_total = spp;
+        hc[k].mc_sample0 = u.mc_sample;
+    }
+    hipLaunchKernelGGL(nd::k_mc_zero, dim3(1024), dim3(256), 0, s, d_rgb, (uint64_t)3 * npix);
+    hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 3);
+    (void)hipEventRecord(c->ev[6], s);
+    for (uint32_t k0 = 0; k0 < n_chunks; k0 += nd::MC_MAX_CHUNKS) {
+        const uint32_t nk = std::min(nd::MC_MAX_CHUNKS, n_chunks - k0);
+        if (k0 > 0 && (e = hipStreamSynchronize(s)) != hipSuccess) return nd_hip_fail(c, e, "path trace execution");
+        if ((e = hipMemcpyAsync(d_chunks, &hc[k0], sizeof(nd::McChunk) * nk, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return nd_hip_fail(c, e, "upload chunk records");
+        for (uint32_t k = 0; k < nk; ++k) {
+            // the passes read chunk k0 + k through a.ch: a one-thread kernel points it there
+            hipLaunchKernelGGL(nd::k_mc_select_chunk, dim3(1), dim3(64), 0, s, d_chunks, k);
+            if (use_graph) {
+                if ((e = hipGraphLaunch(c->mc_graph_exec, s)) != hipSuccess) return nd_hip_fail(c, e, "launch path-trace graph");
+            } else {
+                enqueue_chunk(s);
+            }
+            if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch path trace");
+            if (c->cancel && *c->cancel) {
+                (void)hipStreamSynchronize(s);
+                return nd_fail(c, ND_E_CANCELLED, "cancelled");
+            }
+        }
+    }
+    unsigned long long segs_total = 0, de_total = 0, shadows_total = 0;
+    {
         uint32_t h_ctr[16];
         if ((e = hipMemcpyAsync(h_ctr, Pp.ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, s)) != hipSuccess) return nd_hip_fail(c, e, "read counters");
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return nd_hip_fail(c, e, "path trace execution");
-        segs_total += h_ctr[nd::C_SEG];
-        de_total += *(unsigned long long*)(h_ctr + nd::C_DE);
-        shadows_total += h_ctr[nd::C_SHTOT];
-        if (c->cancel && *c->cancel) return nd_fail(c, ND_E_CANCELLED, "cancelled");
+        segs_total = h_ctr[nd::C_SEG];
+        de_total = *(unsigned long long*)(h_ctr + nd::C_DE);
+        shadows_total = h_ctr[nd::C_SHTOT];
     }
     (void)hipEventRecord(c->ev[7], s);
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return nd_hip_fail(c, e, "path trace execution");
@@ -979,6 +1048,5 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         stats->ms_total = ms;
         stats->max_depth_reached = (uint32_t)S.max_depth;
     }
-    (void)launches;
     return ND_OK;
 }

@@ -113,3 +113,28 @@ def test_unsupported_materials_are_rejected(renderer):
     u.instances[1].colors[3].transparency = 0.5
     with pytest.raises(nd.NdError):
         renderer.path_trace(u, 1)
+
+
+def test_graph_replays_match_eager_launches(renderer):
+    """Each chunk's passes are replayed from a captured hipGraph. Every replay, with other
+    eager work and synchronisations in between, equals the launch-by-launch result; the
+    second case has two chunks of unequal size (8 + 1 samples at 1080p)."""
+    import torch
+    t = renderer.tuning()
+    cases = [(c3(64, 36), 3), (nd.config_uniforms(3, 0), 9)]
+    try:
+        for u, spp in cases:
+            renderer.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays, graphs=False)
+            eager, _, _, st_e = renderer.path_trace(u, spp)
+            renderer.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays, graphs=True)
+            for rep in range(3):
+                g, _, _, st_g = renderer.path_trace(u, spp)
+                assert np.array_equal(g, eager), (spp, rep)
+                assert st_g.segments == st_e.segments and st_g.de_evals == st_e.de_evals
+                # unrelated eager work on the same context and on torch's stream
+                renderer.render_gbuffer(nd.resize(nd.config_uniforms(2, rep), 64, 36), nd.GBuffer(64, 36))
+                x = torch.ones(1 << 20, device="cuda:0")
+                torch.cuda.synchronize()
+                assert float(x.sum()) == float(1 << 20)
+    finally:
+        renderer.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays)
