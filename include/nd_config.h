@@ -50,6 +50,12 @@ void nd_camera_look_at(nd_uniforms* u, const float pos[3], const float target[3]
  * is a smooth function of the parameters on the scale of the finite-difference step. */
 void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg, float fraction);
 
+/* Enables the empty-space accelerator as the host stamps it (stampAccelUniforms,
+ * fr.zig:3073-3080): accel_res, accel_levels (<= 8), accel_safety, and accel_params from
+ * placeCascades around u->camera_pos for u->max_dist (accel.zig:34-58). The app's defaults are
+ * res 64, levels 6, safety 0.92 (accel.zig:13-14, accel_state.zig:19). */
+void nd_accel_setup(nd_uniforms* u, uint32_t res, uint32_t levels, float safety);
+
 #ifdef __cplusplus
 }
 #endif

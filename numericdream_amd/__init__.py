@@ -68,6 +68,8 @@ def _load():
     lib.nd_camera_look_at.restype = None
     lib.nd_set_pixel_epsilon.argtypes = [P(Uniforms), C.c_float, C.c_float]
     lib.nd_set_pixel_epsilon.restype = None
+    lib.nd_accel_setup.argtypes = [P(Uniforms), u32, u32, C.c_float]
+    lib.nd_accel_setup.restype = None
     return lib
 
 
@@ -102,6 +104,13 @@ def camera_look_at(u, pos, target, fov_deg=60.0):
 
 def set_pixel_epsilon(u, fov_deg=60.0, fraction=1e-3):
     lib.nd_set_pixel_epsilon(C.byref(u), fov_deg, fraction)
+
+
+def enable_accel(u, res=64, levels=6, safety=0.92):
+    """Turns on the empty-space accelerator with cascades around u's camera (the app's
+    defaults: 64^3 cells, 6 levels, safety 0.92)."""
+    lib.nd_accel_setup(C.byref(u), res, levels, safety)
+    return u
 
 
 def pixel_fraction(u, fov_deg=60.0):

@@ -1,0 +1,115 @@
+// nd_accel.hip - building the empty-space accelerator grids of a batch (cs_build_accel,
+// template.wgsl:5409-5431; the host side of buildAccel, fr.zig:3172-3212).
+//
+// One grid per distinct (scene, cascades): jobs that share the scene and the camera-centred
+// cascades share the grid. The grids of the last batch stay cached on the device and are
+// reused while every job of a later batch finds its grid there.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "nd_ctx.h"
+#include "nd_de.hpp"
+
+namespace nd {
+
+struct AccelDesc {
+    float prm[ND_MAX_CASCADES][4];
+    int res, levels;
+    float safety;
+};
+
+// One thread per voxel, voxels in [level][z][y][x] order: the safe skip distance
+// max(DE(centre) - HALF_SQRT3 cell, 0) * safety. The centre and the bound are formed with
+// separate roundings (no fused multiply-add), as the scalar restatement forms them.
+__global__ void k_accel_build(const DevScene* __restrict__ sc, float* __restrict__ grid, AccelDesc d, uint32_t n) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const uint32_t res = (uint32_t)d.res;
+    const uint32_t x = idx % res, y = (idx / res) % res, gz = idx / (res * res);
+    const uint32_t level = gz / res, cz = gz % res;
+    const float cell = d.prm[level][3];
+    const V3<float> c = mk3(__fadd_rn(d.prm[level][0], __fmul_rn((float)x + 0.5f, cell)),
+                            __fadd_rn(d.prm[level][1], __fmul_rn((float)y + 0.5f, cell)),
+                            __fadd_rn(d.prm[level][2], __fmul_rn((float)cz + 0.5f, cell)));
+    const float de = scene_de<false>(*sc, c, nullptr);
+    grid[idx] = __fmul_rn(fmaxf(__fsub_rn(de, __fmul_rn(0.8660254f, cell)), 0.f), d.safety);
+}
+
+}  // namespace nd
+
+static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+static uint64_t accel_key(const DevJob& j) {
+    uint64_t h = 1469598103934665603ull;
+    h = fnv1a(h, &j.scene, sizeof(DevScene));
+    h = fnv1a(h, &j.accel_res, sizeof(int));
+    h = fnv1a(h, &j.accel_levels, sizeof(int));
+    h = fnv1a(h, &j.accel_safety, sizeof(float));
+    h = fnv1a(h, j.accel_prm, sizeof(j.accel_prm));
+    return h;
+}
+
+nd_status nd_accel_prepare(nd_ctx* c, DevJob* jobs, uint32_t n) {
+    std::vector<uint64_t> keys(n, 0);
+    std::vector<uint32_t> uniq_first;
+    std::vector<uint64_t> uniq;
+    for (uint32_t i = 0; i < n; ++i) {
+        jobs[i].accel = nullptr;
+        if (jobs[i].accel_levels <= 0) continue;
+        keys[i] = accel_key(jobs[i]);
+        bool seen = false;
+        for (uint64_t k : uniq) seen = seen || k == keys[i];
+        if (!seen) { uniq.push_back(keys[i]); uniq_first.push_back(i); }
+    }
+    if (uniq.empty()) return ND_OK;
+    auto find = [&](uint64_t k) -> const NdAccelEntry* {
+        for (const NdAccelEntry& e : c->accel_cache)
+            if (e.key == k) return &e;
+        return nullptr;
+    };
+    bool cached = true;
+    for (uint64_t k : uniq) cached = cached && find(k) != nullptr;
+    if (!cached) {
+        size_t total = 0;
+        for (uint32_t f : uniq_first) {
+            const size_t r = (size_t)jobs[f].accel_res;
+            total += r * r * r * (size_t)jobs[f].accel_levels;
+        }
+        if (total > ((size_t)1 << 31)) return nd_fail(c, ND_E_INVALID, "accelerator grids of %zu cells exceed 2^31", total);
+        c->accel_cache.clear();
+        nd_status st = c->accel.ensure(c, total * sizeof(float));
+        if (st != ND_OK) return st;
+        if ((st = c->accel_scenes.ensure(c, uniq.size() * sizeof(DevScene))) != ND_OK) return st;
+        c->accel_scene_host.resize(uniq.size());
+        for (size_t u = 0; u < uniq.size(); ++u) c->accel_scene_host[u] = jobs[uniq_first[u]].scene;
+        hipError_t e = hipMemcpyAsync(c->accel_scenes.p, c->accel_scene_host.data(), uniq.size() * sizeof(DevScene),
+                                      hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return nd_hip_fail(c, e, "upload accelerator scenes");
+        size_t off = 0;
+        for (size_t u = 0; u < uniq.size(); ++u) {
+            const DevJob& j = jobs[uniq_first[u]];
+            nd::AccelDesc d;
+            std::memcpy(d.prm, j.accel_prm, sizeof(d.prm));
+            d.res = j.accel_res;
+            d.levels = j.accel_levels;
+            d.safety = j.accel_safety;
+            const size_t cnt = (size_t)d.res * d.res * d.res * d.levels;
+            hipLaunchKernelGGL(nd::k_accel_build, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, c->stream,
+                               (const DevScene*)c->accel_scenes.p + u, (float*)c->accel.p + off, d, (uint32_t)cnt);
+            if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch accelerator build");
+            c->accel_cache.push_back(NdAccelEntry{uniq[u], off, cnt});
+            off += cnt;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (jobs[i].accel_levels <= 0) continue;
+        jobs[i].accel = (const float*)c->accel.p + find(keys[i])->offset;
+    }
+    return ND_OK;
+}

@@ -122,7 +122,6 @@ static int formula_iters(int f, const float* p) {
 
 nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
     std::memset(&s, 0, sizeof(s));
-    if (u.accel_enabled > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "the empty-space accelerator (accel_enabled) is not implemented");
     if (u.warp_count > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "domain warps (warp_count) are not implemented");
     if (u.fft_active > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "the FFT view (fft_active) is not implemented");
     if (((uint32_t)u.approx_flags & (1u << 8)) != 0u)
@@ -239,6 +238,20 @@ nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, n
         j.shadow_steps = o.shadow_max_steps ? (int)o.shadow_max_steps : std::min(i32(u.max_steps), 40);
     }
     j.normal_mode = (int)o.normal_mode;
+    // empty-space accelerator: the grid is built by nd_accel_prepare
+    j.accel = nullptr;
+    j.accel_res = 0;
+    j.accel_levels = 0;
+    if (u.accel_enabled > 0.5f) {
+        const int res = i32(u.accel_res), levels = std::min(i32(u.accel_levels), ND_MAX_CASCADES);
+        if (res < 1 || res > 1024 || levels < 1)
+            return nd_fail(c, ND_E_INVALID, "accelerator of %d^3 cells x %d levels", res, levels);
+        j.accel_res = res;
+        j.accel_levels = levels;
+        j.accel_safety = u.accel_safety;
+        for (int l = 0; l < ND_MAX_CASCADES; ++l)
+            for (int k = 0; k < 4; ++k) j.accel_prm[l][k] = u.accel_params[l][k];
+    }
     return ND_OK;
 }
 
@@ -286,6 +299,7 @@ void nd_ctx_destroy(nd_ctx* c) {
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     c->jobs.release(); c->job_base.release(); c->ctr.release(); c->hits.release(); c->shared.release();
     c->staging.release(); c->latency.release(); c->adj.release(); c->mc.release();
+    c->accel.release(); c->accel_scenes.release();
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -434,6 +448,7 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     bool shared = true;
     for (uint32_t i = 1; i < n && shared; ++i) shared = std::memcmp(&hj[i].scene, &hj[0].scene, sizeof(DevScene)) == 0;
     c->last_shared = shared;
+    if ((st = nd_accel_prepare(c, hj.data(), n)) != ND_OK) return st;
 
     if ((st = c->jobs.ensure(c, sizeof(DevJob) * n)) != ND_OK) return st;
     if ((st = c->job_base.ensure(c, sizeof(uint32_t) * n)) != ND_OK) return st;

@@ -4,6 +4,7 @@
 // defaults of the Zig extern structs (src/gpu/fractal_renderer.zig:85-363).
 #include "nd_config.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -228,6 +229,26 @@ void nd_camera_look_at(nd_uniforms* u, const float pos[3], const float target[3]
         u->camera_right[i] = (float)(r[i] * k);
         u->camera_up[i] = (float)(up[i] * k);
     }
+}
+
+void nd_accel_setup(nd_uniforms* u, uint32_t res, uint32_t levels, float safety) {
+    // placeCascades (accel.zig:34-58) in the same f32 arithmetic
+    const int n = std::min(std::max((int)levels, 1), ND_MAX_CASCADES);
+    const float res_f = (float)std::max(res, 1u);
+    const float coarsest = std::max(2.0f * u->max_dist / res_f, 1e-6f);
+    for (int l = 0; l < ND_MAX_CASCADES; ++l)
+        for (int k = 0; k < 4; ++k) u->accel_params[l][k] = 0.0f;
+    for (int l = 0; l < n; ++l) {
+        const float cell = coarsest / (float)(1u << (n - 1 - l));
+        const float half = 0.5f * res_f * cell;
+        for (int a = 0; a < 3; ++a) u->accel_params[l][a] = std::floor((u->camera_pos[a] - half) / cell) * cell;
+        u->accel_params[l][3] = cell;
+    }
+    u->accel_enabled = 1.0f;
+    u->accel_levels = (float)n;
+    u->accel_res = (float)res;
+    u->accel_safety = safety;
+    u->accel_z_offset = 0.0f;
 }
 
 void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg, float fraction) {

@@ -15,6 +15,13 @@ struct DevBuf {
     void release();
 };
 
+// A cached accelerator grid: its key (scene + cascades) and place in nd_ctx::accel.
+struct NdAccelEntry {
+    uint64_t key;
+    size_t offset;  // in floats
+    size_t count;
+};
+
 struct nd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -33,6 +40,9 @@ struct nd_ctx {
     hipGraphExec_t mc_graph_exec = nullptr;         // the path trace's per-chunk passes
     alignas(8) unsigned char mc_graph_key[1024] = {};  // what that graph baked in
     DevBuf jobs, job_base, ctr, hits, shared, staging, latency, adj, mc;
+    DevBuf accel, accel_scenes;              // accelerator grids of the last batch, their scenes
+    std::vector<NdAccelEntry> accel_cache;
+    std::vector<DevScene> accel_scene_host;
     bool last_shared = false;       // scene of the last batch was shared by all jobs
     uint32_t last_grid = 0;
     DevHitBuffers last_hb{};        // hit list of the last persistent batch
@@ -52,4 +62,7 @@ nd_status nd_fail(nd_ctx* c, nd_status st, const char* fmt, ...);
 nd_status nd_hip_fail(nd_ctx* c, hipError_t e, const char* what);
 nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s);
 nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, nd_rect rect, DevJob& j);
+// Builds (or finds cached) the accelerator grids of the jobs that enable it and points their
+// accel fields at them; launches on the context stream.
+nd_status nd_accel_prepare(nd_ctx* c, DevJob* jobs, uint32_t n);
 uint32_t nd_persistent_grid(nd_ctx* c);

@@ -7,6 +7,8 @@
 
 #include <stdint.h>
 
+#include "nd_uniforms.h"
+
 #define ND_DEV_MAX_INST 4
 #define ND_DEV_MAX_CARVES 16
 
@@ -74,6 +76,12 @@ struct DevJob {
     float light_k;         // penumbra k = mix(softness, 512, hard)
     float light_far;       // 2 * max_dist (global light distance)
     float _pad2;
+    // empty-space accelerator (template.wgsl:426-447): grid [level][z][y][x] of safe skip
+    // distances, or null; cascade corners and cells from the Uniforms
+    const float* accel;
+    int accel_res, accel_levels;
+    float accel_safety, _pad3;
+    float accel_prm[ND_MAX_CASCADES][4];
     float* out_depth;
     float* out_normal;
     float* out_pos;

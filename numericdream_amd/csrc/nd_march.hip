@@ -13,6 +13,7 @@
 // (:1781-1804), visibility_march / shadow_factor (:2282-2375), frag_out (:3762-3774).
 #include <hip/hip_runtime.h>
 
+#include "nd_accel.hpp"
 #include "nd_de.hpp"
 #include "nd_internal.h"
 #include "nd_march_state.hpp"
@@ -83,7 +84,12 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
     march_init(m);
     float dist = 0.f;
     int st;
+    const bool acc = J.accel != nullptr;
     do {
+        if (acc) {
+            st = march_skips(m, J, [&](float ts) { return accel_skip(J, mk3(org.x + dir.x * ts, org.y + dir.y * ts, org.z + dir.z * ts)); }, &dist);
+            if (st != MS_CONT) break;
+        }
         float s = march_sample_t(m);
         float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
         st = march_update(m, J, s, d, &dist);
@@ -116,6 +122,7 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
             float t = 0.004f, res = 1.f;
             for (int i = 0; i < J.shadow_steps; ++i) {
                 if (t >= ldist) break;
+                if (J.accel && shadow_skips(J, so, ld, J.light_k, res, ldist, J.shadow_steps, &t, &i)) break;
                 float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
                 evals++;
                 if (d < 0.0005f) { res = 0.f; break; }
@@ -213,6 +220,7 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
     March m;
     march_init(m);
     MarchParams mp = {};
+    bool acc = false;  // the ray's job uses the accelerator
     unsigned long long de = 0, nhits = 0;
     uint32_t rounds_used = 0;  // rounds that evaluated at least one DE
     for (uint32_t round = 0;; ++round) {
@@ -250,6 +258,7 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
                             dir = primary_dir(CJ, CJ.rect_x + px, CJ.rect_y + py);
                             org = mk3(CJ.cam[0], CJ.cam[1], CJ.cam[2]);
                             mp = march_params(CJ);
+                            acc = CJ.accel != nullptr;
                             march_init(m);
                             active = true;
                         }
@@ -264,15 +273,22 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
             if (!work_left) break;
             continue;
         }
-        de += (unsigned long long)__popcll(act);
         ++rounds_used;
         bool hit_now = false;
         float dist = 0.f;
+        int st = MS_CONT;
+        if (active && acc) {
+            const DevJob& AJ = jobs[job];
+            st = march_skips(m, mp, [&](float ts) { return accel_skip(AJ, mk3(org.x + dir.x * ts, org.y + dir.y * ts, org.z + dir.z * ts)); }, &dist);
+        }
+        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && st == MS_CONT));
         if (active) {
-            const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
-            const float s = march_sample_t(m);
-            const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
-            const int st = march_update(m, mp, s, d, &dist);
+            if (st == MS_CONT) {
+                const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
+                const float s = march_sample_t(m);
+                const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
+                st = march_update(m, mp, s, d, &dist);
+            }
             if (st != MS_CONT) {
                 const DevJob& J = jobs[job];
                 const size_t o = (size_t)py * J.rect_w + px;
@@ -394,7 +410,8 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
     uint32_t cnext = 0, cend = 0;
     bool work_left = true, active = false;
     uint32_t hit = 0;
-    int job = 0, steps = 0, step = 0;
+    int job = 0, steps = 0, step = 0, sev = 0;  // step: march steps incl. accelerator skips; sev: DEs
+    bool sacc = false;
     V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
     float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
     uint32_t sray = 0;           // ray slot of the current shadow ray
@@ -431,7 +448,8 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                             ld = mk3(a.hb.sdx[hit], a.hb.sdy[hit], a.hb.sdz[hit]);
                             ldist = a.hb.sdist[hit];
                             k = jobs[job].light_k;
-                            t = 0.004f; res = 1.f; step = 0;
+                            sacc = jobs[job].accel != nullptr;
+                            t = 0.004f; res = 1.f; step = 0; sev = 0;
                             active = true;
                         }
                     }
@@ -445,23 +463,27 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
             if (!work_left) break;
             continue;
         }
-        de += (unsigned long long)__popcll(act);
+        bool done = false;
+        if (active && sacc) done = shadow_skips(jobs[job], so, ld, k, res, ldist, steps, &t, &step);
+        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && !done));
         if (active) {
-            const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
-            const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
-            step++;
-            bool done = false;
-            if (d < 0.0005f) { res = 0.f; done = true; }
-            else {
-                res = fminf(res, k * d * frcp(t));
-                t += fmaxf(d, 0.001f);
-                if (step >= steps || t >= ldist) done = true;
+            if (!done) {
+                const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
+                const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
+                step++;
+                sev++;
+                if (d < 0.0005f) { res = 0.f; done = true; }
+                else {
+                    res = fminf(res, k * d * frcp(t));
+                    t += fmaxf(d, 0.001f);
+                    if (step >= steps || t >= ldist) done = true;
+                }
             }
             if (done) {
                 const DevJob& J = jobs[job];
                 const uint32_t o = a.hb.opix[hit];
                 if (J.out_shadow) J.out_shadow[o] = fclamp(res, 0.f, 1.f);
-                if (J.out_evals) J.out_evals[o] += (uint32_t)step;
+                if (J.out_evals) J.out_evals[o] += (uint32_t)sev;
                 active = false;
                 if (a.ray_done) {
                     if (work_left) lat_ray = sray;

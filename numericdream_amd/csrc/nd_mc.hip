@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "nd_accel.hpp"
 #include "nd_ctx.h"
 #include "nd_de.hpp"
 #include "nd_internal.h"
@@ -364,7 +365,9 @@ __global__ void k_mc_generate(McArgs a) {
 template <class RAY>
 NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
     const DevScene& sc = *a.scene;
-    const MarchParams mp = march_params(*a.job);
+    const DevJob& J = *a.job;
+    const MarchParams mp = march_params(J);
+    const bool acc = J.accel != nullptr;
     uint32_t cnext = 0, cend = 0;
     bool work_left = true, active = false;
     uint32_t item = 0;
@@ -406,12 +409,17 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
             if (!work_left) break;
             continue;
         }
-        de += (unsigned long long)__popcll(act);
+        float dist = 0.f;
+        int st = MS_CONT;
+        if (active && acc)
+            st = march_skips(m, mp, [&](float ts) { return accel_skip(J, mk3(org.x + dir.x * ts, org.y + dir.y * ts, org.z + dir.z * ts)); }, &dist);
+        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && st == MS_CONT));
         if (active) {
-            const float s = march_sample_t(m);
-            const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
-            float dist = 0.f;
-            const int st = march_update(m, mp, s, d, &dist);
+            if (st == MS_CONT) {
+                const float s = march_sample_t(m);
+                const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
+                st = march_update(m, mp, s, d, &dist);
+            }
             if (st != MS_CONT) {
                 R.finish(item, st == MS_HIT, dist);
                 active = false;
@@ -667,6 +675,8 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
 // --- shadow: visibility_march on an opaque scene (template.wgsl:2282-2342), refilled
 __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
     const DevScene& sc = *a.scene;
+    const DevJob& J = *a.job;
+    const bool acc = J.accel != nullptr;
     const uint32_t n = a.P.ctr[C_SHADOW];
     const int max_steps = a.S->shadow_steps;
     uint32_t cnext = 0, cend = 0;
@@ -717,16 +727,19 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
             if (!work_left) break;
             continue;
         }
-        de += (unsigned long long)__popcll(act);
+        bool done = false;
+        if (active && acc) done = shadow_skips(J, so, ld, k, res, ldist, max_steps, &t, &step);
+        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && !done));
         if (active) {
-            const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
-            step++;
-            bool done = false;
-            if (d < 0.0005f) { res = 0.f; done = true; }
-            else {
-                res = fminf(res, k * d / t);
-                t += fmaxf(d, 0.001f);
-                if (step >= max_steps || t >= ldist) done = true;
+            if (!done) {
+                const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
+                step++;
+                if (d < 0.0005f) { res = 0.f; done = true; }
+                else {
+                    res = fminf(res, k * d / t);
+                    t += fmaxf(d, 0.001f);
+                    if (step >= max_steps || t >= ldist) done = true;
+                }
             }
             if (done) {
                 const float vis = fclamp(res, 0.f, 1.f);
@@ -875,6 +888,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     std::memset(&job, 0, sizeof(job));
     nd_status st = nd_pack_job(c, u, o, rect, job);
     if (st != ND_OK) return st;
+    if ((st = nd_accel_prepare(c, &job, 1)) != ND_OK) return st;
     nd::McScene S;
     pack_mc_scene(u, S);
     const uint32_t npix = (uint32_t)job.rect_w * (uint32_t)job.rect_h;

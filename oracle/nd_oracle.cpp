@@ -94,6 +94,7 @@ int render_gbuffer_t(const nd_uniforms* u, const int32_t* cm, const GbufOpts& o,
                      float* depth, float* normal, float* trap, uint8_t* hit, float* shadow, uint32_t* evals,
                      int nthreads, const double* cam_offset) {
     Scene<T> s = decode_scene<T>(*u, code_map(cm));
+    attach_accel(*u, s);
     if (cam_offset) for (int k = 0; k < 3; ++k) s.cam_offset[k] = cam_offset[k];
     MarchCfg mc = march_cfg(*u);
     std::atomic<int> next_row{0};
@@ -154,6 +155,7 @@ int ndo_march(const nd_uniforms* u, const int32_t* cm, int prec, const double* o
     MarchCfg mc = march_cfg(*u);
     if (prec == 0) {
         Scene<double> s = decode_scene<double>(*u, code_map(cm));
+        attach_accel(*u, s);
         for (int i = 0; i < n; ++i) {
             auto m = march_to(s, mc, V3<double>(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
                               V3<double>(d[3 * i], d[3 * i + 1], d[3 * i + 2]), (double)u->max_dist);
@@ -161,6 +163,7 @@ int ndo_march(const nd_uniforms* u, const int32_t* cm, int prec, const double* o
         }
     } else {
         Scene<float> s = decode_scene<float>(*u, code_map(cm));
+        attach_accel(*u, s);
         for (int i = 0; i < n; ++i) {
             auto m = march_to(s, mc, V3<float>((float)o[3 * i], (float)o[3 * i + 1], (float)o[3 * i + 2]),
                               V3<float>((float)d[3 * i], (float)d[3 * i + 1], (float)d[3 * i + 2]), u->max_dist);
@@ -185,6 +188,7 @@ int ndo_normal(const nd_uniforms* u, const int32_t* cm, int mode, const double* 
 int ndo_shadow(const nd_uniforms* u, const int32_t* cm, const double* origin, const double* dir, double light_dist,
                double k, int steps, double* vis, int32_t* evals) {
     Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    attach_accel(*u, s);
     int ev = 0;
     *vis = visibility_march_opaque(s, V3<double>(origin[0], origin[1], origin[2]), V3<double>(dir[0], dir[1], dir[2]),
                                    light_dist, k, steps, &ev);
@@ -227,6 +231,7 @@ double ndo_flops_per_de(const nd_uniforms* u, const int32_t* cm, const double* p
 int ndo_flops_render(const nd_uniforms* u, const int32_t* cm, int shadow_max_steps, int rx, int ry, int rw, int rh,
                      int step, double* out) {
     Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    attach_accel(*u, s);
     MarchCfg mc = march_cfg(*u);
     uint64_t ev[2] = {0, 0}, fl[2] = {0, 0}, tp[2] = {0, 0};
     for (int y = 0; y < rh; y += step)
@@ -338,10 +343,15 @@ int ndo_adjoint_pixels(const nd_uniforms* u, const int32_t* cm, const int32_t* p
 int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, int ry, int rw, int rh, int spp,
                   float* rgb, float* depth, float* normal, float* sample_rgb, uint32_t* evals, int nthreads) {
     std::atomic<int> next_row{0};
+    Scene<double> scene64;
+    Scene<float> scene32;
+    if (prec == 0) { scene64 = decode_scene<double>(*u0, code_map(cm)); attach_accel(*u0, scene64); }
+    else { scene32 = decode_scene<float>(*u0, code_map(cm)); attach_accel(*u0, scene32); }
     auto run = [&](auto tag) {
         using T = decltype(tag);
         nd_uniforms u = *u0;
-        Scene<T> s = decode_scene<T>(u, code_map(cm));
+        Scene<T> s;
+        if constexpr (std::is_same<T, double>::value) s = scene64; else s = scene32;
         MarchCfg mc = march_cfg(u);
         for (;;) {
             int y = next_row.fetch_add(1);
@@ -475,6 +485,30 @@ int ndo_helper(int op, const double* a, double* out) {
         case 25: { Spectral<double> sp(a[0]); for (int k = 0; k < 8; ++k) out[k] = sp.lambda[k]; return 0; }
     }
     return -1;
+}
+
+// placeCascades for a camera position (out: 8 x [corner xyz, cell]).
+int ndo_place_cascades(const double* cam, double max_dist, int levels, int res, double* out) {
+    double o[ND_MAX_CASCADES][4];
+    place_cascades(cam, max_dist, levels, res, o);
+    std::memcpy(out, o, sizeof(o));
+    return 0;
+}
+
+// The accelerator grid the oracle builds for u (prec 0: FP64, 1: FP32), levels*res^3 values
+// in [level][z][y][x] order, and accel_skip at n points.
+int ndo_accel(const nd_uniforms* u, const int32_t* cm, int prec, double* grid_out, const double* pts, int n,
+              double* skip_out) {
+    auto go = [&](auto tag) {
+        using T = decltype(tag);
+        Scene<T> s = decode_scene<T>(*u, code_map(cm));
+        auto g = build_accel(*u, s);
+        if (!g) return -1;
+        if (grid_out) std::memcpy(grid_out, g->safe.data(), sizeof(double) * g->safe.size());
+        for (int i = 0; i < n; ++i) skip_out[i] = (double)val(accel_skip(g.get(), V3<T>(T(pts[3 * i]), T(pts[3 * i + 1]), T(pts[3 * i + 2]))));
+        return 0;
+    };
+    return prec == 0 ? go(0.0) : go(0.0f);
 }
 
 // formula step with an explicit layout: a = [f, z3, dr, pos3, p16]

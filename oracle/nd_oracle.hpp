@@ -22,7 +22,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <type_traits>
+#include <vector>
 
 #include "nd_uniforms.h"
 
@@ -433,6 +436,13 @@ template <class P> struct Instance {
     TrapSpec trap;
 };
 
+// Empty-space accelerator grid (built by build_accel below).
+struct AccelGrid {
+    int res = 0, levels = 0;
+    double prm[ND_MAX_CASCADES][4] = {};  // corner xyz, cell
+    std::vector<double> safe;             // [level][z][y][x] safe skip distances
+};
+
 template <class P> struct Scene {
     int count = 1;  // max(i32(instance_count), 1)
     Instance<P> inst[ND_MAX_INSTANCES];
@@ -440,6 +450,7 @@ template <class P> struct Scene {
     int carve_count = 0;
     double carves[ND_MAX_CARVES][4];
     double cam_offset[3] = {0, 0, 0};  // translation added to the camera (adjoint parameter)
+    std::shared_ptr<const AccelGrid> accel;  // null with the accelerator off
 };
 
 // code -> builtin formula. Default: identity for the four built-ins.
@@ -640,6 +651,85 @@ template <class T, class P> T probed_de(const Scene<P>& s, V3<T> pos) {
 }
 
 // ----------------------------------------------------------------------------- march
+// ----------------------------------------------------------------------------- accelerator
+// placeCascades (accel.zig:34-58): `levels` cubic grids of res^3 cells around the camera,
+// level l with cell = (2 max_dist / res) / 2^(levels-1-l) (level 0 finest), each corner
+// snapped down to a multiple of its cell.
+inline void place_cascades(const double cam[3], double max_dist, int levels, int res, double out[ND_MAX_CASCADES][4]) {
+    for (int l = 0; l < ND_MAX_CASCADES; ++l) out[l][0] = out[l][1] = out[l][2] = out[l][3] = 0.0;
+    const int n = std::min(std::max(levels, 1), ND_MAX_CASCADES);
+    const float res_f = (float)std::max(res, 1);
+    const float coarsest = std::max(2.0f * (float)max_dist / res_f, 1e-6f);
+    for (int l = 0; l < n; ++l) {
+        const float cell = coarsest / (float)(1u << (n - 1 - l));
+        const float half = 0.5f * res_f * cell;
+        for (int a = 0; a < 3; ++a) out[l][a] = std::floor(((float)cam[a] - half) / cell) * cell;
+        out[l][3] = cell;
+    }
+}
+
+// cs_build_accel (template.wgsl:5409-5431): a voxel stores max(DE(centre) - HALF_SQRT3 * cell,
+// 0) * safety, DE evaluated in the scene's parameter precision P.
+template <class P> std::shared_ptr<const AccelGrid> build_accel(const nd_uniforms& u, const Scene<P>& s) {
+    auto g = std::make_shared<AccelGrid>();
+    g->res = i32(u.accel_res);
+    g->levels = std::min(i32(u.accel_levels), ND_MAX_CASCADES);
+    if (g->res <= 0 || g->levels <= 0) return nullptr;
+    for (int l = 0; l < ND_MAX_CASCADES; ++l)
+        for (int k = 0; k < 4; ++k) g->prm[l][k] = u.accel_params[l][k];
+    const int res = g->res;
+    const size_t n = (size_t)res * res * res * g->levels;
+    g->safe.assign(n, 0.0);
+    const P safety = P(u.accel_safety);
+    const int slabs = res * g->levels;
+    const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nt; ++w) {
+        pool.emplace_back([&, w] {
+            for (int gz = w; gz < slabs; gz += nt) {
+                const int level = gz / res, cz = gz % res;
+                const P cell = P(g->prm[level][3]);
+                for (int y = 0; y < res; ++y)
+                    for (int x = 0; x < res; ++x) {
+                        V3<P> c(P(g->prm[level][0]) + (P(x) + P(0.5)) * cell, P(g->prm[level][1]) + (P(y) + P(0.5)) * cell,
+                                P(g->prm[level][2]) + (P(cz) + P(0.5)) * cell);
+                        P d = scene_de(s, c);
+                        P v = tmax(d - P(0.8660254) * cell, P(0)) * safety;
+                        g->safe[((size_t)gz * res + y) * res + x] = (double)v;
+                    }
+            }
+        });
+    }
+    for (auto& t : pool) t.join();
+    return g;
+}
+
+// Attaches the accelerator grid when the Uniforms enable it (the host stamps accel_params
+// from place_cascades, fr.zig:3073-3080).
+template <class P> void attach_accel(const nd_uniforms& u, Scene<P>& s) {
+    if (u.accel_enabled > 0.5f) s.accel = build_accel(u, s);
+}
+
+// accel_skip (template.wgsl:428-447): the first level, from the finest, whose grid holds p
+// gives the skip distance; 0 outside every grid or with the accelerator off.
+template <class T> T accel_skip(const AccelGrid* g, V3<T> p) {
+    if (!g) return T(0);
+    for (int l = 0; l < g->levels; ++l) {
+        const double cell = g->prm[l][3];
+        if (cell <= 0.0) continue;
+        const T pc[3] = {p.x, p.y, p.z};
+        int c[3];
+        bool in = true;
+        for (int a = 0; a < 3; ++a) {
+            c[a] = (int)std::floor((double)val((pc[a] - T(g->prm[l][a])) / T(cell)));
+            if (c[a] < 0 || c[a] >= g->res) in = false;
+        }
+        if (!in) continue;
+        return T(g->safe[(((size_t)l * g->res + c[2]) * g->res + c[1]) * g->res + c[0]]);
+    }
+    return T(0);
+}
+
 struct MarchCfg {
     int max_steps;
     double max_dist;
@@ -679,7 +769,17 @@ MarchResult<T> march_to(const Scene<P>& s, const MarchCfg& m, V3<T> origin, V3<T
     for (int n = 0; n < max_samples; ++n) {
         T sample_t = probing ? t : T(0.5) * (lo + hi);
         T eps = T(m.eps_c) * sample_t + T(m.eps_f);
-        // accel_skip() is 0 with the accelerator off, never > eps.
+        if (probing && s.accel) {
+            // accelerator skip (template.wgsl:1890-1900): a step without a DE evaluation
+            T skip = accel_skip(s.accel.get(), origin + dir * sample_t);
+            if (skip > eps) {
+                lo = sample_t;
+                t = sample_t + skip;
+                i++;
+                if (t > max_t || i >= m.max_steps) break;
+                continue;
+            }
+        }
         T d = probed_de(s, origin + dir * sample_t);
         evals++;
         if (bisecting) {
@@ -785,6 +885,14 @@ T visibility_march_opaque(const Scene<P>& s, V3<T> origin, V3<T> dir, T light_di
     for (int i = 0; i < steps; ++i) {
         if (t >= light_dist) break;
         V3<T> p = origin + dir * t;
+        if (s.accel) {
+            // accelerator skip (template.wgsl:2297-2303): only where it cannot lower res
+            T skip = accel_skip(s.accel.get(), p);
+            if (skip > T(0.001) && k * skip >= res * t) {
+                t = t + skip;
+                continue;
+            }
+        }
         T d = probed_de(s, p);
         if (evals) (*evals)++;
         if (d < T(0.0005)) return T(0);  // SHADOW_HIT_EPS
