@@ -131,6 +131,31 @@ def gradient_leg(r, nd, torch, reps=3):
             "views": len(views), "pixels": total, "pixels_in_loss": used}
 
 
+def accel_leg(r, nd, torch, abi, jobs, steps=2):
+    """The same 16-camera batch with the empty-space accelerator as the app enables it by
+    default (64^3 cells x 6 levels, safety 0.92; one grid per camera, built on the device in
+    the warmup and reused from the cache afterwards)."""
+    acc_jobs = []
+    for u, g in jobs:
+        ua = nd.abi.Uniforms.from_buffer_copy(bytes(u))
+        nd.enable_accel(ua, res=64, levels=6, safety=0.92)
+        acc_jobs.append((ua, g))
+    res = {}
+    for name, kernel in (("persistent", abi.KERNEL_PERSISTENT), ("plain", abi.KERNEL_PLAIN)):
+        r.render_batch(acc_jobs, shadows=True, shadow_max_steps=128, kernel=kernel)  # builds / caches the grids
+        r.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = r.render_batch(acc_jobs, shadows=True, shadow_max_steps=128, kernel=kernel)
+        r.sync()
+        wall = (time.perf_counter() - t0) / steps
+        res[name] = {"mrays_per_s": round(st.rays / wall / 1e6, 2), "ms_per_step": round(wall * 1e3, 3),
+                     "de_evals": int(st.de_evals)}
+        log("c2+accel %s: %.1f ms per step" % (name, wall * 1e3))
+    res["persistent_vs_plain"] = round(res["persistent"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 3)
+    return res
+
+
 def mc_leg(r, nd, spp=64):
     """Wavefront Monte Carlo path trace of the four-fractal smooth-union scene at 1920x1080 and
     64 samples per pixel (one frame)."""
@@ -205,8 +230,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the path-trace, 256-job and gradient legs")
-    ap.add_argument("--legs", default="c3,c4,c5", help="comma list of extra legs: c3 path trace, c4 256 jobs, "
-                                                      "c5 gradient")
+    ap.add_argument("--legs", default="c3,c4,c5,accel", help="comma list of extra legs: c3 path trace, c4 256 "
+                                                            "jobs, c5 gradient, accel the batch with the "
+                                                            "empty-space accelerator")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run one elementwise copy of MIB MiB (reads MIB, writes MIB): "
                          "the reference kernel for scaling hardware byte counters")
@@ -281,6 +307,8 @@ def main():
         res["c5_gradient"] = g
     if "c3" in legs:
         res["c3_path_trace"] = mc_leg(r, nd)
+    if "accel" in legs:
+        res["c2_accelerator"] = accel_leg(r, nd, torch, abi, jobs)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(nd)

@@ -7,37 +7,59 @@
 
 namespace nd {
 
-// Safe skip distance at p: the first level, from the finest, whose grid holds p; 0 outside
-// every grid. Coordinates are compared as floats before the integer conversion.
-NDD float accel_skip(const DevJob& J, const V3<float>& p) {
-    const int res = J.accel_res;
-    const float fres = (float)res;
-    for (int l = 0; l < J.accel_levels; ++l) {
-        const float cell = J.accel_prm[l][3];
-        if (!(cell > 0.f)) continue;
-        const float fx = floorf((p.x - J.accel_prm[l][0]) / cell);
-        const float fy = floorf((p.y - J.accel_prm[l][1]) / cell);
-        const float fz = floorf((p.z - J.accel_prm[l][2]) / cell);
+// What a ray needs to look the grid up, held in registers: the grid, the cascade table and
+// the centre and half-extent of the finest cascade.
+struct AccelView {
+    const float* grid;  // null: accelerator off
+    const float* prm;   // [level][corner xyz, cell]
+    float cx, cy, cz;   // centre of level 0
+    float inv_half0;    // 1 / half-extent of level 0
+    int res, levels;
+};
+
+NDD AccelView accel_view(const DevJob& J) {
+    AccelView a;
+    a.grid = J.accel;
+    a.prm = &J.accel_prm[0][0];
+    a.res = J.accel_res;
+    a.levels = J.accel_levels;
+    const float half0 = 0.5f * (float)J.accel_res * J.accel_prm[0][3];
+    a.cx = J.accel_prm[0][0] + half0;
+    a.cy = J.accel_prm[0][1] + half0;
+    a.cz = J.accel_prm[0][2] + half0;
+    a.inv_half0 = 1.f / half0;
+    return a;
+}
+
+// Safe skip distance at p: the grid value of the finest level whose grid holds p, 0 outside
+// every grid (accel_skip). Level l is centred within one of its cells of level 0's centre
+// and reaches half0 2^l (1 + 2/res) from it at most, so no level below
+// ceil(log2(|p - centre|_inf / half0)) - 1 can hold p; the search starts one level lower
+// still (the hardware log2 is accurate to an ulp), which finds the same level as a search
+// from level 0. Cells are indexed by floor((p - corner) * (1 / cell)).
+NDD float accel_skip(const AccelView& A, const V3<float>& p) {
+    const float u = fmaxf(fabsf(p.x - A.cx), fmaxf(fabsf(p.y - A.cy), fabsf(p.z - A.cz)));
+    int l = (int)ceilf(__builtin_amdgcn_logf(u * A.inv_half0)) - 2;
+    l = l < 0 ? 0 : l;
+    const float fres = (float)A.res;
+    for (int k = 0; k < 4 && l < A.levels; ++k, ++l) {
+        const float* q = A.prm + 4 * l;
+        const float ic = __builtin_amdgcn_rcpf(q[3]);
+        const float fx = floorf((p.x - q[0]) * ic);
+        const float fy = floorf((p.y - q[1]) * ic);
+        const float fz = floorf((p.z - q[2]) * ic);
         if (fx < 0.f || fy < 0.f || fz < 0.f || fx >= fres || fy >= fres || fz >= fres) continue;
-        const size_t idx = (((size_t)l * res + (size_t)fz) * res + (size_t)fy) * res + (size_t)fx;
-        return J.accel[idx];
+        const size_t idx = (((size_t)l * A.res + (size_t)fz) * A.res + (size_t)fy) * A.res + (size_t)fx;
+        return A.grid[idx];
     }
     return 0.f;
 }
 
-// Accelerator steps of a shadow ray before its next DE sample (visibility_march,
-// template.wgsl:2297-2303): a skip is taken where it cannot lower the penumbra estimate
-// (skip > SHADOW_MIN_STEP and k skip >= res t) and uses up one of the ray's steps. Returns
-// true when the ray ended (step budget or light distance reached).
-NDD bool shadow_skips(const DevJob& J, const V3<float>& o, const V3<float>& d, float k, float res, float ldist,
-                      int max_steps, float* t, int* step) {
-    for (;;) {
-        const float skip = accel_skip(J, mk3(o.x + d.x * *t, o.y + d.y * *t, o.z + d.z * *t));
-        if (!(skip > 0.001f && k * skip >= res * *t)) return false;
-        *t += skip;
-        ++*step;
-        if (*step >= max_steps || *t >= ldist) return true;
-    }
+// The accelerator test of a shadow-ray step (visibility_march, template.wgsl:2297-2303): a
+// skip is taken where it cannot lower the penumbra estimate (skip > SHADOW_MIN_STEP and
+// k skip >= res t); it uses up one of the ray's steps.
+NDD bool shadow_skip_applies(float skip, float k, float res, float t) {
+    return skip > 0.001f && k * skip >= res * t;
 }
 
 }  // namespace nd

@@ -516,6 +516,8 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.round_cap = c->round_cap;
         L.any_shadow = any_shadow;
         L.surface = !ex.skip_surface;
+        L.accel = 0;
+        for (uint32_t i = 0; i < n; ++i) L.accel |= hj[i].accel != nullptr;
         L.job_done_cycles = track ? (unsigned long long*)c->latency.p : nullptr;
         L.ray_done = track ? (uint32_t*)((char*)c->latency.p + lat_jobs) : nullptr;
         for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];

@@ -21,6 +21,7 @@ struct NdPassLaunch {
     int surface;                 // run the normal / trap pass
     unsigned long long* job_done_cycles;  // per-job completion clock, or null
     uint32_t* ray_done;          // per ray slot completion tick (total_rays entries) when tracking
+    int accel;                   // some job uses the empty-space accelerator
     hipEvent_t ev[4];            // optional: before march / surface / shadow / after
 };
 

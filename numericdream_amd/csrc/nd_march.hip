@@ -85,14 +85,17 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
     float dist = 0.f;
     int st;
     const bool acc = J.accel != nullptr;
+    const AccelView av = accel_view(J);
     do {
-        if (acc) {
-            st = march_skips(m, J, [&](float ts) { return accel_skip(J, mk3(org.x + dir.x * ts, org.y + dir.y * ts, org.z + dir.z * ts)); }, &dist);
-            if (st != MS_CONT) break;
-        }
+        // with the accelerator the grid value is read while the DE at the same point is
+        // evaluated; a skip then takes the step and the DE sample is not used
         float s = march_sample_t(m);
-        float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
-        st = march_update(m, J, s, d, &dist);
+        const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
+        const bool probe = acc && m.probing;
+        const float skip = probe ? accel_skip(av, ps) : 0.f;
+        float d = scene_de<false>(sc, ps, nullptr);
+        st = probe ? march_skip_step(m, J, skip, &dist) : -1;
+        if (st < 0) st = march_update(m, J, s, d, &dist);
     } while (st == MS_CONT);
     const size_t o = (size_t)y * J.rect_w + x;
     int evals = m.evals;
@@ -122,8 +125,10 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
             float t = 0.004f, res = 1.f;
             for (int i = 0; i < J.shadow_steps; ++i) {
                 if (t >= ldist) break;
-                if (J.accel && shadow_skips(J, so, ld, J.light_k, res, ldist, J.shadow_steps, &t, &i)) break;
-                float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
+                const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
+                const float skip = acc ? accel_skip(av, ps) : 0.f;
+                float d = scene_de<false>(sc, ps, nullptr);
+                if (acc && shadow_skip_applies(skip, J.light_k, res, t)) { t += skip; continue; }
                 evals++;
                 if (d < 0.0005f) { res = 0.f; break; }
                 res = fminf(res, J.light_k * d * frcp(t));
@@ -202,7 +207,7 @@ NDD void write_miss(const DevJob& J, size_t o, int evals) {
 // finished is refilled at once from the wave's current chunk of ray slots. The chunk and
 // its job are wave-uniform, so a refill reads the job's camera and precision with scalar
 // loads and the hot loop touches no per-lane memory.
-template <bool SHARED>
+template <bool SHARED, bool ACC>
 __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     uint32_t cnext = 0, cend = 0;          // wave-uniform chunk of ray slots [cnext, cend)
@@ -221,6 +226,7 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
     march_init(m);
     MarchParams mp = {};
     bool acc = false;  // the ray's job uses the accelerator
+    AccelView av = {};
     unsigned long long de = 0, nhits = 0;
     uint32_t rounds_used = 0;  // rounds that evaluated at least one DE
     for (uint32_t round = 0;; ++round) {
@@ -258,7 +264,8 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
                             dir = primary_dir(CJ, CJ.rect_x + px, CJ.rect_y + py);
                             org = mk3(CJ.cam[0], CJ.cam[1], CJ.cam[2]);
                             mp = march_params(CJ);
-                            acc = CJ.accel != nullptr;
+                            acc = ACC && CJ.accel != nullptr;
+                            if (acc) av = accel_view(CJ);
                             march_init(m);
                             active = true;
                         }
@@ -276,19 +283,23 @@ __global__ void __launch_bounds__(64) k_march(PassArgs a, const DevJob* __restri
         ++rounds_used;
         bool hit_now = false;
         float dist = 0.f;
-        int st = MS_CONT;
-        if (active && acc) {
-            const DevJob& AJ = jobs[job];
-            st = march_skips(m, mp, [&](float ts) { return accel_skip(AJ, mk3(org.x + dir.x * ts, org.y + dir.y * ts, org.z + dir.z * ts)); }, &dist);
-        }
-        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && st == MS_CONT));
+        int st = -1;
+        bool used = false;  // the DE sample decided the step (not an accelerator skip)
         if (active) {
-            if (st == MS_CONT) {
-                const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
-                const float s = march_sample_t(m);
-                const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
-                st = march_update(m, mp, s, d, &dist);
-            }
+            // with the accelerator the grid value is read while the DE at the same point is
+            // evaluated; a skip then takes the step and the DE sample is not used
+            const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
+            const float s = march_sample_t(m);
+            const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
+            const bool probe = ACC && acc && m.probing;
+            const float skip = probe ? accel_skip(av, ps) : 0.f;
+            const float d = scene_de<false>(sc, ps, nullptr);
+            if (probe) st = march_skip_step(m, mp, skip, &dist);
+            used = st < 0;
+            if (used) st = march_update(m, mp, s, d, &dist);
+        }
+        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(used));
+        if (active) {
             if (st != MS_CONT) {
                 const DevJob& J = jobs[job];
                 const size_t o = (size_t)py * J.rect_w + px;
@@ -403,7 +414,7 @@ __global__ void __launch_bounds__(64) k_surface(PassArgs a, const DevJob* __rest
 
 // --- pass 3: light-0 soft shadow of every hit (visibility_march on an opaque scene,
 // template.wgsl:2282-2342), refilled like the march from the SoA shadow rays.
-template <bool SHARED>
+template <bool SHARED, bool ACC>
 __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     const uint32_t n_hits = a.ctr->hit_count;
@@ -412,6 +423,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
     uint32_t hit = 0;
     int job = 0, steps = 0, step = 0, sev = 0;  // step: march steps incl. accelerator skips; sev: DEs
     bool sacc = false;
+    AccelView sav = {};
     V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
     float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
     uint32_t sray = 0;           // ray slot of the current shadow ray
@@ -448,7 +460,8 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                             ld = mk3(a.hb.sdx[hit], a.hb.sdy[hit], a.hb.sdz[hit]);
                             ldist = a.hb.sdist[hit];
                             k = jobs[job].light_k;
-                            sacc = jobs[job].accel != nullptr;
+                            sacc = ACC && jobs[job].accel != nullptr;
+                            if (sacc) sav = accel_view(jobs[job]);
                             t = 0.004f; res = 1.f; step = 0; sev = 0;
                             active = true;
                         }
@@ -463,14 +476,18 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
             if (!work_left) break;
             continue;
         }
-        bool done = false;
-        if (active && sacc) done = shadow_skips(jobs[job], so, ld, k, res, ldist, steps, &t, &step);
-        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && !done));
+        bool done = false, skipped = false;
         if (active) {
-            if (!done) {
-                const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
-                const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
-                step++;
+            const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
+            const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
+            const float skip = (ACC && sacc) ? accel_skip(sav, ps) : 0.f;
+            const float d = scene_de<false>(sc, ps, nullptr);
+            step++;
+            skipped = ACC && sacc && shadow_skip_applies(skip, k, res, t);
+            if (skipped) {
+                t += skip;
+                if (step >= steps || t >= ldist) done = true;
+            } else {
                 sev++;
                 if (d < 0.0005f) { res = 0.f; done = true; }
                 else {
@@ -491,6 +508,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                 }
             }
         }
+        de += (unsigned long long)__popcll(act & ~__builtin_amdgcn_ballot_w64(skipped));
         if (round >= a.round_cap) {
             if (__lane_id() == 0) atomicAdd(&a.ctr->rounds_capped, 1u);
             break;
@@ -561,8 +579,14 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
         hipLaunchKernelGGL(nd::k_zero_u64, dim3((L.n_jobs + 255) / 256), dim3(256), 0, s, L.job_done_cycles, L.n_jobs);
     }
     if (L.ev[0]) hipEventRecord(L.ev[0], s);
-    if (L.shared) hipLaunchKernelGGL(nd::k_march<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
-    else hipLaunchKernelGGL(nd::k_march<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+    // ACC: some job uses the accelerator (the kernels without it keep fewer registers)
+    if (L.shared) {
+        if (L.accel) hipLaunchKernelGGL((nd::k_march<true, true>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL((nd::k_march<true, false>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+    } else {
+        if (L.accel) hipLaunchKernelGGL((nd::k_march<false, true>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL((nd::k_march<false, false>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+    }
     if (L.ev[1]) hipEventRecord(L.ev[1], s);
     if (L.surface) {
         if (L.shared) hipLaunchKernelGGL(nd::k_surface<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
@@ -570,8 +594,13 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
     }
     if (L.ev[2]) hipEventRecord(L.ev[2], s);
     if (L.any_shadow) {
-        if (L.shared) hipLaunchKernelGGL(nd::k_shadow<true>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
-        else hipLaunchKernelGGL(nd::k_shadow<false>, dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+        if (L.shared) {
+            if (L.accel) hipLaunchKernelGGL((nd::k_shadow<true, true>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+            else hipLaunchKernelGGL((nd::k_shadow<true, false>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+        } else {
+            if (L.accel) hipLaunchKernelGGL((nd::k_shadow<false, true>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+            else hipLaunchKernelGGL((nd::k_shadow<false, false>), dim3(L.grid), dim3(64), 0, s, a, L.jobs, L.shared);
+        }
     }
     if (a.ray_done)
         hipLaunchKernelGGL(nd::k_job_done, dim3(L.n_jobs, nd::JOB_DONE_SLICES), dim3(256), 0, s, L.jobs, a.ray_done,
@@ -583,7 +612,7 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
 // Resident wave64 workgroups of the march kernel per CU at full occupancy.
 uint32_t nd_march_occupancy() {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, nd::k_march<true>, 64, 0) != hipSuccess || blocks <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, nd::k_march<true, false>, 64, 0) != hipSuccess || blocks <= 0)
         return 8;
     return (uint32_t)blocks;
 }

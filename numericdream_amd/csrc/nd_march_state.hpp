@@ -61,22 +61,20 @@ NDD int march_update_norefine(March& m, const MP& J, float s, float d, float* di
     return MS_CONT;
 }
 
-// Accelerator steps before the next probe (template.wgsl:1890-1900): while the grid's safe
-// distance at t exceeds eps(t), t advances by it without a DE evaluation; each such step
-// counts as a march step. skip_at(t) returns the safe distance. Returns MS_MISS (with
-// *dist = t) when a skip ends the ray at the distance or step limit, else MS_CONT.
-template <class MP, class SKIP>
-NDD int march_skips(March& m, const MP& J, const SKIP& skip_at, float* dist) {
-    while (m.probing) {
-        const float s = m.t;
-        const float skip = skip_at(s);
-        if (!(skip > J.eps_c * s + J.eps_f)) break;
-        m.lo = s;
-        m.t = s + skip;
-        m.i++;
-        m.n++;
-        if (m.t > J.max_dist || m.i >= J.max_steps) { *dist = m.t; return MS_MISS; }
-    }
+// One accelerator step at a probe (template.wgsl:1890-1900), given the grid's safe distance
+// at the probe point: when it exceeds eps(t), t advances by it without using a DE sample and
+// the step counts as a march step. Returns MS_CONT after such a skip, MS_MISS (*dist = t)
+// when the skip ends the ray at the distance or step limit, and -1 when no skip applies and
+// the DE sample at t decides (march_update).
+template <class MP>
+NDD int march_skip_step(March& m, const MP& J, float skip, float* dist) {
+    const float s = m.t;
+    if (!(skip > J.eps_c * s + J.eps_f)) return -1;
+    m.lo = s;
+    m.t = s + skip;
+    m.i++;
+    m.n++;
+    if (m.t > J.max_dist || m.i >= J.max_steps) { *dist = m.t; return MS_MISS; }
     return MS_CONT;
 }
 

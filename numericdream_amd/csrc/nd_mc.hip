@@ -368,6 +368,7 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
     const DevJob& J = *a.job;
     const MarchParams mp = march_params(J);
     const bool acc = J.accel != nullptr;
+    const AccelView av = accel_view(J);
     uint32_t cnext = 0, cend = 0;
     bool work_left = true, active = false;
     uint32_t item = 0;
@@ -410,21 +411,23 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
             continue;
         }
         float dist = 0.f;
-        int st = MS_CONT;
-        if (active && acc)
-            st = march_skips(m, mp, [&](float ts) { return accel_skip(J, mk3(org.x + dir.x * ts, org.y + dir.y * ts, org.z + dir.z * ts)); }, &dist);
-        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && st == MS_CONT));
+        int st = -1;
+        bool used = false;  // the DE sample decided the step (not an accelerator skip)
         if (active) {
-            if (st == MS_CONT) {
-                const float s = march_sample_t(m);
-                const float d = scene_de<false>(sc, mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s), nullptr);
-                st = march_update(m, mp, s, d, &dist);
-            }
+            const float s = march_sample_t(m);
+            const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
+            const bool probe = acc && m.probing;
+            const float skip = probe ? accel_skip(av, ps) : 0.f;
+            const float d = scene_de<false>(sc, ps, nullptr);
+            if (probe) st = march_skip_step(m, mp, skip, &dist);
+            used = st < 0;
+            if (used) st = march_update(m, mp, s, d, &dist);
             if (st != MS_CONT) {
                 R.finish(item, st == MS_HIT, dist);
                 active = false;
             }
         }
+        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(used));
     }
     if (__lane_id() == 0) atomicAdd(de_ctr(a), de);
 }
@@ -677,6 +680,7 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
     const DevScene& sc = *a.scene;
     const DevJob& J = *a.job;
     const bool acc = J.accel != nullptr;
+    const AccelView av = accel_view(J);
     const uint32_t n = a.P.ctr[C_SHADOW];
     const int max_steps = a.S->shadow_steps;
     uint32_t cnext = 0, cend = 0;
@@ -727,13 +731,17 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
             if (!work_left) break;
             continue;
         }
-        bool done = false;
-        if (active && acc) done = shadow_skips(J, so, ld, k, res, ldist, max_steps, &t, &step);
-        de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(active && !done));
+        bool done = false, skipped = false;
         if (active) {
-            if (!done) {
-                const float d = scene_de<false>(sc, mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t), nullptr);
-                step++;
+            const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
+            const float skip = acc ? accel_skip(av, ps) : 0.f;
+            const float d = scene_de<false>(sc, ps, nullptr);
+            step++;
+            skipped = acc && shadow_skip_applies(skip, k, res, t);
+            if (skipped) {
+                t += skip;
+                if (step >= max_steps || t >= ldist) done = true;
+            } else {
                 if (d < 0.0005f) { res = 0.f; done = true; }
                 else {
                     res = fminf(res, k * d / t);
@@ -750,6 +758,7 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
                 active = false;
             }
         }
+        de += (unsigned long long)__popcll(act & ~__builtin_amdgcn_ballot_w64(skipped));
     }
     if (__lane_id() == 0) atomicAdd(de_ctr(a), de);
 }
