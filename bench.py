@@ -27,7 +27,7 @@ METRIC = "Mrays/s persistent-batched vs plain @1080p Mandelbox, %FP32 peak; grad
 # HBM bytes per k_march launch on the 16-camera batch, from two rocprofv3 --pmc runs
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy; profiles/k_march_c2_traffic.md).
 # Hardware counters cannot be read inside the timed run, so the measured figure is carried here.
-K_MARCH_TRAFFIC_BYTES_16CAM = 1.100e9
+K_MARCH_TRAFFIC_BYTES_16CAM = 1.462e9
 
 
 def log(msg):
@@ -286,7 +286,7 @@ def main():
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": K_MARCH_TRAFFIC_BYTES_16CAM if len(jobs) == 16 else None,
                      "frac_unpacked_peak": round(achieved / FP32_PEAK_UNPACKED_TFLOPS, 4),
-                     "kernel": "k_march<true> (persistent march pass)",
+                     "kernel": "k_march<true, false> (persistent march pass, shared scene, no accelerator)",
                      "flops_per_de": round(f_march, 2), "de_evals_per_launch": pers["de_march"] // steps,
                      "ms_per_launch": round(pers["ms_march"] / steps, 3),
                      "lane_occupancy": round(pers["de_march"] / (64.0 * pers["rounds"]), 4) if pers["rounds"] else None},

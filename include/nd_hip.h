@@ -179,8 +179,9 @@ nd_status nd_batch_job_latency(nd_ctx* ctx, float* ms_out, uint32_t n_jobs);
                                      replaying the captured hipGraph */
 
 typedef struct nd_tuning {
-    uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups; 0 = twice the
-                                  resident waves per CU (the second half starts as the first drains) */
+    uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups for every pass;
+                                  0 = each kernel's resident waves per CU, fewer (at least 4) when the
+                                  batch has under two claims per wave */
     uint32_t refill_min_idle;  /* a wave pulls new rays when at least this many lanes are idle (default 8) */
     uint32_t chunk_rays;       /* rays a wave claims from the global queue per atomic (default 256) */
     uint32_t flags;            /* ND_TUNE_* */

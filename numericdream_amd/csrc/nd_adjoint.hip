@@ -100,6 +100,14 @@ __global__ void k_fill_f64(double* p, int n) {
 
 }  // namespace nd
 
+void nd_resident_adjoint(uint32_t* r) {
+    int b = 0;
+    r[RES_ADJOINT] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_adjoint<true>, 64, 0) == hipSuccess && b > 0 ? b : 8;
+    b = 0;
+    r[RES_ADJOINT_MIXED] =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_adjoint<false>, 64, 0) == hipSuccess && b > 0 ? b : 8;
+}
+
 extern "C" nd_status nd_adjoint(nd_ctx* c, const nd_uniforms* views, uint32_t n, const nd_adjoint_io* io,
                                 nd_batch_stats* stats) {
     if (!c) return ND_E_INVALID;
@@ -172,8 +180,10 @@ extern "C" nd_status nd_adjoint(nd_ctx* c, const nd_uniforms* views, uint32_t n,
     a.pix_base = d_base;
     a.acc = acc;
     (void)hipEventRecord(c->ev[4], c->stream);
-    if (a.shared) hipLaunchKernelGGL(nd::k_adjoint<true>, dim3(c->last_grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
-    else hipLaunchKernelGGL(nd::k_adjoint<false>, dim3(c->last_grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
+    // grid-stride over the hits, at most one per ray of the batch
+    const uint32_t grid = nd_pass_grid(c, a.shared ? RES_ADJOINT : RES_ADJOINT_MIXED, ((uint64_t)c->last_rays + 63) / 64);
+    if (a.shared) hipLaunchKernelGGL(nd::k_adjoint<true>, dim3(grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
+    else hipLaunchKernelGGL(nd::k_adjoint<false>, dim3(grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
     if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch adjoint");
     (void)hipEventRecord(c->ev[5], c->stream);
     double h_acc[8];

@@ -281,8 +281,10 @@ nd_status nd_ctx_create(int hip_device, nd_ctx** out) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, hip_device) == hipSuccess && khz > 0)
         c->wall_khz = khz;
     for (auto& e : c->ev) (void)hipEventCreate(&e);
-    c->occupancy_waves = nd_march_occupancy();
-    c->tuning.waves_per_cu = 0;  // 0 = twice the resident waves per CU
+    nd_resident_march(c->resident);
+    nd_resident_adjoint(c->resident);
+    nd_resident_mc(c->resident);
+    c->tuning.waves_per_cu = 0;  // 0 = each kernel's resident waves per CU, fewer for small batches
     c->tuning.refill_min_idle = 8;
     c->tuning.chunk_rays = 256;
     c->tuning.flags = 0;
@@ -362,15 +364,18 @@ static size_t out_bytes(const nd_gbuffer_out& o, size_t px) {
     return b;
 }
 
-uint32_t nd_persistent_grid(nd_ctx* c) {
-    uint32_t wpc = c->tuning.waves_per_cu;
+uint32_t nd_pass_grid(const nd_ctx* c, int res, uint64_t units) {
+    const uint64_t cu = (uint64_t)std::max(c->cu_count, 1);
+    uint64_t wpc = c->tuning.waves_per_cu;
     if (wpc == 0) {
-        // k_march bounds occupancy (one wave64 per workgroup). Twice the resident count: the
-        // second half starts on a CU as soon as waves of the first drain, which evens out the
-        // tail of the persistent passes.
-        wpc = 2 * (c->occupancy_waves ? c->occupancy_waves : 8);
+        // As many waves as fit at once: a wave beyond that waits for a slot and then finds
+        // the queue drained, and more waves per SIMD only slow the resident ones. A small
+        // batch gets about two claims per wave (at least 4 waves per CU), which measured
+        // faster than one.
+        wpc = c->resident[res] ? c->resident[res] : 8;
+        if (units) wpc = std::min<uint64_t>(wpc, std::max<uint64_t>(4, (units + 2 * cu - 1) / (2 * cu)));
     }
-    return (uint32_t)std::max(c->cu_count, 1) * wpc;
+    return (uint32_t)(cu * wpc);
 }
 
 // Tangent slots of the adjoint: 0..2 spatial, 3 Mandelbulb power, 4 Mandelbox scale,
@@ -483,7 +488,12 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         s.kernels = n;
     } else {
         // hit slots: every ray may hit, plus at most one partly used 64-slot chunk per wave
-        size_t hit_cap = (size_t)total + (size_t)nd_persistent_grid(c) * 64;
+        const bool acc = [&] { for (uint32_t i = 0; i < n; ++i) if (hj[i].accel) return true; return false; }();
+        const uint32_t chunk = c->tuning.chunk_rays;
+        const uint64_t claims = ((uint64_t)total + chunk - 1) / chunk;
+        const uint32_t grid_march = nd_pass_grid(
+            c, acc ? (shared ? RES_MARCH_ACC : RES_MARCH_ACC_MIXED) : (shared ? RES_MARCH : RES_MARCH_MIXED), claims);
+        size_t hit_cap = (size_t)total + (size_t)grid_march * 64;
         size_t hb_bytes = hit_cap * (4 + 2 + 4 + 4 + 2 + 7 * 4) + 16 * 256;
         if ((st = c->hits.ensure(c, hb_bytes)) != ND_OK) return st;
         char* hp = (char*)c->hits.p;
@@ -508,16 +518,18 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.n_jobs = (int)n;
         L.total_rays = (uint32_t)total;
         L.ctr = (DevCounters*)c->ctr.p;
-        L.grid = nd_persistent_grid(c);
-        c->last_grid = L.grid;
+        L.grid_march = grid_march;
+        L.grid_surface = nd_pass_grid(c, shared ? RES_SURFACE : RES_SURFACE_MIXED, ((uint64_t)total + 63) / 64);
+        L.grid_shadow = nd_pass_grid(
+            c, acc ? (shared ? RES_SHADOW_ACC : RES_SHADOW_ACC_MIXED) : (shared ? RES_SHADOW : RES_SHADOW_MIXED), claims);
+        c->last_rays = (uint32_t)total;
         c->last_hb = L.hb;
         L.chunk = c->tuning.chunk_rays;
         L.min_idle = c->tuning.refill_min_idle;
         L.round_cap = c->round_cap;
         L.any_shadow = any_shadow;
         L.surface = !ex.skip_surface;
-        L.accel = 0;
-        for (uint32_t i = 0; i < n; ++i) L.accel |= hj[i].accel != nullptr;
+        L.accel = acc;
         L.job_done_cycles = track ? (unsigned long long*)c->latency.p : nullptr;
         L.ray_done = track ? (uint32_t*)((char*)c->latency.p + lat_jobs) : nullptr;
         for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];
@@ -615,7 +627,7 @@ nd_status nd_autotune(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_
     if (opts) o = *opts;
     o.kernel = ND_KERNEL_PERSISTENT;
     const nd_tuning saved = c->tuning;
-    const uint32_t occ = c->occupancy_waves ? c->occupancy_waves : 8;
+    const uint32_t occ = c->resident[RES_MARCH] ? c->resident[RES_MARCH] : 8;
     const uint32_t wpcs[] = {occ / 2 ? occ / 2 : 1, occ, occ * 2};
     const uint32_t idles[] = {1, 8, 32};
     const uint32_t chunks[] = {64, 256, 1024};

@@ -7,6 +7,7 @@
 
 #include "nd_dev.h"
 #include "nd_hip.h"
+#include "nd_internal.h"
 
 struct DevBuf {
     void* p = nullptr;
@@ -29,7 +30,7 @@ struct nd_ctx {
     int code_map[ND_MAX_CODES];
     nd_tuning tuning{};
     int cu_count = 256;
-    uint32_t occupancy_waves = 0;  // resident k_march waves per CU at full occupancy
+    uint32_t resident[RES_COUNT] = {};  // resident waves per CU of each persistent kernel
     char arch[64] = {0};
     int wall_khz = 100000;
     uint32_t round_cap = 0xffffffffu;
@@ -44,7 +45,7 @@ struct nd_ctx {
     std::vector<NdAccelEntry> accel_cache;
     std::vector<DevScene> accel_scene_host;
     bool last_shared = false;       // scene of the last batch was shared by all jobs
-    uint32_t last_grid = 0;
+    uint32_t last_rays = 0;         // rays of the last persistent batch
     DevHitBuffers last_hb{};        // hit list of the last persistent batch
     std::vector<DevJob> host_jobs;
     std::vector<float> job_latency_ms;
@@ -65,4 +66,6 @@ nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, n
 // Builds (or finds cached) the accelerator grids of the jobs that enable it and points their
 // accel fields at them; launches on the context stream.
 nd_status nd_accel_prepare(nd_ctx* c, DevJob* jobs, uint32_t n);
-uint32_t nd_persistent_grid(nd_ctx* c);
+// Workgroups (one wave64 each) of a persistent pass whose kernel has residency class `res`,
+// over about `units` claims of work (0 = unknown).
+uint32_t nd_pass_grid(const nd_ctx* c, int res, uint64_t units);

@@ -13,7 +13,9 @@ struct NdPassLaunch {
     uint32_t total_rays;
     DevCounters* ctr;
     DevHitBuffers hb;
-    uint32_t grid;               // persistent workgroups (one wave64 each)
+    uint32_t grid_march;         // persistent workgroups (one wave64 each) of each pass
+    uint32_t grid_surface;
+    uint32_t grid_shadow;
     uint32_t chunk;
     uint32_t min_idle;
     uint32_t round_cap;
@@ -28,4 +30,17 @@ struct NdPassLaunch {
 hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, hipStream_t s);
 hipError_t nd_launch_zero_counters(DevCounters* c, hipStream_t s);
 hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s);
-uint32_t nd_march_occupancy();
+
+// Resident wave64 workgroups per CU of each persistent kernel (blockDim 64, no dynamic LDS).
+// "_MIXED" variants take the scene from each job, the others share one scene.
+enum NdResident {
+    RES_MARCH, RES_MARCH_MIXED, RES_MARCH_ACC, RES_MARCH_ACC_MIXED,
+    RES_SURFACE, RES_SURFACE_MIXED,
+    RES_SHADOW, RES_SHADOW_MIXED, RES_SHADOW_ACC, RES_SHADOW_ACC_MIXED,
+    RES_ADJOINT, RES_ADJOINT_MIXED,
+    RES_MC_EXTEND, RES_MC_SHADE, RES_MC_SHADOW, RES_MC_IND_MARCH, RES_MC_IND_SHADE,
+    RES_COUNT
+};
+void nd_resident_march(uint32_t* res);    // fills RES_MARCH .. RES_SHADOW_ACC_MIXED
+void nd_resident_adjoint(uint32_t* res);  // RES_ADJOINT, RES_ADJOINT_MIXED
+void nd_resident_mc(uint32_t* res);       // RES_MC_*

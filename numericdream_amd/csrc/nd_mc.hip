@@ -872,6 +872,20 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     S.max_dist = u.max_dist;
 }
 
+template <typename K>
+static uint32_t mc_resident(K kernel) {
+    int b = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, 64, 0) == hipSuccess && b > 0 ? (uint32_t)b : 8u;
+}
+
+void nd_resident_mc(uint32_t* r) {
+    r[RES_MC_EXTEND] = mc_resident(nd::k_mc_extend);
+    r[RES_MC_SHADE] = mc_resident(nd::k_mc_shade);
+    r[RES_MC_SHADOW] = mc_resident(nd::k_mc_shadow);
+    r[RES_MC_IND_MARCH] = mc_resident(nd::k_mc_indirect_march);
+    r[RES_MC_IND_SHADE] = mc_resident(nd::k_mc_indirect_shade);
+}
+
 extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size, uint32_t spp, nd_rect rect,
                                    const nd_image_out* out, nd_trace_stats* stats) {
     if (!c) return ND_E_INVALID;
@@ -961,7 +975,11 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     a.npix = npix;
     a.chunk = 64;  // bounce queues are short: small claims balance better
     a.out_rgb = d_rgb; a.out_depth = d_depth; a.out_normal = d_normal;
-    const uint32_t grid = nd_persistent_grid(c);
+    // each pass at its kernel's residency; claims of 64 paths over the pool
+    const uint64_t claims = ((uint64_t)P + 63) / 64;
+    const uint32_t g_ext = nd_pass_grid(c, RES_MC_EXTEND, claims), g_shade = nd_pass_grid(c, RES_MC_SHADE, claims);
+    const uint32_t g_shadow = nd_pass_grid(c, RES_MC_SHADOW, claims), g_im = nd_pass_grid(c, RES_MC_IND_MARCH, claims);
+    const uint32_t g_is = nd_pass_grid(c, RES_MC_IND_SHADE, claims);
     const int max_depth = S.max_depth;
     const bool indirect = S.indirect_on != 0;
     // The passes of one chunk, in stream order. Captured once into a graph per shape and
@@ -973,12 +991,12 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         int ping = 0;
         for (int depth = 0; depth < max_depth; ++depth) {
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 1);
-            hipLaunchKernelGGL(nd::k_mc_extend, dim3(grid), dim3(64), 0, st, a, ping);
-            hipLaunchKernelGGL(nd::k_mc_shade, dim3(grid), dim3(64), 0, st, a, ping);
-            hipLaunchKernelGGL(nd::k_mc_shadow, dim3(grid), dim3(64), 0, st, a);
+            hipLaunchKernelGGL(nd::k_mc_extend, dim3(g_ext), dim3(64), 0, st, a, ping);
+            hipLaunchKernelGGL(nd::k_mc_shade, dim3(g_shade), dim3(64), 0, st, a, ping);
+            hipLaunchKernelGGL(nd::k_mc_shadow, dim3(g_shadow), dim3(64), 0, st, a);
             if (depth == 0 && indirect) {
-                hipLaunchKernelGGL(nd::k_mc_indirect_march, dim3(grid), dim3(64), 0, st, a);
-                hipLaunchKernelGGL(nd::k_mc_indirect_shade, dim3(grid), dim3(64), 0, st, a);
+                hipLaunchKernelGGL(nd::k_mc_indirect_march, dim3(g_im), dim3(64), 0, st, a);
+                hipLaunchKernelGGL(nd::k_mc_indirect_shade, dim3(g_is), dim3(64), 0, st, a);
             }
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 2);
             ping ^= 1;
@@ -990,14 +1008,15 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         // everything the captured launches bake in: kernel arguments and launch shapes
         struct McGraphKey {
             nd::McArgs args;
-            uint32_t grid;
+            uint32_t grid[5];
             int max_depth, indirect;
             uint64_t paths;
         } key;
         static_assert(sizeof(McGraphKey) <= sizeof(c->mc_graph_key), "graph key storage");
         std::memset(&key, 0, sizeof(key));
         key.args = a;
-        key.grid = grid; key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
+        key.grid[0] = g_ext; key.grid[1] = g_shade; key.grid[2] = g_shadow; key.grid[3] = g_im; key.grid[4] = g_is;
+        key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
         if (!c->mc_graph_exec || !c->graph_valid || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
             if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }
             hipGraph_t g = nullptr;
