@@ -58,14 +58,63 @@ typedef enum nd_status {
     ND_E_UNSUPPORTED = 7           /* a Uniforms feature outside this library (accel, warps, fog...) */
 } nd_status;
 
-/* Built-in distance estimators (the four formula plug-ins on the path). */
+/* Built-in distance estimators: one per formula plug-in (a formulas/*.wgsl file, or the
+ * built-in mandelbrot.wgsl), with the @param slots of that file. 0-3 are the tuned ones. */
 typedef enum nd_builtin_formula {
     ND_FORMULA_NONE = -1,
     ND_FORMULA_MANDELBULB = 0,  /* src/gpu/shaders/mandelbrot.wgsl */
     ND_FORMULA_MANDELBOX = 1,   /* formulas/amazingbox_box.wgsl */
     ND_FORMULA_MENGER3 = 2,     /* formulas/menger3_ifs.wgsl */
-    ND_FORMULA_KLEINIAN = 3     /* formulas/kleinian_kleinian.wgsl */
+    ND_FORMULA_KLEINIAN = 3,    /* formulas/kleinian_kleinian.wgsl */
+    ND_FORMULA_SPHERE = 4,             /* formulas/sphere_primitive.wgsl */
+    ND_FORMULA_CUBE = 5,               /* formulas/cube_primitive.wgsl */
+    ND_FORMULA_CYLINDER = 6,           /* formulas/cylinder_primitive.wgsl */
+    ND_FORMULA_TORUS = 7,              /* formulas/torus_primitive.wgsl */
+    ND_FORMULA_PYRAMID = 8,            /* formulas/pyramid_primitive.wgsl */
+    ND_FORMULA_PRISM = 9,              /* formulas/prism_primitive.wgsl */
+    ND_FORMULA_SIERPINSKI3 = 10,       /* formulas/sierpinski3_ifs.wgsl */
+    ND_FORMULA_CUBEJULIA = 11,         /* formulas/cubejulia_ifs_julia.wgsl */
+    ND_FORMULA_JULIAKIFS = 12,         /* formulas/juliakifs_ifs_julia.wgsl */
+    ND_FORMULA_DIAGONALTETRA = 13,     /* formulas/diagonaltetra_ifs_fold.wgsl */
+    ND_FORMULA_STARKIFS = 14,          /* formulas/starkifs_ifs_fold.wgsl */
+    ND_FORMULA_KALEIDOJULIA = 15,      /* formulas/kaleidojulia_ifs_fold_julia.wgsl */
+    ND_FORMULA_BULBTETRA = 16,         /* formulas/bulbtetra_hybrid_bulb_ifs.wgsl */
+    ND_FORMULA_ICOSAHEDRALIFS = 17,    /* formulas/icosahedralifs_ifs.wgsl */
+    ND_FORMULA_ICOSAHEDRALJULIA = 18,  /* formulas/icosahedraljulia_ifs_julia.wgsl */
+    ND_FORMULA_ICOSAHEDRALTWIST = 19,  /* formulas/icosahedraltwist_ifs_twist.wgsl */
+    ND_FORMULA_DIAGONALICOSA = 20,     /* formulas/diagonalicosa_ifs_fold.wgsl */
+    ND_FORMULA_HYBRIDKIFS = 21,        /* formulas/hybridkifs_hybrid_ifs.wgsl */
+    ND_FORMULA_KALEIDOSCOPICIFS = 22,  /* formulas/kaleidoscopicifs_ifs_twist.wgsl */
+    ND_FORMULA_JULIABOX = 23,          /* formulas/juliabox_box_julia.wgsl */
+    ND_FORMULA_AMAZINGSURF = 24,       /* formulas/amazingsurf_box_fold.wgsl */
+    ND_FORMULA_MANDELBOXFOAM = 25,     /* formulas/mandelboxfoam_box_fold.wgsl */
+    ND_FORMULA_MANDELBOXSORT = 26,     /* formulas/mandelboxsort_box_fold.wgsl */
+    ND_FORMULA_MANDELBOXDIAG = 27,     /* formulas/mandelboxdiag_box_fold.wgsl */
+    ND_FORMULA_MANDELBOXINV = 28,      /* formulas/mandelboxinv_box_fold.wgsl */
+    ND_FORMULA_MANDELBOXTWIST = 29,    /* formulas/mandelboxtwist_box_twist.wgsl */
+    ND_FORMULA_PRISMKIFS = 30,         /* formulas/prismkifs_box_fold.wgsl */
+    ND_FORMULA_BUFFALO = 31,           /* formulas/buffalo_box.wgsl */
+    ND_FORMULA_BUFFALOJULIA = 32,      /* formulas/buffalojulia_box_julia.wgsl */
+    ND_FORMULA_BUFFALOTWIST = 33,      /* formulas/buffalotwist_box_twist.wgsl */
+    ND_FORMULA_APOLLONIAN = 34,        /* formulas/apollonian_kleinian.wgsl */
+    ND_FORMULA_APOLLONIANJULIA = 35,   /* formulas/apollonianjulia_kleinian_julia.wgsl */
+    ND_FORMULA_KLEINIANJULIA = 36,     /* formulas/kleinianjulia_kleinian_julia.wgsl */
+    ND_FORMULA_PSEUDOKLEINIAN = 37,    /* formulas/pseudokleinian_kleinian.wgsl */
+    ND_FORMULA_JULIABULB = 38,         /* formulas/juliabulb_bulb_julia.wgsl */
+    ND_FORMULA_MANDELBULBMIRROR = 39,  /* formulas/mandelbulbmirror_bulb_fold.wgsl */
+    ND_FORMULA_MANDELBULBTWIST = 40,   /* formulas/mandelbulbtwist_bulb_twist.wgsl */
+    ND_FORMULA_MANDELBULBDUAL = 41,    /* formulas/mandelbulbdual_hybrid_bulb.wgsl */
+    ND_FORMULA_QUATJULIA = 42,         /* formulas/quatjulia_bulb_julia.wgsl */
+    ND_FORMULA_BULBBOX = 43,           /* formulas/bulbbox_hybrid_bulb_box.wgsl */
+    ND_FORMULA_MANDALAY = 44,          /* formulas/mandalay_box_fold.wgsl */
+    ND_FORMULA_WOBBLEKIFS = 45,        /* formulas/wobblekifs_ifs_twist_julia.wgsl */
+    ND_FORMULA_RADIOACTIVE = 46,       /* formulas/radioactive_ifs_carve_twist.wgsl */
+    ND_FORMULA_RECURSIVESPHERE = 47,   /* formulas/recursivesphere_ifs_carve_twist.wgsl */
+    ND_FORMULA_TREE = 48,              /* formulas/tree_ifs.wgsl */
+    ND_FORMULA_KLEINIANLIMIT = 49,     /* formulas/kleinianlimit_kleinian.wgsl */
+    ND_FORMULA_FBMSDF = 50             /* formulas/fbmsdf_primitive_carve.wgsl */
 } nd_builtin_formula;
+#define ND_FORMULA_COUNT 51
 
 #define ND_MAX_CODES 16
 

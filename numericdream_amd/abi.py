@@ -165,6 +165,24 @@ STATUS_NAMES = {0: "ND_OK", 1: "ND_E_INVALID", 2: "ND_E_UNSUPPORTED_FORMULA", 3:
 NORMAL_ANALYTIC, NORMAL_TETRA, NORMAL_CENTRAL = 0, 1, 2
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1, 2
 FORMULA_MANDELBULB, FORMULA_MANDELBOX, FORMULA_MENGER3, FORMULA_KLEINIAN = 0, 1, 2, 3
+# nd_builtin_formula id -> the formula file it implements (include/nd_hip.h); a host maps
+# each entry of its formula code table to an id by the file it was loaded from.
+FORMULA_FILES = [
+    "src/gpu/shaders/mandelbrot.wgsl", "formulas/amazingbox_box.wgsl", "formulas/menger3_ifs.wgsl",
+    "formulas/kleinian_kleinian.wgsl"] + ["formulas/%s.wgsl" % f for f in (
+        "sphere_primitive", "cube_primitive", "cylinder_primitive", "torus_primitive", "pyramid_primitive",
+        "prism_primitive", "sierpinski3_ifs", "cubejulia_ifs_julia", "juliakifs_ifs_julia", "diagonaltetra_ifs_fold",
+        "starkifs_ifs_fold", "kaleidojulia_ifs_fold_julia", "bulbtetra_hybrid_bulb_ifs", "icosahedralifs_ifs",
+        "icosahedraljulia_ifs_julia", "icosahedraltwist_ifs_twist", "diagonalicosa_ifs_fold", "hybridkifs_hybrid_ifs",
+        "kaleidoscopicifs_ifs_twist", "juliabox_box_julia", "amazingsurf_box_fold", "mandelboxfoam_box_fold",
+        "mandelboxsort_box_fold", "mandelboxdiag_box_fold", "mandelboxinv_box_fold", "mandelboxtwist_box_twist",
+        "prismkifs_box_fold", "buffalo_box", "buffalojulia_box_julia", "buffalotwist_box_twist",
+        "apollonian_kleinian", "apollonianjulia_kleinian_julia", "kleinianjulia_kleinian_julia",
+        "pseudokleinian_kleinian", "juliabulb_bulb_julia", "mandelbulbmirror_bulb_fold", "mandelbulbtwist_bulb_twist",
+        "mandelbulbdual_hybrid_bulb", "quatjulia_bulb_julia", "bulbbox_hybrid_bulb_box", "mandalay_box_fold",
+        "wobblekifs_ifs_twist_julia", "radioactive_ifs_carve_twist", "recursivesphere_ifs_carve_twist", "tree_ifs",
+        "kleinianlimit_kleinian", "fbmsdf_primitive_carve")]
+FORMULA_COUNT = len(FORMULA_FILES)
 PARAM_COUNT = 6
 
 # every symbol the headers declare (include/nd_hip.h, include/nd_config.h)

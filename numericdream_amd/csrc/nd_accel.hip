@@ -23,6 +23,7 @@ struct AccelDesc {
 // One thread per voxel, voxels in [level][z][y][x] order: the safe skip distance
 // max(DE(centre) - HALF_SQRT3 cell, 0) * safety. The centre and the bound are formed with
 // separate roundings (no fused multiply-add), as the scalar restatement forms them.
+template <bool EXT>
 __global__ void k_accel_build(const DevScene* __restrict__ sc, float* __restrict__ grid, AccelDesc d, uint32_t n) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
@@ -33,7 +34,7 @@ __global__ void k_accel_build(const DevScene* __restrict__ sc, float* __restrict
     const V3<float> c = mk3(__fadd_rn(d.prm[level][0], __fmul_rn((float)x + 0.5f, cell)),
                             __fadd_rn(d.prm[level][1], __fmul_rn((float)y + 0.5f, cell)),
                             __fadd_rn(d.prm[level][2], __fmul_rn((float)cz + 0.5f, cell)));
-    const float de = scene_de<false>(*sc, c, nullptr);
+    const float de = scene_de<false, EXT>(*sc, c, nullptr);
     grid[idx] = __fmul_rn(fmaxf(__fsub_rn(de, __fmul_rn(0.8660254f, cell)), 0.f), d.safety);
 }
 
@@ -100,8 +101,12 @@ nd_status nd_accel_prepare(nd_ctx* c, DevJob* jobs, uint32_t n) {
             d.levels = j.accel_levels;
             d.safety = j.accel_safety;
             const size_t cnt = (size_t)d.res * d.res * d.res * d.levels;
-            hipLaunchKernelGGL(nd::k_accel_build, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, c->stream,
-                               (const DevScene*)c->accel_scenes.p + u, (float*)c->accel.p + off, d, (uint32_t)cnt);
+            const dim3 g((uint32_t)((cnt + 255) / 256));
+            const DevScene* dsc = (const DevScene*)c->accel_scenes.p + u;
+            if (nd_scene_ext(j.scene))
+                hipLaunchKernelGGL(nd::k_accel_build<true>, g, dim3(256), 0, c->stream, dsc, (float*)c->accel.p + off, d, (uint32_t)cnt);
+            else
+                hipLaunchKernelGGL(nd::k_accel_build<false>, g, dim3(256), 0, c->stream, dsc, (float*)c->accel.p + off, d, (uint32_t)cnt);
             if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch accelerator build");
             c->accel_cache.push_back(NdAccelEntry{uniq[u], off, cnt});
             off += cnt;

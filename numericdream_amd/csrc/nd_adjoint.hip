@@ -35,7 +35,7 @@ struct AdjArgs {
     double* acc;               // [loss, 6 gradient components, pixels used]
 };
 
-template <bool SHARED>
+template <bool SHARED, bool EXT>
 __global__ void __launch_bounds__(64) k_adjoint(AdjArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     using D6 = DualN<6>;
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(64) k_adjoint(AdjArgs a, const DevJob* __restr
         const float t = a.hb.t[i];
         const V3<float> hp = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
         V3<D6> q = mk3(dvar<6>(hp.x, 0), dvar<6>(hp.y, 1), dvar<6>(hp.z, 2));
-        const D6 d = scene_de_d<D6>(sc, q);
+        const D6 d = scene_de_d<D6, EXT>(sc, q);
         const float den = d.d[0] * dir.x + d.d[1] * dir.y + d.d[2] * dir.z - J.eps_c;
         float g[6] = {0, 0, 0, 0, 0, 0};
         if (den != 0.f) {
@@ -102,10 +102,13 @@ __global__ void k_fill_f64(double* p, int n) {
 
 void nd_resident_adjoint(uint32_t* r) {
     int b = 0;
-    r[RES_ADJOINT] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_adjoint<true>, 64, 0) == hipSuccess && b > 0 ? b : 8;
+    r[RES_ADJOINT] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_adjoint<true, false>, 64, 0) == hipSuccess && b > 0 ? b : 8;
     b = 0;
     r[RES_ADJOINT_MIXED] =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_adjoint<false>, 64, 0) == hipSuccess && b > 0 ? b : 8;
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_adjoint<false, false>, 64, 0) == hipSuccess && b > 0 ? b : 8;
+    b = 0;
+    r[RES_ADJOINT_EXT] =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_adjoint<false, true>, 64, 0) == hipSuccess && b > 0 ? b : 8;
 }
 
 extern "C" nd_status nd_adjoint(nd_ctx* c, const nd_uniforms* views, uint32_t n, const nd_adjoint_io* io,
@@ -181,9 +184,11 @@ extern "C" nd_status nd_adjoint(nd_ctx* c, const nd_uniforms* views, uint32_t n,
     a.acc = acc;
     (void)hipEventRecord(c->ev[4], c->stream);
     // grid-stride over the hits, at most one per ray of the batch
-    const uint32_t grid = nd_pass_grid(c, a.shared ? RES_ADJOINT : RES_ADJOINT_MIXED, ((uint64_t)c->last_rays + 63) / 64);
-    if (a.shared) hipLaunchKernelGGL(nd::k_adjoint<true>, dim3(grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
-    else hipLaunchKernelGGL(nd::k_adjoint<false>, dim3(grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
+    const int res = c->last_ext ? RES_ADJOINT_EXT : a.shared ? RES_ADJOINT : RES_ADJOINT_MIXED;
+    const uint32_t grid = nd_pass_grid(c, res, ((uint64_t)c->last_rays + 63) / 64);
+    if (c->last_ext) hipLaunchKernelGGL((nd::k_adjoint<false, true>), dim3(grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
+    else if (a.shared) hipLaunchKernelGGL((nd::k_adjoint<true, false>), dim3(grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
+    else hipLaunchKernelGGL((nd::k_adjoint<false, false>), dim3(grid), dim3(64), 0, c->stream, a, a.jobs, a.shared);
     if ((e = hipGetLastError()) != hipSuccess) return nd_hip_fail(c, e, "launch adjoint");
     (void)hipEventRecord(c->ev[5], c->stream);
     double h_acc[8];

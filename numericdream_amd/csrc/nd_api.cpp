@@ -61,6 +61,65 @@ void DevBuf::release() {
 // ----------------------------------------------------------------------------- packing
 static int i32(float x) { return (int)x; }
 
+// The @param slots each formula file reads outside its step arithmetic: its de_iterations
+// count (-1: one iteration), the degrees of its rotate_xyz_deg (-1: a literal 0, all -1: no
+// rotation) and of its twist about z (-1: none). From the @param lines and de_iterations
+// of each formula file.
+struct FormulaSlots { int8_t iters, rx, ry, rz, twist; };
+static const FormulaSlots kSlots[ND_FORMULA_COUNT] = {
+    {1, -1, -1, -1, -1},   // mandelbrot.wgsl:3-5
+    {4, 5, 6, 7, -1},      // amazingbox_box.wgsl:13-15, 29
+    {0, 5, 6, 7, -1},      // menger3_ifs.wgsl:9-11, 17
+    {6, -1, -1, -1, -1},   // kleinian_kleinian.wgsl:12-14
+    {-1, -1, -1, -1, -1},  // sphere_primitive.wgsl:2-4
+    {-1, -1, -1, -1, -1},  // cube_primitive.wgsl:2-4
+    {-1, -1, -1, -1, -1},  // cylinder_primitive.wgsl:2-4
+    {-1, -1, -1, -1, -1},  // torus_primitive.wgsl:3-5
+    {-1, -1, -1, -1, -1},  // pyramid_primitive.wgsl:5-7
+    {-1, -1, -1, -1, -1},  // prism_primitive.wgsl:4-6
+    {1, 5, 6, 7, -1},      // sierpinski3_ifs.wgsl:9-11, 17
+    {4, 5, 6, 7, -1},      // cubejulia_ifs_julia.wgsl:9-11, 17
+    {4, 5, 6, 7, -1},      // juliakifs_ifs_julia.wgsl:9-11, 17
+    {1, 5, 6, 7, -1},      // diagonaltetra_ifs_fold.wgsl:9-11, 25
+    {2, 6, 7, 8, -1},      // starkifs_ifs_fold.wgsl:12-14, 31
+    {5, 6, 7, 8, -1},      // kaleidojulia_ifs_fold_julia.wgsl:12-14, 31
+    {2, -1, -1, -1, -1},   // bulbtetra_hybrid_bulb_ifs.wgsl:9-11
+    {3, 5, 6, 7, -1},      // icosahedralifs_ifs.wgsl:9-11, 36
+    {6, 8, 9, 10, -1},     // icosahedraljulia_ifs_julia.wgsl:12-14, 40
+    {4, 6, 7, -1, 3},      // icosahedraltwist_ifs_twist.wgsl:9-11, 16, 41
+    {3, 5, 6, 7, -1},      // diagonalicosa_ifs_fold.wgsl:9-11, 43
+    {1, 5, 6, 7, -1},      // hybridkifs_hybrid_ifs.wgsl:9-11, 21
+    {2, 6, 7, -1, 1},      // kaleidoscopicifs_ifs_twist.wgsl:9-11, 15, 28
+    {7, 8, 9, 10, -1},     // juliabox_box_julia.wgsl:12-14, 29
+    {4, 5, 6, 7, -1},      // amazingsurf_box_fold.wgsl:13-15, 29
+    {4, 5, 6, 7, -1},      // mandelboxfoam_box_fold.wgsl:13-15, 29
+    {4, 5, 6, 7, -1},      // mandelboxsort_box_fold.wgsl:13-15, 32
+    {4, 5, 6, 7, -1},      // mandelboxdiag_box_fold.wgsl:13-15, 37
+    {5, 6, 7, 8, -1},      // mandelboxinv_box_fold.wgsl:14-16, 38
+    {5, 6, 7, -1, 4},      // mandelboxtwist_box_twist.wgsl:13-15, 22, 34
+    {5, 6, 7, 8, -1},      // prismkifs_box_fold.wgsl:16-18, 44
+    {3, 4, 5, 6, -1},      // buffalo_box.wgsl:12-14, 28
+    {6, 7, 8, 9, -1},      // buffalojulia_box_julia.wgsl:11-13, 28
+    {4, 5, 6, -1, 3},      // buffalotwist_box_twist.wgsl:12-14, 20, 33
+    {3, 4, 5, 6, -1},      // apollonian_kleinian.wgsl:12-14, 28
+    {6, 7, 8, 9, -1},      // apollonianjulia_kleinian_julia.wgsl:11-13, 28
+    {7, -1, -1, -1, -1},   // kleinianjulia_kleinian_julia.wgsl:10-12
+    {5, -1, -1, -1, -1},   // pseudokleinian_kleinian.wgsl:11-13
+    {1, -1, -1, -1, -1},   // juliabulb_bulb_julia.wgsl:8-10
+    {1, -1, -1, -1, -1},   // mandelbulbmirror_bulb_fold.wgsl:9-11
+    {1, -1, -1, -1, 2},    // mandelbulbtwist_bulb_twist.wgsl:10-12, 16
+    {1, -1, -1, -1, -1},   // mandelbulbdual_hybrid_bulb.wgsl:10-12
+    {5, -1, -1, -1, -1},   // quatjulia_bulb_julia.wgsl:9-11
+    {5, 8, 9, 10, -1},     // bulbbox_hybrid_bulb_box.wgsl:16-18, 44
+    {7, -1, -1, -1, -1},   // mandalay_box_fold.wgsl:23-25
+    {1, -1, -1, -1, -1},   // wobblekifs_ifs_twist_julia.wgsl:27-29
+    {0, -1, -1, -1, -1},   // radioactive_ifs_carve_twist.wgsl:12-14
+    {0, -1, -1, -1, -1},   // recursivesphere_ifs_carve_twist.wgsl:18-20
+    {0, -1, -1, -1, -1},   // tree_ifs.wgsl:15-17
+    {5, -1, -1, -1, -1},   // kleinianlimit_kleinian.wgsl:21-23
+    {0, -1, -1, -1, -1},   // fbmsdf_primitive_carve.wgsl:29-31
+};
+
 static void hoist(DevInst& in) {
     std::memset(in.h, 0, sizeof(in.h));
     const float* p = in.p;
@@ -96,10 +155,16 @@ static void hoist(DevInst& in) {
             in.h[11] = std::fmax(std::fabs(p[7]), 1.0f);
             break;
     }
-    // formula-internal Euler rotation (degrees): amazingbox p[5..7], menger3 p[5..7]
+    // formula-internal Euler rotation (degrees) and twist angle
     in.rot_on = 0;
-    if (in.formula == ND_FORMULA_MANDELBOX || in.formula == ND_FORMULA_MENGER3) {
-        float rx = p[5], ry = p[6], rz = p[7];
+    if (in.formula >= 0 && in.formula < ND_FORMULA_COUNT) {
+        const FormulaSlots& fs = kSlots[in.formula];
+        const float k = 3.14159265358979323846f / 180.0f;
+        if (fs.twist >= 0) {
+            in.h[14] = std::cos(p[fs.twist] * k);
+            in.h[15] = std::sin(p[fs.twist] * k);
+        }
+        float rx = fs.rx >= 0 ? p[fs.rx] : 0.0f, ry = fs.ry >= 0 ? p[fs.ry] : 0.0f, rz = fs.rz >= 0 ? p[fs.rz] : 0.0f;
         if (!(rx == 0.0f && ry == 0.0f && rz == 0.0f)) {
             in.rot_on = 1;
             const float k = 3.14159265358979323846f / 180.0f;
@@ -110,14 +175,15 @@ static void hoist(DevInst& in) {
     }
 }
 
+bool nd_scene_ext(const DevScene& s) {
+    for (int i = 0; i < s.count && i < ND_MAX_INSTANCES; ++i)
+        if (!s.inst[i].hidden && s.inst[i].formula > ND_FORMULA_KLEINIAN) return true;
+    return false;
+}
+
 static int formula_iters(int f, const float* p) {
-    switch (f) {
-        case ND_FORMULA_MANDELBULB: return i32(p[1]);  // mandelbrot.wgsl:3-5
-        case ND_FORMULA_MANDELBOX: return i32(p[4]);   // amazingbox_box.wgsl:13-15
-        case ND_FORMULA_MENGER3: return i32(p[0]);     // menger3_ifs.wgsl:9-11
-        case ND_FORMULA_KLEINIAN: return i32(p[6]);    // kleinian_kleinian.wgsl:12-14
-    }
-    return 0;
+    if (f < 0 || f >= ND_FORMULA_COUNT) return 0;
+    return kSlots[f].iters >= 0 ? i32(p[kSlots[f].iters]) : 1;
 }
 
 nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
@@ -327,7 +393,7 @@ nd_status nd_bind_formulas(nd_ctx* c, const int32_t* map, uint32_t n) {
     if (!c) return ND_E_INVALID;
     if ((!map && n) || n > ND_MAX_CODES) return nd_fail(c, ND_E_INVALID, "code table of %u entries (max %d)", n, ND_MAX_CODES);
     for (uint32_t i = 0; i < n; ++i)
-        if (map[i] < -1 || map[i] > ND_FORMULA_KLEINIAN)
+        if (map[i] < -1 || map[i] >= ND_FORMULA_COUNT)
             return nd_fail(c, ND_E_INVALID, "code %u maps to unknown built-in %d", i, map[i]);
     for (int i = 0; i < ND_MAX_CODES; ++i) c->code_map[i] = (uint32_t)i < n ? map[i] : -1;
     return ND_OK;
@@ -452,7 +518,10 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
 
     bool shared = true;
     for (uint32_t i = 1; i < n && shared; ++i) shared = std::memcmp(&hj[i].scene, &hj[0].scene, sizeof(DevScene)) == 0;
-    c->last_shared = shared;
+    bool ext = false;
+    for (uint32_t i = 0; i < n && !ext; ++i) ext = nd_scene_ext(hj[i].scene);
+    c->last_shared = shared && !ext;  // the EXT kernels read each job's own scene
+    c->last_ext = ext;
     if ((st = nd_accel_prepare(c, hj.data(), n)) != ND_OK) return st;
 
     if ((st = c->jobs.ensure(c, sizeof(DevJob) * n)) != ND_OK) return st;
@@ -481,7 +550,7 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         (void)hipEventRecord(c->ev[0], c->stream);
         for (uint32_t i = 0; i < n; ++i) {
             e = nd_launch_plain((const DevJob*)c->jobs.p + i, hj[i].tiles_x, hj[i].tiles_y, (DevCounters*)c->ctr.p,
-                                c->stream);
+                                ext, c->stream);
             if (e != hipSuccess) return nd_hip_fail(c, e, "launch plain march");
         }
         (void)hipEventRecord(c->ev[3], c->stream);
@@ -491,8 +560,12 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         const bool acc = [&] { for (uint32_t i = 0; i < n; ++i) if (hj[i].accel) return true; return false; }();
         const uint32_t chunk = c->tuning.chunk_rays;
         const uint64_t claims = ((uint64_t)total + chunk - 1) / chunk;
-        const uint32_t grid_march = nd_pass_grid(
-            c, acc ? (shared ? RES_MARCH_ACC : RES_MARCH_ACC_MIXED) : (shared ? RES_MARCH : RES_MARCH_MIXED), claims);
+        const int res_march = ext ? (acc ? RES_MARCH_ACC_EXT : RES_MARCH_EXT)
+                                  : acc ? (shared ? RES_MARCH_ACC : RES_MARCH_ACC_MIXED) : (shared ? RES_MARCH : RES_MARCH_MIXED);
+        const int res_shadow = ext ? (acc ? RES_SHADOW_ACC_EXT : RES_SHADOW_EXT)
+                                   : acc ? (shared ? RES_SHADOW_ACC : RES_SHADOW_ACC_MIXED) : (shared ? RES_SHADOW : RES_SHADOW_MIXED);
+        const int res_surface = ext ? RES_SURFACE_EXT : (shared ? RES_SURFACE : RES_SURFACE_MIXED);
+        const uint32_t grid_march = nd_pass_grid(c, res_march, claims);
         size_t hit_cap = (size_t)total + (size_t)grid_march * 64;
         size_t hb_bytes = hit_cap * (4 + 2 + 4 + 4 + 2 + 7 * 4) + 16 * 256;
         if ((st = c->hits.ensure(c, hb_bytes)) != ND_OK) return st;
@@ -519,9 +592,8 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.total_rays = (uint32_t)total;
         L.ctr = (DevCounters*)c->ctr.p;
         L.grid_march = grid_march;
-        L.grid_surface = nd_pass_grid(c, shared ? RES_SURFACE : RES_SURFACE_MIXED, ((uint64_t)total + 63) / 64);
-        L.grid_shadow = nd_pass_grid(
-            c, acc ? (shared ? RES_SHADOW_ACC : RES_SHADOW_ACC_MIXED) : (shared ? RES_SHADOW : RES_SHADOW_MIXED), claims);
+        L.grid_surface = nd_pass_grid(c, res_surface, ((uint64_t)total + 63) / 64);
+        L.grid_shadow = nd_pass_grid(c, res_shadow, claims);
         c->last_rays = (uint32_t)total;
         c->last_hb = L.hb;
         L.chunk = c->tuning.chunk_rays;
@@ -530,6 +602,7 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.any_shadow = any_shadow;
         L.surface = !ex.skip_surface;
         L.accel = acc;
+        L.ext = ext;
         L.job_done_cycles = track ? (unsigned long long*)c->latency.p : nullptr;
         L.ray_done = track ? (uint32_t*)((char*)c->latency.p + lat_jobs) : nullptr;
         for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];

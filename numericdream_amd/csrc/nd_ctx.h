@@ -45,6 +45,7 @@ struct nd_ctx {
     std::vector<NdAccelEntry> accel_cache;
     std::vector<DevScene> accel_scene_host;
     bool last_shared = false;       // scene of the last batch was shared by all jobs
+    bool last_ext = false;          // the last batch ran the kernels with every formula
     uint32_t last_rays = 0;         // rays of the last persistent batch
     DevHitBuffers last_hb{};        // hit list of the last persistent batch
     std::vector<DevJob> host_jobs;
@@ -62,6 +63,8 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
 nd_status nd_fail(nd_ctx* c, nd_status st, const char* fmt, ...);
 nd_status nd_hip_fail(nd_ctx* c, hipError_t e, const char* what);
 nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s);
+// Some visible instance uses a formula beyond the four tuned ones.
+bool nd_scene_ext(const DevScene& s);
 nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, nd_rect rect, DevJob& j);
 // Builds (or finds cached) the accelerator grids of the jobs that enable it and points their
 // accel fields at them; launches on the context stream.

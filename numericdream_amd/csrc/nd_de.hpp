@@ -1,7 +1,8 @@
 // nd_de.hpp - device distance estimators for gfx950.
 //
-// The four formula plug-ins on the path (mandelbrot.wgsl, formulas/amazingbox_box.wgsl,
-// formulas/menger3_ifs.wgsl, formulas/kleinian_kleinian.wgsl), the formula wrapper with
+// The four tuned formula plug-ins (mandelbrot.wgsl, formulas/amazingbox_box.wgsl,
+// formulas/menger3_ifs.wgsl, formulas/kleinian_kleinian.wgsl; the others are in
+// nd_formulas.hpp), the formula wrapper with
 // orbit traps (fr.zig:490-513, template.wgsl:1207-1289), the instance transform and the
 // seven combine modes (template.wgsl:1600-1779), written as HIP device templates over the
 // scalar type:
@@ -415,12 +416,20 @@ NDD Carry<float> box_loop(Carry<float> c, const V3<float>& posc, const BoxF& b, 
     return c;
 }
 
+}  // namespace nd
+
+#include "nd_formulas.hpp"
+
+namespace nd {
+
 // ----------------------------------------------------------------------------- formula wrapper
 // formula_N (fr.zig:490-513): iterate de_step de_iterations times from (pos, 1), update the
 // trap after every step, finalize. The float path exits early once every lane of the wave
 // has escaped a Mandelbulb (bulb_power_step is the identity after escape, and so is the
 // trap update), which changes no result.
-template <bool TRAP>
+// EXT: the instance may use any formula of nd_formulas.hpp; without it only the four above
+// are compiled in (others give NO_SURFACE_DE) and the kernel keeps their register budget.
+template <bool TRAP, bool EXT = false>
 NDD float formula_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
     Carry<float> c; c.z = pos; c.dr = 1.f;
     const int iters = in.iters;
@@ -483,13 +492,15 @@ NDD float formula_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
             if (TRAP) *trap = trap_finish(acc, in, iters);
             return fdiv(len3(c.z), fabsf(c.dr));
         }
+        default:
+            if constexpr (EXT) return ext_de_f<TRAP>(in, pos, trap);
+            if (TRAP) *trap = 0.f;
+            return 1e6f;  // NO_SURFACE_DE
     }
-    if (TRAP) *trap = 0.f;
-    return 1e6f;  // NO_SURFACE_DE
 }
 
 // Dual-number version (normals, adjoint). The trap is not needed on these paths.
-template <class S>
+template <class S, bool EXT = false>
 NDD S formula_de_d(const DevInst& in, const V3<S>& pos) {
     Carry<S> c; c.z = pos; c.dr = Sc<S>::c(1.f);
     const int iters = in.iters;
@@ -520,8 +531,10 @@ NDD S formula_de_d(const DevInst& in, const V3<S>& pos) {
             for (int it = 0; it < iters; ++it) c = kleinian_step<S>(c, posc, in);
             return sdiv(len3(c.z), sabs(c.dr));
         }
+        default:
+            if constexpr (EXT) return ext_de_d<S>(in, pos);
+            return Sc<S>::c(1e6f);
     }
-    return Sc<S>::c(1e6f);
 }
 
 // ----------------------------------------------------------------------------- instance & scene
@@ -577,7 +590,7 @@ NDD void combine_de(int mode, float d, float di, float k, float* od, float* oh) 
 
 // scene_de (template.wgsl:1756-1779) over a DevScene; TRAP also returns the trap blended
 // like hit_material blends materials (template.wgsl:2128-2155).
-template <bool TRAP>
+template <bool TRAP, bool EXT = false>
 NDD float scene_de(const DevScene& sc, const V3<float>& p, float* trap) {
     float d = sc.empty_de, tr = 0.f;
     bool first = true;
@@ -586,7 +599,7 @@ NDD float scene_de(const DevScene& sc, const V3<float>& p, float* trap) {
         const DevInst& in = sc.inst[i];
         if (in.hidden) continue;
         float ti = 0.f;
-        float di = formula_de_f<TRAP>(in, instance_local<float>(in, p), &ti) * in.de_factor;
+        float di = formula_de_f<TRAP, EXT>(in, instance_local<float>(in, p), &ti) * in.de_factor;
         if (first) {
             d = di; tr = ti; first = false;
         } else {
@@ -612,14 +625,14 @@ NDD float scene_de(const DevScene& sc, const V3<float>& p, float* trap) {
 }
 
 // Dual-number scene DE (union/smooth combines differentiated through their active branch).
-template <class S>
+template <class S, bool EXT = false>
 NDD S scene_de_d(const DevScene& sc, const V3<S>& p) {
     S d = Sc<S>::c(sc.empty_de);
     bool first = true;
     for (int i = 0; i < sc.count; ++i) {
         const DevInst& in = sc.inst[i];
         if (in.hidden) continue;
-        S di = formula_de_d<S>(in, instance_local<S>(in, p)) * in.de_factor;
+        S di = formula_de_d<S, EXT>(in, instance_local<S>(in, p)) * in.de_factor;
         if (first) { d = di; first = false; continue; }
         const float k = in.blend_k;
         const int mode = in.combine;

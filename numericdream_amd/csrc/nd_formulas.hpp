@@ -1,0 +1,1043 @@
+// nd_formulas.hpp - the formula plug-ins beyond the four tuned ones.
+//
+// One struct per formulas/*.wgsl file (nd_builtin_formula 4 and up): de_step as step<S>(),
+// de_finalize as fin<S>(), over the scalar type S of nd_de.hpp (float for the march,
+// shadow and trap; DualN<N> for analytic normals). de_iterations is resolved on the host
+// (DevInst::iters), and so are the formula rotation (rotate_xyz_deg of the rotation
+// @params: DevInst::rot_on / rot_cs) and the cos / sin of a twist @param (h[14], h[15]).
+// Parameters are read with prm<S>(in, k) = FractalInstance.params[k], @param order.
+//
+// Included by nd_de.hpp after the shared helpers, before the formula wrapper.
+#pragma once
+
+#include "nd_hip.h"
+
+namespace nd {
+
+// template.wgsl:59-68
+constexpr float K_PI = 3.14159265f;
+constexpr float K_TAU = 6.28318530718f;
+constexpr float K_EPS = 1e-4f;
+constexpr float K_EPS_FINE = 1e-6f;
+constexpr float K_EPS_TINY = 1e-12f;
+constexpr float K_DEG = 3.14159265358979323846f / 180.0f;  // WGSL radians()
+constexpr float K_INFINITE = 1e30f;
+
+// ----------------------------------------------------------------------------- scalar helpers
+template <class S> NDD S C(float x) { return Sc<S>::c(x); }
+template <class S> NDD S P(const DevInst& in, int k) { return prm<S>(in, k); }
+
+NDD float sfloor(float a) { return floorf(a); }
+template <int N> NDD DualN<N> sfloor(const DualN<N>& a) { return dconst<N>(floorf(a.v)); }
+NDD float sexp(float a) { return expf(a); }
+template <int N> NDD DualN<N> sexp(const DualN<N>& a) { float e = expf(a.v); return dchain(a, e, e); }
+NDD float ssign(float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); }  // WGSL sign
+template <int N> NDD DualN<N> ssign(const DualN<N>& a) { return dconst<N>(ssign(a.v)); }
+template <class S> NDD S ssin(const S& a) { S s, c; ssincos(a, &s, &c); return s; }
+template <class S> NDD S scos(const S& a) { S s, c; ssincos(a, &s, &c); return c; }
+NDD float satan(float a) { return atanf(a); }
+template <int N> NDD DualN<N> satan(const DualN<N>& a) { return dchain(a, atanf(a.v), 1.f / (1.f + a.v * a.v)); }
+NDD float stan(float a) { return tanf(a); }
+template <int N> NDD DualN<N> stan(const DualN<N>& a) {
+    float t = tanf(a.v);
+    return dchain(a, t, 1.f + t * t); }
+// pow(x, y) of WGSL for x > 0
+template <class S> NDD S spowc(const S& x, float y) { return spow(x, C<S>(y)); }
+template <class S> NDD S smix(const S& a, const S& b, const S& t) { return a * (C<S>(1.f) - t) + b * t; }
+
+// The float bits of a value whose low bits carry a level counter (the packed-carry formulas).
+template <class S> NDD uint32_t sbits(const S& x) { return __float_as_uint(vv(x)); }
+NDD float with_value(float, float v) { return v; }
+template <int N> NDD DualN<N> with_value(DualN<N> x, float v) { x.v = v; return x; }
+// select(m, min_mag, |m| < min_mag) with its low bits replaced by code (radioactive_pack
+// and its siblings): the derivative passes through unchanged.
+template <class S> NDD S pack_low(const S& m, uint32_t mask, uint32_t code) {
+    const float v = vv(m);
+    const float safe = fabsf(v) < 1e-30f ? 1e-30f : v;
+    return with_value(m, __uint_as_float((__float_as_uint(safe) & ~mask) | code));
+}
+
+// ----------------------------------------------------------------------------- vec3 helpers
+template <class S> NDD V3<S> vneg(const V3<S>& a) { return mk3(-a.x, -a.y, -a.z); }
+template <class S> NDD V3<S> vabs3(const V3<S>& a) { return mk3(sabs(a.x), sabs(a.y), sabs(a.z)); }
+template <class S> NDD V3<S> vmax0(const V3<S>& a) {
+    return mk3(smax_(a.x, C<S>(0.f)), smax_(a.y, C<S>(0.f)), smax_(a.z, C<S>(0.f))); }
+template <class S> NDD V3<S> vsub1(const V3<S>& a, float s) { return mk3(a.x - s, a.y - s, a.z - s); }
+template <class S> NDD S len2(const S& x, const S& y) { return ssqrt(x * x + y * y); }
+template <class S> NDD V3<S> vrot(const V3<S>& z, const DevInst& in) { return in.rot_on ? rotate_cs(z, in.rot_cs) : z; }
+// rotation about z by the hoisted twist angle (x ca - y sa, x sa + y ca, z)
+template <class S> NDD V3<S> vtwist(const V3<S>& z, const DevInst& in) {
+    const float ca = in.h[14], sa = in.h[15];
+    return mk3(z.x * ca - z.y * sa, z.x * sa + z.y * ca, z.z);
+}
+
+// box_fold (template.wgsl:949-951)
+template <class S> NDD V3<S> box_fold3(const V3<S>& z, const S& L) {
+    const S nL = -L;
+    return mk3(sclamp(z.x, nL, L) * 2.f - z.x, sclamp(z.y, nL, L) * 2.f - z.y, sclamp(z.z, nL, L) * 2.f - z.z);
+}
+// sphere_fold (template.wgsl:959-969), radii squared
+template <class S> NDD void sphere_fold3(V3<S>& z, S& dr, const S& m2, const S& F2) {
+    const S r2 = dot3(z, z);
+    if (vv(r2) < vv(m2)) {
+        const S t = sdiv(F2, m2);
+        z = z * t; dr = dr * t;
+    } else if (vv(r2) < vv(F2)) {
+        const S t = sdiv(F2, r2);
+        z = z * t; dr = dr * t;
+    }
+}
+// sphere_invert_offset (template.wgsl:975-983)
+template <class S> NDD void sphere_invert3(V3<S>& z, S& dr, const V3<S>& ctr, const S& R2) {
+    const V3<S> d = z - ctr;
+    const S r2 = smax_(dot3(d, d), C<S>(K_EPS_FINE));
+    if (vv(r2) < vv(R2)) {
+        const S t = sdiv(R2, r2);
+        z = ctr + d * t; dr = dr * t;
+    }
+}
+// lattice_wrap (template.wgsl:987-989); round() is half to even
+template <class S> NDD V3<S> lattice_wrap3(const V3<S>& p, const S& cell) {
+    return mk3(p.x - cell * srint(sdiv(p.x, cell)), p.y - cell * srint(sdiv(p.y, cell)),
+               p.z - cell * srint(sdiv(p.z, cell)));
+}
+// tetra_fold (template.wgsl:1058-1076)
+template <class S> NDD V3<S> tetra_fold3(V3<S> v) {
+    if (vv(v.x) + vv(v.y) < 0.f) { S t = -v.y; v.y = -v.x; v.x = t; }
+    if (vv(v.x) + vv(v.z) < 0.f) { S t = -v.z; v.z = -v.x; v.x = t; }
+    if (vv(v.y) + vv(v.z) < 0.f) { S t = -v.z; v.z = -v.y; v.y = t; }
+    return v;
+}
+// the x < y swap that opens the diagonal folds
+template <class S> NDD V3<S> swap_xy_desc(V3<S> z) {
+    if (vv(z.x) < vv(z.y)) { S t = z.x; z.x = z.y; z.y = t; }
+    return z;
+}
+// Icosahedral mirrors n1 = normalize(-phi, phi - 1, 1), n2 = normalize(1, -phi, phi + 1),
+// n3 = (0, 0, -1), each applied when dot(z, n) > 0 (formulas/icosahedralifs_ifs.wgsl:18-34).
+constexpr float ICO_N1X = -0.80901699437494742f, ICO_N1Y = 0.30901699437494742f, ICO_N1Z = 0.5f;
+constexpr float ICO_N2X = 0.30901699437494742f, ICO_N2Y = -0.5f, ICO_N2Z = 0.80901699437494742f;
+template <class S> NDD V3<S> mirror_pos(V3<S> z, float nx, float ny, float nz) {
+    const S t = z.x * nx + z.y * ny + z.z * nz;
+    if (vv(t) > 0.f) {
+        const S t2 = t * 2.f;
+        z = mk3(z.x - t2 * nx, z.y - t2 * ny, z.z - t2 * nz);
+    }
+    return z;
+}
+template <class S> NDD V3<S> icosa_mirrors(V3<S> z) {
+    z = mirror_pos(z, ICO_N1X, ICO_N1Y, ICO_N1Z);
+    z = mirror_pos(z, ICO_N2X, ICO_N2Y, ICO_N2Z);
+    return mirror_pos(z, 0.f, 0.f, -1.f);
+}
+// wrap_angle (template.wgsl:1080-1082)
+template <class S> NDD S wrap_angle(const S& a, float period) { return a - sfloor(a * (1.f / period)) * period; }
+// Polar wedge fold of the xy plane into [0, period / 2] (starkifs / kaleidojulia / prismkifs).
+template <class S> NDD V3<S> wedge_fold(const V3<S>& z, float degree) {
+    const float period = K_TAU / degree;
+    S a = wrap_angle(satan2(z.y, z.x), period);
+    if (vv(a) > period * 0.5f) a = C<S>(period) - a;
+    const S r = len2(z.x, z.y);
+    S sa, ca;
+    ssincos(a, &sa, &ca);
+    return mk3(r * ca, r * sa, z.z);
+}
+// z * s - o * (s - 1), the IFS scaling about the offset point
+template <class S> NDD V3<S> ifs_scale(const V3<S>& z, const S& s, const V3<S>& o) {
+    const S sm1 = s - 1.f;
+    return mk3(z.x * s - o.x * sm1, z.y * s - o.y * sm1, z.z * s - o.z * sm1);
+}
+template <class S> NDD V3<S> p3(const DevInst& in, int k0, int k1, int k2) { return mk3(P<S>(in, k0), P<S>(in, k1), P<S>(in, k2)); }
+
+// Spherical power map of the bulb family: dir(theta * power + tp, phi * power + pp) for the
+// unit direction of z (bulb_power_step, template.wgsl:1019-1031, with the phase @params).
+template <class S> NDD V3<S> bulb_dir(const V3<S>& z, const S& r, const S& power, float tp, float pp) {
+    const S theta = sacos(sclamp(sdiv(z.z, r), C<S>(-1.f), C<S>(1.f))) * power + tp;
+    const S phi = satan2(z.y, z.x) * power + pp;
+    S st, ct, sp, cp;
+    ssincos(theta, &st, &ct);
+    ssincos(phi, &sp, &cp);
+    return mk3(st * cp, st * sp, ct);
+}
+
+// ----------------------------------------------------------------------------- finalizers
+template <class S> NDD S fin_scaled(const Carry<S>& c) { return sdiv(len3(c.z), sabs(c.dr)); }  // template.wgsl:993-995
+template <class S> NDD S fin_plain(const Carry<S>& c) { return sdiv(len3(c.z), c.dr); }         // template.wgsl:999-1001
+template <class S> NDD S fin_unit_box(const Carry<S>& c) { return sdiv(len3(vmax0(vsub1(vabs3(c.z), 1.f))), c.dr); }
+template <class S> NDD S prim_div(const S& d, const Carry<S>& c) { return sdiv(d, smax_(sabs(c.dr), C<S>(K_EPS_FINE))); }
+
+// primitive_de_step (template.wgsl:1105-1109)
+template <class S> NDD Carry<S> prim_step(const Carry<S>& c, const S& size) {
+    const S unit = sdiv(C<S>(1.f), smax_(size, C<S>(K_EPS)));
+    const S f = sdiv(unit, smax_(sabs(c.dr), C<S>(K_EPS_FINE)));
+    Carry<S> o; o.z = c.z * f; o.dr = unit;
+    return o;
+}
+
+#define ND_STEP template <class S> static NDD Carry<S> step(const Carry<S>& c, const V3<S>& pos, const DevInst& in)
+#define ND_FIN template <class S> static NDD S fin(const Carry<S>& c)
+#define ND_RET(zz, dd) do { Carry<S> o_; o_.z = (zz); o_.dr = (dd); return o_; } while (0)
+
+// ============================================================================= primitives
+// formulas/sphere_primitive.wgsl:6-12
+struct FSphere {
+    ND_STEP { (void)pos; return prim_step(c, P<S>(in, 0)); }
+    ND_FIN { return prim_div(len3(c.z) - 1.f, c); }
+};
+// formulas/cube_primitive.wgsl:6-15
+struct FCube {
+    ND_STEP { (void)pos; return prim_step(c, P<S>(in, 0)); }
+    ND_FIN {
+        const V3<S> q = vsub1(vabs3(c.z), 1.f);
+        const S inside = smin_(smax_(q.x, smax_(q.y, q.z)), C<S>(0.f));
+        return prim_div(len3(vmax0(q)) + inside, c);
+    }
+};
+// formulas/cylinder_primitive.wgsl:6-15
+struct FCylinder {
+    ND_STEP { (void)pos; return prim_step(c, P<S>(in, 0)); }
+    ND_FIN {
+        const S ex = len2(c.z.x, c.z.z) - 1.f, ey = sabs(c.z.y) - 1.f;
+        const S ox = smax_(ex, C<S>(0.f)), oy = smax_(ey, C<S>(0.f));
+        return prim_div(len2(ox, oy) + smin_(smax_(ex, ey), C<S>(0.f)), c);
+    }
+};
+// formulas/torus_primitive.wgsl:7-16 (the step reads pos, not the carry)
+struct FTorus {
+    ND_STEP {
+        (void)c;
+        const S tube = smax_(P<S>(in, 1), C<S>(K_EPS));
+        const S it = sdiv(C<S>(1.f), tube);
+        const V3<S> sc = pos * it;
+        ND_RET(mk3(len2(sc.x, sc.z) - P<S>(in, 0) * it, sc.y, C<S>(0.f)), it);
+    }
+    ND_FIN { return prim_div(len2(c.z.x, c.z.y) - 1.f, c); }
+};
+// formulas/pyramid_primitive.wgsl:2-21; side plane normal (2, 1) / sqrt(5)
+struct FPyramid {
+    ND_STEP { (void)pos; return prim_step(c, P<S>(in, 0)); }
+    ND_FIN {
+        const S m = smax_(sabs(c.z.x), sabs(c.z.z));
+        const S side = (m * 2.f + c.z.y - 1.f) * 0.44721359549995794f;
+        const S base = -(c.z.y + 1.f);
+        return prim_div(smax_(side, base), c);
+    }
+};
+// formulas/prism_primitive.wgsl:8-27
+struct FPrism {
+    ND_STEP {
+        (void)pos;
+        const Carry<S> q0 = prim_step(c, P<S>(in, 0));
+        const S hh = sdiv(P<S>(in, 1), smax_(P<S>(in, 0), C<S>(K_EPS)));
+        const float wedge = K_PI / fmaxf(rintf(in.p[2]), 3.f);
+        float cw, sw;
+        sincosf(wedge, &sw, &cw);
+        const S a = wrap_angle(satan2(q0.z.x, q0.z.z), 2.f * wedge) - wedge;
+        S sa, ca;
+        ssincos(a, &sa, &ca);
+        const S l = len2(q0.z.x, q0.z.z);
+        S qx = l * ca - cw, qy = l * sabs(sa) - sw;
+        qy = qy + sclamp(-qy, C<S>(0.f), C<S>(sw));
+        const S edge = vv(qx) > 0.f ? len2(qx, qy) : -len2(qx, qy);
+        ND_RET(mk3(edge, sabs(q0.z.y) - hh, C<S>(0.f)), q0.dr);
+    }
+    ND_FIN {
+        const S ex = c.z.x, ey = c.z.y;
+        return prim_div(len2(smax_(ex, C<S>(0.f)), smax_(ey, C<S>(0.f))) + smin_(smax_(ex, ey), C<S>(0.f)), c);
+    }
+};
+
+// ============================================================================= tetrahedral / cubic IFS
+// formulas/sierpinski3_ifs.wgsl:13-25
+struct FSierpinski3 {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        ND_RET(ifs_scale(vrot(tetra_fold3(c.z), in), s, p3<S>(in, 2, 3, 4)), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/cubejulia_ifs_julia.wgsl:13-25
+struct FCubeJulia {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        ND_RET(ifs_scale(vrot(sort_desc_abs(c.z), in), s, p3<S>(in, 1, 2, 3)), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/juliakifs_ifs_julia.wgsl:13-25
+struct FJuliaKifs {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        ND_RET(ifs_scale(vrot(tetra_fold3(c.z), in), s, p3<S>(in, 1, 2, 3)), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/diagonaltetra_ifs_fold.wgsl:13-33
+struct FDiagonalTetra {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        ND_RET(ifs_scale(vrot(tetra_fold3(swap_xy_desc(c.z)), in), s, p3<S>(in, 2, 3, 4)), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/starkifs_ifs_fold.wgsl:10-40
+struct FStarKifs {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 1);
+        const V3<S> z = wedge_fold(c.z, fmaxf(floorf(in.p[0]), 2.f));
+        ND_RET(ifs_scale(vrot(tetra_fold3(z), in), s, p3<S>(in, 3, 4, 5)), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/kaleidojulia_ifs_fold_julia.wgsl:10-40
+struct FKaleidoJulia {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 1);
+        const V3<S> z = wedge_fold(c.z, fmaxf(floorf(in.p[0]), 2.f));
+        ND_RET(ifs_scale(vrot(sort_desc_abs(z), in), s, p3<S>(in, 2, 3, 4)), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/bulbtetra_hybrid_bulb_ifs.wgsl:13-38: a bulb step without the trailing +1 on dr
+// (skipped once r > 2), then the tetrahedral IFS step
+struct FBulbTetra {
+    ND_STEP {
+        const S power = P<S>(in, 0), s = P<S>(in, 1);
+        const S r0 = len3(c.z);
+        V3<S> z = c.z;
+        S dr = c.dr;
+        if (vv(r0) <= 2.f) {
+            const V3<S> d = bulb_dir(z, r0, power, in.p[6] * K_DEG, in.p[7] * K_DEG);
+            const S rp = spow(r0, power - 1.f);
+            const S zr = rp * r0;
+            dr = rp * power * dr;
+            z = mk3(d.x * zr + pos.x, d.y * zr + pos.y, d.z * zr + pos.z);
+        }
+        ND_RET(ifs_scale(tetra_fold3(z), s, p3<S>(in, 3, 4, 5)), dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/hybridkifs_hybrid_ifs.wgsl:13-35
+struct FHybridKifs {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        const V3<S> o = p3<S>(in, 2, 3, 4);
+        V3<S> z = ifs_scale(vrot(sort_desc_abs(tetra_fold3(c.z)), in), s, o);
+        const S oz = o.z * (s - 1.f);
+        if (vv(z.z) < -0.5f * vv(oz)) z.z = z.z + oz;
+        ND_RET(z, c.dr * s);
+    }
+    ND_FIN { return fin_unit_box(c); }
+};
+// formulas/kaleidoscopicifs_ifs_twist.wgsl:13-37
+struct FKaleidoscopicIfs {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        const V3<S> o = p3<S>(in, 3, 4, 5);
+        V3<S> z = ifs_scale(sort_desc_abs(c.z), s, o);
+        const S oz = o.z * (s - 1.f);
+        if (vv(z.z) < -0.5f * vv(oz)) z.z = z.z + oz;
+        ND_RET(vrot(vtwist(z, in), in), c.dr * s);
+    }
+    ND_FIN { return fin_unit_box(c); }
+};
+
+// ============================================================================= icosahedral IFS
+// formulas/icosahedralifs_ifs.wgsl:13-44
+struct FIcosahedralIfs {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        const V3<S> o = p3<S>(in, 1, 2, 4);
+        const V3<S> z = vrot(icosa_mirrors(vabs3(c.z)), in);
+        ND_RET((z - o) * s + o, c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/icosahedraljulia_ifs_julia.wgsl:16-48
+struct FIcosahedralJulia {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        const V3<S> o = p3<S>(in, 1, 2, 7);
+        const V3<S> z = vrot(icosa_mirrors(vabs3(c.z)), in);
+        ND_RET((z - o) * s + o + p3<S>(in, 3, 4, 5), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/icosahedraltwist_ifs_twist.wgsl:13-49
+struct FIcosahedralTwist {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        const V3<S> o = p3<S>(in, 1, 2, 5);
+        const V3<S> z = (icosa_mirrors(vabs3(c.z)) - o) * s + o;
+        ND_RET(vrot(vtwist(z, in), in), c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+// formulas/diagonalicosa_ifs_fold.wgsl:13-51
+struct FDiagonalIcosa {
+    ND_STEP {
+        (void)pos;
+        const S s = P<S>(in, 0);
+        const V3<S> o = p3<S>(in, 1, 2, 4);
+        const V3<S> z = vrot(icosa_mirrors(swap_xy_desc(vabs3(c.z))), in);
+        ND_RET((z - o) * s + o, c.dr * s);
+    }
+    ND_FIN { return fin_plain(c); }
+};
+
+// ============================================================================= box-fold family
+// z * scale + add, dr * |scale| + dr_add: the tail shared by the Mandelbox relatives
+template <class S> NDD Carry<S> box_tail(const V3<S>& z, const S& dr, const S& scale, const V3<S>& add, const S& dr_add) {
+    Carry<S> o;
+    o.z = mk3(z.x * scale + add.x, z.y * scale + add.y, z.z * scale + add.z);
+    o.dr = dr * sabs(scale) + dr_add;
+    return o;
+}
+// pos * cs + julia, max(|cs|, 1)
+template <class S> NDD V3<S> cj_add(const V3<S>& pos, const DevInst& in, int cs) {
+    const S k = P<S>(in, cs);
+    return mk3(pos.x * k + P<S>(in, cs + 1), pos.y * k + P<S>(in, cs + 2), pos.z * k + P<S>(in, cs + 3));
+}
+template <class S> NDD S cj_dr(const DevInst& in, int cs) { return C<S>(fmaxf(fabsf(in.p[cs]), 1.f)); }
+template <class S> NDD S sq(const S& x) { return x * x; }
+
+// formulas/juliabox_box_julia.wgsl:16-37
+struct FJuliaBox {
+    ND_STEP {
+        (void)pos;
+        V3<S> z = box_fold3(c.z, P<S>(in, 3));
+        S dr = c.dr;
+        sphere_fold3(z, dr, sq(P<S>(in, 1)), sq(P<S>(in, 2)));
+        return box_tail(vrot(z, in), dr, P<S>(in, 0), p3<S>(in, 4, 5, 6), C<S>(1.f));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/amazingsurf_box_fold.wgsl:17-38 (TAIL 1: |z.z|), formulas/mandelboxfoam_box_fold.wgsl
+// (TAIL 2: |z.x| and |z.z|), formulas/mandelboxsort_box_fold.wgsl (PRE 1: sort_desc_abs
+// first), formulas/mandelboxdiag_box_fold.wgsl (PRE 2: the x < y swap first); slots
+// [scale, minR, fixR, limit, iters, rx, ry, rz, cs, jx, jy, jz]
+template <int PRE, int TAIL>
+struct FBoxVariant {
+    ND_STEP {
+        V3<S> z = c.z;
+        if (PRE == 1) z = sort_desc_abs(z);
+        if (PRE == 2) z = swap_xy_desc(z);
+        z = box_fold3(z, P<S>(in, 3));
+        S dr = c.dr;
+        sphere_fold3(z, dr, sq(P<S>(in, 1)), sq(P<S>(in, 2)));
+        Carry<S> o = box_tail(vrot(z, in), dr, P<S>(in, 0), cj_add(pos, in, 8), cj_dr<S>(in, 8));
+        if (TAIL == 2) o.z.x = sabs(o.z.x);
+        if (TAIL >= 1) o.z.z = sabs(o.z.z);
+        return o;
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/mandelboxinv_box_fold.wgsl:18-46
+struct FMandelboxInv {
+    ND_STEP {
+        const S tin = sdiv(sq(P<S>(in, 0)), smax_(dot3(c.z, c.z), C<S>(K_EPS_FINE)));
+        V3<S> z = box_fold3(c.z * tin, P<S>(in, 4));
+        S dr = c.dr * tin;
+        sphere_fold3(z, dr, sq(P<S>(in, 2)), sq(P<S>(in, 3)));
+        return box_tail(vrot(z, in), dr, P<S>(in, 1), cj_add(pos, in, 9), cj_dr<S>(in, 9));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/mandelboxtwist_box_twist.wgsl:17-42
+struct FMandelboxTwist {
+    ND_STEP {
+        V3<S> z = box_fold3(c.z, P<S>(in, 3));
+        S dr = c.dr;
+        sphere_fold3(z, dr, sq(P<S>(in, 1)), sq(P<S>(in, 2)));
+        return box_tail(vrot(vtwist(z, in), in), dr, P<S>(in, 0), cj_add(pos, in, 8), cj_dr<S>(in, 8));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/prismkifs_box_fold.wgsl:14-52
+struct FPrismKifs {
+    ND_STEP {
+        V3<S> z = box_fold3(wedge_fold(c.z, fmaxf(floorf(in.p[0]), 2.f)), P<S>(in, 4));
+        S dr = c.dr;
+        sphere_fold3(z, dr, sq(P<S>(in, 2)), sq(P<S>(in, 3)));
+        return box_tail(vrot(z, in), dr, P<S>(in, 1), cj_add(pos, in, 9), cj_dr<S>(in, 9));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/buffalo_box.wgsl:16-36 (ABS) and formulas/apollonian_kleinian.wgsl:16-36 (no
+// abs); slots [scale, minR, fixR, iters, rx, ry, rz, cs, jx, jy, jz]
+template <bool ABS>
+struct FSphereFoldMandel {
+    ND_STEP {
+        V3<S> z = ABS ? vabs3(c.z) : c.z;
+        S dr = c.dr;
+        sphere_fold3(z, dr, sq(P<S>(in, 1)), sq(P<S>(in, 2)));
+        return box_tail(vrot(z, in), dr, P<S>(in, 0), cj_add(pos, in, 7), cj_dr<S>(in, 7));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/buffalojulia_box_julia.wgsl:15-36 (ABS) and
+// formulas/apollonianjulia_kleinian_julia.wgsl:15-36; slots [scale, minR, fixR, jx, jy, jz,
+// iters, rx, ry, rz]
+template <bool ABS>
+struct FSphereFoldJulia {
+    ND_STEP {
+        (void)pos;
+        V3<S> z = ABS ? vabs3(c.z) : c.z;
+        S dr = c.dr;
+        sphere_fold3(z, dr, sq(P<S>(in, 1)), sq(P<S>(in, 2)));
+        return box_tail(vrot(z, in), dr, P<S>(in, 0), p3<S>(in, 3, 4, 5), C<S>(1.f));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/buffalotwist_box_twist.wgsl:16-41
+struct FBuffaloTwist {
+    ND_STEP {
+        V3<S> z = vabs3(c.z);
+        S dr = c.dr;
+        sphere_fold3(z, dr, sq(P<S>(in, 1)), sq(P<S>(in, 2)));
+        return box_tail(vrot(vtwist(z, in), in), dr, P<S>(in, 0), cj_add(pos, in, 7), cj_dr<S>(in, 7));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+
+// ============================================================================= Kleinian relatives
+// formulas/kleinianjulia_kleinian_julia.wgsl:14-34
+struct FKleinianJulia {
+    ND_STEP {
+        (void)pos;
+        V3<S> z = lattice_wrap3(c.z, P<S>(in, 8));
+        S dr = c.dr;
+        sphere_invert3(z, dr, p3<S>(in, 0, 1, 2), sq(P<S>(in, 3)));
+        ND_RET(z + p3<S>(in, 4, 5, 6), dr + 1.f);
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+// formulas/pseudokleinian_kleinian.wgsl:15-34
+struct FPseudoKleinian {
+    ND_STEP {
+        V3<S> z = lattice_wrap3(c.z, P<S>(in, 4));
+        S dr = c.dr;
+        sphere_invert3(z, dr, p3<S>(in, 0, 1, 2), sq(P<S>(in, 3)));
+        ND_RET(z + cj_add(pos, in, 6), dr + cj_dr<S>(in, 6));
+    }
+    ND_FIN { return fin_scaled(c); }
+};
+
+// ============================================================================= bulb family
+// formulas/juliabulb_bulb_julia.wgsl:12-30
+struct FJuliaBulb {
+    ND_STEP {
+        (void)pos;
+        const S r = len3(c.z);
+        if (vv(r) > 2.f) return c;
+        const S power = P<S>(in, 0);
+        const V3<S> d = bulb_dir(c.z, r, power, in.p[5] * K_DEG, in.p[6] * K_DEG);
+        const S rp = spow(r, power - 1.f);
+        const S zr = rp * r;
+        ND_RET(d * zr + p3<S>(in, 2, 3, 4), rp * power * c.dr + 1.f);
+    }
+    ND_FIN { return bulb_finalize(c); }
+};
+// formulas/mandelbulbmirror_bulb_fold.wgsl:13-32
+struct FMandelbulbMirror {
+    ND_STEP {
+        const V3<S> zin = mk3(c.z.x, -c.z.y, c.z.z);
+        const S r = len3(zin);
+        if (vv(r) > 2.f) return c;
+        const S power = P<S>(in, 0);
+        const V3<S> d = bulb_dir(zin, r, power, in.p[2] * K_DEG, in.p[3] * K_DEG);
+        const S zr = spow(r, power);
+        const S dr = spow(r, power - 1.f) * power * c.dr + cj_dr<S>(in, 4);
+        ND_RET(d * zr + cj_add(pos, in, 4), dr);
+    }
+    ND_FIN { return bulb_finalize(c); }
+};
+// formulas/mandelbulbtwist_bulb_twist.wgsl:14-37
+struct FMandelbulbTwist {
+    ND_STEP {
+        const S r = len3(c.z);
+        if (vv(r) > 2.f) return c;
+        const S power = P<S>(in, 0);
+        const V3<S> d = bulb_dir(c.z, r, power, in.p[3] * K_DEG, in.p[4] * K_DEG);
+        const S zr = spow(r, power);
+        const S dr = spow(r, power - 1.f) * power * c.dr + cj_dr<S>(in, 5);
+        ND_RET(vtwist(d * zr, in) + cj_add(pos, in, 5), dr);
+    }
+    ND_FIN { return bulb_finalize(c); }
+};
+// formulas/mandelbulbdual_hybrid_bulb.wgsl:14-45
+struct FMandelbulbDual {
+    ND_STEP {
+        const S r1 = len3(c.z);
+        if (vv(r1) > 2.f) return c;
+        const float tp = in.p[3] * K_DEG, pp = in.p[4] * K_DEG;
+        const S pw1 = P<S>(in, 0), pw2 = P<S>(in, 2);
+        V3<S> z = bulb_dir(c.z, r1, pw1, tp, pp) * spow(r1, pw1);
+        S dr = spow(r1, pw1 - 1.f) * pw1 * c.dr;
+        const S r2 = len3(z);
+        const V3<S> d2 = bulb_dir(z, r2, pw2, tp, pp);
+        dr = spow(r2, pw2 - 1.f) * pw2 * dr;
+        ND_RET(d2 * spow(r2, pw2) + cj_add(pos, in, 5), dr + cj_dr<S>(in, 5));
+    }
+    ND_FIN { return bulb_finalize(c); }
+};
+// formulas/quatjulia_bulb_julia.wgsl:7-38: the quaternion Julia set z^2 + c in the plane
+// spanned by 1 and the imaginary axis of c; the sign of dr marks the first iteration.
+struct FQuatJulia {
+    ND_STEP {
+        (void)pos;
+        const float cx = in.p[1], cy = in.p[2], cz = in.p[3];
+        const float cl = sqrtf(cx * cx + cy * cy + cz * cz);
+        V3<S> s = c.z;
+        S dr = -c.dr;
+        if (vv(c.dr) > 0.f) {
+            float ax = 1.f, ay = 0.f, az = 0.f;
+            if (cl > K_EPS_FINE) { ax = cx / cl; ay = cy / cl; az = cz / cl; }
+            const V3<S> v = mk3(c.z.y, c.z.z, P<S>(in, 4));
+            const S along = v.x * ax + v.y * ay + v.z * az;
+            const V3<S> w = mk3(v.x - along * ax, v.y - along * ay, v.z - along * az);
+            s = mk3(c.z.x, along, len3(w));
+            dr = c.dr;
+        }
+        const S r2 = dot3(s, s);
+        if (vv(r2) > 256.f) ND_RET(s, -dr);
+        const V3<S> q = mk3(s.x * s.x - s.y * s.y - s.z * s.z + P<S>(in, 0), s.x * s.y * 2.f + cl, s.x * s.z * 2.f);
+        ND_RET(q, ssqrt(r2) * dr * -2.f);
+    }
+    ND_FIN {
+        const S r = len3(c.z);
+        return sdiv(sln(smax_(r, C<S>(K_EPS_FINE))) * r * 0.5f, smax_(sabs(c.dr), C<S>(K_EPS_FINE)));
+    }
+};
+// formulas/bulbbox_hybrid_bulb_box.wgsl:20-52
+struct FBulbBox {
+    ND_STEP {
+        const S power = P<S>(in, 0);
+        const S r0 = len3(c.z);
+        V3<S> z = c.z;
+        S dr = c.dr;
+        if (vv(r0) <= 2.f) {
+            const V3<S> d = bulb_dir(z, r0, power, in.p[6] * K_DEG, in.p[7] * K_DEG);
+            dr = spow(r0, power - 1.f) * power * dr;
+            z = d * spow(r0, power);
+        }
+        z = box_fold3(z, P<S>(in, 4));
+        sphere_fold3(z, dr, sq(P<S>(in, 2)), sq(P<S>(in, 3)));
+        return box_tail(vrot(z, in), dr, P<S>(in, 1), cj_add(pos, in, 11), cj_dr<S>(in, 11));
+    }
+    ND_FIN { return bulb_finalize(c); }
+};
+
+// ============================================================================= other folds
+// formulas/mandalay_box_fold.wgsl:19-74
+template <class S> NDD S mandalay_fold(V3<S> q, const S& fo, const S& g) {
+    if (vv(q.z) > vv(q.y)) { S t = q.y; q.y = q.z; q.z = t; }
+    const S vx = q.x - fo * 2.f;
+    const S vy = q.y - fo * 4.f;
+    const S v = smax_(sabs(vx + fo) - fo, vy);
+    const S v1 = smax_(vx - g, q.y);
+    return smin_(smin_(v, v1), q.x);
+}
+struct FMandalay {
+    ND_STEP {
+        const V3<S> cc = pos * 4.f + p3<S>(in, 12, 13, 14);
+        V3<S> z = c.z;
+        S dr = c.dr;
+        if (vv(dr) == 1.f) { z = cc; dr = C<S>(4.f); }
+        if (vv(dot3(z, z)) > 1e20f) ND_RET(z, dr);
+        const S L = P<S>(in, 8);
+        z = mk3(z.x - sclamp(z.x, -L, L) * 2.f, z.y - sclamp(z.y, -L, L) * 2.f, z.z - sclamp(z.z, -L, L) * 2.f);
+        const V3<S> sg = mk3(ssign(z.x), ssign(z.y), ssign(z.z));
+        const V3<S> a = vabs3(z);
+        const S gap = P<S>(in, 5);
+        z = mk3(mandalay_fold(a, P<S>(in, 2), P<S>(in, 9) * gap) * sg.x,
+                mandalay_fold(mk3(a.y, a.z, a.x), P<S>(in, 3), P<S>(in, 10) * gap) * sg.y,
+                mandalay_fold(mk3(a.z, a.x, a.y), P<S>(in, 4), P<S>(in, 11) * gap) * sg.z);
+        const float mr = in.p[1];
+        const S t = sclamp(sdiv(C<S>(1.f), smax_(dot3(z, z), C<S>(K_EPS_TINY))), C<S>(1.f), C<S>(1.f / (mr * mr)));
+        z = z * t;
+        dr = dr * t;
+        const S s = P<S>(in, 0);
+        z = mk3(z.x * s + cc.x, z.y * s + cc.y, (z.z * s + cc.z) * P<S>(in, 6));
+        ND_RET(z, dr * sabs(s) + 4.f);
+    }
+    ND_FIN { return sdiv(len3(c.z) - 30.f, sabs(c.dr)); }
+};
+// formulas/wobblekifs_ifs_twist_julia.wgsl:19-65
+template <class S> NDD void wob_rot(S& x, S& y, const S& a) {
+    S sa, ca;
+    ssincos(a, &sa, &ca);
+    const S nx = ca * x + sa * y, ny = ca * y - sa * x;
+    x = nx; y = ny;
+}
+struct FWobbleKifs {
+    ND_STEP {
+        const S s = P<S>(in, 0);
+        const float t = in.p[7] * 4.f;
+        const S len = len3(pos);
+        V3<S> z = c.z;
+        if (vv(c.dr) == 1.f) {
+            z = mk3(z.y, z.x, z.z);
+            wob_rot(z.x, z.y, C<S>(in.p[15] + sinf(t - 0.25f) * in.p[14]));
+        }
+        const S phx = ssin(len * in.p[12] + t) * in.p[8] + in.p[5];
+        const S phy = scos(len * in.p[13] + (t - 0.2f)) * in.p[9] + in.p[6];
+        z = vabs3(z) * s + p3<S>(in, 2, 3, 4);
+        wob_rot(z.x, z.z, phx * 3.14f + scos(len + t) * in.p[10]);
+        wob_rot(z.y, z.z, phy * 3.14f + ssin(len + t) * in.p[11]);
+        ND_RET(z, c.dr * s);
+    }
+    ND_FIN {
+        const V3<S> a = vabs3(c.z);
+        const S m = smax_(smax_(a.x, a.y), smax_(a.z, C<S>(1e-20f)));
+        const V3<S> q = mk3(sdiv(a.x, m), sdiv(a.y, m), sdiv(a.z, m));
+        const V3<S> q3 = mk3(q.x * q.x * q.x, q.y * q.y * q.y, q.z * q.z * q.z);
+        const S n = spowc(q3.x * q3.x + q3.y * q3.y + q3.z * q3.z, 1.f / 6.f);
+        return sdiv(m * n, c.dr) - 0.25f;
+    }
+};
+
+// ============================================================================= packed-carry formulas
+// These keep a running minimum in dr, with an iteration level in its low mantissa bits.
+// formulas/radioactive_ifs_carve_twist.wgsl:9-42
+struct FRadioactive {
+    ND_STEP {
+        (void)pos;
+        const uint32_t level = sbits(c.dr) & 31u;
+        V3<S> v = c.z;
+        S sv = c.dr;
+        if (level == 0u) sv = v.y - P<S>(in, 5);
+        const S i = P<S>(in, 1) * spow(P<S>(in, 4), C<S>((float)level));
+        float sn, cs;
+        sincosf(in.p[2], &sn, &cs);
+        v = mk3(v.x * cs + v.z * sn, v.y, v.z * cs - v.x * sn);
+        const S w = i * 2.f;
+        const S ic = i * P<S>(in, 3);
+        v = mk3(ic - sabs(v.x - w * sfloor(sdiv(v.x, w)) - i), ic - sabs(v.y - w * sfloor(sdiv(v.y, w)) - i),
+                ic - sabs(v.z - w * sfloor(sdiv(v.z, w)) - i));
+        sv = smax_(sv, smin_(smin_(v.x, v.y), v.z));
+        ND_RET(v, pack_low(sv, 31u, min(level + 1u, 31u)));
+    }
+    ND_FIN { return c.dr; }
+};
+// formulas/recursivesphere_ifs_carve_twist.wgsl:12-65
+template <class S> NDD S smin_poly(const S& a, const S& b, const S& k) {  // smin, template.wgsl:1653-1659
+    if (vv(k) <= 0.0001f) return smin_(a, b);
+    const S h = sclamp(sdiv(b - a, k) * 0.5f + 0.5f, C<S>(0.f), C<S>(1.f));
+    return smix(b, a, h) - k * h * (C<S>(1.f) - h);
+}
+struct FRecursiveSphere {
+    ND_STEP {
+        (void)pos;
+        const uint32_t level = sbits(c.dr) & 31u;
+        const S world = sdiv(P<S>(in, 1), C<S>(5.2f));
+        V3<S> z = c.z;
+        S best = c.dr;
+        if (level == 0u) {
+            z = mk3(sdiv(z.x, world), sdiv(z.y, world), sdiv(z.z, world));
+            best = C<S>(K_INFINITE);
+        }
+        const S shrink = spow(P<S>(in, 5), C<S>((float)level));
+        const S radius = sdiv(C<S>(5.2f), P<S>(in, 8)) * shrink;
+        const S fin = P<S>(in, 6) * shrink;
+        S d = len3(z) - radius * P<S>(in, 8);
+        V3<S> corners = vabs3(z) + mk3(radius * -0.57735027f, radius * -0.57735027f, radius * -0.57735027f);
+        const S hole = len3(corners) - radius * P<S>(in, 2);
+        const V3<S> ap = vabs3(z) * 0.7071f;
+        S cut = smax_(hole, fin - sabs(ap.x - ap.y));
+        cut = smax_(cut, fin - sabs(ap.y - ap.z));
+        cut = smax_(cut, fin - sabs(ap.z - ap.x));
+        cut = smin_(cut, hole + P<S>(in, 3) * shrink);
+        d = -smin_poly(-d, cut, P<S>(in, 7) * shrink);
+        best = smin_(best, d * world);
+        const S a = sdiv(P<S>(in, 4), shrink);
+        S sa, ca;
+        ssincos(a, &sa, &ca);
+        corners = mk3(ca * corners.x - sa * corners.z, corners.y, sa * corners.x + ca * corners.z);
+        z = mk3(corners.x, corners.z, -corners.y);
+        if (level + 1u >= (uint32_t)fmaxf(in.p[0], 1.f))
+            best = smin_(best, (len3(z) - radius * P<S>(in, 5) * P<S>(in, 8)) * world);
+        ND_RET(z, pack_low(best, 31u, min(level + 1u, 31u)));
+    }
+    ND_FIN { return c.dr; }
+};
+// formulas/tree_ifs.wgsl:9-75
+template <class S> NDD S round_cone(const V3<S>& p, const S& r1, const S& r2, const S& h) {
+    const S qx = len2(p.x, p.z), qy = p.y;
+    const S b = sdiv(r1 - r2, h);
+    const S a = ssqrt(C<S>(1.f) - b * b);
+    const S k = qy * a - qx * b;
+    if (vv(k) < 0.f) return len2(qx, qy) - r1;
+    if (vv(k) > vv(a * h)) return len2(qx, qy - h) - r2;
+    return qx * a + qy * b - r1;
+}
+struct FTree {
+    ND_STEP {
+        (void)pos;
+        const uint32_t level = sbits(c.dr) & 31u;
+        const S ratio = P<S>(in, 3), thick = P<S>(in, 5);
+        V3<S> z = c.z;
+        S m = c.dr;
+        S lh = C<S>(1.f);
+        if (level == 0u) {
+            z = mk3(sdiv(z.x, C<S>(0.6f)), sdiv(z.y + 1.f, C<S>(0.6f)), sdiv(z.z, C<S>(0.6f)));
+            lh = P<S>(in, 6);
+            m = round_cone(z, thick, thick * ratio, lh) * 0.6f;
+        }
+        z.y = z.y - lh;
+        float stw, ctw;
+        sincosf(in.p[4] * K_DEG, &stw, &ctw);
+        z = mk3(z.x * ctw - z.z * stw, z.y, z.x * stw + z.z * ctw);
+        const float sector = K_TAU / fmaxf(rintf(in.p[1]), 1.f);
+        S ang = satan2(z.z, z.x);
+        ang = sabs(ang - srint(ang * (1.f / sector)) * sector);
+        const S rxz = len2(z.x, z.z);
+        S sa, ca;
+        ssincos(ang, &sa, &ca);
+        z = mk3(rxz * ca, z.y, rxz * sa);
+        float sth, cth;
+        sincosf(in.p[2] * K_DEG, &sth, &cth);
+        z = mk3(sdiv(z.x * cth - z.y * sth, ratio), sdiv(z.x * sth + z.y * cth, ratio), sdiv(z.z, ratio));
+        const uint32_t next = min(level + 1u, 31u);
+        const S wpl = spow(ratio, C<S>((float)next)) * 0.6f;
+        m = smin_(m, round_cone(z, thick, thick * ratio, C<S>(1.f)) * wpl);
+        if ((int)next >= (int)in.p[0] && in.p[7] > 0.f)
+            m = smin_(m, (len3(mk3(z.x, z.y - 1.f, z.z)) - P<S>(in, 7)) * wpl);
+        ND_RET(z, pack_low(m, 31u, next));
+    }
+    ND_FIN { return c.dr; }
+};
+// formulas/kleinianlimit_kleinian.wgsl:9-76 (7-bit step counter, 127 = done)
+struct FKleinianLimit {
+    ND_STEP {
+        const uint32_t stp = sbits(c.dr) & 127u;
+        if (stp == 127u) return c;
+        const S a = P<S>(in, 0), b = P<S>(in, 1);
+        const S bx = P<S>(in, 2), bz = P<S>(in, 3);
+        V3<S> z = c.z;
+        S df = c.dr;
+        if (stp == 0u) { z.y = z.y + 1.f; df = C<S>(1.f); }
+        const S ba = sdiv(b, a);
+        z.x = z.x + ba * z.y;
+        z.x = z.x - bx * 2.f * sfloor(sdiv(z.x + bx, bx * 2.f));
+        z.z = z.z - bz * 2.f * sfloor(sdiv(z.z + bz, bz * 2.f));
+        z.x = z.x - ba * z.y;
+        const S xs = z.x + b * 0.5f;
+        const S sep = a * 0.5f + (a * 2.f - 1.95f) * (ssign(vv(b)) * 0.25f) * ssign(vv(xs)) *
+                      (C<S>(1.f) - sexp(-(C<S>(7.2f) - (C<S>(1.95f) - a) * 15.f) * sabs(xs)));
+        if (vv(z.y) >= vv(sep)) z = mk3(-b - z.x, a - z.y, -z.z);
+        const S ir = sdiv(C<S>(1.f), smax_(dot3(z, z), C<S>(1e-20f)));
+        z = z * -ir;
+        z.x = -b - z.x;
+        z.y = a + z.y;
+        df = df * ir;
+        if ((int)stp + 1 >= (int)in.p[5]) {
+            const S y = smin_(z.y, a - z.y);
+            const S de = sdiv(smin_(y, C<S>(0.3f)), smax_(df, C<S>(2.f)));
+            const S cut = pos.y + 1.f - P<S>(in, 4) * a;
+            ND_RET(z, pack_low(smax_(de, cut), 127u, 127u));
+        }
+        ND_RET(z, pack_low(df, 127u, stp + 1u));
+    }
+    ND_FIN { return c.dr; }
+};
+// formulas/fbmsdf_primitive_carve.wgsl:11-107
+// The hash's product reaches ~2.5e5 before its fract, so it is computed exactly as written
+// in FP32: every product and sum rounded on its own (no fused multiply-add).
+NDD float fbm_fract(float x) { return __fsub_rn(x, floorf(x)); }
+NDD float fbm_hash(float x, float y, float z) {
+    const float qx = __fmul_rn(17.f, fbm_fract(__fadd_rn(__fmul_rn(x, 0.3183099f), 0.11f)));
+    const float qy = __fmul_rn(17.f, fbm_fract(__fadd_rn(__fmul_rn(y, 0.3183099f), 0.17f)));
+    const float qz = __fmul_rn(17.f, fbm_fract(__fadd_rn(__fmul_rn(z, 0.3183099f), 0.13f)));
+    return fbm_fract(__fmul_rn(__fmul_rn(__fmul_rn(qx, qy), qz), __fadd_rn(__fadd_rn(qx, qy), qz)));
+}
+template <class S> NDD S fbm_lattice(const V3<S>& p, float radius) {
+    const float ix = floorf(vv(p.x)), iy = floorf(vv(p.y)), iz = floorf(vv(p.z));
+    const V3<S> f = mk3(p.x - ix, p.y - iy, p.z - iz);
+    S d = C<S>(1e10f);
+    for (int k = 0; k < 8; ++k) {
+        const float cx = (float)(k & 1), cy = (float)((k >> 1) & 1), cz = (float)((k >> 2) & 1);
+        const float r = fbm_hash(ix + cx, iy + cy, iz + cz);
+        d = smin_(d, len3(mk3(f.x - cx, f.y - cy, f.z - cz)) - r * r * radius);
+    }
+    return d;
+}
+template <class S> NDD S fbm_simplex(const V3<S>& p, float radius) {
+    const float k1 = 0.333333333f, k2 = 0.166666667f;
+    const float s = (vv(p.x) + vv(p.y) + vv(p.z)) * k1;
+    const float ix = floorf(vv(p.x) + s), iy = floorf(vv(p.y) + s), iz = floorf(vv(p.z) + s);
+    const float u = (ix + iy + iz) * k2;
+    const V3<S> d0 = mk3(p.x - (ix - u), p.y - (iy - u), p.z - (iz - u));
+    // e = step(d0.yzx, d0)
+    const float ex = vv(d0.x) >= vv(d0.y) ? 1.f : 0.f, ey = vv(d0.y) >= vv(d0.z) ? 1.f : 0.f,
+                ez = vv(d0.z) >= vv(d0.x) ? 1.f : 0.f;
+    const float i1x = ex * (1.f - ez), i1y = ey * (1.f - ex), i1z = ez * (1.f - ey);
+    const float i2x = 1.f - ez * (1.f - ex), i2y = 1.f - ex * (1.f - ey), i2z = 1.f - ey * (1.f - ez);
+    const V3<S> d1 = mk3(d0.x - (i1x - k2), d0.y - (i1y - k2), d0.z - (i1z - k2));
+    const V3<S> d2 = mk3(d0.x - (i2x - 2.f * k2), d0.y - (i2y - 2.f * k2), d0.z - (i2z - 2.f * k2));
+    const float k3 = 1.f - 3.f * k2;
+    const V3<S> d3 = mk3(d0.x - k3, d0.y - k3, d0.z - k3);
+    const float r0 = fbm_hash(ix, iy, iz);
+    const float r1 = fbm_hash(ix + i1x, iy + i1y, iz + i1z);
+    const float r2 = fbm_hash(ix + i2x, iy + i2y, iz + i2z);
+    const float r3 = fbm_hash(ix + 1.f, iy + 1.f, iz + 1.f);
+    const float rs = radius * (0.55f / 0.7f);
+    return smin_(smin_(len3(d0) - r0 * r0 * rs, len3(d1) - r1 * r1 * rs),
+                 smin_(len3(d2) - r2 * r2 * rs, len3(d3) - r3 * r3 * rs));
+}
+struct FFbmSdf {
+    ND_STEP {
+        (void)pos;
+        const uint32_t level = sbits(c.dr) & 31u;
+        V3<S> z = c.z;
+        S d = c.dr;
+        if (level == 0u) {
+            const V3<S> q = vsub1(vabs3(z), in.p[1]);
+            d = smin_(smax_(q.x, smax_(q.y, q.z)), C<S>(0.f)) + len3(vmax0(q));
+            const float o = 0.5f + in.p[7];
+            z = mk3(z.x + o, z.y + o, z.z + o);
+        }
+        const S sc = spow(P<S>(in, 4), C<S>((float)level));
+        const S base = in.p[6] > 0.5f ? fbm_simplex(z, in.p[2]) : fbm_lattice(z, in.p[2]);
+        const S n = sc * base;
+        const S k = smax_(P<S>(in, 3) * sc, C<S>(K_EPS_FINE));
+        const S h = smax_(k - sabs(d + n), C<S>(0.f));
+        d = smax_(d, -n) + sdiv(h * h * 0.25f, k);
+        const S l = P<S>(in, 5);
+        z = mk3((z.y * -0.80f - z.z * 0.60f) * l, (z.x * 0.80f + z.y * 0.36f - z.z * 0.48f) * l,
+                (z.x * 0.60f - z.y * 0.48f + z.z * 0.64f) * l);
+        ND_RET(z, pack_low(d, 31u, min(level + 1u, 31u)));
+    }
+    ND_FIN { return c.dr; }
+};
+
+#undef ND_STEP
+#undef ND_FIN
+#undef ND_RET
+
+using FAmazingSurf = FBoxVariant<0, 1>;
+using FMandelboxFoam = FBoxVariant<0, 2>;
+using FMandelboxSort = FBoxVariant<1, 0>;
+using FMandelboxDiag = FBoxVariant<2, 0>;
+using FBuffalo = FSphereFoldMandel<true>;
+using FApollonian = FSphereFoldMandel<false>;
+using FBuffaloJulia = FSphereFoldJulia<true>;
+using FApollonianJulia = FSphereFoldJulia<false>;
+
+// ----------------------------------------------------------------------------- the loop
+template <class F, bool TRAP>
+NDD float ext_run_f(const DevInst& in, const V3<float>& pos, float* trap) {
+    Carry<float> c; c.z = pos; c.dr = 1.f;
+    TrapAcc acc;
+    if (TRAP) acc = trap_begin(in);
+    const int iters = in.iters;
+    for (int it = 0; it < iters; ++it) {
+        const V3<float> zp = c.z;
+        c = F::template step<float>(c, pos, in);
+        if (TRAP) trap_update(acc, in, c.z, zp, it);
+    }
+    if (TRAP) *trap = trap_finish(acc, in, iters);
+    return F::template fin<float>(c);
+}
+template <class F, class S>
+NDD S ext_run_d(const DevInst& in, const V3<S>& pos) {
+    Carry<S> c; c.z = pos; c.dr = C<S>(1.f);
+    for (int it = 0; it < in.iters; ++it) c = F::template step<S>(c, pos, in);
+    return F::template fin<S>(c);
+}
+
+// X(id, struct) over every formula of this file
+#define ND_EXT_FORMULAS(X)                                                 \
+    X(ND_FORMULA_SPHERE, FSphere)                                          \
+    X(ND_FORMULA_CUBE, FCube)                                              \
+    X(ND_FORMULA_CYLINDER, FCylinder)                                      \
+    X(ND_FORMULA_TORUS, FTorus)                                            \
+    X(ND_FORMULA_PYRAMID, FPyramid)                                        \
+    X(ND_FORMULA_PRISM, FPrism)                                            \
+    X(ND_FORMULA_SIERPINSKI3, FSierpinski3)                                \
+    X(ND_FORMULA_CUBEJULIA, FCubeJulia)                                    \
+    X(ND_FORMULA_JULIAKIFS, FJuliaKifs)                                    \
+    X(ND_FORMULA_DIAGONALTETRA, FDiagonalTetra)                            \
+    X(ND_FORMULA_STARKIFS, FStarKifs)                                      \
+    X(ND_FORMULA_KALEIDOJULIA, FKaleidoJulia)                              \
+    X(ND_FORMULA_BULBTETRA, FBulbTetra)                                    \
+    X(ND_FORMULA_ICOSAHEDRALIFS, FIcosahedralIfs)                          \
+    X(ND_FORMULA_ICOSAHEDRALJULIA, FIcosahedralJulia)                      \
+    X(ND_FORMULA_ICOSAHEDRALTWIST, FIcosahedralTwist)                      \
+    X(ND_FORMULA_DIAGONALICOSA, FDiagonalIcosa)                            \
+    X(ND_FORMULA_HYBRIDKIFS, FHybridKifs)                                  \
+    X(ND_FORMULA_KALEIDOSCOPICIFS, FKaleidoscopicIfs)                      \
+    X(ND_FORMULA_JULIABOX, FJuliaBox)                                      \
+    X(ND_FORMULA_AMAZINGSURF, FAmazingSurf)                      \
+    X(ND_FORMULA_MANDELBOXFOAM, FMandelboxFoam)                    \
+    X(ND_FORMULA_MANDELBOXSORT, FMandelboxSort)                    \
+    X(ND_FORMULA_MANDELBOXDIAG, FMandelboxDiag)                    \
+    X(ND_FORMULA_MANDELBOXINV, FMandelboxInv)                              \
+    X(ND_FORMULA_MANDELBOXTWIST, FMandelboxTwist)                          \
+    X(ND_FORMULA_PRISMKIFS, FPrismKifs)                                    \
+    X(ND_FORMULA_BUFFALO, FBuffalo)                         \
+    X(ND_FORMULA_BUFFALOJULIA, FBuffaloJulia)                     \
+    X(ND_FORMULA_BUFFALOTWIST, FBuffaloTwist)                              \
+    X(ND_FORMULA_APOLLONIAN, FApollonian)                     \
+    X(ND_FORMULA_APOLLONIANJULIA, FApollonianJulia)                 \
+    X(ND_FORMULA_KLEINIANJULIA, FKleinianJulia)                            \
+    X(ND_FORMULA_PSEUDOKLEINIAN, FPseudoKleinian)                          \
+    X(ND_FORMULA_JULIABULB, FJuliaBulb)                                    \
+    X(ND_FORMULA_MANDELBULBMIRROR, FMandelbulbMirror)                      \
+    X(ND_FORMULA_MANDELBULBTWIST, FMandelbulbTwist)                        \
+    X(ND_FORMULA_MANDELBULBDUAL, FMandelbulbDual)                          \
+    X(ND_FORMULA_QUATJULIA, FQuatJulia)                                    \
+    X(ND_FORMULA_BULBBOX, FBulbBox)                                        \
+    X(ND_FORMULA_MANDALAY, FMandalay)                                      \
+    X(ND_FORMULA_WOBBLEKIFS, FWobbleKifs)                                  \
+    X(ND_FORMULA_RADIOACTIVE, FRadioactive)                                \
+    X(ND_FORMULA_RECURSIVESPHERE, FRecursiveSphere)                        \
+    X(ND_FORMULA_TREE, FTree)                                              \
+    X(ND_FORMULA_KLEINIANLIMIT, FKleinianLimit)                            \
+    X(ND_FORMULA_FBMSDF, FFbmSdf)
+
+// The dispatch is a real call, not inlined: it keeps the 47 formula loops out of the
+// register allocation of the kernels' hot paths, which only ever run the four tuned DEs
+// inline.
+struct ExtDe { float d, trap; };
+template <bool TRAP>
+__device__ __noinline__ ExtDe ext_de_call(const DevInst* in, float x, float y, float z) {
+    const V3<float> pos = mk3(x, y, z);
+    ExtDe r; r.trap = 0.f; r.d = 1e6f;  // NO_SURFACE_DE
+    switch (in->formula) {
+#define ND_CASE_F(ID, F) case ID: r.d = ext_run_f<F, TRAP>(*in, pos, &r.trap); break;
+        ND_EXT_FORMULAS(ND_CASE_F)
+#undef ND_CASE_F
+    }
+    return r;
+}
+template <bool TRAP>
+NDD float ext_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
+    const ExtDe r = ext_de_call<TRAP>(&in, pos.x, pos.y, pos.z);
+    if (TRAP) *trap = r.trap;
+    return r.d;
+}
+template <class S>
+__device__ __noinline__ S ext_de_d_call(const DevInst* in, V3<S> pos) {
+    switch (in->formula) {
+#define ND_CASE_D(ID, F) case ID: return ext_run_d<F, S>(*in, pos);
+        ND_EXT_FORMULAS(ND_CASE_D)
+#undef ND_CASE_D
+    }
+    return C<S>(1e6f);
+}
+template <class S>
+NDD S ext_de_d(const DevInst& in, const V3<S>& pos) { return ext_de_d_call<S>(&in, pos); }
+
+}  // namespace nd

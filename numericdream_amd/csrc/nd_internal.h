@@ -24,21 +24,24 @@ struct NdPassLaunch {
     unsigned long long* job_done_cycles;  // per-job completion clock, or null
     uint32_t* ray_done;          // per ray slot completion tick (total_rays entries) when tracking
     int accel;                   // some job uses the empty-space accelerator
+    int ext;                     // some instance uses a formula beyond the four tuned ones
     hipEvent_t ev[4];            // optional: before march / surface / shadow / after
 };
 
-hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, hipStream_t s);
+hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, bool ext, hipStream_t s);
 hipError_t nd_launch_zero_counters(DevCounters* c, hipStream_t s);
 hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s);
 
 // Resident wave64 workgroups per CU of each persistent kernel (blockDim 64, no dynamic LDS).
-// "_MIXED" variants take the scene from each job, the others share one scene.
+// "_MIXED" variants take the scene from each job, the others share one scene; "_EXT"
+// variants also run the formulas beyond the four tuned ones (and take each job's scene).
 enum NdResident {
-    RES_MARCH, RES_MARCH_MIXED, RES_MARCH_ACC, RES_MARCH_ACC_MIXED,
-    RES_SURFACE, RES_SURFACE_MIXED,
-    RES_SHADOW, RES_SHADOW_MIXED, RES_SHADOW_ACC, RES_SHADOW_ACC_MIXED,
-    RES_ADJOINT, RES_ADJOINT_MIXED,
+    RES_MARCH, RES_MARCH_MIXED, RES_MARCH_ACC, RES_MARCH_ACC_MIXED, RES_MARCH_EXT, RES_MARCH_ACC_EXT,
+    RES_SURFACE, RES_SURFACE_MIXED, RES_SURFACE_EXT,
+    RES_SHADOW, RES_SHADOW_MIXED, RES_SHADOW_ACC, RES_SHADOW_ACC_MIXED, RES_SHADOW_EXT, RES_SHADOW_ACC_EXT,
+    RES_ADJOINT, RES_ADJOINT_MIXED, RES_ADJOINT_EXT,
     RES_MC_EXTEND, RES_MC_SHADE, RES_MC_SHADOW, RES_MC_IND_MARCH, RES_MC_IND_SHADE,
+    RES_MC_EXTEND_EXT, RES_MC_SHADE_EXT, RES_MC_SHADOW_EXT, RES_MC_IND_MARCH_EXT, RES_MC_IND_SHADE_EXT,
     RES_COUNT
 };
 void nd_resident_march(uint32_t* res);    // fills RES_MARCH .. RES_SHADOW_ACC_MIXED

@@ -26,13 +26,14 @@ NDD V3<float> vnorm(const V3<float>& g) {
     float il = __frsqrt_rn(dot3(g, g));
     return mk3(g.x * il, g.y * il, g.z * il);
 }
+template <bool EXT>
 NDD V3<float> surface_normal(const DevScene& sc, int mode, const V3<float>& p, float eps, int* evals) {
     if (mode == 1) {  // tetrahedral, e = eps(t)
         const float K[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
         V3<float> g = mk3(0.f, 0.f, 0.f);
         for (int i = 0; i < 4; ++i) {
             V3<float> k = mk3(K[i][0], K[i][1], K[i][2]);
-            float d = scene_de<false>(sc, mk3(p.x + k.x * eps, p.y + k.y * eps, p.z + k.z * eps), nullptr);
+            float d = scene_de<false, EXT>(sc, mk3(p.x + k.x * eps, p.y + k.y * eps, p.z + k.z * eps), nullptr);
             g = mk3(g.x + k.x * d, g.y + k.y * d, g.z + k.z * d);
         }
         *evals += 4;
@@ -40,16 +41,16 @@ NDD V3<float> surface_normal(const DevScene& sc, int mode, const V3<float>& p, f
     }
     if (mode == 2) {  // central, h = eps / 2
         float h = 0.5f * eps;
-        float gx = scene_de<false>(sc, mk3(p.x + h, p.y, p.z), nullptr) - scene_de<false>(sc, mk3(p.x - h, p.y, p.z), nullptr);
-        float gy = scene_de<false>(sc, mk3(p.x, p.y + h, p.z), nullptr) - scene_de<false>(sc, mk3(p.x, p.y - h, p.z), nullptr);
-        float gz = scene_de<false>(sc, mk3(p.x, p.y, p.z + h), nullptr) - scene_de<false>(sc, mk3(p.x, p.y, p.z - h), nullptr);
+        float gx = scene_de<false, EXT>(sc, mk3(p.x + h, p.y, p.z), nullptr) - scene_de<false, EXT>(sc, mk3(p.x - h, p.y, p.z), nullptr);
+        float gy = scene_de<false, EXT>(sc, mk3(p.x, p.y + h, p.z), nullptr) - scene_de<false, EXT>(sc, mk3(p.x, p.y - h, p.z), nullptr);
+        float gz = scene_de<false, EXT>(sc, mk3(p.x, p.y, p.z + h), nullptr) - scene_de<false, EXT>(sc, mk3(p.x, p.y, p.z - h), nullptr);
         *evals += 6;
         return vnorm(mk3(gx, gy, gz));
     }
     // analytic: forward-mode derivative of the DE
     using D3 = DualN<3>;
     V3<D3> q = mk3(dvar<3>(p.x, 0), dvar<3>(p.y, 1), dvar<3>(p.z, 2));
-    D3 d = scene_de_d<D3>(sc, q);
+    D3 d = scene_de_d<D3, EXT>(sc, q);
     *evals += 1;
     return vnorm(mk3(d.d[0], d.d[1], d.d[2]));
 }
@@ -77,6 +78,7 @@ NDD const DevScene& scene_of(const DevJob* __restrict__ jobs, const DevScene* __
 }
 
 // The whole per-pixel pipeline of the plain schedule; returns its DE evaluations.
+template <bool EXT>
 NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
     const V3<float> dir = primary_dir(J, J.rect_x + x, J.rect_y + y);
     const V3<float> org = mk3(J.cam[0], J.cam[1], J.cam[2]);
@@ -93,7 +95,7 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
         const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
         const bool probe = acc && m.probing;
         const float skip = probe ? accel_skip(av, ps) : 0.f;
-        float d = scene_de<false>(sc, ps, nullptr);
+        float d = scene_de<false, EXT>(sc, ps, nullptr);
         st = probe ? march_skip_step(m, J, skip, &dist) : -1;
         if (st < 0) st = march_update(m, J, s, d, &dist);
     } while (st == MS_CONT);
@@ -111,9 +113,9 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
     }
     const V3<float> hp = mk3(org.x + dir.x * dist, org.y + dir.y * dist, org.z + dir.z * dist);
     int sev = 0;
-    const V3<float> n = surface_normal(sc, J.normal_mode, hp, J.eps_c * dist + J.eps_f, &sev);
+    const V3<float> n = surface_normal<EXT>(sc, J.normal_mode, hp, J.eps_c * dist + J.eps_f, &sev);
     float trap = 0.f;
-    (void)scene_de<true>(sc, hp, &trap);
+    (void)scene_de<true, EXT>(sc, hp, &trap);
     float shadow = 1.f;
     if (J.shadow_on) {
         V3<float> ld; float ldist;
@@ -127,7 +129,7 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
                 if (t >= ldist) break;
                 const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
                 const float skip = acc ? accel_skip(av, ps) : 0.f;
-                float d = scene_de<false>(sc, ps, nullptr);
+                float d = scene_de<false, EXT>(sc, ps, nullptr);
                 if (acc && shadow_skip_applies(skip, J.light_k, res, t)) { t += skip; continue; }
                 evals++;
                 if (d < 0.0005f) { res = 0.f; break; }
@@ -150,13 +152,14 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
 // ============================================================================ plain kernel
 // One lane per pixel of one job; blockDim 64 = one 8x8 tile per wave64. 6 waves per SIMD
 // (80 VGPRs) is this kernel's fastest setting, about 7% ahead of the compiler's 4.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) k_plain(const DevJob* __restrict__ jobp, DevCounters* ctr) {
+template <bool EXT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 6, EXT ? 8 : 6))) k_plain(const DevJob* __restrict__ jobp, DevCounters* ctr) {
     const DevJob& J = *jobp;
     const DevScene& sc = J.scene;
     const int x = blockIdx.x * 8 + (threadIdx.x & 7);
     const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
     const bool live = x < J.rect_w && y < J.rect_h;
-    const int ev = live ? plain_pixel(J, sc, x, y) : 0;
+    const int ev = live ? plain_pixel<EXT>(J, sc, x, y) : 0;
     unsigned long long v = (unsigned long long)ev;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if (threadIdx.x == 0 && ctr) atomicAdd(&ctr->de_march, v);
@@ -213,9 +216,9 @@ NDD void write_miss(const DevJob& J, size_t o, int evals) {
 // pass about 7% faster, and 6 waves without one about 4%: more waves hide the DE's dependent
 // latency better than the spills cost. The accelerated variant waits on grid gathers and
 // keeps the compiler's choice.
-template <bool SHARED, bool ACC>
+template <bool SHARED, bool ACC, bool EXT>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(ACC ? 1 : (SHARED ? 7 : 6), ACC ? 8 : (SHARED ? 7 : 6))))
+__attribute__((amdgpu_waves_per_eu((ACC || EXT) ? 1 : (SHARED ? 7 : 6), (ACC || EXT) ? 8 : (SHARED ? 7 : 6))))
 k_march(PassArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     uint32_t cnext = 0, cend = 0;          // wave-uniform chunk of ray slots [cnext, cend)
@@ -301,7 +304,7 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
             const bool probe = ACC && acc && m.probing;
             const float skip = probe ? accel_skip(av, ps) : 0.f;
-            const float d = scene_de<false>(sc, ps, nullptr);
+            const float d = scene_de<false, EXT>(sc, ps, nullptr);
             if (probe) st = march_skip_step(m, mp, skip, &dist);
             used = st < 0;
             if (used) st = march_update(m, mp, s, d, &dist);
@@ -366,7 +369,7 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
 
 // --- pass 2: surface. Normal + trap of every hit (fixed cost per hit), and the shadow
 // ray of the hit, stored SoA for pass 3.
-template <bool SHARED>
+template <bool SHARED, bool EXT>
 __global__ void __launch_bounds__(64) k_surface(PassArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     const uint32_t n_hits = a.ctr->hit_count;
@@ -385,9 +388,9 @@ __global__ void __launch_bounds__(64) k_surface(PassArgs a, const DevJob* __rest
             const float t = a.hb.t[i];
             const V3<float> hp = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
             int ev = 0;
-            const V3<float> n = surface_normal(sc, J.normal_mode, hp, J.eps_c * t + J.eps_f, &ev);
+            const V3<float> n = surface_normal<EXT>(sc, J.normal_mode, hp, J.eps_c * t + J.eps_f, &ev);
             float trap = 0.f;
-            (void)scene_de<true>(sc, hp, &trap);
+            (void)scene_de<true, EXT>(sc, hp, &trap);
             ev += 1;
             de += (unsigned long long)ev;
             if (J.out_normal) { J.out_normal[3 * o] = n.x; J.out_normal[3 * o + 1] = n.y; J.out_normal[3 * o + 2] = n.z; }
@@ -422,7 +425,7 @@ __global__ void __launch_bounds__(64) k_surface(PassArgs a, const DevJob* __rest
 
 // --- pass 3: light-0 soft shadow of every hit (visibility_march on an opaque scene,
 // template.wgsl:2282-2342), refilled like the march from the SoA shadow rays.
-template <bool SHARED, bool ACC>
+template <bool SHARED, bool ACC, bool EXT>
 __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     const uint32_t n_hits = a.ctr->hit_count;
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
             const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
             const float skip = (ACC && sacc) ? accel_skip(sav, ps) : 0.f;
-            const float d = scene_de<false>(sc, ps, nullptr);
+            const float d = scene_de<false, EXT>(sc, ps, nullptr);
             step++;
             skipped = ACC && sacc && shadow_skip_applies(skip, k, res, t);
             if (skipped) {
@@ -563,8 +566,9 @@ hipError_t nd_launch_zero_counters(DevCounters* c, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, hipStream_t s) {
-    hipLaunchKernelGGL(nd::k_plain, dim3(tiles_x, tiles_y), dim3(64), 0, s, d_job, ctr);
+hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, bool ext, hipStream_t s) {
+    if (ext) hipLaunchKernelGGL(nd::k_plain<true>, dim3(tiles_x, tiles_y), dim3(64), 0, s, d_job, ctr);
+    else hipLaunchKernelGGL(nd::k_plain<false>, dim3(tiles_x, tiles_y), dim3(64), 0, s, d_job, ctr);
     return hipGetLastError();
 }
 
@@ -587,27 +591,37 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
         hipLaunchKernelGGL(nd::k_zero_u64, dim3((L.n_jobs + 255) / 256), dim3(256), 0, s, L.job_done_cycles, L.n_jobs);
     }
     if (L.ev[0]) hipEventRecord(L.ev[0], s);
-    // ACC: some job uses the accelerator (the kernels without it keep fewer registers)
-    if (L.shared) {
-        if (L.accel) hipLaunchKernelGGL((nd::k_march<true, true>), dim3(L.grid_march), dim3(64), 0, s, a, L.jobs, L.shared);
-        else hipLaunchKernelGGL((nd::k_march<true, false>), dim3(L.grid_march), dim3(64), 0, s, a, L.jobs, L.shared);
+    // ACC: some job uses the accelerator; EXT: some instance uses a formula beyond the four
+    // tuned ones (those kernels read each job's own scene). The kernels without either keep
+    // fewer registers.
+    const dim3 b64(64);
+    if (L.ext) {
+        if (L.accel) hipLaunchKernelGGL((nd::k_march<false, true, true>), dim3(L.grid_march), b64, 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL((nd::k_march<false, false, true>), dim3(L.grid_march), b64, 0, s, a, L.jobs, L.shared);
+    } else if (L.shared) {
+        if (L.accel) hipLaunchKernelGGL((nd::k_march<true, true, false>), dim3(L.grid_march), b64, 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL((nd::k_march<true, false, false>), dim3(L.grid_march), b64, 0, s, a, L.jobs, L.shared);
     } else {
-        if (L.accel) hipLaunchKernelGGL((nd::k_march<false, true>), dim3(L.grid_march), dim3(64), 0, s, a, L.jobs, L.shared);
-        else hipLaunchKernelGGL((nd::k_march<false, false>), dim3(L.grid_march), dim3(64), 0, s, a, L.jobs, L.shared);
+        if (L.accel) hipLaunchKernelGGL((nd::k_march<false, true, false>), dim3(L.grid_march), b64, 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL((nd::k_march<false, false, false>), dim3(L.grid_march), b64, 0, s, a, L.jobs, L.shared);
     }
     if (L.ev[1]) hipEventRecord(L.ev[1], s);
     if (L.surface) {
-        if (L.shared) hipLaunchKernelGGL(nd::k_surface<true>, dim3(L.grid_surface), dim3(64), 0, s, a, L.jobs, L.shared);
-        else hipLaunchKernelGGL(nd::k_surface<false>, dim3(L.grid_surface), dim3(64), 0, s, a, L.jobs, L.shared);
+        if (L.ext) hipLaunchKernelGGL((nd::k_surface<false, true>), dim3(L.grid_surface), b64, 0, s, a, L.jobs, L.shared);
+        else if (L.shared) hipLaunchKernelGGL((nd::k_surface<true, false>), dim3(L.grid_surface), b64, 0, s, a, L.jobs, L.shared);
+        else hipLaunchKernelGGL((nd::k_surface<false, false>), dim3(L.grid_surface), b64, 0, s, a, L.jobs, L.shared);
     }
     if (L.ev[2]) hipEventRecord(L.ev[2], s);
     if (L.any_shadow) {
-        if (L.shared) {
-            if (L.accel) hipLaunchKernelGGL((nd::k_shadow<true, true>), dim3(L.grid_shadow), dim3(64), 0, s, a, L.jobs, L.shared);
-            else hipLaunchKernelGGL((nd::k_shadow<true, false>), dim3(L.grid_shadow), dim3(64), 0, s, a, L.jobs, L.shared);
+        if (L.ext) {
+            if (L.accel) hipLaunchKernelGGL((nd::k_shadow<false, true, true>), dim3(L.grid_shadow), b64, 0, s, a, L.jobs, L.shared);
+            else hipLaunchKernelGGL((nd::k_shadow<false, false, true>), dim3(L.grid_shadow), b64, 0, s, a, L.jobs, L.shared);
+        } else if (L.shared) {
+            if (L.accel) hipLaunchKernelGGL((nd::k_shadow<true, true, false>), dim3(L.grid_shadow), b64, 0, s, a, L.jobs, L.shared);
+            else hipLaunchKernelGGL((nd::k_shadow<true, false, false>), dim3(L.grid_shadow), b64, 0, s, a, L.jobs, L.shared);
         } else {
-            if (L.accel) hipLaunchKernelGGL((nd::k_shadow<false, true>), dim3(L.grid_shadow), dim3(64), 0, s, a, L.jobs, L.shared);
-            else hipLaunchKernelGGL((nd::k_shadow<false, false>), dim3(L.grid_shadow), dim3(64), 0, s, a, L.jobs, L.shared);
+            if (L.accel) hipLaunchKernelGGL((nd::k_shadow<false, true, false>), dim3(L.grid_shadow), b64, 0, s, a, L.jobs, L.shared);
+            else hipLaunchKernelGGL((nd::k_shadow<false, false, false>), dim3(L.grid_shadow), b64, 0, s, a, L.jobs, L.shared);
         }
     }
     if (a.ray_done)
@@ -625,14 +639,19 @@ static uint32_t resident_of(K kernel) {
 }
 
 void nd_resident_march(uint32_t* r) {
-    r[RES_MARCH] = resident_of(nd::k_march<true, false>);
-    r[RES_MARCH_MIXED] = resident_of(nd::k_march<false, false>);
-    r[RES_MARCH_ACC] = resident_of(nd::k_march<true, true>);
-    r[RES_MARCH_ACC_MIXED] = resident_of(nd::k_march<false, true>);
-    r[RES_SURFACE] = resident_of(nd::k_surface<true>);
-    r[RES_SURFACE_MIXED] = resident_of(nd::k_surface<false>);
-    r[RES_SHADOW] = resident_of(nd::k_shadow<true, false>);
-    r[RES_SHADOW_MIXED] = resident_of(nd::k_shadow<false, false>);
-    r[RES_SHADOW_ACC] = resident_of(nd::k_shadow<true, true>);
-    r[RES_SHADOW_ACC_MIXED] = resident_of(nd::k_shadow<false, true>);
+    r[RES_MARCH] = resident_of(nd::k_march<true, false, false>);
+    r[RES_MARCH_MIXED] = resident_of(nd::k_march<false, false, false>);
+    r[RES_MARCH_ACC] = resident_of(nd::k_march<true, true, false>);
+    r[RES_MARCH_ACC_MIXED] = resident_of(nd::k_march<false, true, false>);
+    r[RES_MARCH_EXT] = resident_of(nd::k_march<false, false, true>);
+    r[RES_MARCH_ACC_EXT] = resident_of(nd::k_march<false, true, true>);
+    r[RES_SURFACE] = resident_of(nd::k_surface<true, false>);
+    r[RES_SURFACE_MIXED] = resident_of(nd::k_surface<false, false>);
+    r[RES_SURFACE_EXT] = resident_of(nd::k_surface<false, true>);
+    r[RES_SHADOW] = resident_of(nd::k_shadow<true, false, false>);
+    r[RES_SHADOW_MIXED] = resident_of(nd::k_shadow<false, false, false>);
+    r[RES_SHADOW_ACC] = resident_of(nd::k_shadow<true, true, false>);
+    r[RES_SHADOW_ACC_MIXED] = resident_of(nd::k_shadow<false, true, false>);
+    r[RES_SHADOW_EXT] = resident_of(nd::k_shadow<false, false, true>);
+    r[RES_SHADOW_ACC_EXT] = resident_of(nd::k_shadow<false, true, true>);
 }

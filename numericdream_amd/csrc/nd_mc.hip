@@ -209,6 +209,7 @@ NDD Mat strip_mat(const McInst& I, float trap) {
     return mat_stop(I.stops[count - 1]);
 }
 // hit_material (template.wgsl:2128-2155)
+template <bool EXT>
 NDD Mat hit_material(const DevScene& sc, const McScene& S, const V3<float>& p) {
     bool first = true;
     float d = sc.empty_de;
@@ -217,7 +218,7 @@ NDD Mat hit_material(const DevScene& sc, const McScene& S, const V3<float>& p) {
         const DevInst& in = sc.inst[i];
         if (in.hidden) continue;
         float ti = 0.f;
-        float di = formula_de_f<true>(in, instance_local<float>(in, p), &ti) * in.de_factor;
+        float di = formula_de_f<true, EXT>(in, instance_local<float>(in, p), &ti) * in.de_factor;
         Mat mi = strip_mat(S.inst[i], ti);
         if (first) { d = di; m = mi; first = false; continue; }
         float nd, h;
@@ -233,11 +234,12 @@ NDD V3<float> vnorm3(const V3<float>& g) {
     return mk3(g.x * il, g.y * il, g.z * il);
 }
 // tetra_gradient / estimate_normal (template.wgsl:1781-1804), e = eps(t)
+template <bool EXT>
 NDD V3<float> normal_tetra(const DevScene& sc, const V3<float>& p, float e) {
     const float K[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
     V3<float> g = mk3(0.f, 0.f, 0.f);
     for (int i = 0; i < 4; ++i) {
-        float d = scene_de<false>(sc, mk3(p.x + K[i][0] * e, p.y + K[i][1] * e, p.z + K[i][2] * e), nullptr);
+        float d = scene_de<false, EXT>(sc, mk3(p.x + K[i][0] * e, p.y + K[i][1] * e, p.z + K[i][2] * e), nullptr);
         g = mk3(g.x + K[i][0] * d, g.y + K[i][1] * d, g.z + K[i][2] * d);
     }
     return vnorm3(g);
@@ -362,7 +364,7 @@ __global__ void k_mc_generate(McArgs a) {
 
 // --- persistent march over a queue of rays. RAY provides setup(i, org, dir) and
 // finish(i, hit, t). All rays share the job's march precision.
-template <class RAY>
+template <bool EXT, class RAY>
 NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
     const DevScene& sc = *a.scene;
     const DevJob& J = *a.job;
@@ -418,7 +420,7 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
             const bool probe = acc && m.probing;
             const float skip = probe ? accel_skip(av, ps) : 0.f;
-            const float d = scene_de<false>(sc, ps, nullptr);
+            const float d = scene_de<false, EXT>(sc, ps, nullptr);
             if (probe) st = march_skip_step(m, mp, skip, &dist);
             used = st < 0;
             if (used) st = march_update(m, mp, s, d, &dist);
@@ -458,16 +460,18 @@ struct IndirectRays {
     }
 };
 
+template <bool EXT>
 __global__ void __launch_bounds__(64) k_mc_extend(McArgs a, int ping) {
     const uint32_t* q = ping ? a.P.live_b : a.P.live_a;
     const uint32_t n = a.P.ctr[C_LIVE_IN];
     ExtendRays R{a, q};
-    persistent_march(a, R, n, a.P.ctr + C_HEAD_EXT);
+    persistent_march<EXT>(a, R, n, a.P.ctr + C_HEAD_EXT);
 }
 
+template <bool EXT>
 __global__ void __launch_bounds__(64) k_mc_indirect_march(McArgs a) {
     IndirectRays R{a};
-    persistent_march(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
+    persistent_march<EXT>(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
 }
 
 // shade_local (template.wgsl:3222-3331) for opaque, film- and subsurface-free materials.
@@ -514,6 +518,7 @@ NDD Sp8 shade_lights(const McArgs& a, const Basis& B, const V3<float>& pos, cons
 }
 
 // --- shade (path_shade, template.wgsl:3902-4124). One path per lane.
+template <bool EXT>
 __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
     const uint32_t* qin = ping ? a.P.live_b : a.P.live_a;
     uint32_t* qout = ping ? a.P.live_a : a.P.live_b;
@@ -552,8 +557,8 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                 const float t = a.P.hit_t[p];
                 const float eps = J.eps_c * t + J.eps_f;
                 const V3<float> hp = mk3(pos.x + dir.x * t, pos.y + dir.y * t, pos.z + dir.z * t);
-                const V3<float> nrm = normal_tetra(sc, hp, eps);
-                const Mat mat = hit_material(sc, S, hp);
+                const V3<float> nrm = normal_tetra<EXT>(sc, hp, eps);
+                const Mat mat = hit_material<EXT>(sc, S, hp);
                 de += 5;
                 if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
                     if (a.out_depth) a.out_depth[pix] = t;
@@ -600,7 +605,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                         float occ = 0.f, w = 1.f;
                         for (int k = 0; k < 5; ++k) {
                             const float step = radius * (0.2f + 0.2f * (float)k);
-                            const float dd = scene_de<false>(sc, mk3(hp.x + nrm.x * step, hp.y + nrm.y * step, hp.z + nrm.z * step), nullptr);
+                            const float dd = scene_de<false, EXT>(sc, mk3(hp.x + nrm.x * step, hp.y + nrm.y * step, hp.z + nrm.z * step), nullptr);
                             occ += fmaxf(step - dd, 0.f) * w;
                             w *= 0.6f;
                         }
@@ -647,6 +652,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
 
 // --- indirect shading: shade_local at the indirect hit, no shadows, ambient 0.15
 // (indirect_light_traced, template.wgsl:3359-3377)
+template <bool EXT>
 __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
     const uint32_t n = a.P.ctr[C_IND];
     const DevJob& J = *a.job;
@@ -664,8 +670,8 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
         } else {
             const float t = a.P.it[i];
             const V3<float> bp = mk3(a.P.iox[i] + dir.x * t, a.P.ioy[i] + dir.y * t, a.P.ioz[i] + dir.z * t);
-            const V3<float> bn = normal_tetra(sc, bp, J.eps_c * t + J.eps_f);
-            const Mat bm = hit_material(sc, S, bp);
+            const V3<float> bn = normal_tetra<EXT>(sc, bp, J.eps_c * t + J.eps_f);
+            const Mat bm = hit_material<EXT>(sc, S, bp);
             de += 5;
             L = shade_lights(a, B, bp, dir, bn, bm, sp_splat(0.15f), false, sp_splat(0.f), p);
         }
@@ -676,6 +682,7 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
 }
 
 // --- shadow: visibility_march on an opaque scene (template.wgsl:2282-2342), refilled
+template <bool EXT>
 __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
     const DevScene& sc = *a.scene;
     const DevJob& J = *a.job;
@@ -735,7 +742,7 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
         if (active) {
             const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
             const float skip = acc ? accel_skip(av, ps) : 0.f;
-            const float d = scene_de<false>(sc, ps, nullptr);
+            const float d = scene_de<false, EXT>(sc, ps, nullptr);
             step++;
             skipped = acc && shadow_skip_applies(skip, k, res, t);
             if (skipped) {
@@ -879,11 +886,16 @@ static uint32_t mc_resident(K kernel) {
 }
 
 void nd_resident_mc(uint32_t* r) {
-    r[RES_MC_EXTEND] = mc_resident(nd::k_mc_extend);
-    r[RES_MC_SHADE] = mc_resident(nd::k_mc_shade);
-    r[RES_MC_SHADOW] = mc_resident(nd::k_mc_shadow);
-    r[RES_MC_IND_MARCH] = mc_resident(nd::k_mc_indirect_march);
-    r[RES_MC_IND_SHADE] = mc_resident(nd::k_mc_indirect_shade);
+    r[RES_MC_EXTEND] = mc_resident(nd::k_mc_extend<false>);
+    r[RES_MC_SHADE] = mc_resident(nd::k_mc_shade<false>);
+    r[RES_MC_SHADOW] = mc_resident(nd::k_mc_shadow<false>);
+    r[RES_MC_IND_MARCH] = mc_resident(nd::k_mc_indirect_march<false>);
+    r[RES_MC_IND_SHADE] = mc_resident(nd::k_mc_indirect_shade<false>);
+    r[RES_MC_EXTEND_EXT] = mc_resident(nd::k_mc_extend<true>);
+    r[RES_MC_SHADE_EXT] = mc_resident(nd::k_mc_shade<true>);
+    r[RES_MC_SHADOW_EXT] = mc_resident(nd::k_mc_shadow<true>);
+    r[RES_MC_IND_MARCH_EXT] = mc_resident(nd::k_mc_indirect_march<true>);
+    r[RES_MC_IND_SHADE_EXT] = mc_resident(nd::k_mc_indirect_shade<true>);
 }
 
 extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size, uint32_t spp, nd_rect rect,
@@ -977,9 +989,11 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     a.out_rgb = d_rgb; a.out_depth = d_depth; a.out_normal = d_normal;
     // each pass at its kernel's residency; claims of 64 paths over the pool
     const uint64_t claims = ((uint64_t)P + 63) / 64;
-    const uint32_t g_ext = nd_pass_grid(c, RES_MC_EXTEND, claims), g_shade = nd_pass_grid(c, RES_MC_SHADE, claims);
-    const uint32_t g_shadow = nd_pass_grid(c, RES_MC_SHADOW, claims), g_im = nd_pass_grid(c, RES_MC_IND_MARCH, claims);
-    const uint32_t g_is = nd_pass_grid(c, RES_MC_IND_SHADE, claims);
+    const bool ext = nd_scene_ext(job.scene);
+    const int rx = ext ? RES_MC_EXTEND_EXT - RES_MC_EXTEND : 0;
+    const uint32_t g_ext = nd_pass_grid(c, RES_MC_EXTEND + rx, claims), g_shade = nd_pass_grid(c, RES_MC_SHADE + rx, claims);
+    const uint32_t g_shadow = nd_pass_grid(c, RES_MC_SHADOW + rx, claims), g_im = nd_pass_grid(c, RES_MC_IND_MARCH + rx, claims);
+    const uint32_t g_is = nd_pass_grid(c, RES_MC_IND_SHADE + rx, claims);
     const int max_depth = S.max_depth;
     const bool indirect = S.indirect_on != 0;
     // The passes of one chunk, in stream order. Captured once into a graph per shape and
@@ -991,12 +1005,23 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         int ping = 0;
         for (int depth = 0; depth < max_depth; ++depth) {
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 1);
-            hipLaunchKernelGGL(nd::k_mc_extend, dim3(g_ext), dim3(64), 0, st, a, ping);
-            hipLaunchKernelGGL(nd::k_mc_shade, dim3(g_shade), dim3(64), 0, st, a, ping);
-            hipLaunchKernelGGL(nd::k_mc_shadow, dim3(g_shadow), dim3(64), 0, st, a);
+            if (ext) {
+                hipLaunchKernelGGL(nd::k_mc_extend<true>, dim3(g_ext), dim3(64), 0, st, a, ping);
+                hipLaunchKernelGGL(nd::k_mc_shade<true>, dim3(g_shade), dim3(64), 0, st, a, ping);
+                hipLaunchKernelGGL(nd::k_mc_shadow<true>, dim3(g_shadow), dim3(64), 0, st, a);
+            } else {
+                hipLaunchKernelGGL(nd::k_mc_extend<false>, dim3(g_ext), dim3(64), 0, st, a, ping);
+                hipLaunchKernelGGL(nd::k_mc_shade<false>, dim3(g_shade), dim3(64), 0, st, a, ping);
+                hipLaunchKernelGGL(nd::k_mc_shadow<false>, dim3(g_shadow), dim3(64), 0, st, a);
+            }
             if (depth == 0 && indirect) {
-                hipLaunchKernelGGL(nd::k_mc_indirect_march, dim3(g_im), dim3(64), 0, st, a);
-                hipLaunchKernelGGL(nd::k_mc_indirect_shade, dim3(g_is), dim3(64), 0, st, a);
+                if (ext) {
+                    hipLaunchKernelGGL(nd::k_mc_indirect_march<true>, dim3(g_im), dim3(64), 0, st, a);
+                    hipLaunchKernelGGL(nd::k_mc_indirect_shade<true>, dim3(g_is), dim3(64), 0, st, a);
+                } else {
+                    hipLaunchKernelGGL(nd::k_mc_indirect_march<false>, dim3(g_im), dim3(64), 0, st, a);
+                    hipLaunchKernelGGL(nd::k_mc_indirect_shade<false>, dim3(g_is), dim3(64), 0, st, a);
+                }
             }
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 2);
             ping ^= 1;
@@ -1009,13 +1034,14 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         struct McGraphKey {
             nd::McArgs args;
             uint32_t grid[5];
-            int max_depth, indirect;
+            int max_depth, indirect, ext;
             uint64_t paths;
         } key;
         static_assert(sizeof(McGraphKey) <= sizeof(c->mc_graph_key), "graph key storage");
         std::memset(&key, 0, sizeof(key));
         key.args = a;
         key.grid[0] = g_ext; key.grid[1] = g_shade; key.grid[2] = g_shadow; key.grid[3] = g_im; key.grid[4] = g_is;
+        key.ext = ext;
         key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
         if (!c->mc_graph_exec || !c->graph_valid || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
             if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }

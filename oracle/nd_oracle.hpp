@@ -342,6 +342,12 @@ template <class T, class P> Carry<T> kleinian_step(const Carry<T>& c, V3<T> pos,
     return {z, dr};
 }
 
+}  // namespace ndo
+
+#include "nd_oracle_formulas.hpp"
+
+namespace ndo {
+
 template <class P> int formula_iterations(int f, const Params<P>& p) {
     switch (f) {
         case F_MANDELBULB: return bulb_iterations(p);
@@ -349,7 +355,7 @@ template <class P> int formula_iterations(int f, const Params<P>& p) {
         case F_MENGER3: return menger_iterations(p);
         case F_KLEINIAN: return kleinian_iterations(p);
     }
-    return 0;
+    return ext_iterations(f, p);
 }
 template <class T, class P> Carry<T> formula_step(int f, const Carry<T>& c, V3<T> pos, const Params<P>& p) {
     switch (f) {
@@ -358,7 +364,7 @@ template <class T, class P> Carry<T> formula_step(int f, const Carry<T>& c, V3<T
         case F_MENGER3: return menger_step(c, pos, p);
         case F_KLEINIAN: return kleinian_step(c, pos, p);
     }
-    return c;
+    return ext_step(f, c, pos, p);
 }
 template <class T> T formula_finalize(int f, const Carry<T>& c) {
     switch (f) {
@@ -367,7 +373,7 @@ template <class T> T formula_finalize(int f, const Carry<T>& c) {
         case F_MENGER3: return menger_finalize(c);
         case F_KLEINIAN: return de_finalize_scaled(c);
     }
-    return T(NO_SURFACE_DE);
+    return ext_finalize(f, c);
 }
 
 // ----------------------------------------------------------------------------- orbit traps

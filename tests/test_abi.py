@@ -110,3 +110,16 @@ def test_context_without_gpu_reports_device_error():
         pytest.skip("a GPU is present")
     h = C.c_void_p()
     assert nd.lib.nd_ctx_create(0, C.byref(h)) == abi.ND_E_DEVICE
+
+
+def test_formula_ids_match_the_header():
+    """abi.FORMULA_FILES, the id order of nd_builtin_formula and the test table agree."""
+    import re
+    from tests import formula_table as ft
+    hdr = open(os.path.join(ROOT, "include", "nd_hip.h")).read()
+    ids = {int(m.group(2)): m.group(3) for m in re.finditer(r"ND_FORMULA_(\w+) = (\d+),?\s*/\* (\S+) \*/", hdr)}
+    assert len(ids) == abi.FORMULA_COUNT == int(re.search(r"#define ND_FORMULA_COUNT (\d+)", hdr).group(1))
+    for i, f in enumerate(abi.FORMULA_FILES):
+        assert ids[i] == f, (i, ids[i], f)
+    for name in ft.FILES:
+        assert abi.FORMULA_FILES[ft.ID[name]] == "formulas/%s.wgsl" % name
