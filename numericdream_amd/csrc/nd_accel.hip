@@ -31,11 +31,11 @@ __global__ void k_accel_build(const DevScene* __restrict__ sc, float* __restrict
     const uint32_t x = idx % res, y = (idx / res) % res, gz = idx / (res * res);
     const uint32_t level = gz / res, cz = gz % res;
     const float cell = d.prm[level][3];
-    const V3<float> c = mk3(__fadd_rn(d.prm[level][0], __fmul_rn((float)x + 0.5f, cell)),
-                            __fadd_rn(d.prm[level][1], __fmul_rn((float)y + 0.5f, cell)),
-                            __fadd_rn(d.prm[level][2], __fmul_rn((float)cz + 0.5f, cell)));
+    const V3<float> c = mk3(add_rn(d.prm[level][0], mul_rn((float)x + 0.5f, cell)),
+                            add_rn(d.prm[level][1], mul_rn((float)y + 0.5f, cell)),
+                            add_rn(d.prm[level][2], mul_rn((float)cz + 0.5f, cell)));
     const float de = scene_de<false, EXT>(*sc, c, nullptr);
-    grid[idx] = __fmul_rn(fmaxf(__fsub_rn(de, __fmul_rn(0.8660254f, cell)), 0.f), d.safety);
+    grid[idx] = mul_rn(fmaxf(sub_rn(de, mul_rn(0.8660254f, cell)), 0.f), d.safety);
 }
 
 }  // namespace nd

@@ -33,6 +33,22 @@ NDD float fmin_(float a, float b) { return fminf(a, b); }
 NDD float fmax_(float a, float b) { return fmaxf(a, b); }
 NDD float fclamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }  // v_med3_f32
 NDD float fabs_(float x) { return fabsf(x); }
+// One rounding per operation, for arithmetic the reference's FP32 results depend on bit for
+// bit (hashes of float bits, lattice hashes, grid centres). HIP compiles with
+// -ffp-contract=fast and this toolchain's __fmul_rn / __fadd_rn are plain operators, so
+// a*b + c would otherwise become one fused multiply-add; contraction is off inside these.
+NDD float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+NDD float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+NDD float sub_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
 
 // ----------------------------------------------------------------------------- dual numbers
 template <int N>

@@ -854,13 +854,13 @@ struct FKleinianLimit {
 };
 // formulas/fbmsdf_primitive_carve.wgsl:11-107
 // The hash's product reaches ~2.5e5 before its fract, so it is computed exactly as written
-// in FP32: every product and sum rounded on its own (no fused multiply-add).
-NDD float fbm_fract(float x) { return __fsub_rn(x, floorf(x)); }
+// in FP32: every product and sum rounded on its own (mul_rn / add_rn, no fused multiply-add).
+NDD float fbm_fract(float x) { return sub_rn(x, floorf(x)); }
 NDD float fbm_hash(float x, float y, float z) {
-    const float qx = __fmul_rn(17.f, fbm_fract(__fadd_rn(__fmul_rn(x, 0.3183099f), 0.11f)));
-    const float qy = __fmul_rn(17.f, fbm_fract(__fadd_rn(__fmul_rn(y, 0.3183099f), 0.17f)));
-    const float qz = __fmul_rn(17.f, fbm_fract(__fadd_rn(__fmul_rn(z, 0.3183099f), 0.13f)));
-    return fbm_fract(__fmul_rn(__fmul_rn(__fmul_rn(qx, qy), qz), __fadd_rn(__fadd_rn(qx, qy), qz)));
+    const float qx = mul_rn(17.f, fbm_fract(add_rn(mul_rn(x, 0.3183099f), 0.11f)));
+    const float qy = mul_rn(17.f, fbm_fract(add_rn(mul_rn(y, 0.3183099f), 0.17f)));
+    const float qz = mul_rn(17.f, fbm_fract(add_rn(mul_rn(z, 0.3183099f), 0.13f)));
+    return fbm_fract(mul_rn(mul_rn(mul_rn(qx, qy), qz), add_rn(add_rn(qx, qy), qz)));
 }
 template <class S> NDD S fbm_lattice(const V3<S>& p, float radius) {
     const float ix = floorf(vv(p.x)), iy = floorf(vv(p.y)), iz = floorf(vv(p.z));

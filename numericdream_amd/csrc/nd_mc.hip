@@ -77,7 +77,7 @@ NDD uint32_t hash_u32(float px, float py) {
 }
 NDD float hash12w(float px, float py) { return (float)hash_u32(px, py) * 2.3283064365386963e-10f; }
 // seed arithmetic without contraction, as the shader's separate multiply and add
-NDD float mad_rn(float a, float b, float c) { return __fadd_rn(__fmul_rn(a, b), c); }
+NDD float mad_rn(float a, float b, float c) { return add_rn(mul_rn(a, b), c); }
 
 // ----------------------------------------------------------------------------- spectra
 struct Sp8 { float v[8]; };
@@ -247,7 +247,7 @@ NDD V3<float> normal_tetra(const DevScene& sc, const V3<float>& p, float e) {
 // cosine_hemisphere_from (template.wgsl:3341-3350)
 NDD V3<float> cosine_sample(const V3<float>& n, float sx, float sy) {
     const float u1 = hash12w(sx, sy);
-    const float u2 = hash12w(__fadd_rn(sx, 91.73f), __fadd_rn(sy, 31.13f));
+    const float u2 = hash12w(add_rn(sx, 91.73f), add_rn(sy, 31.13f));
     const float r = sqrtf(u1);
     float sp, cp;
     sincosf(u2 * 6.28318530718f, &sp, &cp);
@@ -346,7 +346,7 @@ __global__ void k_mc_generate(McArgs a) {
     const int x = (int)(pix % (uint32_t)J.rect_w), y = (int)(pix / (uint32_t)J.rect_w);
     const int gx = J.rect_x + x, gy = J.rect_y + y;
     const V3<float> d = primary_dir(J, gx, gy);
-    const float ms = __fadd_rn(a.ch->mc_sample0, (float)s);
+    const float ms = add_rn(a.ch->mc_sample0, (float)s);
     // base seed = frag + mc_sample * (91.73, 57.31) (fs_main, template.wgsl:4818)
     const float bx = mad_rn(ms, 91.73f, (float)gx + 0.5f);
     const float by = mad_rn(ms, 57.31f, (float)gy + 0.5f);
@@ -356,7 +356,7 @@ __global__ void k_mc_generate(McArgs a) {
     // path_begin (template.wgsl:3819-3839)
     a.P.seedx[p] = mad_rn(ms, 27.31f, bx);
     a.P.seedy[p] = mad_rn(ms, 45.17f, by);
-    a.P.hero[p] = mad_rn(ms, 0.61803399f, hash12w(__fadd_rn(bx, 19.31f), __fadd_rn(by, 47.11f)));
+    a.P.hero[p] = mad_rn(ms, 0.61803399f, hash12w(add_rn(bx, 19.31f), add_rn(by, 47.11f)));
     a.P.depth[p] = 0;
     a.P.flags[p] = 0;
     a.P.live_a[p] = p;
@@ -542,8 +542,8 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
             Sp8 thr, rad;
             for (int k = 0; k < 8; ++k) { thr.v[k] = a.P.thr[8 * p + k]; rad.v[k] = a.P.rad[8 * p + k]; }
             // path_seed (template.wgsl:3845-3847)
-            const float sx = __fadd_rn(a.P.seedx[p], mad_rn((float)depth, 17.77f, 3.1f));
-            const float sy = __fadd_rn(a.P.seedy[p], mad_rn((float)depth, 41.13f, 7.7f));
+            const float sx = add_rn(a.P.seedx[p], mad_rn((float)depth, 17.77f, 3.1f));
+            const float sy = add_rn(a.P.seedy[p], mad_rn((float)depth, 41.13f, 7.7f));
             const uint32_t pix = p % a.npix;
             if (!(fl & 2u)) {
                 const V3<float> sky = sky_color(S, dir);
@@ -570,7 +570,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                 const float alpha = rough * rough;
                 V3<float> nr = nf;
                 if (alpha > 1e-4f) {
-                    const V3<float> hs = cosine_sample(nf, __fadd_rn(sx, 5.19f), __fadd_rn(sy, 12.73f));
+                    const V3<float> hs = cosine_sample(nf, add_rn(sx, 5.19f), add_rn(sy, 12.73f));
                     nr = vnorm3(mk3(nf.x + (hs.x - nf.x) * alpha, nf.y + (hs.y - nf.y) * alpha, nf.z + (hs.z - nf.z) * alpha));
                 }
                 const float ior = fmaxf(mat.ior, 1.f);
@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                     if (depth == 0 && S.indirect_on) {
                         // deferred indirect_light: weight throughput * local_share * albedo
                         const Sp8 albedo = from_rgb(B, mat.color[0], mat.color[1], mat.color[2]);
-                        const V3<float> idir = cosine_sample(nrm, __fadd_rn(sx, 13.13f), __fadd_rn(sy, 71.71f));
+                        const V3<float> idir = cosine_sample(nrm, add_rn(sx, 13.13f), add_rn(sy, 71.71f));
                         uint32_t slot;
                         append(a.P.ctr + C_IND, true, &slot);
                         a.P.ind_path[slot] = p;
@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                     cont = true;
                     if (depth >= 4) {
                         const float survive = fclamp(sp_max(thr), 0.05f, 1.f);
-                        if (hash12w(__fadd_rn(sx, 59.3f), __fadd_rn(sy, 23.9f)) > survive) cont = false;
+                        if (hash12w(add_rn(sx, 59.3f), add_rn(sy, 23.9f)) > survive) cont = false;
                         else for (int k = 0; k < 8; ++k) thr.v[k] *= 1.f / survive;
                     }
                     if (cont && depth + 1 < S.max_depth) {
