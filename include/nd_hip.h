@@ -112,9 +112,12 @@ typedef enum nd_builtin_formula {
     ND_FORMULA_RECURSIVESPHERE = 47,   /* formulas/recursivesphere_ifs_carve_twist.wgsl */
     ND_FORMULA_TREE = 48,              /* formulas/tree_ifs.wgsl */
     ND_FORMULA_KLEINIANLIMIT = 49,     /* formulas/kleinianlimit_kleinian.wgsl */
-    ND_FORMULA_FBMSDF = 50             /* formulas/fbmsdf_primitive_carve.wgsl */
+    ND_FORMULA_FBMSDF = 50,            /* formulas/fbmsdf_primitive_carve.wgsl */
+    ND_FORMULA_SEASCAPE = 51,          /* formulas/seascape_primitive.wgsl */
+    ND_FORMULA_DRAGON = 52,            /* formulas/dragon_ifs_twist.wgsl */
+    ND_FORMULA_ROMANESCO = 53          /* formulas/romanesco_ifs.wgsl */
 } nd_builtin_formula;
-#define ND_FORMULA_COUNT 51
+#define ND_FORMULA_COUNT 54
 
 #define ND_MAX_CODES 16
 

@@ -181,7 +181,8 @@ FORMULA_FILES = [
         "pseudokleinian_kleinian", "juliabulb_bulb_julia", "mandelbulbmirror_bulb_fold", "mandelbulbtwist_bulb_twist",
         "mandelbulbdual_hybrid_bulb", "quatjulia_bulb_julia", "bulbbox_hybrid_bulb_box", "mandalay_box_fold",
         "wobblekifs_ifs_twist_julia", "radioactive_ifs_carve_twist", "recursivesphere_ifs_carve_twist", "tree_ifs",
-        "kleinianlimit_kleinian", "fbmsdf_primitive_carve")]
+        "kleinianlimit_kleinian", "fbmsdf_primitive_carve", "seascape_primitive", "dragon_ifs_twist",
+        "romanesco_ifs")]
 FORMULA_COUNT = len(FORMULA_FILES)
 PARAM_COUNT = 6
 

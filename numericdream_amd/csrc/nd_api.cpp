@@ -118,6 +118,9 @@ static const FormulaSlots kSlots[ND_FORMULA_COUNT] = {
     {0, -1, -1, -1, -1},   // tree_ifs.wgsl:15-17
     {5, -1, -1, -1, -1},   // kleinianlimit_kleinian.wgsl:21-23
     {0, -1, -1, -1, -1},   // fbmsdf_primitive_carve.wgsl:29-31
+    {-1, -1, -1, -1, -1},  // seascape_primitive.wgsl:22-24
+    {-1, -1, -1, -1, -1},  // dragon_ifs_twist.wgsl:13-15
+    {0, -1, -1, -1, -1},   // romanesco_ifs.wgsl:24-26
 };
 
 static void hoist(DevInst& in) {

@@ -922,6 +922,299 @@ struct FFbmSdf {
     ND_FIN { return c.dr; }
 };
 
+// ============================================================================= procedural
+// formulas/seascape_primitive.wgsl:14-108: a height field of summed choppy wave octaves,
+// bounded by a slab; the integer hash (LCG_MULTIPLIER / LCG_INCREMENT, template.wgsl:76-81)
+// of the lattice cell.
+NDD float sea_hash(int cx, int cy) {
+    uint32_t vx = (uint32_t)cx * 1664525u + 1013904223u, vy = (uint32_t)cy * 1664525u + 1013904223u;
+    vx += vy * 1664525u;
+    vy += vx * 1664525u;
+    vx ^= vx >> 16;
+    vy ^= vy >> 16;
+    vx += vy * 1664525u;
+    vy += vx * 1664525u;
+    vx ^= vx >> 16;
+    vy ^= vy >> 16;
+    return (float)(vx ^ vy) * 2.3283064365386963e-10f;
+}
+template <class S> NDD S sea_noise(const S& px, const S& py) {
+    const float ix = floorf(vv(px)), iy = floorf(vv(py));
+    const S fx = px - ix, fy = py - iy;
+    const int cx = (int)ix, cy = (int)iy;
+    const S ux = fx * fx * (C<S>(3.f) - fx * 2.f), uy = fy * fy * (C<S>(3.f) - fy * 2.f);
+    const S a = smix(C<S>(sea_hash(cx, cy)), C<S>(sea_hash(cx + 1, cy)), ux);
+    const S b = smix(C<S>(sea_hash(cx, cy + 1)), C<S>(sea_hash(cx + 1, cy + 1)), ux);
+    return smix(a, b, uy) * 2.f - 1.f;
+}
+template <class S> NDD S sea_octave(S ux, S uy, float choppy) {
+    const S n = sea_noise(ux, uy);
+    ux = ux + n;
+    uy = uy + n;
+    S sx, cx, sy, cy;
+    ssincos(ux, &sx, &cx);
+    ssincos(uy, &sy, &cy);
+    S wx = C<S>(1.f) - sabs(sx), wy = C<S>(1.f) - sabs(sy);
+    wx = smix(wx, sabs(cx), wx);
+    wy = smix(wy, sabs(cy), wy);
+    const S crest = spowc(smax_(wx * wy, C<S>(0.f)), 0.65f);
+    return spowc(smax_(C<S>(1.f) - crest, C<S>(0.f)), choppy);
+}
+struct FSeascape {
+    ND_STEP {
+        (void)pos;
+        const int octaves = min(max((int)in.p[0], 1), 8);
+        const float size = fmaxf(in.p[1], K_EPS);
+        const S iq = sdiv(C<S>(1.f), smax_(sabs(c.dr), C<S>(K_EPS_FINE)) * size);
+        const V3<S> q = c.z * iq;
+        const float sea = fmaxf(in.p[2], 0.f), time = in.p[5], extent = fmaxf(in.p[6], K_EPS);
+        const float depth = fmaxf(in.p[7], 0.f), fgain = fmaxf(in.p[8], 0.f);
+        const float again = fminf(fmaxf(in.p[9], 0.f), 0.95f), relax = fminf(fmaxf(in.p[10], 0.f), 1.f);
+        const float hmax = 2.f * sea * (1.f - powf(again, (float)octaves)) / (1.f - again);
+        float reach = hmax;
+        S above = q.y - reach, below = -q.y;
+        S ux = q.x * 0.75f, uy = q.z;
+        float freq = in.p[4], amp = sea, choppy = fmaxf(in.p[3], 1.f), stretch = 1.f, slope = 0.f;
+        S h = C<S>(0.f);
+        for (int i = 0; i < octaves; ++i) {
+            const S d = sea_octave((ux + time) * freq, (uy + time) * freq, choppy) +
+                        sea_octave((ux - time) * freq, (uy - time) * freq, choppy);
+            h = h + d * amp;
+            reach -= 2.f * amp;
+            slope += 2.f * amp * freq * stretch * (3.5f + 1.05f * choppy);
+            const float is = 1.f / sqrtf(1.f + slope * slope);
+            above = smax_(above, (q.y - h - fmaxf(reach, 0.f)) * is);
+            below = smax_(below, (h - q.y) * is);
+            const S nx = ux * 1.6f + uy * 1.2f, ny = uy * 1.6f - ux * 1.2f;  // uv * mat2x2f(1.6, 1.2, -1.2, 1.6)
+            ux = nx;
+            uy = ny;
+            freq *= fgain;
+            amp *= again;
+            stretch *= 2.f;
+            choppy = choppy * (1.f - relax) + relax;
+        }
+        const S wave = vv(q.y) >= vv(h) ? above : -below;
+        const S ex = sabs(q.x) - extent, ey = smax_(-q.y - depth, q.y - hmax), ez = sabs(q.z) - extent;
+        const S bounds = len3(vmax0(mk3(ex, ey, ez))) + smin_(smax_(ex, smax_(ey, ez)), C<S>(0.f));
+        const S d = smax_(wave, bounds);
+        const S elev = sclamp(h * (1.f / fmaxf(hmax, K_EPS)), C<S>(0.f), C<S>(1.f));
+        ND_RET(mk3(C<S>(0.f), elev, C<S>(0.f)), d * size);
+    }
+    ND_FIN { return c.dr; }
+};
+
+// formulas/dragon_ifs_twist.wgsl:7-123: the nearest point of a twisted dragon (or Levy)
+// curve by a depth-first descent over its two affine maps, pruned by bounding spheres.
+struct M3 { float m[3][3]; };  // m[col][row], WGSL mat3x3f columns
+NDD M3 m3_rz(float a) { float c = cosf(a), s = sinf(a); return M3{{{c, s, 0.f}, {-s, c, 0.f}, {0.f, 0.f, 1.f}}}; }
+NDD M3 m3_rx(float a) { float c = cosf(a), s = sinf(a); return M3{{{1.f, 0.f, 0.f}, {0.f, c, s}, {0.f, -s, c}}}; }
+NDD M3 m3_mul(const M3& a, const M3& b) {
+    M3 r;
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) r.m[j][i] = a.m[0][i] * b.m[j][0] + a.m[1][i] * b.m[j][1] + a.m[2][i] * b.m[j][2];
+    return r;
+}
+NDD void m3_apply(const M3& a, const float* v, float* o) {  // a * v
+    for (int i = 0; i < 3; ++i) o[i] = a.m[0][i] * v[0] + a.m[1][i] * v[1] + a.m[2][i] * v[2];
+}
+NDD M3 m3_inverse(const M3& m) {  // dragon_inverse: transpose of the cofactor columns / det
+    const float* c0 = m.m[0]; const float* c1 = m.m[1]; const float* c2 = m.m[2];
+    float a[3] = {c1[1] * c2[2] - c1[2] * c2[1], c1[2] * c2[0] - c1[0] * c2[2], c1[0] * c2[1] - c1[1] * c2[0]};
+    float b[3] = {c2[1] * c0[2] - c2[2] * c0[1], c2[2] * c0[0] - c2[0] * c0[2], c2[0] * c0[1] - c2[1] * c0[0]};
+    float d[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2], c0[0] * c1[1] - c0[1] * c1[0]};
+    const float id = 1.f / (c0[0] * a[0] + c0[1] * a[1] + c0[2] * a[2]);
+    M3 r;
+    for (int i = 0; i < 3; ++i) { r.m[i][0] = a[i] * id; r.m[i][1] = b[i] * id; r.m[i][2] = d[i] * id; }
+    return r;
+}
+NDD float dragon_radius(const float* c, const float* ob, const M3& qa, const M3& qb, float s) {
+    float ac[3], bc[3];
+    m3_apply(qa, c, ac);
+    m3_apply(qb, c, bc);
+    const float x0 = s * ac[0] - c[0], x1 = s * ac[1] - c[1], x2 = s * ac[2] - c[2];
+    const float y0 = ob[0] + s * bc[0] - c[0], y1 = ob[1] + s * bc[1] - c[1], y2 = ob[2] + s * bc[2] - c[2];
+    return fmaxf(sqrtf(x0 * x0 + x1 * x1 + x2 * x2), sqrtf(y0 * y0 + y1 * y1 + y2 * y2)) / (1.f - s);
+}
+NDD float dragon_curve_t(uint32_t path, int n, bool levy, float x) {
+    float t0 = 0.f, span = 1.f;
+    for (int k = 0; k < n; ++k) {
+        if ((path >> k) & 1u) { t0 += levy ? 0.5f * span : span; span *= levy ? 0.5f : -0.5f; }
+        else span *= 0.5f;
+    }
+    return t0 + span * x;
+}
+template <class S> NDD V3<S> vmul_mt(const V3<S>& z, const M3& a) {  // z * a = transpose(a) z
+    return mk3(z.x * a.m[0][0] + z.y * a.m[0][1] + z.z * a.m[0][2], z.x * a.m[1][0] + z.y * a.m[1][1] + z.z * a.m[1][2],
+               z.x * a.m[2][0] + z.y * a.m[2][1] + z.z * a.m[2][2]);
+}
+struct FDragon {
+    ND_STEP {
+        (void)pos;
+        const int n = min(max((int)in.p[0], 1), 20);
+        const float th = in.p[1] * K_DEG;
+        const float s = 0.5f / cosf(th);
+        const bool levy = in.p[4] > 0.5f;
+        const float thick = in.p[5];
+        const M3 qa = m3_mul(m3_rz(th), m3_rx(in.p[2] * K_DEG));
+        const M3 qb = m3_mul(m3_rz(levy ? -th : K_PI - th), m3_rx(in.p[3] * K_DEG));
+        const float ob[3] = {levy ? 0.5f : 1.f, levy ? 0.5f * tanf(th) : 0.f, 0.f};
+        M3 ma, mb;
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                const float e = i == j ? 1.f : 0.f;
+                ma.m[j][i] = e - (qa.m[j][i] + qb.m[j][i]) * (0.5f * s);
+                mb.m[j][i] = e - qb.m[j][i] * s;
+            }
+        const float hob[3] = {0.5f * ob[0], 0.5f * ob[1], 0.5f * ob[2]};
+        float cen[3], mid[3];
+        m3_apply(m3_inverse(ma), hob, cen);
+        m3_apply(m3_inverse(mb), ob, mid);
+        for (int i = 0; i < 3; ++i) mid[i] *= 0.5f;
+        const float rc = dragon_radius(cen, ob, qa, qb, s), rm = dragon_radius(mid, ob, qa, qb, s);
+        const float* cc = rm < rc ? mid : cen;
+        const float radius = fminf(rc, rm);
+        const V3<S> cv = mk3(C<S>(cc[0]), C<S>(cc[1]), C<S>(cc[2]));
+        V3<S> zs[21];
+        zs[0] = c.z * 0.5f + cv;
+        int depth = 0;
+        bool fresh = true;
+        uint32_t path = 0u, second = 0u;
+        S best = C<S>(K_INFINITE);
+        float best_t = 0.f;
+        const float is = 1.f / s;
+        for (int visit = 0; visit < 2048 && depth >= 0; ++visit) {
+            const V3<S> z = zs[depth];
+            const uint32_t bit = 1u << depth;
+            if (fresh) {
+                const float here = powf(s, (float)depth);
+                const S bound = (len3(z - cv) - radius - thick * powf(s, (float)(n - depth))) * here;
+                if (vv(bound) >= vv(best)) { --depth; fresh = false; continue; }
+                if (depth == n) {
+                    const S x = sclamp(z.x, C<S>(0.f), C<S>(1.f));
+                    const S d = (len3(mk3(z.x - x, z.y, z.z)) - thick) * here;
+                    if (vv(d) < vv(best)) { best = d; best_t = dragon_curve_t(path, n, levy, vv(x)); }
+                    --depth;
+                    fresh = false;
+                    continue;
+                }
+            } else if (second & bit) {
+                --depth;
+                continue;
+            }
+            const V3<S> za = vmul_mt(z, qa) * is;
+            const V3<S> zb = vmul_mt(mk3(z.x - ob[0], z.y - ob[1], z.z - ob[2]), qb) * is;
+            const bool a_first = vv(dot3(za - cv, za - cv)) <= vv(dot3(zb - cv, zb - cv));
+            const bool pick_b = a_first != fresh;
+            second = fresh ? (second & ~bit) : (second | bit);
+            path = pick_b ? (path | bit) : (path & ~bit);
+            zs[depth + 1] = pick_b ? zb : za;
+            ++depth;
+            fresh = true;
+        }
+        for (int k = 0; k <= depth; ++k)
+            best = smin_(best, (len3(zs[k] - cv) - radius - thick * powf(s, (float)(n - k))) * powf(s, (float)k));
+        ND_RET(mk3(C<S>(best_t), C<S>(0.f), C<S>(0.f)), sdiv(best * 2.f, smax_(sabs(c.dr), C<S>(K_EPS_FINE))));
+    }
+    ND_FIN { return c.dr; }
+};
+
+// formulas/romanesco_ifs.wgsl:8-146: a cone of spiral-packed buds, each bud a smaller cone;
+// dr carries the running minimum with a 4-bit level and a 10-bit world scale code.
+template <class S> NDD S rom_cone(const V3<S>& z, float s, float tip) {
+    const float r2 = fminf(tip, 0.9f * s);
+    return round_cone(z, C<S>(s), C<S>(r2), C<S>(1.f - r2 / s));
+}
+template <class S> NDD S rom_pack(const S& m, float wpl, uint32_t level) {
+    const float q = fminf(fmaxf(rintf((2.f - log2f(wpl)) * 32.f), 1.f), 1023.f);
+    return pack_low(m, 16383u, ((uint32_t)q << 4) | level);
+}
+struct RomFrame { float ex, ez, ax, ay, az, cx, cy, cz, bx, by, bz, scale; };
+NDD RomFrame rom_frame(float c0, float c1, float u0, float sc, float cc, float lean, float gap, float sink) {
+    const float sn = expf(u0 + c0);
+    const float th = c1 / sc;
+    const float erx = cosf(th), erz = sinf(th);
+    const float etx = -erz, etz = erx;  // et = (-er.z, 0, er.x)
+    const float nx = erx * cc, ny = sc, nz = erz * cc;
+    const float ux = -erx * sc, uy = cc, uz = -erz * sc;
+    float sl, cl;
+    sincosf(lean, &sl, &cl);
+    const float ax = nx * cl + ux * sl, ay = ny * cl + uy * sl, az = nz * cl + uz * sl;
+    const float scale = sn * gap;
+    RomFrame f;
+    f.ex = etx; f.ez = etz;
+    f.ax = ax; f.ay = ay; f.az = az;
+    // cross(et, axis) with et.y = 0
+    f.cx = 0.f * az - etz * ay; f.cy = etz * ax - etx * az; f.cz = etx * ay - 0.f * ax;
+    f.bx = (erx * sc) * sn - ax * (sink * scale);
+    f.by = 1.f + (-cc) * sn - ay * (sink * scale);
+    f.bz = (erz * sc) * sn - az * (sink * scale);
+    f.scale = scale;
+    return f;
+}
+template <class S> NDD V3<S> rom_local(const V3<S>& z, const RomFrame& f) {
+    const V3<S> v = mk3(z.x - f.bx, z.y - f.by, z.z - f.bz);
+    const float is = 1.f / f.scale;
+    return mk3((v.x * f.ex + v.z * f.ez) * is, (v.x * f.ax + v.y * f.ay + v.z * f.az) * is,
+               (v.x * f.cx + v.y * f.cy + v.z * f.cz) * is);
+}
+struct FRomanesco {
+    ND_STEP {
+        (void)pos;
+        const uint32_t bits = sbits(c.dr) & 16383u;
+        if (bits != 0u && vv(dot3(c.z, c.z)) > 100.f) return c;
+        const uint32_t level = bits & 15u;
+        float wpl = exp2f(2.f - (float)(bits >> 4) / 32.f);
+        V3<S> z = c.z;
+        S m = c.dr;
+        const float alpha = in.p[1] * K_DEG;
+        const float sa = sinf(alpha);
+        const float lean = in.p[4] * K_DEG, sink = in.p[5], tip = in.p[6];
+        float fi = 3.f, fj = 5.f;
+        for (int k = 1; k < (int)in.p[2]; ++k) { const float nx = fi + fj; fi = fj; fj = nx; }
+        const float g = 0.381966f;
+        const float ei = fi * g - rintf(fi * g), ej = fj * g - rintf(fj * g);
+        const float lam = sqrtf((ei * ei - ej * ej) / (fj * fj - fi * fi));
+        const float kappa = sqrtf(fi * fi * lam * lam + ei * ei);
+        const float t = tanf(alpha) / cosf(lean);
+        const float ac = atanf(t * sa / (K_PI * kappa * in.p[3] + t * cosf(alpha)));
+        const float sc = sinf(ac), cc = cosf(ac);
+        if (bits == 0u) {
+            wpl = 1.5f;
+            z = mk3(z.x * (1.f / 1.5f), (z.y + 0.5f) * (1.f / 1.5f), z.z * (1.f / 1.5f));
+            m = smax_(rom_cone(z, sc, tip), -z.y) * wpl;
+        }
+        const bool last = (int)level + 1 >= (int)in.p[0];
+        const float per = K_TAU * sc;
+        const float vax = -fi * lam * per, vay = ei * per, vbx = -fj * lam * per, vby = ej * per;
+        const float spacing = per * kappa;
+        const float slant = fmaxf(sqrtf(vv(z.x) * vv(z.x) + vv(z.z) * vv(z.z)) * sc - (vv(z.y) - 1.f) * cc, 1e-5f);
+        const float u0 = logf(1.f / cc) - 0.5f * spacing;
+        const float dx = fminf(logf(slant), u0 + spacing) - u0, dy = atan2f(vv(z.z), vv(z.x)) * sc;
+        const float det = vax * vby - vay * vbx;
+        const float x0 = rintf((dx * vby - dy * vbx) / det), y0 = rintf((vax * dy - vay * dx) / det);
+        const float gap = sinf(alpha - ac) / ((1.f - sink) * cosf(lean));
+        RomFrame bf = rom_frame(0.f, 0.f, u0, sc, cc, lean, gap, sink);
+        V3<S> best = rom_local(z, bf);
+        S best_outer = rom_cone(best, sa, tip) * bf.scale;
+        S near = last ? best_outer : rom_cone(best, sc, tip) * bf.scale;
+        float best_scale = bf.scale;
+        for (int i = -1; i <= 1; ++i)
+            for (int j = -1; j <= 1; ++j) {
+                const float cx = x0 + (float)i, cy = y0 + (float)j;
+                if (cx * fi + cy * fj < -0.5f) continue;
+                const RomFrame f = rom_frame(cx * vax + cy * vbx, cx * vay + cy * vby, u0, sc, cc, lean, gap, sink);
+                const V3<S> l = rom_local(z, f);
+                const S outer = rom_cone(l, sa, tip) * f.scale;
+                near = smin_(near, last ? outer : rom_cone(l, sc, tip) * f.scale);
+                if (vv(outer) < vv(best_outer)) { best = l; best_outer = outer; best_scale = f.scale; }
+            }
+        m = smin_(m, near * wpl);
+        wpl = wpl * best_scale;
+        ND_RET(best, rom_pack(m, wpl, min(level + 1u, 15u)));
+    }
+    ND_FIN { return with_value(c.dr, __uint_as_float(sbits(c.dr) & ~16383u)); }
+};
+
 #undef ND_STEP
 #undef ND_FIN
 #undef ND_RET
@@ -1005,7 +1298,10 @@ NDD S ext_run_d(const DevInst& in, const V3<S>& pos) {
     X(ND_FORMULA_RECURSIVESPHERE, FRecursiveSphere)                        \
     X(ND_FORMULA_TREE, FTree)                                              \
     X(ND_FORMULA_KLEINIANLIMIT, FKleinianLimit)                            \
-    X(ND_FORMULA_FBMSDF, FFbmSdf)
+    X(ND_FORMULA_FBMSDF, FFbmSdf)                                          \
+    X(ND_FORMULA_SEASCAPE, FSeascape)                                      \
+    X(ND_FORMULA_DRAGON, FDragon)                                          \
+    X(ND_FORMULA_ROMANESCO, FRomanesco)
 
 // The dispatch is a real call, not inlined: it keeps the 47 formula loops out of the
 // register allocation of the kernels' hot paths, which only ever run the four tuned DEs

@@ -24,7 +24,7 @@ enum {
     F_MANDELBOXTWIST, F_PRISMKIFS, F_BUFFALO, F_BUFFALOJULIA, F_BUFFALOTWIST, F_APOLLONIAN, F_APOLLONIANJULIA,
     F_KLEINIANJULIA, F_PSEUDOKLEINIAN, F_JULIABULB, F_MANDELBULBMIRROR, F_MANDELBULBTWIST, F_MANDELBULBDUAL,
     F_QUATJULIA, F_BULBBOX, F_MANDALAY, F_WOBBLEKIFS, F_RADIOACTIVE, F_RECURSIVESPHERE, F_TREE,
-    F_KLEINIANLIMIT, F_FBMSDF, F_COUNT
+    F_KLEINIANLIMIT, F_FBMSDF, F_SEASCAPE, F_DRAGON, F_ROMANESCO, F_COUNT
 };
 
 constexpr double TAU = 6.28318530718;   // template.wgsl:60
@@ -859,6 +859,285 @@ struct OFbmSdf {  // formulas/fbmsdf_primitive_carve.wgsl:1-107
     O_FIN { return c.dr; }
 };
 
+// ============================================================================= procedural
+// formulas/seascape_primitive.wgsl:14-108 (integer hash with the LCG constants of
+// template.wgsl:76-81)
+inline double sea_hash(int32_t cx, int32_t cy) {
+    uint32_t v[2] = {(uint32_t)cx * 1664525u + 1013904223u, (uint32_t)cy * 1664525u + 1013904223u};
+    for (int round = 0; round < 2; ++round) {
+        v[0] += v[1] * 1664525u;
+        v[1] += v[0] * 1664525u;
+        v[0] ^= v[0] >> 16u;
+        v[1] ^= v[1] >> 16u;
+    }
+    return (double)((float)(v[0] ^ v[1]) * 2.3283064365386963e-10f);
+}
+template <class T> T sea_noise(T px, T py) {
+    double ix = std::floor(val(px)), iy = std::floor(val(py));
+    T fx = px - T(ix), fy = py - T(iy);
+    int32_t cx = (int32_t)ix, cy = (int32_t)iy;
+    T ux = fx * fx * (T(3) - T(2) * fx), uy = fy * fy * (T(3) - T(2) * fy);
+    T lo = wmix(T(sea_hash(cx, cy)), T(sea_hash(cx + 1, cy)), ux);
+    T hi = wmix(T(sea_hash(cx, cy + 1)), T(sea_hash(cx + 1, cy + 1)), ux);
+    return T(-1) + T(2) * wmix(lo, hi, uy);
+}
+template <class T> T sea_octave(T ux, T uy, double choppy) {
+    T n = sea_noise(ux, uy);
+    ux = ux + n;
+    uy = uy + n;
+    T wx = T(1) - abs(sin(ux)), wy = T(1) - abs(sin(uy));
+    wx = wmix(wx, abs(cos(ux)), wx);
+    wy = wmix(wy, abs(cos(uy)), wy);
+    T crest = pow(tmax(wx * wy, T(0)), T(0.65));
+    return pow(tmax(T(1) - crest, T(0)), T(choppy));
+}
+struct OSeascape {
+    O_ONE
+    O_STEP {
+        (void)pos;
+        auto v = [&](int k) { return (double)val(pp.p[k]); };
+        int octaves = std::min(std::max(i32(v(0)), 1), 8);
+        double size = std::max(v(1), EPSILON);
+        V3<T> q = c.z / (T(size) * tmax(abs(c.dr), T(EPSILON_FINE)));
+        double sea = std::max(v(2), 0.0), time = v(5), extent = std::max(v(6), EPSILON), depth = std::max(v(7), 0.0);
+        double fgain = std::max(v(8), 0.0), again = std::min(std::max(v(9), 0.0), 0.95), relax = std::min(std::max(v(10), 0.0), 1.0);
+        double hmax = 2.0 * sea * (1.0 - std::pow(again, (double)octaves)) / (1.0 - again);
+        double reach = hmax;
+        T above = q.y - T(reach), below = -q.y;
+        T ux = q.x * T(0.75), uy = q.z;
+        double freq = v(4), amp = sea, choppy = std::max(v(3), 1.0), stretch = 1.0, slope = 0.0;
+        T h = T(0);
+        for (int i = 0; i < octaves; ++i) {
+            T d = sea_octave((ux + T(time)) * T(freq), (uy + T(time)) * T(freq), choppy) +
+                  sea_octave((ux - T(time)) * T(freq), (uy - T(time)) * T(freq), choppy);
+            h = h + d * T(amp);
+            reach -= 2.0 * amp;
+            slope += 2.0 * amp * freq * stretch * (3.5 + 1.05 * choppy);
+            double sl = std::sqrt(1.0 + slope * slope);
+            above = tmax(above, (q.y - h - T(std::max(reach, 0.0))) / T(sl));
+            below = tmax(below, (h - q.y) / T(sl));
+            T nx = ux * T(1.6) + uy * T(1.2), ny = ux * T(-1.2) + uy * T(1.6);  // uv * mat2x2f(1.6, 1.2, -1.2, 1.6)
+            ux = nx;
+            uy = ny;
+            freq *= fgain;
+            amp *= again;
+            stretch *= 2.0;
+            choppy = choppy * (1.0 - relax) + relax;
+        }
+        T wave = q.y >= h ? above : -below;
+        V3<T> e(abs(q.x) - T(extent), tmax(-T(depth) - q.y, q.y - T(hmax)), abs(q.z) - T(extent));
+        T bounds = length(vmax(e, V3<T>())) + tmin(tmax(e.x, tmax(e.y, e.z)), T(0));
+        T d = tmax(wave, bounds);
+        T elevation = tclamp(h / T(std::max(hmax, EPSILON)), T(0), T(1));
+        return {V3<T>(T(0), elevation, T(0)), d * T(size)};
+    }
+    O_FIN { return c.dr; }
+};
+
+// formulas/dragon_ifs_twist.wgsl:7-123 (matrices as column arrays, as WGSL mat3x3f)
+struct DMat { double c[3][3]; };  // c[col][row]
+inline DMat dm_rz(double a) { return DMat{{{std::cos(a), std::sin(a), 0}, {-std::sin(a), std::cos(a), 0}, {0, 0, 1}}}; }
+inline DMat dm_rx(double a) { return DMat{{{1, 0, 0}, {0, std::cos(a), std::sin(a)}, {0, -std::sin(a), std::cos(a)}}}; }
+inline DMat dm_mul(const DMat& a, const DMat& b) {
+    DMat r{};
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) r.c[j][i] += a.c[k][i] * b.c[j][k];
+    return r;
+}
+inline V3<double> dm_apply(const DMat& a, V3<double> v) {
+    return V3<double>(a.c[0][0] * v.x + a.c[1][0] * v.y + a.c[2][0] * v.z, a.c[0][1] * v.x + a.c[1][1] * v.y + a.c[2][1] * v.z,
+                      a.c[0][2] * v.x + a.c[1][2] * v.y + a.c[2][2] * v.z);
+}
+inline DMat dm_inverse(const DMat& m) {  // dragon_inverse
+    V3<double> c0(m.c[0][0], m.c[0][1], m.c[0][2]), c1(m.c[1][0], m.c[1][1], m.c[1][2]), c2(m.c[2][0], m.c[2][1], m.c[2][2]);
+    V3<double> a = cross(c1, c2), b = cross(c2, c0), d = cross(c0, c1);
+    double id = 1.0 / dot(c0, a);
+    DMat r;
+    V3<double> rows[3] = {a, b, d};
+    for (int i = 0; i < 3; ++i) {  // transpose(mat3x3f(a, b, d)): row i is rows[i]
+        r.c[0][i] = rows[i].x * id; r.c[1][i] = rows[i].y * id; r.c[2][i] = rows[i].z * id;
+    }
+    return r;
+}
+template <class T> V3<T> vec_times_mat(V3<T> z, const DMat& a) {  // WGSL z * a
+    return V3<T>(z.x * T(a.c[0][0]) + z.y * T(a.c[0][1]) + z.z * T(a.c[0][2]),
+                 z.x * T(a.c[1][0]) + z.y * T(a.c[1][1]) + z.z * T(a.c[1][2]),
+                 z.x * T(a.c[2][0]) + z.y * T(a.c[2][1]) + z.z * T(a.c[2][2]));
+}
+inline double dragon_radius(V3<double> c, V3<double> ob, const DMat& qa, const DMat& qb, double s) {
+    return std::max(length(dm_apply(qa, c) * s - c), length(ob + dm_apply(qb, c) * s - c)) / (1.0 - s);
+}
+inline double dragon_curve_t(uint32_t path, int n, bool levy, double x) {
+    double t0 = 0.0, span = 1.0;
+    for (int k = 0; k < n; ++k) {
+        if (((path >> (uint32_t)k) & 1u) == 1u) {
+            t0 += levy ? 0.5 * span : span;
+            span *= levy ? 0.5 : -0.5;
+        } else {
+            span *= 0.5;
+        }
+    }
+    return t0 + span * x;
+}
+struct ODragon {
+    O_ONE
+    O_STEP {
+        (void)pos;
+        auto v = [&](int k) { return (double)val(pp.p[k]); };
+        int n = std::min(std::max(i32(v(0)), 1), 20);
+        double th = v(1) * (3.14159265358979323846 / 180.0);
+        double s = 0.5 / std::cos(th);
+        bool levy = v(4) > 0.5;
+        double thick = v(5);
+        DMat qa = dm_mul(dm_rz(th), dm_rx(v(2) * (3.14159265358979323846 / 180.0)));
+        DMat qb = dm_mul(dm_rz(levy ? -th : PI - th), dm_rx(v(3) * (3.14159265358979323846 / 180.0)));
+        V3<double> ob = levy ? V3<double>(0.5, 0.5 * std::tan(th), 0.0) : V3<double>(1.0, 0.0, 0.0);
+        DMat ma, mb;
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                ma.c[j][i] = (i == j ? 1.0 : 0.0) - (qa.c[j][i] + qb.c[j][i]) * (0.5 * s);
+                mb.c[j][i] = (i == j ? 1.0 : 0.0) - qb.c[j][i] * s;
+            }
+        V3<double> centroid = dm_apply(dm_inverse(ma), ob * 0.5);
+        V3<double> mid = dm_apply(dm_inverse(mb), ob) * 0.5;
+        double rc = dragon_radius(centroid, ob, qa, qb, s), rm = dragon_radius(mid, ob, qa, qb, s);
+        V3<double> cd = rm < rc ? mid : centroid;
+        double radius = std::min(rc, rm);
+        V3<T> cv(T(cd.x), T(cd.y), T(cd.z));
+        V3<T> zs[21];
+        zs[0] = c.z / T(2) + cv;
+        int depth = 0;
+        bool fresh = true;
+        uint32_t path = 0u, second = 0u;
+        T best = T(INFINITE_DISTANCE);
+        double best_t = 0.0;
+        for (int visit = 0; visit < 2048 && depth >= 0; ++visit) {
+            V3<T> z = zs[depth];
+            uint32_t bit = 1u << (uint32_t)depth;
+            if (fresh) {
+                double here = std::pow(s, (double)depth);
+                T bound = (length(z - cv) - T(radius) - T(thick * std::pow(s, (double)(n - depth)))) * T(here);
+                if (bound >= best) { depth--; fresh = false; continue; }
+                if (depth == n) {
+                    T x = tclamp(z.x, T(0), T(1));
+                    T d = (length(z - V3<T>(x, T(0), T(0))) - T(thick)) * T(here);
+                    if (d < best) { best = d; best_t = dragon_curve_t(path, n, levy, (double)val(x)); }
+                    depth--;
+                    fresh = false;
+                    continue;
+                }
+            } else if ((second & bit) != 0u) {
+                depth--;
+                continue;
+            }
+            V3<T> za = vec_times_mat(z, qa) / T(s);
+            V3<T> zb = vec_times_mat(z - V3<T>(T(ob.x), T(ob.y), T(ob.z)), qb) / T(s);
+            bool a_first = dot(za - cv, za - cv) <= dot(zb - cv, zb - cv);
+            bool pick_b = a_first != fresh;
+            second = !fresh ? (second | bit) : (second & ~bit);
+            path = pick_b ? (path | bit) : (path & ~bit);
+            zs[depth + 1] = pick_b ? zb : za;
+            depth++;
+            fresh = true;
+        }
+        for (int k = 0; k <= depth; ++k)
+            best = tmin(best, (length(zs[k] - cv) - T(radius) - T(thick * std::pow(s, (double)(n - k)))) * T(std::pow(s, (double)k)));
+        return {V3<T>(T(best_t), T(0), T(0)), best * T(2) / tmax(abs(c.dr), T(EPSILON_FINE))};
+    }
+    O_FIN { return c.dr; }
+};
+
+// formulas/romanesco_ifs.wgsl:8-146
+template <class T> T rom_cone(V3<T> z, double s, double tip) {
+    double r2 = std::min(tip, 0.9 * s);
+    return tree_limb(z, T(s), T(r2), T(1.0 - r2 / s));
+}
+template <class T> T rom_pack(T m, double wpl, uint32_t level) {
+    double q = std::min(std::max(std::nearbyint((2.0 - std::log2(wpl)) * 32.0), 1.0), 1023.0);
+    return pack_f32(m, 16383u, ((uint32_t)q << 4u) | level);
+}
+template <class T> V3<T> rom_frame(V3<T> z, double c0, double c1, double u0, double sc, double cc, double lean, double gap,
+                                   double sink, double* scale_out) {
+    double sn = std::exp(u0 + c0);
+    double th = c1 / sc;
+    V3<double> er(std::cos(th), 0.0, std::sin(th));
+    V3<double> et(-er.z, 0.0, er.x);
+    V3<double> normal = er * cc + V3<double>(0.0, sc, 0.0);
+    V3<double> up_slope = V3<double>(0.0, cc, 0.0) - er * sc;
+    V3<double> axis = normal * std::cos(lean) + up_slope * std::sin(lean);
+    double scale = sn * gap;
+    V3<double> base = V3<double>(0.0, 1.0, 0.0) + (er * sc - V3<double>(0.0, cc, 0.0)) * sn - axis * (sink * scale);
+    V3<double> w = cross(et, axis);
+    V3<T> v = z - V3<T>(T(base.x), T(base.y), T(base.z));
+    *scale_out = scale;
+    return V3<T>(dot(v, V3<T>(T(et.x), T(et.y), T(et.z))), dot(v, V3<T>(T(axis.x), T(axis.y), T(axis.z))),
+                 dot(v, V3<T>(T(w.x), T(w.y), T(w.z)))) / T(scale);
+}
+struct ORomanesco {
+    O_ITERS(0)
+    O_STEP {
+        (void)pos;
+        auto v = [&](int k) { return (double)val(pp.p[k]); };
+        uint32_t bits = f32_bits(c.dr) & 16383u;
+        if (bits != 0u && dot(c.z, c.z) > T(100)) return c;
+        uint32_t level = bits & 15u;
+        double wpl = std::exp2(2.0 - (double)(bits >> 4u) / 32.0);
+        V3<T> z = c.z;
+        T m = c.dr;
+        const double deg = 3.14159265358979323846 / 180.0;
+        double alpha = v(1) * deg, sa = std::sin(alpha), lean = v(4) * deg, sink = v(5), tip = v(6);
+        double fi = 3.0, fj = 5.0;
+        for (int k = 1; k < i32(v(2)); ++k) { double nx = fi + fj; fi = fj; fj = nx; }
+        const double g = 0.381966;
+        double ei = fi * g - std::nearbyint(fi * g), ej = fj * g - std::nearbyint(fj * g);
+        double lam = std::sqrt((ei * ei - ej * ej) / (fj * fj - fi * fi));
+        double kappa = std::sqrt(fi * fi * lam * lam + ei * ei);
+        double t = std::tan(alpha) / std::cos(lean);
+        double ac = std::atan(t * sa / (PI * kappa * v(3) + t * std::cos(alpha)));
+        double sc = std::sin(ac), cc = std::cos(ac);
+        if (bits == 0u) {
+            wpl = 1.5;
+            z = (z + V3<T>(T(0), T(0.5), T(0))) / T(wpl);
+            m = tmax(rom_cone(z, sc, tip), -z.y) * T(wpl);
+        }
+        bool last = (int)level + 1 >= i32(v(0));
+        double per = TAU * sc;
+        double vax = -fi * lam * per, vay = ei * per, vbx = -fj * lam * per, vby = ej * per;
+        double spacing = per * kappa;
+        double zx = val(z.x), zy = val(z.y), zz = val(z.z);
+        double slant = std::max(std::sqrt(zx * zx + zz * zz) * sc - (zy - 1.0) * cc, 1e-5);
+        double u0 = std::log(1.0 / cc) - 0.5 * spacing;
+        double dx = std::min(std::log(slant), u0 + spacing) - u0, dy = std::atan2(zz, zx) * sc;
+        double det = vax * vby - vay * vbx;
+        double x0 = std::nearbyint((dx * vby - dy * vbx) / det), y0 = std::nearbyint((vax * dy - vay * dx) / det);
+        double gap = std::sin(alpha - ac) / ((1.0 - sink) * std::cos(lean));
+        double bscale;
+        V3<T> best = rom_frame(z, 0.0, 0.0, u0, sc, cc, lean, gap, sink, &bscale);
+        T best_outer = rom_cone(best, sa, tip) * T(bscale);
+        T near = last ? best_outer : rom_cone(best, sc, tip) * T(bscale);
+        for (int i = -1; i <= 1; ++i)
+            for (int j = -1; j <= 1; ++j) {
+                double cx = x0 + i, cy = y0 + j;
+                if (cx * fi + cy * fj < -0.5) continue;
+                double fs;
+                V3<T> f = rom_frame(z, cx * vax + cy * vbx, cx * vay + cy * vby, u0, sc, cc, lean, gap, sink, &fs);
+                T outer = rom_cone(f, sa, tip) * T(fs);
+                near = tmin(near, last ? outer : rom_cone(f, sc, tip) * T(fs));
+                if (outer < best_outer) { best = f; best_outer = outer; bscale = fs; }
+            }
+        m = tmin(m, near * T(wpl));
+        wpl = wpl * bscale;
+        return {best, rom_pack(m, wpl, std::min(level + 1u, 15u))};
+    }
+    O_FIN {
+        uint32_t u = f32_bits(c.dr) & ~16383u;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return setv(c.dr, (double)f);
+    }
+};
+
 #undef O_ITERS
 #undef O_ONE
 #undef O_STEP
@@ -879,7 +1158,8 @@ struct OFbmSdf {  // formulas/fbmsdf_primitive_carve.wgsl:1-107
     X(F_PSEUDOKLEINIAN, OPseudoKleinian) X(F_JULIABULB, OJuliaBulb) X(F_MANDELBULBMIRROR, OMandelbulbMirror)        \
     X(F_MANDELBULBTWIST, OMandelbulbTwist) X(F_MANDELBULBDUAL, OMandelbulbDual) X(F_QUATJULIA, OQuatJulia)          \
     X(F_BULBBOX, OBulbBox) X(F_MANDALAY, OMandalay) X(F_WOBBLEKIFS, OWobbleKifs) X(F_RADIOACTIVE, ORadioactive)     \
-    X(F_RECURSIVESPHERE, ORecursiveSphere) X(F_TREE, OTree) X(F_KLEINIANLIMIT, OKleinianLimit) X(F_FBMSDF, OFbmSdf)
+    X(F_RECURSIVESPHERE, ORecursiveSphere) X(F_TREE, OTree) X(F_KLEINIANLIMIT, OKleinianLimit) X(F_FBMSDF, OFbmSdf) \
+    X(F_SEASCAPE, OSeascape) X(F_DRAGON, ODragon) X(F_ROMANESCO, ORomanesco)
 
 template <class P> int ext_iterations(int f, const Params<P>& p) {
     switch (f) {

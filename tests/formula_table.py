@@ -20,7 +20,7 @@ FILES = [
     "juliabulb_bulb_julia", "mandelbulbmirror_bulb_fold", "mandelbulbtwist_bulb_twist", "mandelbulbdual_hybrid_bulb",
     "quatjulia_bulb_julia", "bulbbox_hybrid_bulb_box", "mandalay_box_fold", "wobblekifs_ifs_twist_julia",
     "radioactive_ifs_carve_twist", "recursivesphere_ifs_carve_twist", "tree_ifs", "kleinianlimit_kleinian",
-    "fbmsdf_primitive_carve",
+    "fbmsdf_primitive_carve", "seascape_primitive", "dragon_ifs_twist", "romanesco_ifs",
 ]
 ID = {f: 4 + i for i, f in enumerate(FILES)}
 BUILTIN_FILES = {0: "mandelbrot", 1: "amazingbox_box", 2: "menger3_ifs", 3: "kleinian_kleinian"}
@@ -106,6 +106,7 @@ SCENES = {
     "mandalay_box_fold": (3.0, {}), "wobblekifs_ifs_twist_julia": (20.0, {1: 8.0}),
     "radioactive_ifs_carve_twist": (9.0, {}), "recursivesphere_ifs_carve_twist": (4.5, {}), "tree_ifs": (2.0, {}),
     "kleinianlimit_kleinian": (6.5, {5: 8.0}), "fbmsdf_primitive_carve": (3.0, {}),
+    "seascape_primitive": (3.0, {}), "dragon_ifs_twist": (1.0, {}), "romanesco_ifs": (2.0, {}),
 }
 
 

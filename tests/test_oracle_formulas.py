@@ -18,7 +18,9 @@ slot, a dropped term, a wrong sign or mirror) fails:
 
 The packed-carry formulas (radioactive, recursivesphere, tree, kleinianlimit, fbmsdf) are
 pinned only by their far-field and trunk closed forms: their interiors are pinned by
-nothing but the GPU parity against this restatement.
+nothing but the GPU parity against this restatement. Seascape is pinned by its calm-sea
+closed form (the wave sum is not), dragon by brute force over its 2^n curve segments, and
+romanesco is parity unpinned.
 """
 import numpy as np
 import pytest
@@ -499,3 +501,64 @@ def test_tree_trunk_is_a_round_cone():
     bf = brute(local, surf) * 0.6
     bf[inside] = -bf[inside]
     assert np.abs(de("tree_ifs", p, pts) - bf).max() < 3e-3
+
+
+def test_seascape_calm_sea_is_its_slab():
+    """Height 0: no waves, the DE is the box [-Extent, Extent] x [-Depth, 0] x [-Extent,
+    Extent] (in units of Size)."""
+    p = f32(ft.defaults("seascape_primitive"))
+    p[2] = 0.0
+    size, ext, depth = p[1], p[6], p[7]
+    pts = pts_box(300, 3.0)
+    q = pts / size
+    e = np.stack([np.abs(q[:, 0]) - ext, np.maximum(-depth - q[:, 1], q[:, 1]), np.abs(q[:, 2]) - ext], 1)
+    box = (np.linalg.norm(np.maximum(e, 0), axis=1) + np.minimum(e.max(1), 0)) * size
+    assert close(de("seascape_primitive", p, pts), box, rtol=1e-9, atol=1e-12)
+
+
+def rot_z(a):
+    return np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1.0]])
+
+
+def rot_x(a):
+    return np.array([[1.0, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+
+
+@pytest.mark.parametrize("levy", [0.0, 1.0])
+def test_dragon_is_the_distance_to_its_curve(levy):
+    """The depth-first descent finds the nearest of the 2^n segments of the curve: the images
+    of [0, 1] x (0, 0) under every n-fold composition of f_a(w) = s qa w and
+    f_b(w) = s qb w + ob, in the frame where the world point is z / 2 + centre."""
+    n, fold, ta, tb, thick = 6, 41.0, 70.0, 110.0, 0.2
+    p = f32([n, fold, ta, tb, levy, thick])
+    th = np.radians(p[1])
+    s = 0.5 / np.cos(th)
+    qa = rot_z(th) @ rot_x(np.radians(p[2]))
+    qb = rot_z(-th if levy else np.pi - th) @ rot_x(np.radians(p[3]))
+    ob = np.array([0.5, 0.5 * np.tan(th), 0.0]) if levy else np.array([1.0, 0.0, 0.0])
+    fa = lambda w: s * (qa @ w)
+    fb = lambda w: s * (qb @ w) + ob
+    segs = []
+    for word in range(2 ** n):
+        a, b = np.zeros(3), np.array([1.0, 0, 0])
+        for k in range(n - 1, -1, -1):
+            f = fb if (word >> k) & 1 else fa
+            a, b = f(a), f(b)
+        segs.append((a, b))
+    A = np.array([x[0] for x in segs]); B = np.array([x[1] for x in segs])
+    # the centre: whichever of the fixed points of the mean map and of f_b gives the tighter
+    # bounding sphere
+    I = np.eye(3)
+    cen = np.linalg.solve(I - (qa + qb) * (0.5 * s), 0.5 * ob)
+    mid = 0.5 * np.linalg.solve(I - qb * s, ob)
+    rad = lambda c: max(np.linalg.norm(s * (qa @ c) - c), np.linalg.norm(ob + s * (qb @ c) - c)) / (1 - s)
+    c = mid if rad(mid) < rad(cen) else cen
+    pts = RNG.uniform(-1.2, 1.2, (60, 3))
+    local = pts / 2 + c
+    ab = B - A
+    want = np.empty(len(pts))
+    for i, q in enumerate(local):
+        t = np.clip(((q - A) * ab).sum(1) / (ab * ab).sum(1), 0, 1)
+        want[i] = np.linalg.norm(A + ab * t[:, None] - q, axis=1).min()
+    want = 2 * (want - thick * s ** n)
+    assert close(de("dragon_ifs_twist", p, pts), want, rtol=1e-7, atol=1e-9)
