@@ -286,7 +286,7 @@ def main():
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": K_MARCH_TRAFFIC_BYTES_16CAM if len(jobs) == 16 else None,
                      "frac_unpacked_peak": round(achieved / FP32_PEAK_UNPACKED_TFLOPS, 4),
-                     "kernel": "k_march<true, false> (persistent march pass, shared scene, no accelerator)",
+                     "kernel": "k_march<true, false, false> (persistent march pass: shared scene, no accelerator, tuned formulas)",
                      "flops_per_de": round(f_march, 2), "de_evals_per_launch": pers["de_march"] // steps,
                      "ms_per_launch": round(pers["ms_march"] / steps, 3),
                      "lane_occupancy": round(pers["de_march"] / (64.0 * pers["rounds"]), 4) if pers["rounds"] else None},

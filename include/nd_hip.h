@@ -235,7 +235,8 @@ typedef struct nd_tuning {
                                   0 = each kernel's resident waves per CU, fewer (at least 4) when the
                                   batch has under two claims per wave */
     uint32_t refill_min_idle;  /* a wave pulls new rays when at least this many lanes are idle (default 8) */
-    uint32_t chunk_rays;       /* rays a wave claims from the global queue per atomic (default 256) */
+    uint32_t chunk_rays;       /* rays a wave claims from the march queue per atomic; shadow claims are
+                                  4x; 0 (default) = 96 when every job has the same scene, else 256 */
     uint32_t flags;            /* ND_TUNE_* */
 } nd_tuning;
 

@@ -251,7 +251,7 @@ class Renderer:
         self._check(lib.nd_get_tuning(self._h, C.byref(t)), "nd_get_tuning")
         return t
 
-    def set_tuning(self, waves_per_cu=0, refill_min_idle=8, chunk_rays=256, track_latency=False, graphs=True):
+    def set_tuning(self, waves_per_cu=0, refill_min_idle=8, chunk_rays=0, track_latency=False, graphs=True):
         flags = (abi.TUNE_TRACK_LATENCY if track_latency else 0) | (0 if graphs else abi.TUNE_NO_GRAPH)
         t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, flags)
         self._check(lib.nd_set_tuning(self._h, C.byref(t)), "nd_set_tuning")
