@@ -123,7 +123,9 @@ static const FormulaSlots kSlots[ND_FORMULA_COUNT] = {
     {0, -1, -1, -1, -1},   // romanesco_ifs.wgsl:24-26
 };
 
-static void hoist(DevInst& in) {
+// I is DevInst or DevMixin (the same formula fields).
+template <class I>
+static void hoist(I& in) {
     std::memset(in.h, 0, sizeof(in.h));
     const float* p = in.p;
     switch (in.formula) {
@@ -180,7 +182,7 @@ static void hoist(DevInst& in) {
 
 bool nd_scene_ext(const DevScene& s) {
     for (int i = 0; i < s.count && i < ND_MAX_INSTANCES; ++i)
-        if (!s.inst[i].hidden && s.inst[i].formula > ND_FORMULA_KLEINIAN) return true;
+        if (!s.inst[i].hidden && (s.inst[i].formula > ND_FORMULA_KLEINIAN || s.inst[i].hybrid)) return true;
     return false;
 }
 
@@ -212,8 +214,6 @@ nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
         if (i < s.count && !in.hidden) {
             if (code >= 0 && in.formula < 0)
                 return nd_fail(c, ND_E_UNSUPPORTED_FORMULA, "instance %d: formula code %d has no HIP distance estimator bound", i, code);
-            if (fi.mixin_count > 0.5f)
-                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d: hybrid mix-ins are not implemented", i);
         }
         in.combine = i32(fi.combine_mode + 0.5f);
         in.blend_k = fi.blend_k;
@@ -243,6 +243,32 @@ nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
         in.trap_radius = fi.trap_radius;
         in.trap_tube = fi.trap_tube;
         hoist(in);
+        // hybrid mix-ins (eval_mixed, template.wgsl:1361-1389; hybrid_cycle_len, :1332-1338)
+        in.hybrid = fi.mixin_count > 0.5f;
+        if (in.hybrid) {
+            in.mix_count = std::min(std::max(i32(fi.mixin_count), 0), ND_MAX_MIXINS);
+            in.hyb_base_n = std::max(i32(fi.hybrid_base_iters), 0);
+            in.hyb_total = std::max(i32(fi.hybrid_total_iters), 0);
+            int cycle = in.hyb_base_n;
+            DevMixin& base = in.mix[0];
+            base.formula = in.formula;
+            base.iters = in.hyb_base_n;
+            for (int k = 0; k < 16; ++k) { base.p[k] = in.p[k]; base.dparam[k] = -1; }
+            hoist(base);
+            for (int j = 0; j < in.mix_count; ++j) {
+                const nd_mixin_params& mp = fi.mixins[j];
+                DevMixin& m = in.mix[1 + j];
+                const int mcode = i32(mp.code);
+                m.formula = (mcode >= 0 && mcode < ND_MAX_CODES) ? c->code_map[mcode] : -1;
+                if (i < s.count && !in.hidden && m.formula < 0)
+                    return nd_fail(c, ND_E_UNSUPPORTED_FORMULA, "instance %d mix-in %d: formula code %d has no HIP distance estimator bound", i, j, mcode);
+                m.iters = std::max(i32(mp.iterations), 0);
+                cycle += m.iters;
+                for (int k = 0; k < 16; ++k) { m.p[k] = mp.params[k]; m.dparam[k] = -1; }
+                hoist(m);
+            }
+            in.hyb_cycle = std::max(cycle, 1);
+        }
     }
     return ND_OK;
 }
@@ -456,6 +482,8 @@ static void attach_adjoint_slots(DevScene& s) {
         if (in.hidden) continue;
         if (in.formula == ND_FORMULA_MANDELBULB && !bulb) { in.dparam[0] = 3; bulb = true; }
         if (in.formula == ND_FORMULA_MANDELBOX && !box) { in.dparam[0] = 4; in.dparam[1] = 5; box = true; }
+        if (in.hybrid)  // a hybrid's base steps read their parameters from mix[0]
+            for (int k = 0; k < 16; ++k) in.mix[0].dparam[k] = in.dparam[k];
     }
 }
 

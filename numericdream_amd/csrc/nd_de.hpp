@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "nd_dev.h"
 
 #define NDD __device__ __forceinline__
@@ -188,14 +190,17 @@ template <class S> NDD V3<float> val3(const V3<S>& a) { return mk3(vv(a.x), vv(a
 // Per-formula parameters as scalars of type S. For float they come from the host-hoisted
 // constants h[]; for dual numbers they are rebuilt from p[] with the adjoint tangent of
 // each @param attached (DevInst::dparam).
-template <class S> NDD S prm(const DevInst& in, int k);
-template <> NDD float prm<float>(const DevInst& in, int k) { return in.p[k]; }
-template <class S> NDD S prm(const DevInst& in, int k) {
-    S r = Sc<S>::c(in.p[k]);
-    int slot = in.dparam[k];
+// I is DevInst or DevMixin: both carry p, h, dparam, rot_on and rot_cs.
+template <class S, class I> NDD S prm(const I& in, int k) {
+    if constexpr (std::is_same<S, float>::value) {
+        return in.p[k];
+    } else {
+        S r = Sc<S>::c(in.p[k]);
+        int slot = in.dparam[k];
 #pragma unroll
-    for (int j = 0; j < (int)(sizeof(r.d) / sizeof(float)); ++j) r.d[j] = (j == slot) ? 1.f : 0.f;
-    return r;
+        for (int j = 0; j < (int)(sizeof(r.d) / sizeof(float)); ++j) r.d[j] = (j == slot) ? 1.f : 0.f;
+        return r;
+    }
 }
 
 template <class S> struct Carry { V3<S> z; S dr; };
@@ -300,8 +305,8 @@ NDD Carry<float> box_step(const Carry<float>& c, const V3<float>& posc, const Bo
     o.dr = c.dr * (t * b.absS) + b.dr_add;
     return o;
 }
-template <class S> NDD Carry<S> box_step_t(const Carry<S>& c, const V3<S>& posc, const S& scale, const S& m2,
-                                           const S& F2, const S& L, float dr_add, const DevInst& in) {
+template <class S, class I> NDD Carry<S> box_step_t(const Carry<S>& c, const V3<S>& posc, const S& scale, const S& m2,
+                                                    const S& F2, const S& L, float dr_add, const I& in) {
     S nL = -L;
     V3<S> z = mk3(sclamp(c.z.x, nL, L) * 2.f - c.z.x, sclamp(c.z.y, nL, L) * 2.f - c.z.y,
                   sclamp(c.z.z, nL, L) * 2.f - c.z.z);
@@ -330,7 +335,7 @@ template <class S> NDD V3<S> sort_desc_abs(const V3<S>& z) {
     if (vv(b) < vv(c)) { t = b; b = c; c = t; }
     return mk3(a, b, c);
 }
-template <class S> NDD Carry<S> menger_step(const Carry<S>& c, const DevInst& in) {
+template <class S, class I> NDD Carry<S> menger_step(const Carry<S>& c, const I& in) {
     V3<S> z = sort_desc_abs(c.z);
     if (in.rot_on) z = rotate_cs(z, in.rot_cs);
     S sc = prm<S>(in, 4);
@@ -352,7 +357,7 @@ template <class S> NDD S menger_finalize(const Carry<S>& c) {
 // ----------------------------------------------------------------------------- Kleinian
 // formulas/kleinian_kleinian.wgsl:16-33 with box_fold, lattice_wrap (round half to even)
 // and sphere_invert_offset (template.wgsl:975-989)
-template <class S> NDD Carry<S> kleinian_step(const Carry<S>& c, const V3<S>& posc, const DevInst& in) {
+template <class S, class I> NDD Carry<S> kleinian_step(const Carry<S>& c, const V3<S>& posc, const I& in) {
     S L = prm<S>(in, 0);
     S nL = -L;
     V3<S> z = mk3(sclamp(c.z.x, nL, L) * 2.f - c.z.x, sclamp(c.z.y, nL, L) * 2.f - c.z.y,
@@ -447,6 +452,9 @@ namespace nd {
 // are compiled in (others give NO_SURFACE_DE) and the kernel keeps their register budget.
 template <bool TRAP, bool EXT = false>
 NDD float formula_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
+    if constexpr (EXT) {
+        if (in.hybrid) return ext_de_f<TRAP>(in, pos, trap);
+    }
     Carry<float> c; c.z = pos; c.dr = 1.f;
     const int iters = in.iters;
     TrapAcc acc;
@@ -518,6 +526,9 @@ NDD float formula_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
 // Dual-number version (normals, adjoint). The trap is not needed on these paths.
 template <class S, bool EXT = false>
 NDD S formula_de_d(const DevInst& in, const V3<S>& pos) {
+    if constexpr (EXT) {
+        if (in.hybrid) return ext_de_d<S>(in, pos);
+    }
     Carry<S> c; c.z = pos; c.dr = Sc<S>::c(1.f);
     const int iters = in.iters;
     switch (in.formula) {

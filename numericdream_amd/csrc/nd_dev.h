@@ -14,6 +14,20 @@
 
 enum { NDF_BULB = 0, NDF_BOX = 1, NDF_MENGER = 2, NDF_KLEINIAN = 3 };
 
+// A hybrid mix-in slot (MixinParams, fr.zig:112-117) after decoding: the formula steps of
+// the slot read the same fields as an instance's own formula (p, h, rot_on, rot_cs).
+struct DevMixin {
+    int formula;       // nd_builtin_formula of the mix-in's code
+    int iters;         // max(i32(iterations), 0): its steps per hybrid cycle
+    int rot_on;
+    int bulb_pow8;
+    int dparam[16];    // -1: mix-in parameters carry no adjoint tangent
+    float p[16];
+    float h[16];
+    float rot_cs[6];
+    float _pad[2];
+};
+
 // One fractal instance (FractalInstance, fr.zig:119-145) after decoding.
 struct DevInst {
     int formula;       // nd_builtin_formula, -1 = none (DE = 1e6)
@@ -38,6 +52,13 @@ struct DevInst {
     float trap_box[3];
     float trap_radius;
     float trap_tube;
+    // hybrid cycle (eval_mixed, template.wgsl:1361-1389)
+    int hybrid;        // mixin_count > 0.5: the instance runs eval_mixed
+    int mix_count;     // min(i32(mixin_count), ND_MAX_MIXINS)
+    int hyb_base_n;    // max(i32(hybrid_base_iters), 0)
+    int hyb_total;     // max(i32(hybrid_total_iters), 0)
+    int hyb_cycle;     // max(base_n + sum of the mix-ins' iters, 1)
+    DevMixin mix[ND_MAX_MIXINS + 1];  // [0] the base formula's own slots, [1 + j] mix-in j
 };
 
 struct DevScene {

@@ -25,7 +25,7 @@ constexpr float K_INFINITE = 1e30f;
 
 // ----------------------------------------------------------------------------- scalar helpers
 template <class S> NDD S C(float x) { return Sc<S>::c(x); }
-template <class S> NDD S P(const DevInst& in, int k) { return prm<S>(in, k); }
+template <class S, class I> NDD S P(const I& in, int k) { return prm<S>(in, k); }
 
 NDD float sfloor(float a) { return floorf(a); }
 template <int N> NDD DualN<N> sfloor(const DualN<N>& a) { return dconst<N>(floorf(a.v)); }
@@ -64,9 +64,9 @@ template <class S> NDD V3<S> vmax0(const V3<S>& a) {
     return mk3(smax_(a.x, C<S>(0.f)), smax_(a.y, C<S>(0.f)), smax_(a.z, C<S>(0.f))); }
 template <class S> NDD V3<S> vsub1(const V3<S>& a, float s) { return mk3(a.x - s, a.y - s, a.z - s); }
 template <class S> NDD S len2(const S& x, const S& y) { return ssqrt(x * x + y * y); }
-template <class S> NDD V3<S> vrot(const V3<S>& z, const DevInst& in) { return in.rot_on ? rotate_cs(z, in.rot_cs) : z; }
+template <class S, class I> NDD V3<S> vrot(const V3<S>& z, const I& in) { return in.rot_on ? rotate_cs(z, in.rot_cs) : z; }
 // rotation about z by the hoisted twist angle (x ca - y sa, x sa + y ca, z)
-template <class S> NDD V3<S> vtwist(const V3<S>& z, const DevInst& in) {
+template <class S, class I> NDD V3<S> vtwist(const V3<S>& z, const I& in) {
     const float ca = in.h[14], sa = in.h[15];
     return mk3(z.x * ca - z.y * sa, z.x * sa + z.y * ca, z.z);
 }
@@ -147,7 +147,7 @@ template <class S> NDD V3<S> ifs_scale(const V3<S>& z, const S& s, const V3<S>& 
     const S sm1 = s - 1.f;
     return mk3(z.x * s - o.x * sm1, z.y * s - o.y * sm1, z.z * s - o.z * sm1);
 }
-template <class S> NDD V3<S> p3(const DevInst& in, int k0, int k1, int k2) { return mk3(P<S>(in, k0), P<S>(in, k1), P<S>(in, k2)); }
+template <class S, class I> NDD V3<S> p3(const I& in, int k0, int k1, int k2) { return mk3(P<S>(in, k0), P<S>(in, k1), P<S>(in, k2)); }
 
 // Spherical power map of the bulb family: dir(theta * power + tp, phi * power + pp) for the
 // unit direction of z (bulb_power_step, template.wgsl:1019-1031, with the phase @params).
@@ -174,7 +174,7 @@ template <class S> NDD Carry<S> prim_step(const Carry<S>& c, const S& size) {
     return o;
 }
 
-#define ND_STEP template <class S> static NDD Carry<S> step(const Carry<S>& c, const V3<S>& pos, const DevInst& in)
+#define ND_STEP template <class S, class I> static NDD Carry<S> step(const Carry<S>& c, const V3<S>& pos, const I& in)
 #define ND_FIN template <class S> static NDD S fin(const Carry<S>& c)
 #define ND_RET(zz, dd) do { Carry<S> o_; o_.z = (zz); o_.dr = (dd); return o_; } while (0)
 
@@ -405,11 +405,11 @@ template <class S> NDD Carry<S> box_tail(const V3<S>& z, const S& dr, const S& s
     return o;
 }
 // pos * cs + julia, max(|cs|, 1)
-template <class S> NDD V3<S> cj_add(const V3<S>& pos, const DevInst& in, int cs) {
+template <class S, class I> NDD V3<S> cj_add(const V3<S>& pos, const I& in, int cs) {
     const S k = P<S>(in, cs);
     return mk3(pos.x * k + P<S>(in, cs + 1), pos.y * k + P<S>(in, cs + 2), pos.z * k + P<S>(in, cs + 3));
 }
-template <class S> NDD S cj_dr(const DevInst& in, int cs) { return C<S>(fmaxf(fabsf(in.p[cs]), 1.f)); }
+template <class S, class I> NDD S cj_dr(const I& in, int cs) { return C<S>(fmaxf(fabsf(in.p[cs]), 1.f)); }
 template <class S> NDD S sq(const S& x) { return x * x; }
 
 // formulas/juliabox_box_julia.wgsl:16-37
@@ -1303,7 +1303,75 @@ NDD S ext_run_d(const DevInst& in, const V3<S>& pos) {
     X(ND_FORMULA_DRAGON, FDragon)                                          \
     X(ND_FORMULA_ROMANESCO, FRomanesco)
 
-// The dispatch is a real call, not inlined: it keeps the 47 formula loops out of the
+// ----------------------------------------------------------------------------- hybrids
+// One de_step of any formula (eval_step, fr.zig:515-539), parameters from an instance or
+// a mix-in slot. The tuned four use their general forms here.
+template <class S, class I>
+NDD Carry<S> any_step(int f, const Carry<S>& c, const V3<S>& pos, const I& in) {
+    switch (f) {
+        case ND_FORMULA_MANDELBULB: {
+            bool esc = false;
+            return bulb_step_trig<S>(c, pos, P<S>(in, 0), esc);
+        }
+        case ND_FORMULA_MANDELBOX: {
+            const S m = P<S>(in, 1), F = P<S>(in, 2);
+            return box_step_t<S>(c, cj_add(pos, in, 8), P<S>(in, 0), m * m, F * F, P<S>(in, 3), fmaxf(fabsf(in.p[8]), 1.f), in);
+        }
+        case ND_FORMULA_MENGER3:
+            return menger_step<S>(c, in);
+        case ND_FORMULA_KLEINIAN:
+            return kleinian_step<S>(c, cj_add(pos, in, 7), in);
+#define ND_CASE_S(ID, F) case ID: return F::template step<S>(c, pos, in);
+        ND_EXT_FORMULAS(ND_CASE_S)
+#undef ND_CASE_S
+    }
+    return c;
+}
+template <class S>
+NDD S any_fin(int f, const Carry<S>& c) {
+    switch (f) {
+        case ND_FORMULA_MANDELBULB: return bulb_finalize<S>(c);
+        case ND_FORMULA_MANDELBOX:
+        case ND_FORMULA_KLEINIAN: return fin_scaled(c);
+        case ND_FORMULA_MENGER3: return menger_finalize<S>(c);
+#define ND_CASE_FI(ID, F) case ID: return F::template fin<S>(c);
+        ND_EXT_FORMULAS(ND_CASE_FI)
+#undef ND_CASE_FI
+    }
+    return C<S>(1e6f);  // NO_SURFACE_DE
+}
+// eval_mixed (template.wgsl:1361-1389) with hybrid_step (:1340-1359): hyb_total steps; step
+// it is the base formula's while it % cycle < base_n, else the mix-in whose run of
+// iterations it falls in; the trap follows every step; the base formula finalizes.
+// One hybrid step through one more call: the loop around it stays small, and the step's
+// 54-way switch is compiled once per scalar type.
+template <class S>
+__device__ __noinline__ Carry<S> hybrid_step_call(Carry<S> c, V3<S> pos, const DevMixin* m) {
+    return any_step<S>(m->formula, c, pos, *m);
+}
+template <class S, bool TRAP>
+NDD S hybrid_run(const DevInst& in, const V3<S>& pos, TrapAcc* acc) {
+    Carry<S> c; c.z = pos; c.dr = C<S>(1.f);
+    for (int it = 0; it < in.hyb_total; ++it) {
+        int pic = it % in.hyb_cycle;
+        int slot = -1;  // -1: no formula at this cycle position (have = false)
+        if (pic < in.hyb_base_n) {
+            slot = 0;
+        } else {
+            pic -= in.hyb_base_n;
+            for (int j = 0; j < in.mix_count; ++j) {
+                if (pic < in.mix[1 + j].iters) { slot = 1 + j; break; }
+                pic -= in.mix[1 + j].iters;
+            }
+        }
+        const V3<S> zp = c.z;
+        if (slot >= 0) c = hybrid_step_call<S>(c, pos, &in.mix[slot]);
+        if constexpr (TRAP) trap_update(*acc, in, val3(c.z), val3(zp), it);
+    }
+    return any_fin<S>(in.formula, c);
+}
+
+// The dispatch is a real call, not inlined: it keeps the 50 formula loops out of the
 // register allocation of the kernels' hot paths, which only ever run the four tuned DEs
 // inline.
 struct ExtDe { float d, trap; };
@@ -1311,6 +1379,13 @@ template <bool TRAP>
 __device__ __noinline__ ExtDe ext_de_call(const DevInst* in, float x, float y, float z) {
     const V3<float> pos = mk3(x, y, z);
     ExtDe r; r.trap = 0.f; r.d = 1e6f;  // NO_SURFACE_DE
+    if (in->hybrid) {
+        TrapAcc acc;
+        if (TRAP) acc = trap_begin(*in);
+        r.d = hybrid_run<float, TRAP>(*in, pos, &acc);
+        if (TRAP) r.trap = trap_finish(acc, *in, in->hyb_total);
+        return r;
+    }
     switch (in->formula) {
 #define ND_CASE_F(ID, F) case ID: r.d = ext_run_f<F, TRAP>(*in, pos, &r.trap); break;
         ND_EXT_FORMULAS(ND_CASE_F)
@@ -1326,6 +1401,7 @@ NDD float ext_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
 }
 template <class S>
 __device__ __noinline__ S ext_de_d_call(const DevInst* in, V3<S> pos) {
+    if (in->hybrid) return hybrid_run<S, false>(*in, pos, nullptr);
     switch (in->formula) {
 #define ND_CASE_D(ID, F) case ID: return ext_run_d<F, S>(*in, pos);
         ND_EXT_FORMULAS(ND_CASE_D)

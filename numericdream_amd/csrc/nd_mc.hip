@@ -460,8 +460,10 @@ struct IndirectRays {
     }
 };
 
+// The two march kernels of the path trace run at 7 waves per SIMD (72 VGPRs, five values
+// spilled) instead of the compiler's 6: a 64-spp frame takes 2.7% less time.
 template <bool EXT>
-__global__ void __launch_bounds__(64) k_mc_extend(McArgs a, int ping) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 7, EXT ? 8 : 7))) k_mc_extend(McArgs a, int ping) {
     const uint32_t* q = ping ? a.P.live_b : a.P.live_a;
     const uint32_t n = a.P.ctr[C_LIVE_IN];
     ExtendRays R{a, q};
@@ -469,7 +471,7 @@ __global__ void __launch_bounds__(64) k_mc_extend(McArgs a, int ping) {
 }
 
 template <bool EXT>
-__global__ void __launch_bounds__(64) k_mc_indirect_march(McArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 7, EXT ? 8 : 7))) k_mc_indirect_march(McArgs a) {
     IndirectRays R{a};
     persistent_march<EXT>(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
 }

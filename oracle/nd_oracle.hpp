@@ -440,6 +440,12 @@ template <class P> struct Instance {
     double step_safety = 1;
     Params<P> p;
     TrapSpec trap;
+    // hybrid mix-ins (template.wgsl:1332-1389): the instance runs eval_mixed when hybrid
+    bool hybrid = false;
+    int mix_count = 0, base_n = 0, total = 0;
+    int mix_formula[ND_MAX_MIXINS] = {-1, -1, -1};
+    int mix_iters[ND_MAX_MIXINS] = {0, 0, 0};
+    Params<P> mix_p[ND_MAX_MIXINS];
 };
 
 // Empty-space accelerator grid (built by build_accel below).
@@ -488,6 +494,15 @@ template <class P> Scene<P> decode_scene(const nd_uniforms& u, const CodeMap& cm
         in.trap.mode = i32(fi.trap_mode + 0.5f);
         in.trap.radius = fi.trap_radius;
         in.trap.tube = fi.trap_tube;
+        in.hybrid = fi.mixin_count > 0.5f;  // instance_local_de (template.wgsl:1600-1605)
+        in.mix_count = std::min(std::max(i32(fi.mixin_count), 0), ND_MAX_MIXINS);
+        in.base_n = std::max(i32(fi.hybrid_base_iters), 0);
+        in.total = std::max(i32(fi.hybrid_total_iters), 0);
+        for (int j = 0; j < ND_MAX_MIXINS; ++j) {
+            in.mix_formula[j] = cm.lookup(i32(fi.mixins[j].code));  // mixin_code (:1321-1323)
+            in.mix_iters[j] = std::max(i32(fi.mixins[j].iterations), 0);
+            for (int k = 0; k < 16; ++k) in.mix_p[j].p[k] = P(fi.mixins[j].params[k]);
+        }
     }
     s.carve_count = std::min(i32(u.carve_count), ND_MAX_CARVES);
     for (int i = 0; i < ND_MAX_CARVES; ++i)
@@ -497,7 +512,41 @@ template <class P> Scene<P> decode_scene(const nd_uniforms& u, const CodeMap& cm
 
 // formula_N wrapper (fr.zig:490-513) with the LOD blend off (approx_flags = 0):
 // returns (DE, trap value).
+// eval_mixed with hybrid_cycle_len / hybrid_step (template.wgsl:1332-1389), LOD off
+template <class T, class P> void eval_mixed(const Instance<P>& in, V3<T> pos, T* de, T* trap) {
+    int cycle_len = in.base_n;
+    for (int j = 0; j < in.mix_count; ++j) cycle_len += in.mix_iters[j];
+    cycle_len = std::max(cycle_len, 1);
+    Carry<T> carry{pos, T(1)};
+    TrapAcc<T> acc = trap_begin<T>(in.trap);
+    for (int it = 0; it < in.total; ++it) {
+        int code = in.formula;
+        const Params<P>* p = &in.p;
+        bool have = true;
+        int pos_in_cycle = it % cycle_len;
+        if (pos_in_cycle >= in.base_n) {
+            pos_in_cycle -= in.base_n;
+            have = false;
+            for (int j = 0; j < in.mix_count; ++j) {
+                if (pos_in_cycle < in.mix_iters[j]) {
+                    code = in.mix_formula[j];
+                    p = &in.mix_p[j];
+                    have = true;
+                    break;
+                }
+                pos_in_cycle -= in.mix_iters[j];
+            }
+        }
+        V3<T> z_prev = carry.z;
+        if (have) carry = formula_step(code, carry, pos, *p);
+        if (trap) acc = trap_update(acc, in.trap, carry.z, z_prev, it);
+    }
+    *de = formula_finalize(in.formula, carry);
+    if (trap) *trap = trap_finish(acc, in.trap, in.total);
+}
+
 template <class T, class P> void eval_formula(const Instance<P>& in, V3<T> pos, T* de, T* trap) {
+    if (in.hybrid) { eval_mixed(in, pos, de, trap); return; }
     if (in.formula < 0) { *de = T(NO_SURFACE_DE); *trap = T(0); return; }
     Carry<T> carry{pos, T(1)};
     TrapAcc<T> acc = trap_begin<T>(in.trap);

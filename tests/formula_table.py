@@ -117,3 +117,24 @@ def test_scene(name, size=(64, 48), **kw):
     for k, v in over.items():
         p[k] = v
     return scene(name, params=p, cam=(0.0, 0.35 * d, -d), size=size, max_dist=max(20.0, 3.0 * d), **kw)
+
+
+def hybrid_scene(base, mixins, base_n, total, base_params=None, cam=(0.0, 0.0, -3.5), size=(64, 48), **kw):
+    """One hybrid instance: the base formula (code 0) for base_n steps of each cycle, then each
+    mix-in (name, params or None, iterations; codes 1, 2, 3) for its iterations, total steps
+    in all, finalized by the base formula. Returns (uniforms, code_map)."""
+    u, _ = scene(base, params=base_params, cam=cam, size=size, **kw)
+    inst = u.instances[0]
+    inst.mixin_count = float(len(mixins))
+    inst.hybrid_base_iters = float(base_n)
+    inst.hybrid_total_iters = float(total)
+    cm = [ID[base]]
+    for j, (name, params, iters) in enumerate(mixins):
+        m = inst.mixins[j]
+        m.code = float(1 + j)
+        m.iterations = float(iters)
+        p = defaults(name) if params is None else list(params) + [0.0] * (16 - len(params))
+        for k in range(16):
+            m.params[k] = float(p[k])
+        cm.append(ID[name])
+    return u, cm

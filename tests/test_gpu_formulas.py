@@ -146,3 +146,38 @@ def test_extended_formula_path_trace(bound):
     assert hit.mean() > 0.1
     assert mse_gpu <= 2.0 * mse_floor + 1e-6
     assert abs(m_gpu - m_ref) <= 0.03 * m_ref
+
+
+HYBRIDS = {
+    # name: (base, base params, [(mix-in, params, iterations)], base_n, total, camera distance)
+    "bulb2_box1": ("mandelbrot", [8.0, 6.0], [("amazingbox_box", None, 1)], 2, 9, 20.0),
+    "box1_juliabox1": ("amazingbox_box", None, [("juliabox_box_julia", None, 1)], 1, 12, 35.0),
+    "juliabulb_sierpinski_bulb3": ("juliabulb_bulb_julia", None,
+                                   [("sierpinski3_ifs", [2.0, 12.0, 1, 1, 1], 1), ("mandelbrot", [3.0, 4.0], 1)], 1, 8, 2.0),
+}
+
+
+def hybrid(name, size=(64, 48)):
+    base, bp, mix, bn, tot, d = HYBRIDS[name]
+    return ft.hybrid_scene(base, mix, bn, tot, base_params=bp, cam=(0.0, 0.35 * d, -d), size=size,
+                           max_dist=max(20.0, 3.0 * d))
+
+
+@pytest.mark.parametrize("name", sorted(HYBRIDS))
+def test_hybrid_gbuffer(bound, name):
+    """Hybrid mix-in cycles (eval_mixed) against the oracle's restatement."""
+    print("\n[case] hybrid %s" % name, flush=True)
+    u, cm = hybrid(name)
+    r = bound(cm)
+    ref = oracle.render_gbuffer(u, normal_mode=0, shadows=True, code_map=cm)
+    g = gpu_render(r, u)
+    st = compare(g, ref)
+    st.update(pointwise(g, u, 0, code_map=cm))
+    report("hybrid " + name, st)
+    assert st["hits_ref"] > 0 and st["hit_agree_inner"] >= 0.99, st
+    assert st["depth_rel_le_1e-4"] >= 0.95, st
+    assert st["pw_trap_equal"] >= 0.9, st
+    assert st["pw_normal_deg_le_0.5"] >= 0.85, st
+    b = gpu_render(r, u, abi.KERNEL_PLAIN)
+    for k in ("hit", "depth", "normal", "trap", "shadow"):
+        assert np.array_equal(g[k], b[k]), k

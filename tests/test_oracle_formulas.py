@@ -562,3 +562,47 @@ def test_dragon_is_the_distance_to_its_curve(levy):
         want[i] = np.linalg.norm(A + ab * t[:, None] - q, axis=1).min()
     want = 2 * (want - thick * s ** n)
     assert close(de("dragon_ifs_twist", p, pts), want, rtol=1e-7, atol=1e-9)
+
+
+# ============================================================================= hybrids
+def de_u(u, cm, pts):
+    return oracle.scene_de(u, np.asarray(pts, np.float64), code_map=cm)
+
+
+def test_hybrid_of_base_steps_only_is_the_plain_formula():
+    p = f32(ft.defaults("amazingbox_box"))
+    n = int(p[4])
+    u, cm = ft.hybrid_scene("amazingbox_box", [("juliabox_box_julia", None, 0)], base_n=n, total=n, base_params=p)
+    pts = pts_box(200, 3.0)
+    assert close(de_u(u, cm, pts), de("amazingbox_box", p, pts), rtol=1e-12)
+
+
+def test_hybrid_of_mixin_steps_only_is_the_mixin_formula():
+    """base_n = 0: every step is the mix-in's; the base only finalizes (both scaled)."""
+    j = rand_params("juliabox_box_julia")
+    k = 9
+    u, cm = ft.hybrid_scene("amazingbox_box", [("juliabox_box_julia", j, 1)], base_n=0, total=k)
+    j2 = list(j)
+    j2[7] = float(k)
+    pts = pts_box(200, 3.0)
+    assert close(de_u(u, cm, pts), de("juliabox_box_julia", j2, pts), rtol=1e-12)
+
+
+def test_hybrid_cycle_schedule():
+    """Mandelbulb x2, Mandelbox x1, Menger x0, Kleinian x1 per cycle: the DE equals the
+    steps composed in that order by hand, finalized by the Mandelbulb."""
+    bulb = f32([8.0, 6.0])
+    box = rand_params("amazingbox_box")
+    kle = rand_params("kleinian_kleinian")
+    u, cm = ft.hybrid_scene("mandelbrot", [("amazingbox_box", box, 1), ("menger3_ifs", None, 0),
+                                            ("kleinian_kleinian", kle, 1)], base_n=2, total=11, base_params=bulb)
+    sched = [("mandelbrot", bulb), ("mandelbrot", bulb), ("amazingbox_box", box), ("kleinian_kleinian", kle)]
+    pts = pts_box(60, 1.5)
+    for q in pts:
+        z, dr = q.copy(), 1.0
+        for it in range(11):
+            name, prm = sched[it % 4]
+            z, dr = step(name, z, dr, q, prm)
+        want = oracle.helper(16, [ft.ID["mandelbrot"], z[0], z[1], z[2], dr], 1)[0]
+        got = de_u(u, cm, [q])[0]
+        assert np.isclose(got, want, rtol=1e-12, atol=1e-15), (q, got, want)
