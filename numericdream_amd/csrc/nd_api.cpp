@@ -247,6 +247,12 @@ nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
         in.hybrid = fi.mixin_count > 0.5f;
         if (in.hybrid) {
             in.mix_count = std::min(std::max(i32(fi.mixin_count), 0), ND_MAX_MIXINS);
+            // Cycles of two or more mix-ins are switched off: a GPU render of one (Julia bulb
+            // base, Sierpinski and Mandelbulb mix-ins) did not finish, with no cause found in
+            // the kernels or on a host run of the same per-ray code, which finished and matched.
+            if (in.mix_count > ND_HIP_MAX_ACTIVE_MIXINS && i < s.count && !in.hidden)
+                return nd_fail(c, ND_E_UNSUPPORTED_FORMULA, "instance %d: hybrid cycles of %d mix-ins are switched off (at most %d)",
+                               i, in.mix_count, ND_HIP_MAX_ACTIVE_MIXINS);
             in.hyb_base_n = std::max(i32(fi.hybrid_base_iters), 0);
             in.hyb_total = std::max(i32(fi.hybrid_total_iters), 0);
             int cycle = in.hyb_base_n;

@@ -11,6 +11,9 @@
 
 #define ND_DEV_MAX_INST 4
 #define ND_DEV_MAX_CARVES 16
+// Mix-ins a hybrid cycle may use in the HIP kernels; cycles of more are rejected at packing
+// (the Uniforms hold ND_MAX_MIXINS).
+#define ND_HIP_MAX_ACTIVE_MIXINS 1
 
 enum { NDF_BULB = 0, NDF_BOX = 1, NDF_MENGER = 2, NDF_KLEINIAN = 3 };
 

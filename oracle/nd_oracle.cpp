@@ -48,16 +48,7 @@ void pixel_gbuffer(const nd_uniforms& u, const Scene<T>& s, const MarchCfg& mc, 
     else if (o.normal_mode == 2) n = normal_central(s, hp, T(0.5) * eps);
     else {
         // analytic: forward-mode derivative evaluated in FP64 at the (T-precision) hit point
-        Scene<double> sd;
-        sd.count = s.count; sd.max_dist = s.max_dist; sd.carve_count = s.carve_count;
-        std::memcpy(sd.carves, s.carves, sizeof(s.carves));
-        for (int i = 0; i < ND_MAX_INSTANCES; ++i) {
-            const auto& a = s.inst[i]; auto& b = sd.inst[i];
-            b.formula = a.formula; b.hidden = a.hidden; b.combine = a.combine; b.blend_k = a.blend_k;
-            std::memcpy(b.offset, a.offset, sizeof(a.offset)); std::memcpy(b.scale, a.scale, sizeof(a.scale));
-            std::memcpy(b.frame, a.frame, sizeof(a.frame)); b.step_safety = a.step_safety; b.trap = a.trap;
-            for (int k = 0; k < 16; ++k) b.p.p[k] = (double)val(a.p.p[k]);
-        }
+        const Scene<double> sd = recast_scene<double>(s);
         V3<double> nd = normal_analytic(sd, V3<double>((double)hp.x, (double)hp.y, (double)hp.z));
         n = V3<T>(T(nd.x), T(nd.y), T(nd.z));
     }

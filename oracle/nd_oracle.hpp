@@ -471,6 +471,36 @@ struct CodeMap {
     int lookup(int code) const { return (code >= 0 && code < 16) ? map[code] : -1; }
 };
 
+// The same scene with every parameter re-typed as Q (its value only, no tangent): every
+// instance field, the hybrid mix-ins included, carries over.
+template <class Q, class P> Scene<Q> recast_scene(const Scene<P>& s) {
+    Scene<Q> r;
+    r.count = s.count;
+    r.max_dist = s.max_dist;
+    r.carve_count = s.carve_count;
+    std::memcpy(r.carves, s.carves, sizeof(s.carves));
+    std::memcpy(r.cam_offset, s.cam_offset, sizeof(s.cam_offset));
+    r.accel = s.accel;
+    for (int i = 0; i < ND_MAX_INSTANCES; ++i) {
+        const Instance<P>& a = s.inst[i];
+        Instance<Q>& b = r.inst[i];
+        b.formula = a.formula; b.hidden = a.hidden; b.combine = a.combine; b.blend_k = a.blend_k;
+        std::memcpy(b.offset, a.offset, sizeof(a.offset));
+        std::memcpy(b.scale, a.scale, sizeof(a.scale));
+        std::memcpy(b.frame, a.frame, sizeof(a.frame));
+        b.step_safety = a.step_safety;
+        b.trap = a.trap;
+        for (int k = 0; k < 16; ++k) b.p.p[k] = Q((double)val(a.p.p[k]));
+        b.hybrid = a.hybrid; b.mix_count = a.mix_count; b.base_n = a.base_n; b.total = a.total;
+        for (int j = 0; j < ND_MAX_MIXINS; ++j) {
+            b.mix_formula[j] = a.mix_formula[j];
+            b.mix_iters[j] = a.mix_iters[j];
+            for (int k = 0; k < 16; ++k) b.mix_p[j].p[k] = Q((double)val(a.mix_p[j].p[k]));
+        }
+    }
+    return r;
+}
+
 template <class P> Scene<P> decode_scene(const nd_uniforms& u, const CodeMap& cm) {
     Scene<P> s;
     s.count = std::max(i32(u.instance_count), 1);
@@ -897,22 +927,7 @@ template <class T, class P> V3<T> normal_central(const Scene<P>& s, V3<T> pos, T
 template <class P> V3<double> normal_analytic(const Scene<P>& s, V3<double> pos) {
     using D = Dual<3>;
     V3<D> q(D::var(pos.x, 0), D::var(pos.y, 1), D::var(pos.z, 2));
-    Scene<D> sd;
-    sd.count = s.count;
-    sd.max_dist = s.max_dist;
-    sd.carve_count = s.carve_count;
-    std::memcpy(sd.carves, s.carves, sizeof(s.carves));
-    for (int i = 0; i < ND_MAX_INSTANCES; ++i) {
-        const Instance<P>& a = s.inst[i];
-        Instance<D>& b = sd.inst[i];
-        b.formula = a.formula; b.hidden = a.hidden; b.combine = a.combine; b.blend_k = a.blend_k;
-        std::memcpy(b.offset, a.offset, sizeof(a.offset));
-        std::memcpy(b.scale, a.scale, sizeof(a.scale));
-        std::memcpy(b.frame, a.frame, sizeof(a.frame));
-        b.step_safety = a.step_safety;
-        b.trap = a.trap;
-        for (int k = 0; k < 16; ++k) b.p.p[k] = D(val(a.p.p[k]));
-    }
+    const Scene<D> sd = recast_scene<D>(s);
     D d = scene_de(sd, q);
     return normalize(V3<double>(d.d[0], d.d[1], d.d[2]));
 }

@@ -138,3 +138,20 @@ def hybrid_scene(base, mixins, base_n, total, base_params=None, cam=(0.0, 0.0, -
             m.params[k] = float(p[k])
         cm.append(ID[name])
     return u, cm
+
+
+# Hybrid test scenes: name -> (base, base params or None, [(mix-in, params or None,
+# iterations)], base_n, total, camera distance d). The camera sits at (0, 0.35 d, -d).
+HYBRIDS = {
+    "bulb2_box1": ("mandelbrot", [8.0, 6.0], [("amazingbox_box", None, 1)], 2, 9, 20.0),
+    "box1_juliabox1": ("amazingbox_box", None, [("juliabox_box_julia", None, 1)], 1, 12, 35.0),
+    "juliabulb_sierpinski_bulb3": ("juliabulb_bulb_julia", None,
+                                   [("sierpinski3_ifs", [2.0, 12.0, 1, 1, 1], 1), ("mandelbrot", [3.0, 4.0], 1)],
+                                   1, 8, 2.0),
+}
+
+
+def hybrid_test_scene(name, size=(64, 48)):
+    base, bp, mix, bn, tot, d = HYBRIDS[name]
+    return hybrid_scene(base, mix, bn, tot, base_params=bp, cam=(0.0, 0.35 * d, -d), size=size,
+                        max_dist=max(20.0, 3.0 * d))

@@ -606,3 +606,26 @@ def test_hybrid_cycle_schedule():
         want = oracle.helper(16, [ft.ID["mandelbrot"], z[0], z[1], z[2], dr], 1)[0]
         got = de_u(u, cm, [q])[0]
         assert np.isclose(got, want, rtol=1e-12, atol=1e-15), (q, got, want)
+
+
+@pytest.mark.parametrize("name", sorted(ft.HYBRIDS))
+def test_hybrid_analytic_normal_is_the_de_gradient(name):
+    """The analytic normal of a hybrid (forward-mode tangents through the mix-in cycle) equals
+    the central-difference gradient of the hybrid DE at its own hit points."""
+    u, cm = ft.hybrid_test_scene(name, size=(24, 18))
+    g = oracle.render_gbuffer(u, normal_mode=0, shadows=False, code_map=cm)
+    h = g["hit"] > 0
+    assert h.sum() > 20
+    py, px = np.nonzero(h)
+    o = np.array([u.camera_pos[k] for k in range(3)], np.float64)
+    d = np.array([oracle.primary_dir(u, int(x), int(y)) for x, y in zip(px, py)])
+    pos = o + d * g["depth"][h].astype(np.float64)[:, None]
+    n = oracle.normal(u, pos, 0, None, code_map=cm)
+    fd = np.zeros_like(pos)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = 1e-6
+        fd[:, k] = de_u(u, cm, pos + e) - de_u(u, cm, pos - e)
+    fd /= np.linalg.norm(fd, axis=1, keepdims=True)
+    cosang = (n * fd).sum(1) / np.linalg.norm(n, axis=1)
+    assert np.median(cosang) > 1 - 1e-8 and (cosang > 0.999).mean() > 0.95, np.sort(cosang)[:5]
