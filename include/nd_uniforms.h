@@ -301,6 +301,10 @@ typedef struct nd_uniforms {
 #define ND_PART_FOG (1u << 5)
 #define ND_PART_REFLECTIONS (1u << 6)
 #define ND_PART_REFRACTION (1u << 7)
+#define ND_PART_PHOTON_MAP (1u << 8)
+#define ND_PART_SUBSURFACE (1u << 9)
+#define ND_PART_IRIDESCENCE (1u << 10)
+#define ND_PART_DISPERSION (1u << 11)
 #define ND_PART_SKY (1u << 12)
 #define ND_PART_MC_INDIRECT (1u << 14)
 #define ND_PART_GEOMETRY_ONLY (1u << 16)

@@ -1,9 +1,11 @@
-// nd_mc.hip - wavefront Monte Carlo path trace for gfx950 (opaque scenes).
+// nd_mc.hip - wavefront Monte Carlo path trace for gfx950.
 //
 // Transport of the reference's trace_path (template.wgsl:3819-4137): 8 hero-stratified
-// wavelengths, the procedural sky, point / global lights with soft shadows, Blinn-Phong
-// shading, Schlick reflection off roughness-perturbed normals, one cosine-sampled indirect
-// bounce at the primary hit, Russian roulette from depth 4.
+// wavelengths, the procedural sky, point / global lights with soft shadows tinted by
+// transparent occluders, the Monte Carlo light pick, Blinn-Phong shading, Schlick reflection
+// and refraction off roughness-perturbed normals with the inside march through solids,
+// Cauchy dispersion on a picked wavelength, one cosine-sampled indirect bounce at the
+// primary hit, Russian roulette from depth 4.
 //
 // Wavefront schedule (the reference's cs_wf_* sequence, template.wgsl:4898-5407, as a
 // device-side loop): per bounce
@@ -32,7 +34,7 @@ namespace nd {
 
 // ----------------------------------------------------------------------------- scene
 struct McStop {
-    float color[3], position, glossiness, transparency, reflectiveness, ior, roughness;
+    float color[3], position, glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
 };
 struct McInst {
     int color_count, trap_repeat;
@@ -51,6 +53,11 @@ struct McScene {
     McLight lights[8];
     int light_count;
     int shadows_on, specular_on, ao_on, reflections_on, indirect_on, sky_on;
+    int strips_on, refraction_on, dispersion_on;  // apply_part_mask
+    int transp;        // scene_has_transparency: shadow rays cross transparent occluders
+    int transmissive;  // some visible stop has transparency: paths can enter solids
+    int light_pick;    // mc_enabled and mc_light_pick
+    float disp_soft;   // radians(0.5 * photon_dispersion_soft)
     int max_depth;
     int shadow_steps;
     float sky_zenith[3], sky_horizon[3], sky_ground[3], sky_sun_color[3], sky_sun_dir[3];
@@ -175,9 +182,30 @@ NDD V3<float> sky_color(const McScene& S, const V3<float>& dir) {
     return mk3(fmaxf(c.x * S.sky_intensity, 0.f), fmaxf(c.y * S.sky_intensity, 0.f), fmaxf(c.z * S.sky_intensity, 0.f));
 }
 
-struct Mat { float color[3], glossiness, reflectiveness, ior, roughness; };
-NDD Mat mat_default() { Mat m; m.color[0] = 0.8f; m.color[1] = 0.8f; m.color[2] = 0.85f; m.glossiness = 0.3f; m.reflectiveness = 0.f; m.ior = 1.5f; m.roughness = 0.f; return m; }
-NDD Mat mat_stop(const McStop& s) { Mat m; for (int k = 0; k < 3; ++k) m.color[k] = s.color[k]; m.glossiness = s.glossiness; m.reflectiveness = s.reflectiveness; m.ior = s.ior; m.roughness = s.roughness; return m; }
+struct Mat { float color[3], glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps; };
+// default_material (template.wgsl:1965-1978)
+NDD Mat mat_default() {
+    Mat m;
+    m.color[0] = 0.8f; m.color[1] = 0.8f; m.color[2] = 0.85f;
+    m.glossiness = 0.3f; m.transparency = 0.f; m.reflectiveness = 0.f; m.ior = 1.5f; m.roughness = 0.f;
+    m.abbe = 0.f; m.inner_max_steps = 64.f;
+    return m;
+}
+NDD Mat mat_stop(const McStop& s) {
+    Mat m;
+    for (int k = 0; k < 3; ++k) m.color[k] = s.color[k];
+    m.glossiness = s.glossiness; m.transparency = s.transparency; m.reflectiveness = s.reflectiveness; m.ior = s.ior;
+    m.roughness = s.roughness; m.abbe = s.abbe; m.inner_max_steps = s.inner_max_steps;
+    return m;
+}
+// apply_part_mask (template.wgsl:394-402) for the parts this path reads
+NDD Mat part_mask(Mat m, const McScene& S) {
+    if (!S.strips_on) { m.color[0] = 0.72f; m.color[1] = 0.72f; m.color[2] = 0.73f; }  // SIMPLE_ALBEDO
+    if (!S.reflections_on) m.reflectiveness = 0.f;
+    if (!S.refraction_on) m.transparency = 0.f;
+    if (!S.dispersion_on) m.abbe = 0.f;
+    return m;
+}
 // mix_material(a, b, h) = mix(b, a, h) (template.wgsl:1980-2001)
 NDD Mat mat_mix(const Mat& a, const Mat& b, float h) {
     Mat r;
@@ -186,6 +214,9 @@ NDD Mat mat_mix(const Mat& a, const Mat& b, float h) {
     r.reflectiveness = b.reflectiveness * (1.f - h) + a.reflectiveness * h;
     r.ior = b.ior * (1.f - h) + a.ior * h;
     r.roughness = b.roughness * (1.f - h) + a.roughness * h;
+    r.transparency = b.transparency * (1.f - h) + a.transparency * h;
+    r.abbe = b.abbe * (1.f - h) + a.abbe * h;
+    r.inner_max_steps = b.inner_max_steps * (1.f - h) + a.inner_max_steps * h;
     return r;
 }
 // sample_gradient + orbit_trap_to_strip (template.wgsl:2003-2035)
@@ -283,7 +314,9 @@ struct McPaths {
     float* rad;   // 8 per path
     float *hero, *seedx, *seedy;
     float* hit_t;
-    uint8_t *depth, *flags;  // flags: 1 reflected, 2 hit
+    uint8_t *depth, *flags;  // flags: 1 reflected, 2 hit, 4 inside a solid
+    float* lambda;           // picked wavelength in micrometres (0: none yet)
+    uint16_t* inner;         // inside-march step budget of the solid the path is in
     // queues (path indices) and counters
     uint32_t *live_a, *live_b;
     uint32_t* shadow_path;
@@ -359,12 +392,18 @@ __global__ void k_mc_generate(McArgs a) {
     a.P.hero[p] = mad_rn(ms, 0.61803399f, hash12w(add_rn(bx, 19.31f), add_rn(by, 47.11f)));
     a.P.depth[p] = 0;
     a.P.flags[p] = 0;
+    a.P.lambda[p] = 0.f;
+    a.P.inner[p] = 64;  // PATH_DEFAULT_INNER_STEPS
     a.P.live_a[p] = p;
 }
 
-// --- persistent march over a queue of rays. RAY provides setup(i, org, dir) and
-// finish(i, hit, t). All rays share the job's march precision.
-template <bool EXT, class RAY>
+// --- persistent march over a queue of rays. RAY provides setup(i, org, dir), finish(i,
+// hit, t) and inside(i): the inside-march step budget of a ray that starts in a solid, 0
+// for one outside. Outside rays run march_to with the job's march precision. With INSIDE,
+// rays in a solid run march_inside (template.wgsl:3015-3046): steps of max(|DE|, floor)
+// with the floor growing by 1 + 6 / steps, until DE > 0.0005 t + 0.00005 from the third
+// sample on (exited: a hit at t), t > max_dist or the budget ends (no exit).
+template <bool EXT, bool INSIDE, class RAY>
 NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
     const DevScene& sc = *a.scene;
     const DevJob& J = *a.job;
@@ -377,6 +416,10 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
     V3<float> org = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 1.f);
     March m;
     march_init(m);
+    bool ins = false;  // INSIDE: the lane's ray is in a solid
+    int ii = 0, isteps = 0;
+    float it = 0.f, flo = 0.f, grow = 1.f;
+    const float max_dist = a.S->max_dist;
     unsigned long long de = 0;
     for (;;) {
         if (work_left) {
@@ -400,6 +443,12 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
                         item = cnext + rank;
                         R.setup(item, &org, &dir);
                         march_init(m);
+                        if (INSIDE) {
+                            isteps = R.inside(item);
+                            ins = isteps > 0;
+                            ii = 0; it = 0.f; flo = 0.004f;  // max(INSIDE_STEP_BASE, INSIDE_MIN_STEP)
+                            grow = 1.f + 6.f / (float)max(isteps, 1);
+                        }
                         active = true;
                     }
                     cnext += min((uint32_t)__popcll(need), avail);
@@ -415,7 +464,23 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
         float dist = 0.f;
         int st = -1;
         bool used = false;  // the DE sample decided the step (not an accelerator skip)
-        if (active) {
+        if (active && INSIDE && ins) {
+            const V3<float> ps = mk3(org.x + dir.x * it, org.y + dir.y * it, org.z + dir.z * it);
+            const float d = scene_de<false, EXT>(sc, ps, nullptr);
+            used = true;
+            if (ii > 1 && d > 0.0005f * it + 0.00005f) {  // inside_exited
+                R.finish(item, true, it);
+                active = false;
+            } else {
+                it += fmaxf(fabsf(d), flo);
+                flo *= grow;
+                ++ii;
+                if (it > max_dist || ii >= isteps) {
+                    R.finish(item, false, it);
+                    active = false;
+                }
+            }
+        } else if (active) {
             const float s = march_sample_t(m);
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
             const bool probe = acc && m.probing;
@@ -445,7 +510,11 @@ struct ExtendRays {
     NDD void finish(uint32_t i, bool hit, float t) const {
         const uint32_t p = q[i];
         a.P.hit_t[p] = t;
-        a.P.flags[p] = (uint8_t)((a.P.flags[p] & 1u) | (hit ? 2u : 0u));
+        a.P.flags[p] = (uint8_t)((a.P.flags[p] & 5u) | (hit ? 2u : 0u));
+    }
+    NDD int inside(uint32_t i) const {
+        const uint32_t p = q[i];
+        return (a.P.flags[p] & 4u) ? (int)a.P.inner[p] : 0;
     }
 };
 struct IndirectRays {
@@ -458,65 +527,141 @@ struct IndirectRays {
         a.P.it[i] = t;
         a.P.ihit[i] = hit ? 1 : 0;
     }
+    NDD int inside(uint32_t) const { return 0; }
 };
 
 // The two march kernels of the path trace run at 7 waves per SIMD (72 VGPRs, five values
-// spilled) instead of the compiler's 6: a 64-spp frame takes 2.7% less time.
-template <bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 7, EXT ? 8 : 7))) k_mc_extend(McArgs a, int ping) {
+// spilled) instead of the compiler's 6: a 64-spp frame takes 2.7% less time. The extend
+// kernel's INSIDE variant (scenes with transmissive materials) keeps the compiler's choice.
+template <bool EXT, bool INSIDE>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu((EXT || INSIDE) ? 1 : 7, (EXT || INSIDE) ? 8 : 7))) k_mc_extend(McArgs a, int ping) {
     const uint32_t* q = ping ? a.P.live_b : a.P.live_a;
     const uint32_t n = a.P.ctr[C_LIVE_IN];
     ExtendRays R{a, q};
-    persistent_march<EXT>(a, R, n, a.P.ctr + C_HEAD_EXT);
+    persistent_march<EXT, INSIDE>(a, R, n, a.P.ctr + C_HEAD_EXT);
 }
 
 template <bool EXT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 7, EXT ? 8 : 7))) k_mc_indirect_march(McArgs a) {
     IndirectRays R{a};
-    persistent_march<EXT>(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
+    persistent_march<EXT, false>(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
 }
 
-// shade_local (template.wgsl:3222-3331) for opaque, film- and subsurface-free materials.
+// shade_local (template.wgsl:3222-3331) for film- and subsurface-free materials.
 // With defer, each shadowed light's contribution is queued as a shadow ray with weight
-// w_scale * light * (albedo n.l + specular); otherwise it is added unshadowed.
+// w_scale * light * (albedo n.l + specular); otherwise it is added unshadowed. With
+// pick_one (several lights, mc_light_pick), the shadowed lights are not queued each: one is
+// chosen with probability weight / total, weight = (r + g + b) (n.l + specular), from
+// hash12w(seed + (47.23, 113.91)), and queued scaled by total / weight.
+NDD void queue_shadow(const McArgs& a, uint32_t path, const V3<float>& pos, const V3<float>& n, const V3<float>& ld,
+                      float dist, float k, const Sp8& c, const Sp8& w_scale, float scale) {
+    uint32_t slot;
+    append(a.P.ctr + C_SHADOW, true, &slot);
+    a.P.shadow_path[slot] = path;
+    a.P.sox[slot] = pos.x + n.x * 0.004f;
+    a.P.soy[slot] = pos.y + n.y * 0.004f;
+    a.P.soz[slot] = pos.z + n.z * 0.004f;
+    a.P.sdx[slot] = ld.x; a.P.sdy[slot] = ld.y; a.P.sdz[slot] = ld.z;
+    a.P.sdist[slot] = dist;
+    a.P.sk[slot] = k;
+    for (int q = 0; q < 8; ++q) a.P.sw[8 * slot + q] = c.v[q] * w_scale.v[q] * scale;
+}
 NDD Sp8 shade_lights(const McArgs& a, const Basis& B, const V3<float>& pos, const V3<float>& dir, const V3<float>& n,
-                     const Mat& m, const Sp8& ambient, bool shadows, const Sp8& w_scale, uint32_t path) {
+                     const Mat& m, const Sp8& ambient, bool shadows, const Sp8& w_scale, uint32_t path, float sx, float sy) {
     const McScene& S = *a.S;
     const V3<float> view = mk3(-dir.x, -dir.y, -dir.z);
     const float spec_power = 4.f + 124.f * fclamp(m.glossiness, 0.f, 1.f);
     const Sp8 albedo = from_rgb(B, m.color[0], m.color[1], m.color[2]);
+    const bool pick_one = shadows && S.light_pick && S.light_count > 1;
+    // light i's direction, colour, distance, n.l and specular term (light_terms)
+    auto terms = [&](int i, V3<float>* ld, V3<float>* col, float* dist, float* ndotl, float* spec) {
+        const McLight& L = S.lights[i];
+        light_of(L, S.max_dist, pos, ld, col, dist);
+        *ndotl = fmaxf(dot3(n, *ld), 0.f);
+        const V3<float> hd = vnorm3(mk3(ld->x + view.x, ld->y + view.y, ld->z + view.z));
+        *spec = S.specular_on ? powf(fmaxf(dot3(n, hd), 0.f), spec_power) * m.glossiness : 0.f;
+    };
     Sp8 acc = sp_splat(0.f);
+    float pick_total = 0.f;
     for (int i = 0; i < S.light_count; ++i) {
         const McLight& L = S.lights[i];
         V3<float> ld, col;
-        float dist;
-        light_of(L, S.max_dist, pos, &ld, &col, &dist);
+        float dist, ndotl, spec;
+        terms(i, &ld, &col, &dist, &ndotl, &spec);
         if (fmaxf(col.x, fmaxf(col.y, col.z)) < 1e-5f) continue;
-        const float ndotl = fmaxf(dot3(n, ld), 0.f);
-        const V3<float> hd = vnorm3(mk3(ld.x + view.x, ld.y + view.y, ld.z + view.z));
-        const float spec = S.specular_on ? powf(fmaxf(dot3(n, hd), 0.f), spec_power) * m.glossiness : 0.f;
         if (ndotl <= 0.f) continue;
+        if (pick_one && L.cast) {
+            pick_total += (col.x + col.y + col.z) * (ndotl + spec);
+            continue;
+        }
         const Sp8 ls = from_rgb(B, col.x, col.y, col.z);
         Sp8 c;
         for (int k = 0; k < 8; ++k) c.v[k] = ls.v[k] * (albedo.v[k] * ndotl + spec);
         if (shadows && L.cast && S.shadow_steps > 0) {
-            uint32_t slot;
-            append(a.P.ctr + C_SHADOW, true, &slot);
-            a.P.shadow_path[slot] = path;
-            a.P.sox[slot] = pos.x + n.x * 0.004f;
-            a.P.soy[slot] = pos.y + n.y * 0.004f;
-            a.P.soz[slot] = pos.z + n.z * 0.004f;
-            a.P.sdx[slot] = ld.x; a.P.sdy[slot] = ld.y; a.P.sdz[slot] = ld.z;
-            a.P.sdist[slot] = dist;
-            a.P.sk[slot] = L.k;
-            for (int k = 0; k < 8; ++k) a.P.sw[8 * slot + k] = c.v[k] * w_scale.v[k];
+            queue_shadow(a, path, pos, n, ld, dist, L.k, c, w_scale, 1.f);
         } else {
             for (int k = 0; k < 8; ++k) acc.v[k] += c.v[k];
+        }
+    }
+    if (pick_total > 0.f) {
+        const float at = hash12w(add_rn(sx, 47.23f), add_rn(sy, 113.91f)) * pick_total;
+        int pick = 0;
+        float running = 0.f, pick_w = 0.f;
+        for (int i = 0; i < S.light_count; ++i) {
+            const McLight& L = S.lights[i];
+            if (!L.cast) continue;
+            V3<float> ld, col;
+            float dist, ndotl, spec;
+            terms(i, &ld, &col, &dist, &ndotl, &spec);
+            if (fmaxf(col.x, fmaxf(col.y, col.z)) < 1e-5f || ndotl <= 0.f) continue;
+            const float w = (col.x + col.y + col.z) * (ndotl + spec);
+            if (w <= 0.f) continue;
+            pick = i;
+            pick_w = w;
+            running += w;
+            if (running > at) break;
+        }
+        V3<float> ld, col;
+        float dist, ndotl, spec;
+        terms(pick, &ld, &col, &dist, &ndotl, &spec);
+        const Sp8 ls = from_rgb(B, col.x, col.y, col.z);
+        Sp8 c;
+        for (int k = 0; k < 8; ++k) c.v[k] = ls.v[k] * (albedo.v[k] * ndotl + spec);
+        const float scale = pick_total / pick_w;
+        if (S.shadow_steps > 0) {
+            queue_shadow(a, path, pos, n, ld, dist, S.lights[pick].k, c, w_scale, scale);
+        } else {
+            for (int k = 0; k < 8; ++k) acc.v[k] += c.v[k] * scale;
         }
     }
     Sp8 r;
     for (int k = 0; k < 8; ++k) r.v[k] = albedo.v[k] * ambient.v[k] + acc.v[k];
     return r;
+}
+
+// refract (WGSL builtin) and refract_safe (template.wgsl:3387-3394): the normal is turned to
+// face the ray; total internal reflection returns the mirror direction.
+NDD V3<float> refract_safe(const V3<float>& d, const V3<float>& raw_n, float eta, bool* tir) {
+    const V3<float> n = dot3(d, raw_n) > 0.f ? mk3(-raw_n.x, -raw_n.y, -raw_n.z) : raw_n;
+    const float c = dot3(n, d);
+    const float k = 1.f - eta * eta * (1.f - c * c);
+    V3<float> r = mk3(0.f, 0.f, 0.f);
+    if (k >= 0.f) {
+        const float s = eta * c + sqrtf(k);
+        r = mk3(d.x * eta - n.x * s, d.y * eta - n.y * s, d.z * eta - n.z * s);
+    }
+    *tir = !(dot3(r, r) > 1e-4f);  // REFRACT_TIR_EPSILON
+    if (*tir) {
+        const float dn = 2.f * c;
+        r = mk3(d.x - n.x * dn, d.y - n.y * dn, d.z - n.z * dn);
+    }
+    return r;
+}
+// cauchy_ior (template.wgsl:3403-3409), lambda in micrometres
+NDD float cauchy_ior(float ior_d, float abbe, float lambda) {
+    const float b = (ior_d - 1.f) / (fmaxf(abbe, 1.f) * 1.9099f);
+    return ior_d + b * (1.f / (lambda * lambda) - 2.89626f);
 }
 
 // --- shade (path_shade, template.wgsl:3902-4124). One path per lane.
@@ -537,6 +682,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
             const int depth = a.P.depth[p];
             const uint8_t fl = a.P.flags[p];
             const bool reflected = fl & 1u;
+            const bool in_solid = fl & 4u;
             const V3<float> pos = mk3(a.P.px[p], a.P.py[p], a.P.pz[p]);
             const V3<float> dir = mk3(a.P.dx[p], a.P.dy[p], a.P.dz[p]);
             const float hero = a.P.hero[p];
@@ -560,14 +706,30 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                 const float eps = J.eps_c * t + J.eps_f;
                 const V3<float> hp = mk3(pos.x + dir.x * t, pos.y + dir.y * t, pos.z + dir.z * t);
                 const V3<float> nrm = normal_tetra<EXT>(sc, hp, eps);
-                const Mat mat = hit_material<EXT>(sc, S, hp);
+                const Mat mat = part_mask(hit_material<EXT>(sc, S, hp), S);
                 de += 5;
                 if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
                     if (a.out_depth) a.out_depth[pix] = t;
                     if (a.out_normal) { a.out_normal[3 * pix] = nrm.x; a.out_normal[3 * pix + 1] = nrm.y; a.out_normal[3 * pix + 2] = nrm.z; }
                 }
-                // scatter_at with allow_diffuse = false (template.wgsl:3552-3616), opaque
+                // dispersion: the first dispersive hit picks one of the 8 wavelengths
+                // (template.wgsl:3995-4003); without soft weights only that lane carries on
+                float lambda = a.P.lambda[p];
+                int collapse = -1;
+                if (lambda == 0.f && mat.abbe > 0.5f) {  // MIN_ABBE
+                    const float hx = hash12w(add_rn(sx, 3.71f), add_rn(sy, 8.13f));  // DISPERSION_PICK_SEED_OFFSET
+                    const int pick = min((int)(hx * 8.f), 7);
+                    float lam[8];
+                    lambdas_of(hero, lam);
+                    lambda = lam[pick] * 0.001f;
+                    a.P.lambda[p] = lambda;
+                    if (S.disp_soft <= 0.f) collapse = pick;
+                }
+                // scatter_at with allow_diffuse = false (template.wgsl:3552-3616)
                 const V3<float> nf = dot3(dir, nrm) > 0.f ? mk3(-nrm.x, -nrm.y, -nrm.z) : nrm;
+                float ior = fmaxf(mat.ior, 1.f);
+                if (lambda > 0.f && mat.abbe > 0.5f) ior = fmaxf(cauchy_ior(ior, mat.abbe, lambda), 1.f);
+                const float eta = in_solid ? ior : 1.f / ior;
                 const float rough = fclamp(mat.roughness, 0.f, 1.f);
                 const float alpha = rough * rough;
                 V3<float> nr = nf;
@@ -575,15 +737,18 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                     const V3<float> hs = cosine_sample(nf, add_rn(sx, 5.19f), add_rn(sy, 12.73f));
                     nr = vnorm3(mk3(nf.x + (hs.x - nf.x) * alpha, nf.y + (hs.y - nf.y) * alpha, nf.z + (hs.z - nf.z) * alpha));
                 }
-                const float ior = fmaxf(mat.ior, 1.f);
                 const float r0r = (1.f - ior) / (1.f + ior);
                 const float r0 = r0r * r0r;
                 const float cth = fclamp(1.f - fabsf(dot3(dir, nr)), 0.f, 1.f);
                 const float c2 = cth * cth;
                 const float f = r0 + (1.f - r0) * c2 * c2 * cth;
+                const float transparency = fclamp(mat.transparency, 0.f, 1.f);
                 const float refl = fclamp(mat.reflectiveness, 0.f, 1.f);
                 const float p_reflect = fclamp(refl + (1.f - refl) * f, 0.f, 1.f);
-                const float local_share = 1.f - p_reflect;
+                const float rest = 1.f - p_reflect;
+                const float p_transmit = rest * transparency;
+                const float local_share = rest * (1.f - transparency);
+                const float p_cont = p_reflect + p_transmit;
                 if (local_share > 0.001f) {
                     Sp8 ambient = sp_splat(0.15f);
                     Sp8 wscale;
@@ -614,16 +779,58 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                         de += 5;
                         ambient = sp_splat(0.15f * fclamp(1.f - 2.f * occ / radius, 0.f, 1.f));
                     }
-                    const Sp8 lit = shade_lights(a, B, hp, dir, nrm, mat, ambient, !reflected && S.shadows_on, wscale, p);
+                    const Sp8 lit = shade_lights(a, B, hp, dir, nrm, mat, ambient, !reflected && S.shadows_on, wscale, p, sx, sy);
                     for (int k = 0; k < 8; ++k) rad.v[k] += wscale.v[k] * lit.v[k];
                 }
-                if (p_reflect > 0.f && S.reflections_on) {
-                    // reflect: weight mix(color, 1, f) * p_reflect
-                    const Sp8 w = from_rgb(B, mat.color[0] + (1.f - mat.color[0]) * f, mat.color[1] + (1.f - mat.color[1]) * f,
-                                           mat.color[2] + (1.f - mat.color[2]) * f);
-                    for (int k = 0; k < 8; ++k) thr.v[k] *= w.v[k] * p_reflect;
+                // the continuation: r = hash * p_cont picks reflection below p_reflect, else
+                // refraction; nothing is left for r >= p_cont (absorbed)
+                const float r = hash12w(sx, sy) * p_cont;
+                int kind = 0;  // 0 absorbed, 1 reflect, 2 transmit
+                V3<float> nd = dir;
+                float wrgb[3] = {0.f, 0.f, 0.f};
+                bool soft_w = false;
+                if (r < p_reflect) {
                     const float dn = 2.f * dot3(nr, dir);
-                    const V3<float> nd = mk3(dir.x - nr.x * dn, dir.y - nr.y * dn, dir.z - nr.z * dn);
+                    nd = mk3(dir.x - nr.x * dn, dir.y - nr.y * dn, dir.z - nr.z * dn);
+                    for (int k = 0; k < 3; ++k) wrgb[k] = mat.color[k] + (1.f - mat.color[k]) * f;
+                    kind = 1;
+                } else if (r < p_reflect + p_transmit) {
+                    bool tir;
+                    nd = refract_safe(dir, nr, eta, &tir);
+                    soft_w = S.disp_soft > 0.f && lambda > 0.f && mat.abbe > 0.5f;
+                    for (int k = 0; k < 3; ++k) wrgb[k] = mat.color[k];
+                    kind = tir ? 1 : 2;
+                }
+                if (kind != 0 && !(kind == 1 && !in_solid && !S.reflections_on)) {
+                    const Sp8 w = from_rgb(B, wrgb[0], wrgb[1], wrgb[2]);
+                    Sp8 dw = sp_splat(1.f);
+                    if (collapse >= 0)
+                        for (int k = 0; k < 8; ++k) dw.v[k] = k == collapse ? 8.f : 0.f;  // spec_collapse_mask
+                    if (soft_w) {
+                        // dispersion_angles2 + dispersion_weights (template.wgsl:3521-3548)
+                        float lam[8];
+                        lambdas_of(hero, lam);
+                        const float ior_d = fmaxf(mat.ior, 1.f);
+                        const float kk = -0.5f / fmaxf(S.disp_soft * S.disp_soft, 1e-12f);
+                        float total = 0.f;
+                        for (int k = 0; k < 8; ++k) {
+                            const float ior_k = fmaxf(cauchy_ior(ior_d, mat.abbe, lam[k] * 0.001f), 1.f);
+                            bool tk;
+                            const V3<float> dk = refract_safe(dir, nr, in_solid ? ior_k : 1.f / ior_k, &tk);
+                            const float ang = 2.f * (1.f - fclamp(dot3(nd, dk), -1.f, 1.f));
+                            dw.v[k] = expf(ang * kk);
+                            total += dw.v[k];
+                        }
+                        const float sc8 = 8.f / fmaxf(total, 1e-6f);
+                        for (int k = 0; k < 8; ++k) dw.v[k] *= sc8;
+                    }
+                    for (int k = 0; k < 8; ++k) thr.v[k] *= dw.v[k] * w.v[k] * p_cont;
+                    bool solid = in_solid;
+                    int inner = a.P.inner[p];
+                    if (kind == 2) {
+                        solid = !in_solid;
+                        if (solid) inner = min(max((int)mat.inner_max_steps, 4), 512);  // MIN/MAX_INNER_STEPS
+                    }
                     const float bias = fmaxf(0.003f, 2.f * eps);
                     cont = true;
                     if (depth >= 4) {
@@ -635,7 +842,8 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                         a.P.px[p] = hp.x + nd.x * bias; a.P.py[p] = hp.y + nd.y * bias; a.P.pz[p] = hp.z + nd.z * bias;
                         a.P.dx[p] = nd.x; a.P.dy[p] = nd.y; a.P.dz[p] = nd.z;
                         a.P.depth[p] = (uint8_t)(depth + 1);
-                        a.P.flags[p] = 1u;
+                        a.P.flags[p] = (uint8_t)(((reflected || kind == 1) ? 1u : 0u) | (solid ? 4u : 0u));
+                        a.P.inner[p] = (uint16_t)inner;
                     } else {
                         cont = false;
                     }
@@ -673,9 +881,9 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
             const float t = a.P.it[i];
             const V3<float> bp = mk3(a.P.iox[i] + dir.x * t, a.P.ioy[i] + dir.y * t, a.P.ioz[i] + dir.z * t);
             const V3<float> bn = normal_tetra<EXT>(sc, bp, J.eps_c * t + J.eps_f);
-            const Mat bm = hit_material<EXT>(sc, S, bp);
+            const Mat bm = part_mask(hit_material<EXT>(sc, S, bp), S);
             de += 5;
-            L = shade_lights(a, B, bp, dir, bn, bm, sp_splat(0.15f), false, sp_splat(0.f), p);
+            L = shade_lights(a, B, bp, dir, bn, bm, sp_splat(0.15f), false, sp_splat(0.f), p, 0.f, 0.f);
         }
         for (int k = 0; k < 8; ++k) atomicAdd(&a.P.rad[8 * p + k], a.P.iw[8 * i + k] * L.v[k]);
     }
@@ -683,8 +891,12 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
     if (threadIdx.x == 0) atomicAdd(de_ctr(a), de);
 }
 
-// --- shadow: visibility_march on an opaque scene (template.wgsl:2282-2342), refilled
-template <bool EXT>
+// --- shadow: visibility_march (template.wgsl:2282-2342) with shadow_factor's parameters,
+// refilled. An opaque hit blocks the light. With TRANSP (scene_has_transparency), a hit on
+// a surface whose material has transparency a >= 0.01 tints the ray by
+// from_rgb(mix(1, colour, a)) * a in the path's spectral basis and steps through the solid
+// (steps of max(|DE|, floor), floor 0.004 growing by 1.15) until DE > 0.001 t + 0.0005.
+template <bool EXT, bool TRANSP>
 __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
     const DevScene& sc = *a.scene;
     const DevJob& J = *a.job;
@@ -698,6 +910,9 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
     V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
     float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
     int step = 0;
+    bool ins = false;          // TRANSP: inside a transparent occluder
+    float flo = 0.004f;        // SHADOW_STEP_FLOOR, grown by SHADOW_FLOOR_GROWTH inside
+    Sp8 tint = sp_splat(1.f);  // TRANSP: accumulated transmission
     unsigned long long de = 0;
     for (;;) {
         if (work_left) {
@@ -724,6 +939,7 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
                         ldist = a.P.sdist[item];
                         k = a.P.sk[item];
                         t = 0.004f; res = 1.f; step = 0;
+                        if (TRANSP) { ins = false; flo = 0.004f; tint = sp_splat(1.f); }
                         if (t < ldist) active = true;
                         else {
                             const uint32_t p = a.P.shadow_path[item];
@@ -743,26 +959,46 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
         bool done = false, skipped = false;
         if (active) {
             const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
-            const float skip = acc ? accel_skip(av, ps) : 0.f;
+            const float skip = (acc && !(TRANSP && ins)) ? accel_skip(av, ps) : 0.f;
             const float d = scene_de<false, EXT>(sc, ps, nullptr);
             step++;
-            skipped = acc && shadow_skip_applies(skip, k, res, t);
+            skipped = acc && !(TRANSP && ins) && shadow_skip_applies(skip, k, res, t);
             if (skipped) {
                 t += skip;
-                if (step >= max_steps || t >= ldist) done = true;
-            } else {
-                if (d < 0.0005f) { res = 0.f; done = true; }
-                else {
-                    res = fminf(res, k * d / t);
-                    t += fmaxf(d, 0.001f);
-                    if (step >= max_steps || t >= ldist) done = true;
+            } else if (TRANSP && ins) {
+                if (d > 0.001f * t + 0.0005f) ins = false;  // SHADOW_EXIT_SLOPE, _BIAS
+                t += fmaxf(fabsf(d), flo);
+                flo *= 1.15f;
+            } else if (d < 0.0005f) {  // SHADOW_HIT_EPS
+                bool blocked = true;
+                if (TRANSP) {
+                    // the occluder's material, unmasked (hit_material(p).mat)
+                    const Mat om = hit_material<EXT>(sc, *a.S, ps);
+                    const float al = fclamp(om.transparency, 0.f, 1.f);
+                    if (al >= 0.01f) {  // SHADOW_OPAQUE_ALPHA
+                        const Basis B = basis_of(a.P.hero[a.P.shadow_path[item]]);
+                        const Sp8 tc = from_rgb(B, 1.f + (om.color[0] - 1.f) * al, 1.f + (om.color[1] - 1.f) * al,
+                                                1.f + (om.color[2] - 1.f) * al);
+                        for (int q = 0; q < 8; ++q) tint.v[q] *= tc.v[q] * al;
+                        if (sp_max(tint) >= 0.004f) {  // SHADOW_TINT_CUT
+                            blocked = false;
+                            ins = true;
+                            t += flo;
+                        }
+                    }
                 }
+                if (blocked) { res = 0.f; done = true; }
+            } else {
+                res = fminf(res, k * d / t);
+                t += fmaxf(d, 0.001f);
             }
+            if (!done && (step >= max_steps || t >= ldist)) done = true;
             if (done) {
                 const float vis = fclamp(res, 0.f, 1.f);
                 if (vis > 0.f) {
                     const uint32_t p = a.P.shadow_path[item];
-                    for (int q = 0; q < 8; ++q) atomicAdd(&a.P.rad[8 * p + q], a.P.sw[8 * item + q] * vis);
+                    for (int q = 0; q < 8; ++q)
+                        atomicAdd(&a.P.rad[8 * p + q], a.P.sw[8 * item + q] * vis * (TRANSP ? tint.v[q] : 1.f));
                 }
                 active = false;
             }
@@ -837,6 +1073,7 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
             for (int k = 0; k < 3; ++k) d.color[k] = c.color[k];
             d.position = c.position; d.glossiness = c.glossiness; d.transparency = c.transparency;
             d.reflectiveness = c.reflectiveness; d.ior = c.ior; d.roughness = c.roughness;
+            d.abbe = c.abbe; d.inner_max_steps = c.inner_max_steps;
         }
     }
     S.light_count = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min((int)u.light_count, ND_MAX_LIGHTS);
@@ -862,6 +1099,17 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     S.reflections_on = !(parts & ND_PART_REFLECTIONS);
     S.indirect_on = u.mc_enabled > 0.5f && !(parts & ND_PART_MC_INDIRECT);
     S.sky_on = !(parts & ND_PART_SKY);
+    S.strips_on = !(parts & ND_PART_COLOR_STRIPS);
+    S.refraction_on = !(parts & ND_PART_REFRACTION);
+    S.dispersion_on = !(parts & ND_PART_DISPERSION);
+    S.transp = u.has_transparency > 0.5f && S.refraction_on;
+    S.transmissive = 0;
+    if (S.refraction_on)
+        for (int i = 0; i < ND_MAX_INSTANCES; ++i)
+            for (int c = 0; c < std::min((int)u.instances[i].color_count, 16); ++c)
+                if (u.instances[i].colors[c].transparency > 0.f) S.transmissive = 1;
+    S.light_pick = u.mc_enabled > 0.5f && u.mc_light_pick > 0.5f;
+    S.disp_soft = (0.5f * u.photon_dispersion_soft) * 0.017453292519943295f;
     S.max_depth = std::min(std::max((int)u.max_reflection_bounces + 8, 1), 16);
     S.shadow_steps = std::min((int)u.max_steps, 40);
     for (int k = 0; k < 3; ++k) {
@@ -888,14 +1136,14 @@ static uint32_t mc_resident(K kernel) {
 }
 
 void nd_resident_mc(uint32_t* r) {
-    r[RES_MC_EXTEND] = mc_resident(nd::k_mc_extend<false>);
+    r[RES_MC_EXTEND] = mc_resident(nd::k_mc_extend<false, false>);
     r[RES_MC_SHADE] = mc_resident(nd::k_mc_shade<false>);
-    r[RES_MC_SHADOW] = mc_resident(nd::k_mc_shadow<false>);
+    r[RES_MC_SHADOW] = mc_resident(nd::k_mc_shadow<false, false>);
     r[RES_MC_IND_MARCH] = mc_resident(nd::k_mc_indirect_march<false>);
     r[RES_MC_IND_SHADE] = mc_resident(nd::k_mc_indirect_shade<false>);
-    r[RES_MC_EXTEND_EXT] = mc_resident(nd::k_mc_extend<true>);
+    r[RES_MC_EXTEND_EXT] = mc_resident(nd::k_mc_extend<true, false>);
     r[RES_MC_SHADE_EXT] = mc_resident(nd::k_mc_shade<true>);
-    r[RES_MC_SHADOW_EXT] = mc_resident(nd::k_mc_shadow<true>);
+    r[RES_MC_SHADOW_EXT] = mc_resident(nd::k_mc_shadow<true, false>);
     r[RES_MC_IND_MARCH_EXT] = mc_resident(nd::k_mc_indirect_march<true>);
     r[RES_MC_IND_SHADE_EXT] = mc_resident(nd::k_mc_indirect_shade<true>);
 }
@@ -907,7 +1155,6 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     if (!out || !out->color) return nd_fail(c, ND_E_INVALID, "path trace needs a colour output");
     if (spp == 0 || spp > 65536) return nd_fail(c, ND_E_INVALID, "spp %u", spp);
     const nd_uniforms& u = *(const nd_uniforms*)uniforms;
-    if (u.has_transparency > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "transmissive materials are not implemented in the path trace");
     if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.dof_enabled > 0.5f || u.adapt_enabled > 0.5f ||
         u.approx_flags != 0.0f || u.primary_cache_key > 0.5f)
         return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field, adaptive sampling, approximations and the hit cache are not implemented");
@@ -916,8 +1163,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     for (int i = 0; i < ND_MAX_INSTANCES; ++i)
         for (int s = 0; s < (int)u.instances[i].color_count && s < 16; ++s) {
             const nd_color_stop& cs = u.instances[i].colors[s];
-            if (cs.transparency > 0.f || cs.subsurface > 0.001f || cs.film_strength > 0.001f || cs.abbe > 0.5f)
-                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d stop %d: transmission, subsurface, film and dispersion are not implemented", i, s);
+            if (cs.subsurface > 0.001f || cs.film_strength > 0.001f)
+                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d stop %d: subsurface scattering and thin film are not implemented", i, s);
         }
     if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
     nd_march_opts o{ND_NORMAL_TETRA, 0, 0, ND_KERNEL_PERSISTENT};
@@ -940,6 +1187,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     sz(32 * P); sz(32 * P);                 // thr, rad
     for (int i = 0; i < 4; ++i) sz(4 * P);  // hero, seeds, hit_t
     sz(P); sz(P);                           // depth, flags
+    sz(4 * P); sz(2 * P);                   // lambda, inner
     sz(4 * P); sz(4 * P);                   // live queues
     const size_t SQ = P * (size_t)std::max(S.light_count, 1);
     sz(4 * SQ); for (int i = 0; i < 8; ++i) sz(4 * SQ); sz(32 * SQ);
@@ -963,6 +1211,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     Pp.hero = (float*)carve(4 * P); Pp.seedx = (float*)carve(4 * P); Pp.seedy = (float*)carve(4 * P);
     Pp.hit_t = (float*)carve(4 * P);
     Pp.depth = (uint8_t*)carve(P); Pp.flags = (uint8_t*)carve(P);
+    Pp.lambda = (float*)carve(4 * P); Pp.inner = (uint16_t*)carve(2 * P);
     Pp.live_a = (uint32_t*)carve(4 * P); Pp.live_b = (uint32_t*)carve(4 * P);
     Pp.shadow_path = (uint32_t*)carve(4 * SQ);
     Pp.sox = (float*)carve(4 * SQ); Pp.soy = (float*)carve(4 * SQ); Pp.soz = (float*)carve(4 * SQ);
@@ -998,6 +1247,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     const uint32_t g_is = nd_pass_grid(c, RES_MC_IND_SHADE + rx, claims);
     const int max_depth = S.max_depth;
     const bool indirect = S.indirect_on != 0;
+    const bool inside = S.transmissive != 0;  // paths can enter solids: extend runs march_inside
+    const bool transp = S.transp != 0;        // shadow rays cross transparent occluders
     // The passes of one chunk, in stream order. Captured once into a graph per shape and
     // replayed for every chunk (the chunk's sample range is read from a.ch).
     auto enqueue_chunk = [&](hipStream_t st) {
@@ -1008,13 +1259,17 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         for (int depth = 0; depth < max_depth; ++depth) {
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 1);
             if (ext) {
-                hipLaunchKernelGGL(nd::k_mc_extend<true>, dim3(g_ext), dim3(64), 0, st, a, ping);
+                if (inside) hipLaunchKernelGGL((nd::k_mc_extend<true, true>), dim3(g_ext), dim3(64), 0, st, a, ping);
+                else hipLaunchKernelGGL((nd::k_mc_extend<true, false>), dim3(g_ext), dim3(64), 0, st, a, ping);
                 hipLaunchKernelGGL(nd::k_mc_shade<true>, dim3(g_shade), dim3(64), 0, st, a, ping);
-                hipLaunchKernelGGL(nd::k_mc_shadow<true>, dim3(g_shadow), dim3(64), 0, st, a);
+                if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<true, true>), dim3(g_shadow), dim3(64), 0, st, a);
+                else hipLaunchKernelGGL((nd::k_mc_shadow<true, false>), dim3(g_shadow), dim3(64), 0, st, a);
             } else {
-                hipLaunchKernelGGL(nd::k_mc_extend<false>, dim3(g_ext), dim3(64), 0, st, a, ping);
+                if (inside) hipLaunchKernelGGL((nd::k_mc_extend<false, true>), dim3(g_ext), dim3(64), 0, st, a, ping);
+                else hipLaunchKernelGGL((nd::k_mc_extend<false, false>), dim3(g_ext), dim3(64), 0, st, a, ping);
                 hipLaunchKernelGGL(nd::k_mc_shade<false>, dim3(g_shade), dim3(64), 0, st, a, ping);
-                hipLaunchKernelGGL(nd::k_mc_shadow<false>, dim3(g_shadow), dim3(64), 0, st, a);
+                if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<false, true>), dim3(g_shadow), dim3(64), 0, st, a);
+                else hipLaunchKernelGGL((nd::k_mc_shadow<false, false>), dim3(g_shadow), dim3(64), 0, st, a);
             }
             if (depth == 0 && indirect) {
                 if (ext) {
@@ -1036,7 +1291,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         struct McGraphKey {
             nd::McArgs args;
             uint32_t grid[5];
-            int max_depth, indirect, ext;
+            int max_depth, indirect, ext, inside, transp;
             uint64_t paths;
         } key;
         static_assert(sizeof(McGraphKey) <= sizeof(c->mc_graph_key), "graph key storage");
@@ -1044,6 +1299,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         key.args = a;
         key.grid[0] = g_ext; key.grid[1] = g_shade; key.grid[2] = g_shadow; key.grid[3] = g_im; key.grid[4] = g_is;
         key.ext = ext;
+        key.inside = inside;
+        key.transp = transp;
         key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
         if (!c->mc_graph_exec || !c->graph_valid || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
             if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }

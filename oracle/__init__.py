@@ -38,6 +38,8 @@ def lib():
         L.ndo_normal.argtypes = [vp, vp, i32, vp, vp, i32, vp]
         L.ndo_shadow.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]
         L.ndo_primary_dir.argtypes = [vp, i32, i32, vp]
+        L.ndo_march_inside.argtypes = [vp, vp, vp, vp, i32, vp]
+        L.ndo_visibility.argtypes = [vp, vp, C.c_double, vp, vp, C.c_double, C.c_double, i32, vp]
         L.ndo_render_gbuffer.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp,
                                          i32, vp]
         L.ndo_flops_per_de.argtypes = [vp, vp, vp, i32, vp]
@@ -97,6 +99,31 @@ def shadow(u, origin, direction, light_dist, k, steps, code_map=None):
     cm, cmp_ = _cm(code_map)
     lib().ndo_shadow(_u(u), cmp_, _p(o), _p(d), float(light_dist), float(k), int(steps), _p(vis), _p(ev))
     return float(vis[0]), int(ev[0])
+
+
+def spectral_basis(hero, rgb):
+    """The 8-wavelength spectrum of an RGB colour in the hero wavelength's basis."""
+    return helper(29, [hero] + list(rgb), 8)
+
+
+def march_inside(u, origin, direction, steps, code_map=None):
+    """(exited, t, DE evaluations) of the inside march from origin."""
+    out = np.zeros(3)
+    o = np.ascontiguousarray(origin, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    cm, cmp_ = _cm(code_map)
+    lib().ndo_march_inside(_u(u), cmp_, _p(o), _p(d), int(steps), _p(out))
+    return bool(out[0]), float(out[1]), int(out[2])
+
+
+def visibility(u, hero, origin, direction, light_dist, k, steps, code_map=None):
+    """The path trace's spectral shadow visibility: 8 lanes for the hero wavelength."""
+    out = np.zeros(8)
+    o = np.ascontiguousarray(origin, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    cm, cmp_ = _cm(code_map)
+    lib().ndo_visibility(_u(u), cmp_, float(hero), _p(o), _p(d), float(light_dist), float(k), int(steps), _p(out))
+    return out
 
 
 def primary_dir(u, px, py):

@@ -474,8 +474,41 @@ int ndo_helper(int op, const double* a, double* out) {
         case 23: { out[0] = a[0]; out[1] = a[1]; out[2] = a[2]; gamut_fit(out); return 0; }
         case 24: out[0] = fresnel_schlick(a[0], a[1]); return 0;
         case 25: { Spectral<double> sp(a[0]); for (int k = 0; k < 8; ++k) out[k] = sp.lambda[k]; return 0; }
+        case 26: out[0] = cauchy_ior(a[0], a[1], a[2]); return 0;
+        case 27: { bool tir; D3 r = refract_safe(D3(a[0], a[1], a[2]), D3(a[3], a[4], a[5]), a[6], &tir);
+                   out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = tir ? 1.0 : 0.0; return 0; }
+        case 29: { Spectral<double> sp(a[0]); Spec<double> v = sp.from_rgb(a[1], a[2], a[3]);
+                   for (int k = 0; k < 8; ++k) out[k] = v.v[k];
+                   return 0; }
+        case 28: { Spectral<double> sp(a[0]); Mat m = default_mat(); m.ior = a[7]; m.abbe = a[8];
+                   Spec<double> w = dispersion_weights(sp, D3(a[1], a[2], a[3]), D3(a[4], a[5], a[6]), m, a[9] > 0.5,
+                                                       D3(a[10], a[11], a[12]), a[13]);
+                   for (int k = 0; k < 8; ++k) out[k] = w.v[k];
+                   return 0; }
     }
     return -1;
+}
+
+// march_inside from origin along dir with a step budget: out = (exited, t, evals).
+int ndo_march_inside(const nd_uniforms* u, const int32_t* cm, const double* origin, const double* dir, int steps,
+                     double* out) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    int ev = 0;
+    MarchResult<double> r = march_inside(s, V3<double>(origin[0], origin[1], origin[2]), V3<double>(dir[0], dir[1], dir[2]), steps,
+                                         (double)u->max_dist, &ev);
+    out[0] = r.hit ? 1.0 : 0.0; out[1] = r.dist; out[2] = ev;
+    return 0;
+}
+
+// The path trace's spectral shadow visibility (8 lanes of the hero wavelength's basis).
+int ndo_visibility(const nd_uniforms* u, const int32_t* cm, double hero, const double* origin, const double* dir,
+                   double light_dist, double k, int steps, double* out) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    Spectral<double> sp(hero);
+    Spec<double> v = visibility_march(*u, s, sp, V3<double>(origin[0], origin[1], origin[2]), V3<double>(dir[0], dir[1], dir[2]),
+                                      light_dist, k, steps, (int*)nullptr);
+    for (int i = 0; i < 8; ++i) out[i] = v.v[i];
+    return 0;
 }
 
 // placeCascades for a camera position (out: 8 x [corner xyz, cell]).

@@ -143,12 +143,21 @@ template <class T> V3<T> sky_color(const nd_uniforms& u, V3<T> dir) {
 // ----------------------------------------------------------------------------- materials
 struct Mat {
     double color[3];
-    double glossiness, transparency, reflectiveness, ior, roughness;
+    double glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
 };
 inline Mat stop_mat(const nd_color_stop& c) {
-    return Mat{{c.color[0], c.color[1], c.color[2]}, c.glossiness, c.transparency, c.reflectiveness, c.ior, c.roughness};
+    return Mat{{c.color[0], c.color[1], c.color[2]}, c.glossiness, c.transparency, c.reflectiveness, c.ior, c.roughness,
+               c.abbe, c.inner_max_steps};
 }
-inline Mat default_mat() { return Mat{{0.8, 0.8, 0.85}, 0.3, 0.0, 0.0, 1.5, 0.0}; }  // template.wgsl:1976-1978
+inline Mat default_mat() { return Mat{{0.8, 0.8, 0.85}, 0.3, 0.0, 0.0, 1.5, 0.0, 0.0, 64.0}; }  // template.wgsl:1965-1978
+// apply_part_mask (template.wgsl:394-402) for the parts this path reads
+inline Mat apply_part_mask(Mat m, uint32_t parts) {
+    if (parts & ND_PART_COLOR_STRIPS) { m.color[0] = 0.72; m.color[1] = 0.72; m.color[2] = 0.73; }  // SIMPLE_ALBEDO
+    if (parts & ND_PART_REFLECTIONS) m.reflectiveness = 0.0;
+    if (parts & ND_PART_REFRACTION) m.transparency = 0.0;
+    if (parts & ND_PART_DISPERSION) m.abbe = 0.0;
+    return m;
+}
 // mix_material(a, b, h) = mix(b, a, h) per field (template.wgsl:1980-2001)
 inline Mat mix_mat(const Mat& a, const Mat& b, double h) {
     Mat r;
@@ -158,6 +167,8 @@ inline Mat mix_mat(const Mat& a, const Mat& b, double h) {
     r.reflectiveness = b.reflectiveness * (1 - h) + a.reflectiveness * h;
     r.ior = b.ior * (1 - h) + a.ior * h;
     r.roughness = b.roughness * (1 - h) + a.roughness * h;
+    r.abbe = b.abbe * (1 - h) + a.abbe * h;
+    r.inner_max_steps = b.inner_max_steps * (1 - h) + a.inner_max_steps * h;
     return r;
 }
 // sample_gradient (template.wgsl:2003-2023)
@@ -239,10 +250,64 @@ template <class T> void light_sample(const nd_light& L, double max_dist, V3<T> p
     }
 }
 
-// shade_local for opaque, film-free, subsurface-free materials (template.wgsl:3222-3331)
+// visibility_march (template.wgsl:2282-2342) with shadow_factor's parameters (:2343-2375):
+// an opaque occluder blocks the light; with transparency in the scene, a transparent one
+// tints it by mix(1, colour, a) * a per crossing while the march steps through its inside.
+template <class T, class P>
+Spec<T> visibility_march(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& sp, V3<T> origin, V3<T> dir,
+                         T light_dist, T k, int steps, int* evals) {
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    const bool opaque_only = !(u.has_transparency > 0.5f && !(parts & ND_PART_REFRACTION));
+    T t = T(0.004);  // SHADOW_BIAS as t0
+    T res = T(1);
+    Spec<T> tint = splat(T(1));
+    bool in_solid = false;
+    T step_floor = T(0.004);  // SHADOW_STEP_FLOOR
+    for (int i = 0; i < steps; ++i) {
+        if (t >= light_dist) break;
+        V3<T> p = origin + dir * t;
+        if (!in_solid && s.accel) {
+            // accelerator skip (template.wgsl:2297-2303): only where it cannot lower res
+            T skip = accel_skip(s.accel.get(), p);
+            if (skip > T(0.001) && k * skip >= res * t) {
+                t = t + skip;
+                continue;
+            }
+        }
+        T d = probed_de(s, p);
+        if (evals) (*evals)++;
+        if (in_solid) {
+            if (d > T(0.001) * t + T(0.0005)) in_solid = false;  // SHADOW_EXIT_SLOPE, _BIAS
+            t = t + tmax(abs(d), step_floor);
+            step_floor = step_floor * T(1.15);  // SHADOW_FLOOR_GROWTH
+            continue;
+        }
+        if (d < T(0.0005)) {  // SHADOW_HIT_EPS
+            if (opaque_only) return splat(T(0));
+            const Mat m = hit_material(u, s, p);  // unmasked, as the reference reads it here
+            if (evals) (*evals)++;
+            const T a = tclamp(T(m.transparency), T(0), T(1));
+            if (a < T(0.01)) return splat(T(0));  // SHADOW_OPAQUE_ALPHA
+            tint = tint * (sp.from_rgb(T(1) + (T(m.color[0]) - T(1)) * a, T(1) + (T(m.color[1]) - T(1)) * a,
+                                       T(1) + (T(m.color[2]) - T(1)) * a) * a);
+            if (smax(tint) < T(0.004)) return splat(T(0));  // SHADOW_TINT_CUT
+            in_solid = true;
+            t = t + step_floor;
+            continue;
+        }
+        res = tmin(res, k * d / t);
+        t = t + tmax(d, T(0.001));  // SHADOW_MIN_STEP
+    }
+    return tint * tclamp(res, T(0), T(1));
+}
+
+// shade_local for film- and subsurface-free materials (template.wgsl:3222-3331), with the
+// Monte Carlo light pick: when several shadowed lights reach the point and mc_light_pick is
+// set, one of them is chosen in proportion to its unshadowed weight and scaled by
+// total / weight.
 template <class T, class P>
 Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& sp, V3<T> pos, V3<T> dir, V3<T> n,
-                    const Mat& m, const Spec<T>& ambient, bool use_shadows, int* evals) {
+                    const Mat& m, const Spec<T>& ambient, bool use_shadows, Seed seed, int* evals) {
     V3<T> view = -dir;
     T spec_power = T(4.0) + (T(128.0) - T(4.0)) * tclamp(T(m.glossiness), T(0), T(1));
     Spec<T> albedo = sp.from_rgb(T(m.color[0]), T(m.color[1]), T(m.color[2]));
@@ -250,24 +315,56 @@ Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& 
     const uint32_t parts = (uint32_t)u.debug_parts;
     int lc = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min(i32(u.light_count), ND_MAX_LIGHTS);
     bool shadows_on = use_shadows && !(parts & ND_PART_SHADOWS);
+    const bool pick_one = shadows_on && u.mc_enabled > 0.5f && u.mc_light_pick > 0.5f && lc > 1;
+    T pick_w[ND_MAX_LIGHTS] = {};
+    T pick_total = T(0);
+    auto terms = [&](const nd_light& L, V3<T>* ld, V3<T>* col, T* dist, T* ndotl, T* spec) {
+        light_sample(L, u.max_dist, pos, ld, col, dist);
+        *ndotl = tmax(dot(n, *ld), T(0));
+        V3<T> hd = normalize(*ld + view);
+        *spec = (parts & ND_PART_SPECULAR) ? T(0) : pow(tmax(dot(n, hd), T(0)), spec_power) * T(m.glossiness);
+    };
+    auto shadow_of = [&](const nd_light& L, V3<T> ld, T dist) {
+        double k = L.shadow_softness + (512.0 - L.shadow_softness) * L.hard_shadows;
+        int steps = std::min(i32(u.max_steps), 40);
+        return visibility_march(u, s, sp, pos + n * T(0.004), ld, dist, T(k), steps, evals);
+    };
     for (int i = 0; i < lc; ++i) {
         const nd_light& L = u.lights[i];
-        V3<T> ld, col; T dist;
-        light_sample(L, u.max_dist, pos, &ld, &col, &dist);
+        V3<T> ld, col; T dist, ndotl, spec;
+        terms(L, &ld, &col, &dist, &ndotl, &spec);
         if (tmax(col.x, tmax(col.y, col.z)) < T(1e-5)) continue;
         Spec<T> ls = sp.from_rgb(col.x, col.y, col.z);
-        T ndotl = tmax(dot(n, ld), T(0));
-        V3<T> hd = normalize(ld + view);
-        T spec = (parts & ND_PART_SPECULAR) ? T(0) : pow(tmax(dot(n, hd), T(0)), spec_power) * T(m.glossiness);
         if (ndotl <= T(0)) continue;
-        T shadow = T(1);
+        if (pick_one && L.cast_shadows > 0.5f) {
+            pick_w[i] = (col.x + col.y + col.z) * (ndotl + spec);
+            pick_total = pick_total + pick_w[i];
+            continue;
+        }
+        Spec<T> shadow = splat(T(1));
         if (shadows_on && L.cast_shadows > 0.5f) {
-            double k = L.shadow_softness + (512.0 - L.shadow_softness) * L.hard_shadows;
-            int steps = std::min(i32(u.max_steps), 40);
-            shadow = visibility_march_opaque(s, pos + n * T(0.004), ld, dist, T(k), steps, evals);
-            if (shadow <= T(0)) continue;
+            shadow = shadow_of(L, ld, dist);
+            if (smax(shadow) <= T(0)) continue;
         }
         Spec<T> reaching = ls * shadow;
+        diffuse = diffuse + reaching * ndotl;
+        specular = specular + reaching * spec;
+    }
+    if (pick_total > T(0)) {
+        const T at = T(hash12w(seed.x + 47.23f, seed.y + 113.91f)) * pick_total;  // LIGHT_PICK_SEED_OFFSET
+        int pick = 0;
+        T running = T(0);
+        for (int i = 0; i < lc; ++i) {
+            if (pick_w[i] <= T(0)) continue;
+            pick = i;
+            running = running + pick_w[i];
+            if (running > at) break;
+        }
+        const nd_light& L = u.lights[pick];
+        V3<T> ld, col; T dist, ndotl, spec;
+        terms(L, &ld, &col, &dist, &ndotl, &spec);
+        const T scale = pick_total / pick_w[pick];
+        Spec<T> reaching = sp.from_rgb(col.x, col.y, col.z) * shadow_of(L, ld, dist) * scale;
         diffuse = diffuse + reaching * ndotl;
         specular = specular + reaching * spec;
     }
@@ -283,12 +380,85 @@ template <class T> T fresnel_schlick(T cos_theta, T ior) {
     return r0 + (T(1) - r0) * c2 * c2 * c;
 }
 
+// cauchy_ior (template.wgsl:3403-3409): n(lambda) = n_d + B (1 / lambda^2 - 1 / lambda_d^2),
+// B = (n_d - 1) / (V (1 / lambda_F^2 - 1 / lambda_C^2)), lambda in micrometres
+template <class T> T cauchy_ior(T ior_d, T abbe, T lambda) {
+    T b = (ior_d - T(1)) / (tmax(abbe, T(1)) * T(1.9099));
+    return ior_d + b * (T(1) / (lambda * lambda) - T(2.89626));
+}
+
+// refract (WGSL builtin) and refract_safe (template.wgsl:3387-3394): the normal faces the
+// incoming ray; total internal reflection returns the mirror direction
+template <class T> V3<T> refract3(V3<T> i, V3<T> n, T eta) {
+    T c = dot(n, i);
+    T k = T(1) - eta * eta * (T(1) - c * c);
+    if (k < T(0)) return V3<T>();
+    return i * eta - n * (eta * c + sqrt(k));
+}
+template <class T> V3<T> refract_safe(V3<T> dir, V3<T> raw_n, T eta, bool* tir) {
+    V3<T> n = dot(dir, raw_n) > T(0) ? -raw_n : raw_n;
+    V3<T> r = refract3(dir, n, eta);
+    *tir = !(dot(r, r) > T(1e-4));  // REFRACT_TIR_EPSILON
+    return *tir ? reflect3(dir, n) : r;
+}
+
+// dispersion_angles2 / dispersion_weights (template.wgsl:3521-3548): each wavelength's
+// refracted direction against the hero's, as 2 (1 - cos), turned into Gaussian weights in
+// angle that sum to 8
+template <class T>
+Spec<T> dispersion_weights(const Spectral<T>& sp, V3<T> in_dir, V3<T> n, const Mat& m, bool inside, V3<T> hero_dir,
+                           T sigma) {
+    const T ior_d = tmax(T(m.ior), T(1));
+    Spec<T> g;
+    T total = T(0);
+    const T kk = T(-0.5) / tmax(sigma * sigma, T(1e-12));
+    for (int k = 0; k < 8; ++k) {
+        T ior_k = tmax(cauchy_ior(ior_d, T(m.abbe), sp.lambda[k] * T(0.001)), T(1));
+        T eta_k = inside ? ior_k : T(1) / ior_k;
+        bool tir;
+        V3<T> d_k = refract_safe(in_dir, n, eta_k, &tir);
+        T a = T(2) * (T(1) - tclamp(dot(hero_dir, d_k), T(-1), T(1)));
+        g.v[k] = exp(a * kk);
+        total = total + g.v[k];
+    }
+    return g * (T(8) / tmax(total, T(1e-6)));  // EPSILON_FINE
+}
+
+// march_inside (template.wgsl:3015-3046): from a point just inside a solid, steps of
+// max(|DE|, floor) with a floor growing by 1 + 6 / steps, until the DE shows the ray has
+// left (DE > 0.0005 t + 0.00005, from the third sample on) or the step budget ends
+template <class T, class P>
+MarchResult<T> march_inside(const Scene<P>& s, V3<T> org, V3<T> dir, int max_inner_steps, T max_t, int* evals) {
+    MarchResult<T> r{};
+    T t = T(0);
+    const int steps = std::max(max_inner_steps, 1);
+    const T growth = T(1) + T(6) / T((double)steps);
+    T step_floor = T(0.004);  // max(INSIDE_STEP_BASE, INSIDE_MIN_STEP)
+    r.hit = false;
+    for (int i = 0; i < steps; ++i) {
+        T d = scene_de(s, org + dir * t);
+        if (evals) (*evals)++;
+        if (i > 1 && d > T(0.0005) * t + T(0.00005)) {
+            r.hit = true;
+            r.dist = t;
+            return r;
+        }
+        t = t + tmax(abs(d), step_floor);
+        step_floor = step_floor * growth;
+        if (t > max_t) break;
+    }
+    r.dist = t;
+    return r;
+}
+
 // ----------------------------------------------------------------------------- trace_path
 struct PathResult { double rgb[3]; double depth; double normal[3]; int evals; int vertices; };
 
 // trace_path (template.wgsl:4126-4137) with path_begin / path_extend / path_shade
-// (:3819-4124) for an opaque scene: no fog, particles, photons, dispersion, film,
-// subsurface, transmission or hit cache.
+// (:3819-4124): opaque and transmissive materials (Schlick reflection / refraction through
+// march_inside), Cauchy dispersion on the picked wavelength with the Gaussian angle
+// weights or the one-wavelength collapse, and the light pick. No fog, particles, photons,
+// film, subsurface, approximations or hit cache.
 template <class T, class P>
 PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, Seed seed) {
     PathResult out{};
@@ -298,14 +468,19 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
     Spectral<T> sp{T(hero_f)};
     Spec<T> radiance = splat(T(0)), throughput = splat(T(1));
     V3<T> pos = org, d = dir;
-    bool reflected = false;
+    bool reflected = false, in_solid = false;
+    int inner_steps = 64;  // PATH_DEFAULT_INNER_STEPS
+    T lambda = T(0);        // picked wavelength in micrometres, 0 before the first dispersive hit
     const int max_depth = std::min(std::max(i32(u.max_reflection_bounces) + 8, 1), 16);
     const uint32_t parts = (uint32_t)u.debug_parts;
+    // radians(0.5 * photon_dispersion_soft)
+    const T soft = T((double)(0.5f * u.photon_dispersion_soft) * 0.017453292519943295);
     for (int depth = 0; depth < max_depth; ++depth) {
         out.vertices = depth + 1;
         Seed sd = seed_add(st_seed, (float)depth * 17.77f + 3.1f, (float)depth * 41.13f + 7.7f);
-        MarchResult<T> m = march_to(s, mc, pos, d, T(u.max_dist));
-        out.evals += m.evals;
+        MarchResult<T> m = in_solid ? march_inside(s, pos, d, inner_steps, T(u.max_dist), &out.evals)
+                                    : march_to(s, mc, pos, d, T(u.max_dist));
+        if (!in_solid) out.evals += m.evals;
         if (!m.hit) {
             V3<T> sky = sky_color(u, d);
             radiance = radiance + throughput * sp.from_rgb(sky.x, sky.y, sky.z);
@@ -315,14 +490,27 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
         V3<T> hp = pos + d * m.dist;
         V3<T> n = normal_tetra(s, hp, T(mc.eps_c) * m.dist + T(mc.eps_f));
         out.evals += 4;
-        Mat mat = hit_material(u, s, hp);
+        Mat mat = apply_part_mask(hit_material(u, s, hp), parts);
         out.evals += 1;
         if (depth == 0) {
             out.depth = (double)val(m.dist);
             out.normal[0] = (double)val(n.x); out.normal[1] = (double)val(n.y); out.normal[2] = (double)val(n.z);
         }
-        // scatter_at, allow_diffuse = false (template.wgsl:3552-3616), opaque material
+        Spec<T> disp_w = splat(T(1));
+        if (lambda == T(0) && mat.abbe > 0.5) {  // MIN_ABBE
+            const float hx = hash12w(sd.x + 3.71f, sd.y + 8.13f);  // DISPERSION_PICK_SEED_OFFSET
+            const int pick = std::min((int)(hx * 8.0f), 7);
+            lambda = sp.lambda[pick] * T(0.001);
+            if (soft <= T(0)) {
+                disp_w = splat(T(0));
+                disp_w.v[pick] = T(8);  // spec_collapse_mask
+            }
+        }
+        // scatter_at with allow_diffuse = false (template.wgsl:3552-3616)
         V3<T> nf = dot(d, n) > T(0) ? -n : n;
+        T ior = tmax(T(mat.ior), T(1));
+        if (lambda > T(0) && mat.abbe > 0.5) ior = tmax(cauchy_ior(ior, T(mat.abbe), lambda), T(1));
+        const T eta = in_solid ? ior : T(1) / ior;
         T rough = tclamp(T(mat.roughness), T(0), T(1));
         T alpha = rough * rough;
         V3<T> nr = nf;
@@ -330,13 +518,15 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
             V3<T> h = cosine_sample(nf, seed_add(sd, 5.19f, 12.73f));
             nr = normalize(nf + (h - nf) * alpha);
         }
-        T ior = tmax(T(mat.ior), T(1));
         T f = fresnel_schlick(dot(d, nr), ior);
+        T transparency = tclamp(T(mat.transparency), T(0), T(1));
         T refl = tclamp(T(mat.reflectiveness), T(0), T(1));
         T p_reflect = tclamp(refl + (T(1) - refl) * f, T(0), T(1));
-        T p_diffuse = T(1) - p_reflect;  // transparency 0: p_transmit = 0
-        T local_share = p_diffuse;
-        if (local_share > T(0.001)) {
+        T rest = T(1) - p_reflect;
+        T p_transmit = rest * transparency;
+        T local_share = rest * (T(1) - transparency);
+        T p_cont = p_reflect + p_transmit;
+        if (local_share > T(0.001)) {  // MIN_LOCAL_SHARE
             Spec<T> ambient = splat(T(0.15));
             bool mc_indirect = u.mc_enabled > 0.5f && !(parts & ND_PART_MC_INDIRECT);
             if (depth == 0 && mc_indirect) {
@@ -352,9 +542,9 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
                 } else {
                     V3<T> bp = io + idir * b.dist;
                     V3<T> bn = normal_tetra(s, bp, T(mc.eps_c) * b.dist + T(mc.eps_f));
-                    Mat bm = hit_material(u, s, bp);
+                    Mat bm = apply_part_mask(hit_material(u, s, bp), parts);
                     out.evals += 5;
-                    ambient = shade_local(u, s, sp, bp, idir, bn, bm, splat(T(0.15)), false, &out.evals);
+                    ambient = shade_local(u, s, sp, bp, idir, bn, bm, splat(T(0.15)), false, is, &out.evals);
                 }
             } else if (!reflected) {
                 // calc_ao (template.wgsl:1825-1841)
@@ -371,18 +561,36 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
                     ambient = ambient * tclamp(T(1) - T(2) * occ / radius, T(0), T(1));
                 }
             }
-            Spec<T> lit = shade_local(u, s, sp, hp, d, n, mat, ambient, !reflected, &out.evals);
+            Spec<T> lit = shade_local(u, s, sp, hp, d, n, mat, ambient, !reflected, sd, &out.evals);
             radiance = radiance + throughput * lit * local_share;
         }
-        if (!(p_reflect > T(0))) break;  // r = hash * p_reflect < p_reflect fails: absorbed
-        if (parts & ND_PART_REFLECTIONS) break;
-        // reflect: weight = mix(color, 1, f) * p_cont (= p_reflect)
-        T w0 = T(mat.color[0]) + (T(1) - T(mat.color[0])) * f;
-        T w1 = T(mat.color[1]) + (T(1) - T(mat.color[1])) * f;
-        T w2 = T(mat.color[2]) + (T(1) - T(mat.color[2])) * f;
-        throughput = throughput * (sp.from_rgb(w0, w1, w2) * p_reflect);
-        reflected = true;
-        V3<T> nd = reflect3(d, nr);
+        // the continuation: r = hash * p_cont picks reflection below p_reflect, else refraction
+        const T r = T(hash12w(sd.x, sd.y)) * p_cont;
+        int kind;  // 0 absorbed, 1 reflect, 2 transmit
+        V3<T> nd;
+        Spec<T> w;
+        if (r < p_reflect) {
+            nd = reflect3(d, nr);
+            w = sp.from_rgb(T(mat.color[0]) + (T(1) - T(mat.color[0])) * f, T(mat.color[1]) + (T(1) - T(mat.color[1])) * f,
+                            T(mat.color[2]) + (T(1) - T(mat.color[2])) * f) * p_cont;
+            kind = 1;
+        } else if (r < p_reflect + p_transmit) {
+            bool tir;
+            nd = refract_safe(d, nr, eta, &tir);
+            if (soft > T(0) && lambda > T(0) && mat.abbe > 0.5)
+                disp_w = dispersion_weights(sp, d, nr, mat, in_solid, nd, soft);
+            w = sp.from_rgb(T(mat.color[0]), T(mat.color[1]), T(mat.color[2])) * p_cont;
+            kind = tir ? 1 : 2;
+        } else {
+            break;  // SCATTER_ABSORBED
+        }
+        if (kind == 1 && !in_solid && (parts & ND_PART_REFLECTIONS)) break;
+        throughput = throughput * (disp_w * w);
+        reflected = reflected || kind == 1;
+        if (kind == 2) {
+            in_solid = !in_solid;
+            if (in_solid) inner_steps = std::min(std::max(i32((float)mat.inner_max_steps), 4), 512);
+        }
         T bias = tmax(T(0.003), T(2) * (T(mc.eps_c) * m.dist + T(mc.eps_f)));
         pos = hp + nd * bias;
         d = nd;
