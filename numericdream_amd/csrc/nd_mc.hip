@@ -35,6 +35,7 @@ namespace nd {
 // ----------------------------------------------------------------------------- scene
 struct McStop {
     float color[3], position, glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
+    float film_thickness, film_ior, film_strength, film_angle_scale, film_perturb, film_perturb_scale;
 };
 struct McInst {
     int color_count, trap_repeat;
@@ -53,7 +54,7 @@ struct McScene {
     McLight lights[8];
     int light_count;
     int shadows_on, specular_on, ao_on, reflections_on, indirect_on, sky_on;
-    int strips_on, refraction_on, dispersion_on;  // apply_part_mask
+    int strips_on, refraction_on, dispersion_on, iridescence_on;  // apply_part_mask
     int transp;        // scene_has_transparency: shadow rays cross transparent occluders
     int transmissive;  // some visible stop has transparency: paths can enter solids
     int light_pick;    // mc_enabled and mc_light_pick
@@ -182,13 +183,18 @@ NDD V3<float> sky_color(const McScene& S, const V3<float>& dir) {
     return mk3(fmaxf(c.x * S.sky_intensity, 0.f), fmaxf(c.y * S.sky_intensity, 0.f), fmaxf(c.z * S.sky_intensity, 0.f));
 }
 
-struct Mat { float color[3], glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps; };
+struct Mat {
+    float color[3], glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
+    float film_thickness, film_ior, film_strength, film_angle_scale, film_perturb, film_perturb_scale;
+};
 // default_material (template.wgsl:1965-1978)
 NDD Mat mat_default() {
     Mat m;
     m.color[0] = 0.8f; m.color[1] = 0.8f; m.color[2] = 0.85f;
     m.glossiness = 0.3f; m.transparency = 0.f; m.reflectiveness = 0.f; m.ior = 1.5f; m.roughness = 0.f;
     m.abbe = 0.f; m.inner_max_steps = 64.f;
+    m.film_thickness = 400.f; m.film_ior = 1.8f; m.film_strength = 0.f; m.film_angle_scale = 1.f;
+    m.film_perturb = 0.f; m.film_perturb_scale = 0.2f;
     return m;
 }
 NDD Mat mat_stop(const McStop& s) {
@@ -196,6 +202,8 @@ NDD Mat mat_stop(const McStop& s) {
     for (int k = 0; k < 3; ++k) m.color[k] = s.color[k];
     m.glossiness = s.glossiness; m.transparency = s.transparency; m.reflectiveness = s.reflectiveness; m.ior = s.ior;
     m.roughness = s.roughness; m.abbe = s.abbe; m.inner_max_steps = s.inner_max_steps;
+    m.film_thickness = s.film_thickness; m.film_ior = s.film_ior; m.film_strength = s.film_strength;
+    m.film_angle_scale = s.film_angle_scale; m.film_perturb = s.film_perturb; m.film_perturb_scale = s.film_perturb_scale;
     return m;
 }
 // apply_part_mask (template.wgsl:394-402) for the parts this path reads
@@ -204,6 +212,7 @@ NDD Mat part_mask(Mat m, const McScene& S) {
     if (!S.reflections_on) m.reflectiveness = 0.f;
     if (!S.refraction_on) m.transparency = 0.f;
     if (!S.dispersion_on) m.abbe = 0.f;
+    if (!S.iridescence_on) m.film_strength = 0.f;
     return m;
 }
 // mix_material(a, b, h) = mix(b, a, h) (template.wgsl:1980-2001)
@@ -217,6 +226,12 @@ NDD Mat mat_mix(const Mat& a, const Mat& b, float h) {
     r.transparency = b.transparency * (1.f - h) + a.transparency * h;
     r.abbe = b.abbe * (1.f - h) + a.abbe * h;
     r.inner_max_steps = b.inner_max_steps * (1.f - h) + a.inner_max_steps * h;
+    r.film_thickness = b.film_thickness * (1.f - h) + a.film_thickness * h;
+    r.film_ior = b.film_ior * (1.f - h) + a.film_ior * h;
+    r.film_strength = b.film_strength * (1.f - h) + a.film_strength * h;
+    r.film_angle_scale = b.film_angle_scale * (1.f - h) + a.film_angle_scale * h;
+    r.film_perturb = b.film_perturb * (1.f - h) + a.film_perturb * h;
+    r.film_perturb_scale = b.film_perturb_scale * (1.f - h) + a.film_perturb_scale * h;
     return r;
 }
 // sample_gradient + orbit_trap_to_strip (template.wgsl:2003-2035)
@@ -274,6 +289,69 @@ NDD V3<float> normal_tetra(const DevScene& sc, const V3<float>& p, float e) {
         g = mk3(g.x + K[i][0] * d, g.y + K[i][1] * d, g.z + K[i][2] * d);
     }
     return vnorm3(g);
+}
+// hash33 (template.wgsl:3439-3450): three values in [0, 1) per integer lattice cell
+NDD void hash33(float px, float py, float pz, float* out) {
+    uint32_t v0 = (uint32_t)(int32_t)px, v1 = (uint32_t)(int32_t)py, v2 = (uint32_t)(int32_t)pz;
+    const uint32_t M = 1664525u, A = 1013904223u;
+    v0 = v0 * M + A; v1 = v1 * M + A; v2 = v2 * M + A;
+    v0 += v1 * v2; v1 += v2 * v0; v2 += v0 * v1;
+    v0 ^= v0 >> 16; v1 ^= v1 >> 16; v2 ^= v2 >> 16;
+    v0 += v1 * v2; v1 += v2 * v0; v2 += v0 * v1;
+    out[0] = (float)v0 * 2.3283064365386963e-10f;
+    out[1] = (float)v1 * 2.3283064365386963e-10f;
+    out[2] = (float)v2 * 2.3283064365386963e-10f;
+}
+// film_normal with film_noise (template.wgsl:3452-3476): the normal bent by smooth value
+// noise of the position, for the thin-film view angle
+NDD V3<float> film_normal(const V3<float>& n, const V3<float>& pos, const Mat& m) {
+    const float amount = fclamp(m.film_perturb, 0.f, 1.f);
+    if (amount <= 0.001f) return n;  // MATERIAL_EPSILON
+    const float is = 1.f / fmaxf(m.film_perturb_scale, 1e-4f);
+    const float p[3] = {pos.x * is, pos.y * is, pos.z * is};
+    float i[3], w[3];
+    for (int k = 0; k < 3; ++k) {
+        i[k] = floorf(p[k]);
+        const float f = p[k] - i[k];
+        w[k] = f * f * (3.f - 2.f * f);
+    }
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        float h[3];
+        hash33(i[0] + (float)dx, i[1] + (float)dy, i[2] + (float)dz, h);
+        const float wt = (dx ? w[0] : 1.f - w[0]) * (dy ? w[1] : 1.f - w[1]) * (dz ? w[2] : 1.f - w[2]);
+        for (int k = 0; k < 3; ++k) acc[k] += h[k] * wt;
+    }
+    return vnorm3(mk3(n.x + (acc[0] * 2.f - 1.f) * amount, n.y + (acc[1] * 2.f - 1.f) * amount,
+                      n.z + (acc[2] * 2.f - 1.f) * amount));
+}
+// thin_film_tint (template.wgsl:3478-3519): Airy reflectance of the film over its base per
+// wavelength (nm), s and p averaged, divided by its peak over the phase and blended with 1
+NDD Sp8 thin_film_tint(float hero, float cos_view, const Mat& m) {
+    const float strength = fclamp(m.film_strength, 0.f, 1.f);
+    const float theta = acosf(fclamp(cos_view, 0.f, 1.f)) * fmaxf(m.film_angle_scale, 0.f);
+    const float c1 = cosf(fminf(theta, 1.5607f));
+    const float s1 = sqrtf(fmaxf(1.f - c1 * c1, 0.f));
+    const float n2 = fmaxf(m.film_ior, 1.f), n3 = fmaxf(m.ior, 1.f);
+    const float s2 = s1 / n2, c2 = sqrtf(fmaxf(1.f - s2 * s2, 0.f));
+    const float s3 = s1 / n3, c3 = sqrtf(fmaxf(1.f - s3 * s3, 0.f));
+    const float r12s = (c1 - n2 * c2) / (c1 + n2 * c2), r12p = (n2 * c1 - c2) / (n2 * c1 + c2);
+    const float r23s = (n2 * c2 - n3 * c3) / (n2 * c2 + n3 * c3), r23p = (n3 * c2 - n2 * c3) / (n3 * c2 + n2 * c3);
+    auto peak_of = [](float a, float b) { const float x = (fabsf(a) + fabsf(b)) / (1.f + fabsf(a * b)); return x * x; };
+    const float peak = 0.5f * (peak_of(r12s, r23s) + peak_of(r12p, r23p));
+    if (peak < 1e-5f) return sp_splat(1.f);  // FILM_MIN_PEAK
+    auto airy = [](float a, float b, float cd) { const float x = 2.f * a * b * cd; return (a * a + b * b + x) / (1.f + a * a * b * b + x); };
+    const float path = 4.f * 3.14159265358979f * n2 * fmaxf(m.film_thickness, 0.f) * c2;
+    float lam[8];
+    lambdas_of(hero, lam);
+    Sp8 r;
+    for (int k = 0; k < 8; ++k) {
+        const float cd = cosf(path / lam[k]);
+        const float R = 0.5f * (airy(r12s, r23s, cd) + airy(r12p, r23p, cd));
+        r.v[k] = 1.f + (R / peak - 1.f) * strength;
+    }
+    return r;
 }
 // cosine_hemisphere_from (template.wgsl:3341-3350)
 NDD V3<float> cosine_sample(const V3<float>& n, float sx, float sy) {
@@ -548,7 +626,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1
     persistent_march<EXT, false>(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
 }
 
-// shade_local (template.wgsl:3222-3331) for film- and subsurface-free materials.
+// shade_local (template.wgsl:3222-3331) for subsurface-free materials: the albedo and the
+// specular term carry the thin-film tint (film).
 // With defer, each shadowed light's contribution is queued as a shadow ray with weight
 // w_scale * light * (albedo n.l + specular); otherwise it is added unshadowed. With
 // pick_one (several lights, mc_light_pick), the shadowed lights are not queued each: one is
@@ -567,12 +646,20 @@ NDD void queue_shadow(const McArgs& a, uint32_t path, const V3<float>& pos, cons
     a.P.sk[slot] = k;
     for (int q = 0; q < 8; ++q) a.P.sw[8 * slot + q] = c.v[q] * w_scale.v[q] * scale;
 }
+// film_albedo: the material's albedo times its thin-film tint at this view (g_shade_albedo).
+NDD Sp8 film_of(float hero, const V3<float>& pos, const V3<float>& dir, const V3<float>& n, const Mat& m) {
+    if (!(m.film_strength > 0.001f)) return sp_splat(1.f);  // MATERIAL_EPSILON
+    const V3<float> fn = film_normal(n, pos, m);
+    return thin_film_tint(hero, -(dir.x * fn.x + dir.y * fn.y + dir.z * fn.z), m);
+}
 NDD Sp8 shade_lights(const McArgs& a, const Basis& B, const V3<float>& pos, const V3<float>& dir, const V3<float>& n,
-                     const Mat& m, const Sp8& ambient, bool shadows, const Sp8& w_scale, uint32_t path, float sx, float sy) {
+                     const Mat& m, const Sp8& film, const Sp8& ambient, bool shadows, const Sp8& w_scale, uint32_t path,
+                     float sx, float sy) {
     const McScene& S = *a.S;
     const V3<float> view = mk3(-dir.x, -dir.y, -dir.z);
     const float spec_power = 4.f + 124.f * fclamp(m.glossiness, 0.f, 1.f);
-    const Sp8 albedo = from_rgb(B, m.color[0], m.color[1], m.color[2]);
+    Sp8 albedo = from_rgb(B, m.color[0], m.color[1], m.color[2]);
+    for (int k = 0; k < 8; ++k) albedo.v[k] *= film.v[k];
     const bool pick_one = shadows && S.light_pick && S.light_count > 1;
     // light i's direction, colour, distance, n.l and specular term (light_terms)
     auto terms = [&](int i, V3<float>* ld, V3<float>* col, float* dist, float* ndotl, float* spec) {
@@ -597,7 +684,7 @@ NDD Sp8 shade_lights(const McArgs& a, const Basis& B, const V3<float>& pos, cons
         }
         const Sp8 ls = from_rgb(B, col.x, col.y, col.z);
         Sp8 c;
-        for (int k = 0; k < 8; ++k) c.v[k] = ls.v[k] * (albedo.v[k] * ndotl + spec);
+        for (int k = 0; k < 8; ++k) c.v[k] = ls.v[k] * (albedo.v[k] * ndotl + spec * film.v[k]);
         if (shadows && L.cast && S.shadow_steps > 0) {
             queue_shadow(a, path, pos, n, ld, dist, L.k, c, w_scale, 1.f);
         } else {
@@ -627,7 +714,7 @@ NDD Sp8 shade_lights(const McArgs& a, const Basis& B, const V3<float>& pos, cons
         terms(pick, &ld, &col, &dist, &ndotl, &spec);
         const Sp8 ls = from_rgb(B, col.x, col.y, col.z);
         Sp8 c;
-        for (int k = 0; k < 8; ++k) c.v[k] = ls.v[k] * (albedo.v[k] * ndotl + spec);
+        for (int k = 0; k < 8; ++k) c.v[k] = ls.v[k] * (albedo.v[k] * ndotl + spec * film.v[k]);
         const float scale = pick_total / pick_w;
         if (S.shadow_steps > 0) {
             queue_shadow(a, path, pos, n, ld, dist, S.lights[pick].k, c, w_scale, scale);
@@ -753,9 +840,12 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                     Sp8 ambient = sp_splat(0.15f);
                     Sp8 wscale;
                     for (int k = 0; k < 8; ++k) wscale.v[k] = thr.v[k] * local_share;
+                    const Sp8 film = film_of(hero, hp, dir, nrm, mat);
                     if (depth == 0 && S.indirect_on) {
                         // deferred indirect_light: weight throughput * local_share * albedo
-                        const Sp8 albedo = from_rgb(B, mat.color[0], mat.color[1], mat.color[2]);
+                        // (with its film tint, g_shade_albedo)
+                        Sp8 albedo = from_rgb(B, mat.color[0], mat.color[1], mat.color[2]);
+                        for (int k = 0; k < 8; ++k) albedo.v[k] *= film.v[k];
                         const V3<float> idir = cosine_sample(nrm, add_rn(sx, 13.13f), add_rn(sy, 71.71f));
                         uint32_t slot;
                         append(a.P.ctr + C_IND, true, &slot);
@@ -779,7 +869,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                         de += 5;
                         ambient = sp_splat(0.15f * fclamp(1.f - 2.f * occ / radius, 0.f, 1.f));
                     }
-                    const Sp8 lit = shade_lights(a, B, hp, dir, nrm, mat, ambient, !reflected && S.shadows_on, wscale, p, sx, sy);
+                    const Sp8 lit = shade_lights(a, B, hp, dir, nrm, mat, film, ambient, !reflected && S.shadows_on, wscale, p, sx, sy);
                     for (int k = 0; k < 8; ++k) rad.v[k] += wscale.v[k] * lit.v[k];
                 }
                 // the continuation: r = hash * p_cont picks reflection below p_reflect, else
@@ -789,10 +879,12 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                 V3<float> nd = dir;
                 float wrgb[3] = {0.f, 0.f, 0.f};
                 bool soft_w = false;
+                bool film_w = false;
                 if (r < p_reflect) {
                     const float dn = 2.f * dot3(nr, dir);
                     nd = mk3(dir.x - nr.x * dn, dir.y - nr.y * dn, dir.z - nr.z * dn);
                     for (int k = 0; k < 3; ++k) wrgb[k] = mat.color[k] + (1.f - mat.color[k]) * f;
+                    film_w = !in_solid;  // the reflection carries the film tint (scatter_at)
                     kind = 1;
                 } else if (r < p_reflect + p_transmit) {
                     bool tir;
@@ -802,7 +894,11 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                     kind = tir ? 1 : 2;
                 }
                 if (kind != 0 && !(kind == 1 && !in_solid && !S.reflections_on)) {
-                    const Sp8 w = from_rgb(B, wrgb[0], wrgb[1], wrgb[2]);
+                    Sp8 w = from_rgb(B, wrgb[0], wrgb[1], wrgb[2]);
+                    if (film_w) {
+                        const Sp8 fr = film_of(hero, hp, dir, nr, mat);
+                        for (int k = 0; k < 8; ++k) w.v[k] *= fr.v[k];
+                    }
                     Sp8 dw = sp_splat(1.f);
                     if (collapse >= 0)
                         for (int k = 0; k < 8; ++k) dw.v[k] = k == collapse ? 8.f : 0.f;  // spec_collapse_mask
@@ -883,7 +979,8 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
             const V3<float> bn = normal_tetra<EXT>(sc, bp, J.eps_c * t + J.eps_f);
             const Mat bm = part_mask(hit_material<EXT>(sc, S, bp), S);
             de += 5;
-            L = shade_lights(a, B, bp, dir, bn, bm, sp_splat(0.15f), false, sp_splat(0.f), p, 0.f, 0.f);
+            L = shade_lights(a, B, bp, dir, bn, bm, film_of(a.P.hero[p], bp, dir, bn, bm), sp_splat(0.15f), false,
+                             sp_splat(0.f), p, 0.f, 0.f);
         }
         for (int k = 0; k < 8; ++k) atomicAdd(&a.P.rad[8 * p + k], a.P.iw[8 * i + k] * L.v[k]);
     }
@@ -1074,6 +1171,9 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
             d.position = c.position; d.glossiness = c.glossiness; d.transparency = c.transparency;
             d.reflectiveness = c.reflectiveness; d.ior = c.ior; d.roughness = c.roughness;
             d.abbe = c.abbe; d.inner_max_steps = c.inner_max_steps;
+            d.film_thickness = c.film_thickness; d.film_ior = c.film_ior; d.film_strength = c.film_strength;
+            d.film_angle_scale = c.film_angle_scale; d.film_perturb = c.film_perturb;
+            d.film_perturb_scale = c.film_perturb_scale;
         }
     }
     S.light_count = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min((int)u.light_count, ND_MAX_LIGHTS);
@@ -1102,6 +1202,7 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     S.strips_on = !(parts & ND_PART_COLOR_STRIPS);
     S.refraction_on = !(parts & ND_PART_REFRACTION);
     S.dispersion_on = !(parts & ND_PART_DISPERSION);
+    S.iridescence_on = !(parts & ND_PART_IRIDESCENCE);
     S.transp = u.has_transparency > 0.5f && S.refraction_on;
     S.transmissive = 0;
     if (S.refraction_on)
@@ -1163,8 +1264,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     for (int i = 0; i < ND_MAX_INSTANCES; ++i)
         for (int s = 0; s < (int)u.instances[i].color_count && s < 16; ++s) {
             const nd_color_stop& cs = u.instances[i].colors[s];
-            if (cs.subsurface > 0.001f || cs.film_strength > 0.001f)
-                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d stop %d: subsurface scattering and thin film are not implemented", i, s);
+            if (cs.subsurface > 0.001f)
+                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d stop %d: subsurface scattering is not implemented", i, s);
         }
     if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
     nd_march_opts o{ND_NORMAL_TETRA, 0, 0, ND_KERNEL_PERSISTENT};

@@ -480,6 +480,13 @@ int ndo_helper(int op, const double* a, double* out) {
         case 29: { Spectral<double> sp(a[0]); Spec<double> v = sp.from_rgb(a[1], a[2], a[3]);
                    for (int k = 0; k < 8; ++k) out[k] = v.v[k];
                    return 0; }
+        case 30: { Spectral<double> sp(a[0]); Mat m = default_mat(); m.ior = a[2]; m.film_ior = a[3];
+                   m.film_thickness = a[4]; m.film_strength = a[5]; m.film_angle_scale = a[6];
+                   Spec<double> w = thin_film_tint(sp, a[1], m);
+                   for (int k = 0; k < 8; ++k) out[k] = w.v[k];
+                   return 0; }
+        case 31: { D3 r = film_noise(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 32: { float h[3]; hash33(a[0], a[1], a[2], h); out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; return 0; }
         case 28: { Spectral<double> sp(a[0]); Mat m = default_mat(); m.ior = a[7]; m.abbe = a[8];
                    Spec<double> w = dispersion_weights(sp, D3(a[1], a[2], a[3]), D3(a[4], a[5], a[6]), m, a[9] > 0.5,
                                                        D3(a[10], a[11], a[12]), a[13]);

@@ -144,17 +144,21 @@ template <class T> V3<T> sky_color(const nd_uniforms& u, V3<T> dir) {
 struct Mat {
     double color[3];
     double glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
+    double film_thickness, film_ior, film_strength, film_angle_scale, film_perturb, film_perturb_scale;
 };
 inline Mat stop_mat(const nd_color_stop& c) {
     return Mat{{c.color[0], c.color[1], c.color[2]}, c.glossiness, c.transparency, c.reflectiveness, c.ior, c.roughness,
-               c.abbe, c.inner_max_steps};
+               c.abbe, c.inner_max_steps, c.film_thickness, c.film_ior, c.film_strength, c.film_angle_scale,
+               c.film_perturb, c.film_perturb_scale};
 }
-inline Mat default_mat() { return Mat{{0.8, 0.8, 0.85}, 0.3, 0.0, 0.0, 1.5, 0.0, 0.0, 64.0}; }  // template.wgsl:1965-1978
+// default_material (template.wgsl:1965-1978)
+inline Mat default_mat() { return Mat{{0.8, 0.8, 0.85}, 0.3, 0.0, 0.0, 1.5, 0.0, 0.0, 64.0, 400.0, 1.8, 0.0, 1.0, 0.0, 0.2}; }
 // apply_part_mask (template.wgsl:394-402) for the parts this path reads
 inline Mat apply_part_mask(Mat m, uint32_t parts) {
     if (parts & ND_PART_COLOR_STRIPS) { m.color[0] = 0.72; m.color[1] = 0.72; m.color[2] = 0.73; }  // SIMPLE_ALBEDO
     if (parts & ND_PART_REFLECTIONS) m.reflectiveness = 0.0;
     if (parts & ND_PART_REFRACTION) m.transparency = 0.0;
+    if (parts & ND_PART_IRIDESCENCE) m.film_strength = 0.0;
     if (parts & ND_PART_DISPERSION) m.abbe = 0.0;
     return m;
 }
@@ -169,6 +173,12 @@ inline Mat mix_mat(const Mat& a, const Mat& b, double h) {
     r.roughness = b.roughness * (1 - h) + a.roughness * h;
     r.abbe = b.abbe * (1 - h) + a.abbe * h;
     r.inner_max_steps = b.inner_max_steps * (1 - h) + a.inner_max_steps * h;
+    r.film_thickness = b.film_thickness * (1 - h) + a.film_thickness * h;
+    r.film_ior = b.film_ior * (1 - h) + a.film_ior * h;
+    r.film_strength = b.film_strength * (1 - h) + a.film_strength * h;
+    r.film_angle_scale = b.film_angle_scale * (1 - h) + a.film_angle_scale * h;
+    r.film_perturb = b.film_perturb * (1 - h) + a.film_perturb * h;
+    r.film_perturb_scale = b.film_perturb_scale * (1 - h) + a.film_perturb_scale * h;
     return r;
 }
 // sample_gradient (template.wgsl:2003-2023)
@@ -301,16 +311,94 @@ Spec<T> visibility_march(const nd_uniforms& u, const Scene<P>& s, const Spectral
     return tint * tclamp(res, T(0), T(1));
 }
 
-// shade_local for film- and subsurface-free materials (template.wgsl:3222-3331), with the
+// ----------------------------------------------------------------------------- thin film
+// hash33 (template.wgsl:3439-3450): three values in [0, 1) per integer lattice cell, from
+// the cell's coordinates truncated to i32 and read as u32, in FP32 as the shader computes
+inline void hash33(double px, double py, double pz, float out[3]) {
+    uint32_t v[3] = {(uint32_t)(int32_t)px, (uint32_t)(int32_t)py, (uint32_t)(int32_t)pz};
+    const uint32_t M = 1664525u, A = 1013904223u;  // LCG_MULTIPLIER, LCG_INCREMENT
+    for (auto& x : v) x = x * M + A;
+    v[0] += v[1] * v[2];
+    v[1] += v[2] * v[0];
+    v[2] += v[0] * v[1];
+    for (auto& x : v) x ^= x >> 16;
+    v[0] += v[1] * v[2];
+    v[1] += v[2] * v[0];
+    v[2] += v[0] * v[1];
+    for (int k = 0; k < 3; ++k) out[k] = (float)v[k] * 2.3283064365386963e-10f;  // INV_U32_RANGE
+}
+// film_noise (template.wgsl:3452-3467): smoothstep-weighted trilinear value noise in [-1, 1]
+template <class T> V3<T> film_noise(V3<T> p) {
+    const T ix = floor(p.x), iy = floor(p.y), iz = floor(p.z);
+    const T fx = p.x - ix, fy = p.y - iy, fz = p.z - iz;
+    const T wx = fx * fx * (T(3) - T(2) * fx), wy = fy * fy * (T(3) - T(2) * fy), wz = fz * fz * (T(3) - T(2) * fz);
+    float h[2][2][2][3];
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                hash33((double)val(ix) + dx, (double)val(iy) + dy, (double)val(iz) + dz, h[dz][dy][dx]);
+    T r[3];
+    for (int k = 0; k < 3; ++k) {
+        auto mx = [&](int dy, int dz) { return T(h[dz][dy][0][k]) * (T(1) - wx) + T(h[dz][dy][1][k]) * wx; };
+        const T y0 = mx(0, 0) * (T(1) - wy) + mx(1, 0) * wy;
+        const T y1 = mx(0, 1) * (T(1) - wy) + mx(1, 1) * wy;
+        r[k] = (y0 * (T(1) - wz) + y1 * wz) * T(2) - T(1);
+    }
+    return V3<T>(r[0], r[1], r[2]);
+}
+// film_normal (template.wgsl:3469-3476)
+template <class T> V3<T> film_normal(V3<T> n, V3<T> pos, const Mat& m) {
+    const T amount = tclamp(T(m.film_perturb), T(0), T(1));
+    if (amount <= T(0.001)) return n;  // MATERIAL_EPSILON
+    const V3<T> bend = film_noise(pos / tmax(T(m.film_perturb_scale), T(1e-4)));
+    return normalize(n + bend * amount);
+}
+// thin_film_tint (template.wgsl:3478-3519): Airy reflectance of a film of index film_ior and
+// thickness film_thickness (nm) on a base of index ior, s and p averaged, per wavelength,
+// normalised by its peak and blended with 1 by film_strength
+template <class T> Spec<T> thin_film_tint(const Spectral<T>& sp, T cos_view, const Mat& m) {
+    const T strength = tclamp(T(m.film_strength), T(0), T(1));
+    const T theta = acos(tclamp(cos_view, T(0), T(1))) * tmax(T(m.film_angle_scale), T(0));
+    const T c1 = cos(tmin(theta, T(1.5607)));  // FILM_MAX_ANGLE
+    const T s1 = sqrt(tmax(T(1) - c1 * c1, T(0)));
+    const T n2 = tmax(T(m.film_ior), T(1)), n3 = tmax(T(m.ior), T(1));
+    const T s2 = s1 / n2, c2 = sqrt(tmax(T(1) - s2 * s2, T(0)));
+    const T s3 = s1 / n3, c3 = sqrt(tmax(T(1) - s3 * s3, T(0)));
+    const T r12s = (c1 - n2 * c2) / (c1 + n2 * c2);
+    const T r12p = (n2 * c1 - c2) / (n2 * c1 + c2);
+    const T r23s = (n2 * c2 - n3 * c3) / (n2 * c2 + n3 * c3);
+    const T r23p = (n3 * c2 - n2 * c3) / (n3 * c2 + n2 * c3);
+    auto peak_of = [](T a, T b) { T x = (abs(a) + abs(b)) / (T(1) + abs(a * b)); return x * x; };
+    const T peak = T(0.5) * (peak_of(r12s, r23s) + peak_of(r12p, r23p));
+    if (peak < T(1e-5)) return splat(T(1));  // FILM_MIN_PEAK
+    auto airy = [](T a, T b, T cd) { T x = T(2) * a * b * cd; return (a * a + b * b + x) / (T(1) + a * a * b * b + x); };
+    const T path = T(4) * T(3.14159265358979) * n2 * tmax(T(m.film_thickness), T(0)) * c2;
+    Spec<T> r;
+    for (int k = 0; k < 8; ++k) {
+        const T cd = cos(path / sp.lambda[k]);
+        const T R = T(0.5) * (airy(r12s, r23s, cd) + airy(r12p, r23p, cd));
+        r.v[k] = T(1) + (R / peak - T(1)) * strength;
+    }
+    return r;
+}
+
+// shade_local for subsurface-free materials (template.wgsl:3222-3331), with the
 // Monte Carlo light pick: when several shadowed lights reach the point and mc_light_pick is
 // set, one of them is chosen in proportion to its unshadowed weight and scaled by
 // total / weight.
 template <class T, class P>
 Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& sp, V3<T> pos, V3<T> dir, V3<T> n,
-                    const Mat& m, const Spec<T>& ambient, bool use_shadows, Seed seed, int* evals) {
+                    const Mat& m, const Spec<T>& ambient, bool use_shadows, Seed seed, int* evals,
+                    Spec<T>* albedo_out = nullptr) {
     V3<T> view = -dir;
     T spec_power = T(4.0) + (T(128.0) - T(4.0)) * tclamp(T(m.glossiness), T(0), T(1));
     Spec<T> albedo = sp.from_rgb(T(m.color[0]), T(m.color[1]), T(m.color[2]));
+    Spec<T> film = splat(T(1));
+    if (m.film_strength > 0.001) {  // MATERIAL_EPSILON
+        film = thin_film_tint(sp, dot(view, film_normal(n, pos, m)), m);
+        albedo = albedo * film;
+    }
+    if (albedo_out) *albedo_out = albedo;  // g_shade_albedo
     Spec<T> diffuse = splat(T(0)), specular = splat(T(0));
     const uint32_t parts = (uint32_t)u.debug_parts;
     int lc = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min(i32(u.light_count), ND_MAX_LIGHTS);
@@ -368,7 +456,7 @@ Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& 
         diffuse = diffuse + reaching * ndotl;
         specular = specular + reaching * spec;
     }
-    return albedo * (ambient + diffuse) + specular;
+    return albedo * (ambient + diffuse) + specular * film;
 }
 
 // fresnel_schlick (template.wgsl:3424-3430)
@@ -457,8 +545,8 @@ struct PathResult { double rgb[3]; double depth; double normal[3]; int evals; in
 // trace_path (template.wgsl:4126-4137) with path_begin / path_extend / path_shade
 // (:3819-4124): opaque and transmissive materials (Schlick reflection / refraction through
 // march_inside), Cauchy dispersion on the picked wavelength with the Gaussian angle
-// weights or the one-wavelength collapse, and the light pick. No fog, particles, photons,
-// film, subsurface, approximations or hit cache.
+// weights or the one-wavelength collapse, thin-film tints, and the light pick. No fog,
+// particles, photons, subsurface, approximations or hit cache.
 template <class T, class P>
 PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, Seed seed) {
     PathResult out{};
@@ -519,6 +607,8 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
             nr = normalize(nf + (h - nf) * alpha);
         }
         T f = fresnel_schlick(dot(d, nr), ior);
+        Spec<T> film = splat(T(1));
+        if (!in_solid && mat.film_strength > 0.001) film = thin_film_tint(sp, dot(-d, film_normal(nr, hp, mat)), mat);
         T transparency = tclamp(T(mat.transparency), T(0), T(1));
         T refl = tclamp(T(mat.reflectiveness), T(0), T(1));
         T p_reflect = tclamp(refl + (T(1) - refl) * f, T(0), T(1));
@@ -572,7 +662,7 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
         if (r < p_reflect) {
             nd = reflect3(d, nr);
             w = sp.from_rgb(T(mat.color[0]) + (T(1) - T(mat.color[0])) * f, T(mat.color[1]) + (T(1) - T(mat.color[1])) * f,
-                            T(mat.color[2]) + (T(1) - T(mat.color[2])) * f) * p_cont;
+                            T(mat.color[2]) + (T(1) - T(mat.color[2])) * f) * film * p_cont;
             kind = 1;
         } else if (r < p_reflect + p_transmit) {
             bool tir;

@@ -109,16 +109,16 @@ def test_rect_matches_full_frame(renderer):
 
 
 def test_unsupported_materials_are_rejected(renderer):
-    """Subsurface scattering and thin film are not implemented and are rejected by name;
-    transmissive materials are accepted."""
-    for field in ("subsurface", "film_strength"):
-        u = c3(16, 9)
-        setattr(u.instances[1].colors[3], field, 0.5)
-        with pytest.raises(nd.NdError) as e:
-            renderer.path_trace(u, 1)
-        assert e.value.status == nd.abi.ND_E_UNSUPPORTED and "subsurface" in str(e.value)
+    """Subsurface scattering is not implemented and is rejected by name; transmissive and
+    thin-film materials are accepted."""
+    u = c3(16, 9)
+    u.instances[1].colors[3].subsurface = 0.5
+    with pytest.raises(nd.NdError) as e:
+        renderer.path_trace(u, 1)
+    assert e.value.status == nd.abi.ND_E_UNSUPPORTED and "subsurface" in str(e.value)
     u = c3(16, 9)
     u.instances[1].colors[3].transparency = 0.5
+    u.instances[1].colors[4].film_strength = 0.5
     u.has_transparency = 1.0
     assert np.isfinite(renderer.path_trace(u, 1)[0]).all()
 

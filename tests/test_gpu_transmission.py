@@ -1,6 +1,6 @@
 """GPU parity of the transmissive path trace (refraction, the inside march, dispersion,
-shadows through transparent occluders, the light pick) against the MC oracle, through the
-C ABI.
+shadows through transparent occluders, thin-film coats, the light pick) against the MC
+oracle, through the C ABI.
 
 Scenes: the four-fractal scene with its Mandelbox (the instance that fills the view; the
 Mandelbulb lies inside the union) made of dispersive glass (the strips keep their colours,
@@ -72,6 +72,27 @@ def test_dispersive_glass_soft_weights(renderer):
 
 def test_dispersive_glass_collapse(renderer):
     compare_full(renderer, glass_c3(soft=0.0), 8, "glass collapse")
+
+
+def filmed_c3(w=64, h=36, glass=False):
+    """The Mandelbox with an iridescent coat: film index 1.8, 350 nm, strength 0.8, its
+    normal bent by the film noise; reflective enough that the film tints the reflections."""
+    u = glass_c3(w, h) if glass else nd.resize(nd.config_uniforms(3, 0), w, h)
+    inst = u.instances[1]
+    for s in range(int(inst.color_count)):
+        c = inst.colors[s]
+        c.film_strength, c.film_ior, c.film_thickness, c.film_angle_scale = 0.8, 1.8, 350.0, 1.0
+        c.film_perturb, c.film_perturb_scale = 0.3, 0.25
+        c.reflectiveness = max(c.reflectiveness, 0.3)
+    return u
+
+
+def test_thin_film(renderer):
+    compare_full(renderer, filmed_c3(), 8, "thin film")
+
+
+def test_thin_film_on_glass(renderer):
+    compare_full(renderer, filmed_c3(glass=True), 8, "thin film on glass")
 
 
 def test_light_pick(renderer):

@@ -249,3 +249,85 @@ def test_light_pick_is_unbiased():
     se = np.sqrt(sa.var(0)[hit] / spp + sb.var(0)[hit] / spp) + 1e-6
     assert np.abs(da / se).mean() < 1.2  # |z| averages 0.8 for unbiased estimates
     assert np.abs(sa.mean() - sb.mean()) < 0.01 * sb.mean()
+
+
+# --------------------------------------------------------------------------- thin film
+def film_reflectance_tm(n1, n2, n3, d_nm, lam_nm, cos1, pol):
+    """Reflectance of one film (index n2, thickness d) between n1 and n3 by the
+    characteristic (transfer) matrix method, for s or p polarisation."""
+    s1 = np.sqrt(max(0.0, 1 - cos1 * cos1))
+    c2 = np.sqrt(1 - (n1 * s1 / n2) ** 2 + 0j)
+    c3 = np.sqrt(1 - (n1 * s1 / n3) ** 2 + 0j)
+    # tilted admittances
+    eta = (lambda n, c: n * c) if pol == "s" else (lambda n, c: n / c)
+    e1, e2, e3 = eta(n1, cos1), eta(n2, c2), eta(n3, c3)
+    delta = 2 * np.pi * n2 * d_nm * c2 / lam_nm
+    M = np.array([[np.cos(delta), 1j * np.sin(delta) / e2], [1j * e2 * np.sin(delta), np.cos(delta)]])
+    B, Cc = M @ np.array([1.0, e3])
+    r = (e1 * B - Cc) / (e1 * B + Cc)
+    return abs(r) ** 2
+
+
+def test_thin_film_tint_is_the_airy_reflectance():
+    """thin_film_tint = 1 + (R / R_peak - 1) strength, R the s/p-averaged reflectance of the
+    film on its base (transfer-matrix method, independent of the Airy sum the oracle uses)
+    and R_peak its maximum over the film phase; no interference (film index = base index or
+    1 on 1) leaves 1."""
+    hero = 0.23
+    lam = H(25, [hero])
+    for cosv in (1.0, 0.8, 0.45):
+        for n2, n3, d in ((1.8, 1.5, 400.0), (1.33, 1.0, 250.0), (2.2, 1.6, 620.0)):
+            tint = H(30, [hero, cosv, n3, n2, d, 1.0, 1.0])
+            R = np.array([0.5 * (film_reflectance_tm(1.0, n2, n3, d, l, cosv, "s") +
+                                 film_reflectance_tm(1.0, n2, n3, d, l, cosv, "p")) for l in lam])
+            ratio = tint  # = R / peak at strength 1
+            # R / ratio is the same peak for every wavelength
+            peak = R / ratio
+            assert np.allclose(peak, peak[0], rtol=1e-9)
+            assert np.all(ratio <= 1 + 1e-12)
+            half = H(30, [hero, cosv, n3, n2, d, 0.5, 1.0])
+            assert np.allclose(half, 1 + (ratio - 1) * 0.5, rtol=1e-12)
+    assert np.allclose(H(30, [hero, 0.7, 1.5, 1.5, 400.0, 1.0, 1.0]), 1.0, atol=1e-12)  # film = base
+    assert np.allclose(H(30, [hero, 0.7, 1.0, 1.0, 400.0, 1.0, 1.0]), 1.0, atol=1e-12)  # no contrast
+    assert np.allclose(H(30, [hero, 0.7, 1.5, 1.8, 400.0, 0.0, 1.0]), 1.0, atol=1e-12)  # strength 0
+
+
+def test_film_noise_is_value_noise_on_the_hash33_lattice():
+    """At lattice points the noise is 2 hash33(p) - 1; in between it is continuous, in
+    [-1, 1], and changes per cell."""
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        i = rng.integers(-20, 20, size=3).astype(np.float64)
+        h = H(32, list(i), 3)
+        assert np.all((h >= 0) & (h < 1))
+        assert np.allclose(H(31, list(i), 3), 2 * h - 1, atol=1e-7)
+        p = i + rng.random(3)
+        e = 1e-7 * unit(rng.normal(size=3))
+        assert np.allclose(H(31, list(p), 3), H(31, list(p + e), 3), atol=1e-5)
+        assert np.all(np.abs(H(31, list(p), 3)) <= 1)
+    # distinct neighbouring cells
+    assert not np.allclose(H(32, [1.0, 2.0, 3.0], 3), H(32, [2.0, 2.0, 3.0], 3))
+
+
+def test_film_of_the_base_index_is_invisible():
+    """A film whose index equals its base's has no interference: the path trace of a mirror
+    ball (reflective, film strength 1, film perturbation on) equals the one without film."""
+    def mirror(film):
+        u, cm = sphere(size=(12, 12), cam=(0.0, 1.5, -3.5))
+        inst = u.instances[0]
+        inst.color_count = 1.0
+        c = inst.colors[0]
+        for k in range(3):
+            c.color[k] = 0.9
+        c.reflectiveness, c.ior, c.glossiness = 1.0, 1.5, 0.5
+        c.film_ior, c.film_thickness, c.film_angle_scale = 1.5, 300.0, 1.0
+        c.film_strength = 1.0 if film else 0.0
+        c.film_perturb, c.film_perturb_scale = 0.5, 0.2
+        return u, cm
+    a = oracle.render_mc(*mirror(True)[:1], 4, code_map=mirror(True)[1])["rgb"]
+    b = oracle.render_mc(*mirror(False)[:1], 4, code_map=mirror(False)[1])["rgb"]
+    assert np.allclose(a, b, rtol=1e-6, atol=1e-8)
+    u, cm = mirror(True)
+    u.instances[0].colors[0].film_ior = 2.2
+    c = oracle.render_mc(u, 4, code_map=cm)["rgb"]
+    assert not np.allclose(c, b, rtol=1e-3)  # a contrasting film shows
