@@ -58,6 +58,7 @@ struct McScene {
     int transp;        // scene_has_transparency: shadow rays cross transparent occluders
     int transmissive;  // some visible stop has transparency: paths can enter solids
     int light_pick;    // mc_enabled and mc_light_pick
+    int cache_on;      // primary_cache_key: the primary hit cache
     float disp_soft;   // radians(0.5 * photon_dispersion_soft)
     int max_depth;
     int shadow_steps;
@@ -290,6 +291,30 @@ NDD V3<float> normal_tetra(const DevScene& sc, const V3<float>& p, float e) {
     }
     return vnorm3(g);
 }
+// oct_encode / oct_decode (template.wgsl:3696-3718): the octahedral map of a unit normal
+// packed as two 16-bit snorm values (pack2x16snorm / unpack2x16snorm)
+NDD uint32_t oct_pack(const V3<float>& n) {
+    const float l1 = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+    float px = n.x / l1, py = n.y / l1;
+    if (n.z < 0.f) {
+        const float qx = (1.f - fabsf(py)) * (px >= 0.f ? 1.f : -1.f);
+        const float qy = (1.f - fabsf(px)) * (py >= 0.f ? 1.f : -1.f);
+        px = qx; py = qy;
+    }
+    const int ix = (int)floorf(0.5f + 32767.f * fminf(1.f, fmaxf(-1.f, px)));
+    const int iy = (int)floorf(0.5f + 32767.f * fminf(1.f, fmaxf(-1.f, py)));
+    return ((uint32_t)ix & 0xffffu) | (((uint32_t)iy & 0xffffu) << 16);
+}
+NDD V3<float> oct_unpack(uint32_t v) {
+    const float fx = fmaxf((float)(int16_t)(v & 0xffffu) / 32767.f, -1.f);
+    const float fy = fmaxf((float)(int16_t)(v >> 16) / 32767.f, -1.f);
+    float x = fx, y = fy;
+    const float z = 1.f - fabsf(fx) - fabsf(fy);
+    const float t = fmaxf(-z, 0.f);
+    x += x >= 0.f ? -t : t;
+    y += y >= 0.f ? -t : t;
+    return vnorm3(mk3(x, y, z));
+}
 // hash33 (template.wgsl:3439-3450): three values in [0, 1) per integer lattice cell
 NDD void hash33(float px, float py, float pz, float* out) {
     uint32_t v0 = (uint32_t)(int32_t)px, v1 = (uint32_t)(int32_t)py, v2 = (uint32_t)(int32_t)pz;
@@ -395,6 +420,12 @@ struct McPaths {
     uint8_t *depth, *flags;  // flags: 1 reflected, 2 hit, 4 inside a solid
     float* lambda;           // picked wavelength in micrometres (0: none yet)
     uint16_t* inner;         // inside-march step budget of the solid the path is in
+    // flags also: 8 the primary segment comes from the cache (no extend), 16 the path's
+    // depth-0 event was a mirror reflection (its depth-1 normal is packed like the primary)
+    // primary hit cache per pixel (adapt_stats[].primary, template.wgsl:3720-3766): bit 0
+    // valid, bit 1 hit; distance; octahedral-packed normal
+    float* ct;
+    uint32_t *cf, *cn;
     // queues (path indices) and counters
     uint32_t *live_a, *live_b;
     uint32_t* shadow_path;
@@ -469,7 +500,16 @@ __global__ void k_mc_generate(McArgs a) {
     a.P.seedy[p] = mad_rn(ms, 45.17f, by);
     a.P.hero[p] = mad_rn(ms, 0.61803399f, hash12w(add_rn(bx, 19.31f), add_rn(by, 47.11f)));
     a.P.depth[p] = 0;
-    a.P.flags[p] = 0;
+    uint8_t fl = 0;
+    if (a.S->cache_on) {
+        // a primary segment stored by sample 0 of this call (an earlier chunk) is reused
+        const uint32_t c = a.P.cf[pix];
+        if (c & 1u) {
+            fl = (uint8_t)(8u | ((c & 2u) ? 2u : 0u));
+            a.P.hit_t[p] = a.P.ct[pix];
+        }
+    }
+    a.P.flags[p] = fl;
     a.P.lambda[p] = 0.f;
     a.P.inner[p] = 64;  // PATH_DEFAULT_INNER_STEPS
     a.P.live_a[p] = p;
@@ -517,7 +557,9 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
                     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
                     const uint32_t rank = (uint32_t)__popcll(need & lt);
                     const bool mine = ((need >> lane) & 1ull) && rank < avail;
-                    if (mine) {
+                    if (mine && R.skip(cnext + rank)) {
+                        // the ray's segment is already known (primary hit cache)
+                    } else if (mine) {
                         item = cnext + rank;
                         R.setup(item, &org, &dir);
                         march_init(m);
@@ -588,8 +630,9 @@ struct ExtendRays {
     NDD void finish(uint32_t i, bool hit, float t) const {
         const uint32_t p = q[i];
         a.P.hit_t[p] = t;
-        a.P.flags[p] = (uint8_t)((a.P.flags[p] & 5u) | (hit ? 2u : 0u));
+        a.P.flags[p] = (uint8_t)((a.P.flags[p] & 21u) | (hit ? 2u : 0u));
     }
+    NDD bool skip(uint32_t i) const { return (a.P.flags[q[i]] & 8u) != 0; }
     NDD int inside(uint32_t i) const {
         const uint32_t p = q[i];
         return (a.P.flags[p] & 4u) ? (int)a.P.inner[p] : 0;
@@ -606,6 +649,7 @@ struct IndirectRays {
         a.P.ihit[i] = hit ? 1 : 0;
     }
     NDD int inside(uint32_t) const { return 0; }
+    NDD bool skip(uint32_t) const { return false; }
 };
 
 // The two march kernels of the path trace run at 7 waves per SIMD (72 VGPRs, five values
@@ -787,14 +831,33 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                 if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
                     if (a.out_depth) a.out_depth[pix] = 0.f;
                     if (a.out_normal) { a.out_normal[3 * pix] = 0.f; a.out_normal[3 * pix + 1] = 0.f; a.out_normal[3 * pix + 2] = 0.f; }
+                    if (S.cache_on) { a.P.ct[pix] = a.P.hit_t[p]; a.P.cf[pix] = 1u; }  // a cached miss
                 }
             } else if (sp_max(thr) >= 0.002f) {
                 const float t = a.P.hit_t[p];
                 const float eps = J.eps_c * t + J.eps_f;
                 const V3<float> hp = mk3(pos.x + dir.x * t, pos.y + dir.y * t, pos.z + dir.z * t);
-                const V3<float> nrm = normal_tetra<EXT>(sc, hp, eps);
+                // path_normal (template.wgsl:3884-3896): with the primary hit cache on, the
+                // primary normal and the normal after a depth-0 mirror are octahedral-packed;
+                // sample 0 stores the primary hit, later samples read it back
+                V3<float> nrm;
+                if (fl & 8u) {
+                    nrm = oct_unpack(a.P.cn[pix]);
+                } else {
+                    nrm = normal_tetra<EXT>(sc, hp, eps);
+                    de += 4;
+                    if (S.cache_on && !in_solid && (depth == 0 || (depth == 1 && (fl & 16u)))) {
+                        const uint32_t packed = oct_pack(nrm);
+                        nrm = oct_unpack(packed);
+                        if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
+                            a.P.cn[pix] = packed;
+                            a.P.ct[pix] = t;
+                            a.P.cf[pix] = 3u;  // valid, hit
+                        }
+                    }
+                }
                 const Mat mat = part_mask(hit_material<EXT>(sc, S, hp), S);
-                de += 5;
+                de += 1;
                 if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
                     if (a.out_depth) a.out_depth[pix] = t;
                     if (a.out_normal) { a.out_normal[3 * pix] = nrm.x; a.out_normal[3 * pix + 1] = nrm.y; a.out_normal[3 * pix + 2] = nrm.z; }
@@ -938,7 +1001,9 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                         a.P.px[p] = hp.x + nd.x * bias; a.P.py[p] = hp.y + nd.y * bias; a.P.pz[p] = hp.z + nd.z * bias;
                         a.P.dx[p] = nd.x; a.P.dy[p] = nd.y; a.P.dz[p] = nd.z;
                         a.P.depth[p] = (uint8_t)(depth + 1);
-                        a.P.flags[p] = (uint8_t)(((reflected || kind == 1) ? 1u : 0u) | (solid ? 4u : 0u));
+                        // a mirror reflection at depth 0 (template.wgsl:4117-4118)
+                        const bool mirror = S.cache_on && depth == 0 && !in_solid && kind == 1 && rough * rough <= 1e-4f;
+                        a.P.flags[p] = (uint8_t)(((reflected || kind == 1) ? 1u : 0u) | (solid ? 4u : 0u) | (mirror ? 16u : 0u));
                         a.P.inner[p] = (uint16_t)inner;
                     } else {
                         cont = false;
@@ -1210,6 +1275,7 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
             for (int c = 0; c < std::min((int)u.instances[i].color_count, 16); ++c)
                 if (u.instances[i].colors[c].transparency > 0.f) S.transmissive = 1;
     S.light_pick = u.mc_enabled > 0.5f && u.mc_light_pick > 0.5f;
+    S.cache_on = u.primary_cache_key > 0.5f;
     S.disp_soft = (0.5f * u.photon_dispersion_soft) * 0.017453292519943295f;
     S.max_depth = std::min(std::max((int)u.max_reflection_bounces + 8, 1), 16);
     S.shadow_steps = std::min((int)u.max_steps, 40);
@@ -1257,8 +1323,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     if (spp == 0 || spp > 65536) return nd_fail(c, ND_E_INVALID, "spp %u", spp);
     const nd_uniforms& u = *(const nd_uniforms*)uniforms;
     if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.dof_enabled > 0.5f || u.adapt_enabled > 0.5f ||
-        u.approx_flags != 0.0f || u.primary_cache_key > 0.5f)
-        return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field, adaptive sampling, approximations and the hit cache are not implemented");
+        u.approx_flags != 0.0f)
+        return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field, adaptive sampling and approximations are not implemented");
     for (int i = 0; i < std::min((int)u.light_count, 8); ++i)
         if ((int)(u.lights[i].light_type + 0.5f) == 2) return nd_fail(c, ND_E_UNSUPPORTED, "ray lights are not implemented");
     for (int i = 0; i < ND_MAX_INSTANCES; ++i)
@@ -1289,6 +1355,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     for (int i = 0; i < 4; ++i) sz(4 * P);  // hero, seeds, hit_t
     sz(P); sz(P);                           // depth, flags
     sz(4 * P); sz(2 * P);                   // lambda, inner
+    sz(4 * (size_t)npix); sz(4 * (size_t)npix); sz(4 * (size_t)npix);  // primary hit cache
     sz(4 * P); sz(4 * P);                   // live queues
     const size_t SQ = P * (size_t)std::max(S.light_count, 1);
     sz(4 * SQ); for (int i = 0; i < 8; ++i) sz(4 * SQ); sz(32 * SQ);
@@ -1313,6 +1380,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     Pp.hit_t = (float*)carve(4 * P);
     Pp.depth = (uint8_t*)carve(P); Pp.flags = (uint8_t*)carve(P);
     Pp.lambda = (float*)carve(4 * P); Pp.inner = (uint16_t*)carve(2 * P);
+    Pp.ct = (float*)carve(4 * (size_t)npix); Pp.cf = (uint32_t*)carve(4 * (size_t)npix); Pp.cn = (uint32_t*)carve(4 * (size_t)npix);
     Pp.live_a = (uint32_t*)carve(4 * P); Pp.live_b = (uint32_t*)carve(4 * P);
     Pp.shadow_path = (uint32_t*)carve(4 * SQ);
     Pp.sox = (float*)carve(4 * SQ); Pp.soy = (float*)carve(4 * SQ); Pp.soz = (float*)carve(4 * SQ);
@@ -1428,6 +1496,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         hc[k].mc_sample0 = u.mc_sample;
     }
     hipLaunchKernelGGL(nd::k_mc_zero, dim3(1024), dim3(256), 0, s, d_rgb, (uint64_t)3 * npix);
+    hipLaunchKernelGGL(nd::k_mc_zero, dim3(256), dim3(256), 0, s, (float*)Pp.cf, (uint64_t)npix);  // empty hit cache
     hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 3);
     (void)hipEventRecord(c->ev[6], s);
     for (uint32_t k0 = 0; k0 < n_chunks; k0 += nd::MC_MAX_CHUNKS) {

@@ -487,6 +487,7 @@ int ndo_helper(int op, const double* a, double* out) {
                    return 0; }
         case 31: { D3 r = film_noise(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
         case 32: { float h[3]; hash33(a[0], a[1], a[2], h); out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; return 0; }
+        case 33: { D3 r = oct_quantize(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
         case 28: { Spectral<double> sp(a[0]); Mat m = default_mat(); m.ior = a[7]; m.abbe = a[8];
                    Spec<double> w = dispersion_weights(sp, D3(a[1], a[2], a[3]), D3(a[4], a[5], a[6]), m, a[9] > 0.5,
                                                        D3(a[10], a[11], a[12]), a[13]);

@@ -539,14 +539,38 @@ MarchResult<T> march_inside(const Scene<P>& s, V3<T> org, V3<T> dir, int max_inn
     return r;
 }
 
+// ----------------------------------------------------------------------------- hit cache
+// oct_encode / oct_decode (template.wgsl:3696-3718): a unit normal through the octahedral
+// map packed as two 16-bit snorm values (pack2x16snorm), and back. The primary hit cache
+// keeps normals in this form, so with the cache on the path uses the decoded normal.
+template <class T> V3<T> oct_quantize(V3<T> n) {
+    const T l1 = abs(n.x) + abs(n.y) + abs(n.z);
+    T px = n.x / l1, py = n.y / l1;
+    if (n.z < T(0)) {
+        const T qx = (T(1) - abs(py)) * (px >= T(0) ? T(1) : T(-1));
+        const T qy = (T(1) - abs(px)) * (py >= T(0) ? T(1) : T(-1));
+        px = qx; py = qy;
+    }
+    auto snorm = [](T x) {  // pack2x16snorm then unpack2x16snorm of one component
+        const double q = std::floor(0.5 + 32767.0 * std::min(1.0, std::max(-1.0, (double)val(x))));
+        return T(std::max(q / 32767.0, -1.0));
+    };
+    px = snorm(px); py = snorm(py);
+    V3<T> r(px, py, T(1) - abs(px) - abs(py));
+    const T t = tmax(-r.z, T(0));
+    r.x = r.x + (r.x >= T(0) ? -t : t);
+    r.y = r.y + (r.y >= T(0) ? -t : t);
+    return normalize(r);
+}
+
 // ----------------------------------------------------------------------------- trace_path
 struct PathResult { double rgb[3]; double depth; double normal[3]; int evals; int vertices; };
 
 // trace_path (template.wgsl:4126-4137) with path_begin / path_extend / path_shade
 // (:3819-4124): opaque and transmissive materials (Schlick reflection / refraction through
 // march_inside), Cauchy dispersion on the picked wavelength with the Gaussian angle
-// weights or the one-wavelength collapse, thin-film tints, and the light pick. No fog,
-// particles, photons, subsurface, approximations or hit cache.
+// weights or the one-wavelength collapse, thin-film tints, the light pick and the primary
+// hit cache's packed normals. No fog, particles, photons, subsurface or approximations.
 template <class T, class P>
 PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, Seed seed) {
     PathResult out{};
@@ -557,6 +581,11 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
     Spec<T> radiance = splat(T(0)), throughput = splat(T(1));
     V3<T> pos = org, d = dir;
     bool reflected = false, in_solid = false;
+    // the primary hit cache (hit_cache_load/store, template.wgsl:3720-3766): on when the host
+    // sets primary_cache_key; the primary hit and the hit after a mirror reflection at depth 0
+    // keep their normals octahedral-packed
+    const bool cache = u.primary_cache_key > 0.5f;
+    bool second = false;
     int inner_steps = 64;  // PATH_DEFAULT_INNER_STEPS
     T lambda = T(0);        // picked wavelength in micrometres, 0 before the first dispersive hit
     const int max_depth = std::min(std::max(i32(u.max_reflection_bounces) + 8, 1), 16);
@@ -578,6 +607,7 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
         V3<T> hp = pos + d * m.dist;
         V3<T> n = normal_tetra(s, hp, T(mc.eps_c) * m.dist + T(mc.eps_f));
         out.evals += 4;
+        if (cache && !in_solid && (depth == 0 || (depth == 1 && second))) n = oct_quantize(n);  // path_normal
         Mat mat = apply_part_mask(hit_material(u, s, hp), parts);
         out.evals += 1;
         if (depth == 0) {
@@ -677,6 +707,7 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
         if (kind == 1 && !in_solid && (parts & ND_PART_REFLECTIONS)) break;
         throughput = throughput * (disp_w * w);
         reflected = reflected || kind == 1;
+        second = depth == 0 && !in_solid && kind == 1 && rough * rough <= T(1e-4);  // MIN_ROUGHNESS
         if (kind == 2) {
             in_solid = !in_solid;
             if (in_solid) inner_steps = std::min(std::max(i32((float)mat.inner_max_steps), 4), 512);

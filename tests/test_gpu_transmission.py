@@ -133,3 +133,32 @@ def test_index_one_clear_glass_is_invisible_on_axis(sphere_bound):
     h = sphere_bound.path_trace(clear_sphere(True), 16)[0]
     assert np.allclose(g[4, 4], h[4, 4], rtol=1e-4, atol=1e-6), (g[4, 4], h[4, 4])
     assert not np.allclose(g, h, rtol=1e-4)  # off axis the sphere shows
+
+
+def test_primary_hit_cache_semantics(renderer):
+    """With primary_cache_key the primary normals are octahedral-packed, as the oracle's."""
+    u = nd.resize(nd.config_uniforms(3, 0), 64, 36)
+    u.primary_cache_key = 1.0
+    compare_full(renderer, u, 8, "primary cache")
+
+
+def test_primary_hit_cache_reuse_is_exact(renderer):
+    """At 1080p, 9 samples run as chunks of 8 + 1; the ninth sample reuses the primary
+    segments stored by sample 0 instead of marching. It must equal that sample rendered on
+    its own (mc_sample 8, one chunk, nothing reused): 9 c9 - 8 c8 = c(sample 8)."""
+    import math
+    u = nd.config_uniforms(3, 0)
+    u.primary_cache_key = 1.0
+    print("\n[case] transmission primary cache reuse 1080p spp 9", flush=True)
+    c9, _, _, st9 = renderer.path_trace(u, 9)
+    c8, _, _, _ = renderer.path_trace(u, 8)
+    u1 = nd.config_uniforms(3, 0)
+    u1.primary_cache_key = 1.0
+    u1.mc_sample = 8.0
+    c1, _, _, st1 = renderer.path_trace(u1, 1)
+    last = c9.astype(np.float64) * 9 - c8.astype(np.float64) * 8
+    err = np.abs(last - c1)
+    print("[mc] cache reuse max_err=%.3g mean=%.5g de_per_path_9=%.1f de_per_path_1=%.1f"
+          % (err.max(), c1.mean(), st9.de_evals / st9.paths, st1.de_evals / st1.paths), flush=True)
+    assert err.max() <= 1e-5 * max(1.0, float(np.abs(c1).max())) * 20
+    assert st9.de_evals / st9.paths < st1.de_evals / st1.paths  # the reused samples skip the primary march

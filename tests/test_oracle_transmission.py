@@ -331,3 +331,39 @@ def test_film_of_the_base_index_is_invisible():
     u.instances[0].colors[0].film_ior = 2.2
     c = oracle.render_mc(u, 4, code_map=cm)["rgb"]
     assert not np.allclose(c, b, rtol=1e-3)  # a contrasting film shows
+
+
+# --------------------------------------------------------------------------- hit cache
+def test_octahedral_normal_packing():
+    """oct_encode/oct_decode through two 16-bit snorm values: unit length, within the
+    lattice's angular resolution (~3e-5 rad, bounded here by 1e-4) over both hemispheres,
+    exact on the axes."""
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(2000):
+        n = unit(rng.normal(size=3))
+        q = H(33, list(n), 3)
+        assert np.isclose(np.linalg.norm(q), 1.0, atol=1e-12)
+        worst = max(worst, np.arccos(np.clip(q @ n, -1, 1)))
+        assert np.sign(q[2]) == np.sign(n[2]) or abs(n[2]) < 1e-3
+    assert worst < 1e-4, worst
+    for a in np.eye(3).tolist() + (-np.eye(3)).tolist():
+        assert np.allclose(H(33, a, 3), a, atol=1e-12)
+
+
+def test_primary_cache_only_quantizes_normals():
+    """With the primary hit cache on, the primary normal (and the normal after a mirror
+    bounce at depth 0) is the packed one: the image moves by far less than its noise."""
+    u, cm = ft.test_scene("mandelboxsort_box_fold", size=(24, 18))
+    u.mc_enabled = 1.0
+    a = oracle.render_mc(u, 4, code_map=cm)
+    u.primary_cache_key = 1.0
+    b = oracle.render_mc(u, 4, code_map=cm)
+    assert np.array_equal(a["depth"], b["depth"])
+    hit = a["depth"] > 0
+    assert hit.mean() > 0.2
+    na, nb = a["normal"][hit].astype(np.float64), b["normal"][hit].astype(np.float64)
+    dn = np.degrees(np.arccos(np.clip((na * nb).sum(-1) / np.linalg.norm(na, axis=-1) / np.linalg.norm(nb, axis=-1),
+                                      -1, 1)))
+    assert 0 < dn.max() < 0.01
+    assert np.abs(a["rgb"] - b["rgb"]).mean() < 0.02 * np.abs(a["rgb"]).mean()
