@@ -186,6 +186,20 @@ def mc_leg(r, nd, spp=64):
         per_sample[name] = round((time.perf_counter() - t0) / 8 * 1e3, 3)
     r.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays)
     u.mc_sample = 0.0
+    # the same frame with the primary hit cache on (primary_cache_key, as the reference's
+    # multi-sample export sets it for a static camera): samples after the first reuse the
+    # primary segment and the packed primary normal
+    uc = nd.config_uniforms(3, 0)
+    uc.primary_cache_key = 1.0
+    r.path_trace(uc, 1, color=col)
+    r.sync()
+    t0 = time.perf_counter()
+    _, _, _, stc = r.path_trace(uc, spp, color=col)
+    r.sync()
+    wall_c = time.perf_counter() - t0
+    log("c3 mc with the primary hit cache: %.1f ms for %d spp" % (wall_c * 1e3, spp))
+    cached = {"ms_per_frame": round(wall_c * 1e3, 2), "msamples_per_s": round(stc.paths / wall_c / 1e6, 3),
+              "de_evals_per_path": round(stc.de_evals / stc.paths, 1)}
     f_de = flops_table()["workloads"]["3"]["march"]["flops_per_de"]
     tflops = f_de * st.de_evals / (st.ms_total / 1e3) / 1e12 if st.ms_total > 0 else 0.0
     return {"tflops": round(tflops, 3), "frac_fp32_peak": round(tflops / FP32_PEAK_TFLOPS, 4),
@@ -194,7 +208,7 @@ def mc_leg(r, nd, spp=64):
             "msamples_per_s": round(st.paths / wall / 1e6, 3), "segments_per_s": round(st.segments / wall, 1),
             "segments_per_path": round(st.segments / st.paths, 3), "shadow_rays": int(st.shadow_rays),
             "de_evals_per_s": round(st.de_evals / wall, 1), "de_evals_per_path": round(st.de_evals / st.paths, 1),
-            "ms_per_1spp_call": per_sample}
+            "ms_per_1spp_call": per_sample, "primary_cache": cached}
 
 
 def cpu_baseline(nd, target_s=40.0):

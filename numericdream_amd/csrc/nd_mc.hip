@@ -1486,15 +1486,15 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
             c->graph_valid = true;
         }
     }
-    // chunk records, uploaded MC_MAX_CHUNKS at a time
-    const uint32_t n_chunks = (spp + chunk - 1) / chunk;
-    std::vector<nd::McChunk> hc(n_chunks);
-    for (uint32_t k = 0; k < n_chunks; ++k) {
-        hc[k].sample_base = k * chunk;
-        hc[k].spp_chunk = std::min(chunk, spp - k * chunk);
-        hc[k].spp_total = spp;
-        hc[k].mc_sample0 = u.mc_sample;
+    // chunk records, uploaded MC_MAX_CHUNKS at a time. With the primary hit cache, sample 0
+    // runs as a chunk of its own so that every later chunk reuses its primary segments.
+    std::vector<nd::McChunk> hc;
+    for (uint32_t base = 0; base < spp;) {
+        const uint32_t n = (S.cache_on && base == 0) ? 1u : std::min(chunk, spp - base);
+        hc.push_back(nd::McChunk{base, n, spp, u.mc_sample});
+        base += n;
     }
+    const uint32_t n_chunks = (uint32_t)hc.size();
     hipLaunchKernelGGL(nd::k_mc_zero, dim3(1024), dim3(256), 0, s, d_rgb, (uint64_t)3 * npix);
     hipLaunchKernelGGL(nd::k_mc_zero, dim3(256), dim3(256), 0, s, (float*)Pp.cf, (uint64_t)npix);  // empty hit cache
     hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 3);
