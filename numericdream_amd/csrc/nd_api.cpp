@@ -402,7 +402,7 @@ void nd_ctx_destroy(nd_ctx* c) {
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     c->jobs.release(); c->job_base.release(); c->ctr.release(); c->hits.release(); c->shared.release();
     c->staging.release(); c->latency.release(); c->adj.release(); c->mc.release();
-    c->accel.release(); c->accel_scenes.release();
+    c->accel.release(); c->accel_scenes.release(); c->mc_adapt.release();
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -59,6 +59,8 @@ struct McScene {
     int transmissive;  // some visible stop has transparency: paths can enter solids
     int light_pick;    // mc_enabled and mc_light_pick
     int cache_on;      // primary_cache_key: the primary hit cache
+    int adapt_on;      // adapt_enabled and mc_enabled
+    float adapt_row_width, adapt_min_samples, adapt_threshold;
     float disp_soft;   // radians(0.5 * photon_dispersion_soft)
     int max_depth;
     int shadow_steps;
@@ -461,6 +463,7 @@ struct McArgs {
     float* out_rgb;
     float* out_depth;
     float* out_normal;
+    float* adapt;  // per pixel: luminance sum, square sum, count, converged; colour sum (3), samples
 };
 
 NDD unsigned long long* de_ctr(const McArgs& a) { return (unsigned long long*)(a.P.ctr + C_DE); }
@@ -1189,6 +1192,49 @@ __global__ void k_mc_resolve(McArgs a) {
     for (int k = 0; k < 3; ++k) a.out_rgb[3 * pix + k] += acc[k] * inv;
 }
 
+// --- resolve with adaptive sampling (adapt_update, template.wgsl:4749-4760; the sample skip
+// of cs_wf_generate, :5134-5146): per pixel, in sample order, a sample after convergence is
+// dropped; the others are added to the pixel's colour sum and luminance statistics, and the
+// output is the mean of the samples taken. The state persists across chunks and calls.
+__global__ void k_mc_resolve_adapt(McArgs a) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= a.npix) return;
+    const McScene& S = *a.S;
+    const DevJob& J = *a.job;
+    float* st = a.adapt + 8 * (size_t)pix;
+    float s4[4] = {st[0], st[1], st[2], st[3]};
+    float sum[3] = {st[4], st[5], st[6]};
+    float taken = st[7];
+    const bool on = (float)(J.rect_x + (int)(pix % (uint32_t)J.rect_w)) < S.adapt_row_width;
+    const float w[3] = {0.2126f, 0.7152f, 0.0722f};
+    for (uint32_t s = 0; s < a.ch->spp_chunk; ++s) {
+        const float ms = add_rn(a.ch->mc_sample0, (float)(a.ch->sample_base + s));
+        if (on && ms > 0.5f && s4[3] > 0.5f) continue;
+        const uint32_t p = s * a.npix + pix;
+        Sp8 r;
+        for (int k = 0; k < 8; ++k) r.v[k] = a.P.rad[8 * p + k];
+        float rgb[3];
+        to_rgb_fit(a.P.hero[p], r, rgb);
+        for (int k = 0; k < 3; ++k) sum[k] += rgb[k];
+        taken += 1.f;
+        if (on) {
+            float l = 0.f;
+            for (int k = 0; k < 3; ++k) l += rgb[k] / (rgb[k] + 1.f) * w[k];
+            s4[0] += l; s4[1] += l * l; s4[2] += 1.f;
+            if (s4[2] >= fmaxf(S.adapt_min_samples, 2.f)) {
+                const float mean = s4[0] / s4[2];
+                const float var = fmaxf(s4[1] / s4[2] - mean * mean, 0.f);
+                s4[3] = sqrtf(var / s4[2]) <= S.adapt_threshold * (mean + 0.05f) ? 1.f : 0.f;
+            }
+        }
+    }
+    for (int k = 0; k < 4; ++k) st[k] = s4[k];
+    for (int k = 0; k < 3; ++k) st[4 + k] = sum[k];
+    st[7] = taken;
+    const float inv = 1.f / fmaxf(taken, 1.f);
+    for (int k = 0; k < 3; ++k) a.out_rgb[3 * pix + k] = sum[k] * inv;
+}
+
 // counters between passes (no host round trip)
 __global__ void k_mc_counters(uint32_t* ctr, int op) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -1276,6 +1322,10 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
                 if (u.instances[i].colors[c].transparency > 0.f) S.transmissive = 1;
     S.light_pick = u.mc_enabled > 0.5f && u.mc_light_pick > 0.5f;
     S.cache_on = u.primary_cache_key > 0.5f;
+    S.adapt_on = u.adapt_enabled > 0.5f && u.mc_enabled > 0.5f;
+    S.adapt_row_width = u.adapt_row_width;
+    S.adapt_min_samples = u.adapt_min_samples;
+    S.adapt_threshold = u.adapt_threshold;
     S.disp_soft = (0.5f * u.photon_dispersion_soft) * 0.017453292519943295f;
     S.max_depth = std::min(std::max((int)u.max_reflection_bounces + 8, 1), 16);
     S.shadow_steps = std::min((int)u.max_steps, 40);
@@ -1322,9 +1372,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     if (!out || !out->color) return nd_fail(c, ND_E_INVALID, "path trace needs a colour output");
     if (spp == 0 || spp > 65536) return nd_fail(c, ND_E_INVALID, "spp %u", spp);
     const nd_uniforms& u = *(const nd_uniforms*)uniforms;
-    if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.dof_enabled > 0.5f || u.adapt_enabled > 0.5f ||
-        u.approx_flags != 0.0f)
-        return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field, adaptive sampling and approximations are not implemented");
+    if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.dof_enabled > 0.5f || u.approx_flags != 0.0f)
+        return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field and approximations are not implemented");
     for (int i = 0; i < std::min((int)u.light_count, 8); ++i)
         if ((int)(u.lights[i].light_type + 0.5f) == 2) return nd_fail(c, ND_E_UNSUPPORTED, "ray lights are not implemented");
     for (int i = 0; i < ND_MAX_INSTANCES; ++i)
@@ -1418,6 +1467,18 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     const bool indirect = S.indirect_on != 0;
     const bool inside = S.transmissive != 0;  // paths can enter solids: extend runs march_inside
     const bool transp = S.transp != 0;        // shadow rays cross transparent occluders
+    const bool adapt = S.adapt_on != 0;       // adaptive sampling: per-pixel state in c->mc_adapt
+    if (adapt) {
+        if ((st = c->mc_adapt.ensure(c, 32 * (size_t)npix)) != ND_OK) return st;
+        a.adapt = (float*)c->mc_adapt.p;
+        const uint32_t key[6] = {(uint32_t)job.rect_x, (uint32_t)job.rect_y, (uint32_t)job.rect_w, (uint32_t)job.rect_h,
+                                 (uint32_t)job.target_w, (uint32_t)job.target_h};
+        // a new sequence (mc_sample 0) or another rect starts from empty statistics
+        if (u.mc_sample < 0.5f || std::memcmp(key, c->mc_adapt_key, sizeof(key)) != 0) {
+            hipLaunchKernelGGL(nd::k_mc_zero, dim3(256), dim3(256), 0, s, a.adapt, (uint64_t)8 * npix);
+            std::memcpy(c->mc_adapt_key, key, sizeof(key));
+        }
+    }
     // The passes of one chunk, in stream order. Captured once into a graph per shape and
     // replayed for every chunk (the chunk's sample range is read from a.ch).
     auto enqueue_chunk = [&](hipStream_t st) {
@@ -1452,7 +1513,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 2);
             ping ^= 1;
         }
-        hipLaunchKernelGGL(nd::k_mc_resolve, dim3((npix + 255) / 256), dim3(256), 0, st, a);
+        if (adapt) hipLaunchKernelGGL(nd::k_mc_resolve_adapt, dim3((npix + 255) / 256), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL(nd::k_mc_resolve, dim3((npix + 255) / 256), dim3(256), 0, st, a);
     };
     const bool use_graph = !(c->tuning.flags & ND_TUNE_NO_GRAPH);
     if (use_graph) {
@@ -1460,7 +1522,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         struct McGraphKey {
             nd::McArgs args;
             uint32_t grid[5];
-            int max_depth, indirect, ext, inside, transp;
+            int max_depth, indirect, ext, inside, transp, adapt;
             uint64_t paths;
         } key;
         static_assert(sizeof(McGraphKey) <= sizeof(c->mc_graph_key), "graph key storage");
@@ -1470,6 +1532,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         key.ext = ext;
         key.inside = inside;
         key.transp = transp;
+        key.adapt = adapt;
         key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
         if (!c->mc_graph_exec || !c->graph_valid || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
             if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }

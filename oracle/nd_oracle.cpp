@@ -354,11 +354,20 @@ int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, in
                 int px = rx + x, py = ry + y;
                 V3<T> dir = primary_dir<T>(u, px, py, (int)u.resolution[0], (int)u.resolution[1]);
                 V3<T> org = V3<T>::from(u.camera_pos);
+                // adaptive sampling (adapt_update, template.wgsl:4749-4760; the sample skip of
+                // fs_main / cs_wf_generate): a converged pixel takes no more samples and
+                // keeps the mean of those it took
+                const bool adapt = u0->adapt_enabled > 0.5f && u0->mc_enabled > 0.5f && (float)px < u0->adapt_row_width;
+                float st4[4] = {0.f, 0.f, 0.f, 0.f};
+                int taken = 0;
                 for (int sidx = 0; sidx < spp; ++sidx) {
                     u.mc_sample = u0->mc_sample + (float)sidx;
                     const float ms = u.mc_sample;
+                    if (adapt && ms > 0.5f && st4[3] > 0.5f) continue;
                     Seed base{((float)px + 0.5f) + ms * 91.73f, ((float)py + 0.5f) + ms * 57.31f};
                     PathResult r = trace_path<T>(u, s, mc, org, dir, base);
+                    ++taken;
+                    if (adapt) adapt_update(st4, r.rgb, u0->adapt_min_samples, u0->adapt_threshold);
                     for (int k = 0; k < 3; ++k) acc[k] += r.rgb[k];
                     if (sample_rgb)
                         for (int k = 0; k < 3; ++k) sample_rgb[((size_t)sidx * rw * rh + i) * 3 + k] = (float)r.rgb[k];
@@ -368,7 +377,7 @@ int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, in
                     }
                     ev += (uint32_t)r.evals;
                 }
-                for (int k = 0; k < 3; ++k) rgb[3 * i + k] = (float)(acc[k] / spp);
+                for (int k = 0; k < 3; ++k) rgb[3 * i + k] = (float)(acc[k] / std::max(taken, 1));
                 if (evals) evals[i] = ev;
             }
         }

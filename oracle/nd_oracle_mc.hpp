@@ -539,6 +539,24 @@ MarchResult<T> march_inside(const Scene<P>& s, V3<T> org, V3<T> dir, int max_inn
     return r;
 }
 
+// ----------------------------------------------------------------------------- adaptive sampling
+// adapt_update (template.wgsl:4749-4760), in FP32 as the shader keeps the statistics: the
+// luminance of the tonemapped sample c / (c + 1) is accumulated with its square and the
+// count; from max(min_samples, 2) samples on, the pixel is converged (stats[3] = 1) when the
+// standard error sqrt(var / n) is at most threshold * (mean + 0.05)
+inline void adapt_update(float* stats, const double* rgb, float min_samples, float threshold) {
+    const float w[3] = {0.2126f, 0.7152f, 0.0722f};  // LUMA_WEIGHTS
+    float l = 0.f;
+    for (int k = 0; k < 3; ++k) { const float c = (float)rgb[k]; l += c / (c + 1.f) * w[k]; }
+    stats[0] += l; stats[1] += l * l; stats[2] += 1.f;
+    if (stats[2] >= std::max(min_samples, 2.f)) {  // ADAPT_MIN_VARIANCE_SAMPLES
+        const float mean = stats[0] / stats[2];
+        const float var = std::max(stats[1] / stats[2] - mean * mean, 0.f);
+        const float err = std::sqrt(var / stats[2]);
+        stats[3] = err <= threshold * (mean + 0.05f) ? 1.f : 0.f;  // ADAPT_DARK_FLOOR
+    }
+}
+
 // ----------------------------------------------------------------------------- hit cache
 // oct_encode / oct_decode (template.wgsl:3696-3718): a unit normal through the octahedral
 // map packed as two 16-bit snorm values (pack2x16snorm), and back. The primary hit cache

@@ -162,3 +162,36 @@ def test_primary_hit_cache_reuse_is_exact(renderer):
           % (err.max(), c1.mean(), st9.de_evals / st9.paths, st1.de_evals / st1.paths), flush=True)
     assert err.max() <= 1e-5 * max(1.0, float(np.abs(c1).max())) * 20
     assert st9.de_evals / st9.paths < st1.de_evals / st1.paths  # the reused samples skip the primary march
+
+
+def adaptive_c3(threshold, min_samples, w=64, h=36):
+    u = nd.resize(nd.config_uniforms(3, 0), w, h)
+    u.adapt_enabled = 1.0
+    u.adapt_threshold = threshold
+    u.adapt_min_samples = min_samples
+    u.adapt_row_width = float(w)
+    return u
+
+
+def test_adaptive_sampling(renderer):
+    """Adaptive sampling at threshold 0.05 from 4 samples, 16 spp: parity like the full
+    transport (converged pixels keep the mean of the samples they took)."""
+    compare_full(renderer, adaptive_c3(0.05, 4.0), 16, "adaptive")
+
+
+def test_adaptive_limit_and_progressive_calls(renderer):
+    """An unbounded threshold stops every pixel at its min-samples count (the plain 5-sample
+    image); sixteen 1-sample calls with increasing mc_sample keep the per-pixel statistics
+    in the context and end at the 16-sample call's image."""
+    u = adaptive_c3(1e9, 5.0)
+    a = renderer.path_trace(u, 12)[0]
+    u.adapt_enabled = 0.0
+    b = renderer.path_trace(u, 5)[0]
+    assert np.allclose(a, b, rtol=2e-6, atol=1e-8)
+    u = adaptive_c3(0.05, 4.0)
+    one = renderer.path_trace(u, 16)[0]
+    for i in range(16):
+        u.mc_sample = float(i)
+        prog = renderer.path_trace(u, 1)[0]
+    print("\n[mc] adaptive progressive max_err=%.3g" % np.abs(prog - one).max(), flush=True)
+    assert np.allclose(prog, one, rtol=2e-6, atol=1e-8)

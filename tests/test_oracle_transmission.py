@@ -367,3 +367,41 @@ def test_primary_cache_only_quantizes_normals():
                                       -1, 1)))
     assert 0 < dn.max() < 0.01
     assert np.abs(a["rgb"] - b["rgb"]).mean() < 0.02 * np.abs(a["rgb"]).mean()
+
+
+# --------------------------------------------------------------------------- adaptive sampling
+def adaptive(threshold, min_samples):
+    u, cm = ft.test_scene("mandelboxsort_box_fold", size=(20, 14))
+    u.mc_enabled = 1.0
+    u.adapt_enabled = 1.0
+    u.adapt_threshold = threshold
+    u.adapt_min_samples = min_samples
+    u.adapt_row_width = 1e6
+    return u, cm
+
+
+def test_adaptive_sampling_limits():
+    """A pixel stops sampling once converged and keeps the mean of the samples it took. With
+    an unbounded threshold every pixel converges at its min-samples count k: the image is the
+    plain k-sample render. With a negative threshold no pixel converges (the standard error is
+    never negative; at threshold 0 a pixel whose FP32 variance cancels to 0 still would):
+    the image is the plain render of all samples. Pixels past adapt_row_width are never
+    adaptive."""
+    for k in (2, 5, 8):
+        u, cm = adaptive(1e9, float(k))
+        a = oracle.render_mc(u, 12, code_map=cm)["rgb"]
+        u.adapt_enabled = 0.0
+        b = oracle.render_mc(u, k, code_map=cm)["rgb"]
+        assert np.allclose(a, b, rtol=1e-6, atol=1e-9), k
+    u, cm = adaptive(-1.0, 2.0)
+    a = oracle.render_mc(u, 6, code_map=cm)["rgb"]
+    u.adapt_enabled = 0.0
+    b = oracle.render_mc(u, 6, code_map=cm)["rgb"]
+    assert np.allclose(a, b, rtol=1e-6, atol=1e-9)
+    u, cm = adaptive(1e9, 2.0)
+    u.adapt_row_width = 10.0  # columns 10.. are outside the adaptive rows
+    a = oracle.render_mc(u, 6, code_map=cm)["rgb"]
+    u.adapt_enabled = 0.0
+    full, two = oracle.render_mc(u, 6, code_map=cm)["rgb"], oracle.render_mc(u, 2, code_map=cm)["rgb"]
+    assert np.allclose(a[:, :10], two[:, :10], rtol=1e-6, atol=1e-9)
+    assert np.allclose(a[:, 10:], full[:, 10:], rtol=1e-6, atol=1e-9)
