@@ -275,6 +275,14 @@ typedef struct nd_adjoint_io {
     uint32_t _pad;
 } nd_adjoint_io;
 
+/* The empty-space accelerator's skip distance (accel_skip, template.wgsl:426-447) at n
+ * points (xyz triples, host memory) with the device grid the context builds or reuses for
+ * the scene and cascades of `uniforms` (accel_enabled must be set). skip_out: n floats, host
+ * memory. No counterpart in the reference: it lets a host or a test check the device grid
+ * and its lookup against another implementation. */
+nd_status nd_accel_probe(nd_ctx* ctx, const void* uniforms, size_t uniforms_size, const float* pts, uint32_t n,
+                         float* skip_out);
+
 nd_status nd_adjoint(nd_ctx* ctx, const nd_uniforms* views, uint32_t n_views,
                      const nd_adjoint_io* io, nd_batch_stats* stats);
 

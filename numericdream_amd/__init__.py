@@ -59,6 +59,7 @@ def _load():
     lib.nd_autotune.argtypes = [vp, P(Job), u32, P(MarchOpts), P(Tuning)]
     lib.nd_adjoint.argtypes = [vp, P(Uniforms), u32, P(AdjointIO), P(BatchStats)]
     lib.nd_path_trace.argtypes = [vp, vp, C.c_size_t, u32, Rect, P(ImageOut), P(TraceStats)]
+    lib.nd_accel_probe.argtypes = [vp, vp, C.c_size_t, vp, u32, vp]
     lib.nd_config_job_count.argtypes = [u32]
     lib.nd_config_job_count.restype = u32
     lib.nd_uniforms_defaults.argtypes = [P(Uniforms)]
@@ -288,6 +289,16 @@ class Renderer:
         st = BatchStats()
         self._check(lib.nd_adjoint(self._h, arr, n, C.byref(io), C.byref(st)), "nd_adjoint")
         return loss.value, np.array(grad[:]), int(used.value), st
+
+    def accel_probe(self, u, pts):
+        """The accelerator's skip distance at points (n, 3) with the device grid of u's scene
+        and cascades (float32, host)."""
+        p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.zeros(len(p), np.float32)
+        ub = Uniforms.from_buffer_copy(bytes(u))
+        self._check(lib.nd_accel_probe(self._h, C.byref(ub), C.sizeof(Uniforms), p.ctypes.data, len(p),
+                                       out.ctypes.data), "nd_accel_probe")
+        return out
 
     def path_trace(self, u, spp, rect=None, color=None, depth=None, normal=None):
         """Monte Carlo path trace of rect (default: the whole frame) with spp samples per pixel

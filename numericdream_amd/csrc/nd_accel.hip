@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "nd_ctx.h"
+#include "nd_accel.hpp"
 #include "nd_de.hpp"
 
 namespace nd {
@@ -116,5 +117,51 @@ nd_status nd_accel_prepare(nd_ctx* c, DevJob* jobs, uint32_t n) {
         if (jobs[i].accel_levels <= 0) continue;
         jobs[i].accel = (const float*)c->accel.p + find(keys[i])->offset;
     }
+    return ND_OK;
+}
+
+namespace nd {
+// accel_skip at n points (xyz triples) with the grid of job J.
+__global__ void k_accel_probe(const DevJob* __restrict__ J, const float* __restrict__ pts, uint32_t n,
+                              float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const AccelView av = accel_view(*J);
+    out[i] = av.grid ? accel_skip(av, mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2])) : 0.f;
+}
+}  // namespace nd
+
+extern "C" nd_status nd_accel_probe(nd_ctx* c, const void* uniforms, size_t size, const float* pts, uint32_t n,
+                                    float* skip_out) {
+    if (!c) return ND_E_INVALID;
+    if (!uniforms || size != sizeof(nd_uniforms)) return nd_fail(c, ND_E_INVALID, "uniforms_size %zu != %zu", size, sizeof(nd_uniforms));
+    if (n == 0) return ND_OK;
+    if (!pts || !skip_out) return nd_fail(c, ND_E_INVALID, "accel probe needs points and an output");
+    if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
+    const nd_uniforms& u = *(const nd_uniforms*)uniforms;
+    if (u.accel_enabled < 0.5f) return nd_fail(c, ND_E_INVALID, "the accelerator is off in these uniforms");
+    nd_march_opts o{ND_NORMAL_TETRA, 0, 0, ND_KERNEL_PERSISTENT};
+    DevJob job;
+    std::memset(&job, 0, sizeof(job));
+    nd_status st = nd_pack_job(c, u, o, nd_rect{0, 0, 0, 0}, job);
+    if (st != ND_OK) return st;
+    if ((st = nd_accel_prepare(c, &job, 1)) != ND_OK) return st;
+    DevJob* d_job = nullptr;
+    float *d_pts = nullptr, *d_out = nullptr;
+    hipError_t e = hipMalloc((void**)&d_job, sizeof(DevJob));
+    if (e == hipSuccess) e = hipMalloc((void**)&d_pts, sizeof(float) * 3 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_out, sizeof(float) * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_job, &job, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pts, pts, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(nd::k_accel_probe, dim3((n + 255) / 256), dim3(256), 0, c->stream, d_job, d_pts, n, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(skip_out, d_out, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_job);
+    (void)hipFree(d_pts);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return nd_hip_fail(c, e, "accelerator probe");
     return ND_OK;
 }

@@ -36,18 +36,31 @@ NDD AccelView accel_view(const DevJob& J) {
 // and reaches half0 2^l (1 + 2/res) from it at most, so no level below
 // ceil(log2(|p - centre|_inf / half0)) - 1 can hold p; the search starts one level lower
 // still (the hardware log2 is accurate to an ulp), which finds the same level as a search
-// from level 0. Cells are indexed by floor((p - corner) * (1 / cell)).
+// from level 0. The log is clamped at 0 before the conversion: at the centre itself (u = 0)
+// it is -inf, whose conversion saturates to INT_MIN and would wrap past every level.
+// Cells are indexed by floor((p - corner) / cell) with a correctly rounded division, as the
+// reference's floor((p - prm.xyz) / cell) (cell_index).
+NDD float cell_index(float x, float cell, float ic) {
+    // The product with the hardware reciprocal decides unless it lies within 1e-4 of a cell
+    // boundary, where it can round to the other side: the camera sits exactly on a boundary
+    // of every cascade (their common centre), so every camera ray's first lookup is one.
+    const float q = x * ic;
+    const float f = floorf(q);
+    const float fr = q - f;
+    if (fr < 1e-4f || fr > 0.9999f) return floorf(__fdiv_rn(x, cell));
+    return f;
+}
 NDD float accel_skip(const AccelView& A, const V3<float>& p) {
     const float u = fmaxf(fabsf(p.x - A.cx), fmaxf(fabsf(p.y - A.cy), fabsf(p.z - A.cz)));
-    int l = (int)ceilf(__builtin_amdgcn_logf(u * A.inv_half0)) - 2;
+    int l = (int)ceilf(fmaxf(__builtin_amdgcn_logf(u * A.inv_half0), 0.f)) - 2;
     l = l < 0 ? 0 : l;
     const float fres = (float)A.res;
     for (int k = 0; k < 4 && l < A.levels; ++k, ++l) {
         const float* q = A.prm + 4 * l;
         const float ic = __builtin_amdgcn_rcpf(q[3]);
-        const float fx = floorf((p.x - q[0]) * ic);
-        const float fy = floorf((p.y - q[1]) * ic);
-        const float fz = floorf((p.z - q[2]) * ic);
+        const float fx = cell_index(p.x - q[0], q[3], ic);
+        const float fy = cell_index(p.y - q[1], q[3], ic);
+        const float fz = cell_index(p.z - q[2], q[3], ic);
         if (fx < 0.f || fy < 0.f || fz < 0.f || fx >= fres || fy >= fres || fz >= fres) continue;
         const size_t idx = (((size_t)l * A.res + (size_t)fz) * A.res + (size_t)fy) * A.res + (size_t)fx;
         return A.grid[idx];

@@ -36,6 +36,7 @@ namespace nd {
 struct McStop {
     float color[3], position, glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
     float film_thickness, film_ior, film_strength, film_angle_scale, film_perturb, film_perturb_scale;
+    float subsurface, subsurface_radius;
 };
 struct McInst {
     int color_count, trap_repeat;
@@ -54,7 +55,9 @@ struct McScene {
     McLight lights[8];
     int light_count;
     int shadows_on, specular_on, ao_on, reflections_on, indirect_on, sky_on;
-    int strips_on, refraction_on, dispersion_on, iridescence_on;  // apply_part_mask
+    int strips_on, refraction_on, dispersion_on, iridescence_on, subsurface_on;  // apply_part_mask
+    int sss;           // some stop has subsurface: the shade pass runs its SSS variant
+    int sss_steps;     // min(max_steps, SSS_PATH_MAX_STEPS = 128)
     int transp;        // scene_has_transparency: shadow rays cross transparent occluders
     int transmissive;  // some visible stop has transparency: paths can enter solids
     int light_pick;    // mc_enabled and mc_light_pick
@@ -189,6 +192,7 @@ NDD V3<float> sky_color(const McScene& S, const V3<float>& dir) {
 struct Mat {
     float color[3], glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
     float film_thickness, film_ior, film_strength, film_angle_scale, film_perturb, film_perturb_scale;
+    float subsurface, subsurface_radius;
 };
 // default_material (template.wgsl:1965-1978)
 NDD Mat mat_default() {
@@ -198,6 +202,7 @@ NDD Mat mat_default() {
     m.abbe = 0.f; m.inner_max_steps = 64.f;
     m.film_thickness = 400.f; m.film_ior = 1.8f; m.film_strength = 0.f; m.film_angle_scale = 1.f;
     m.film_perturb = 0.f; m.film_perturb_scale = 0.2f;
+    m.subsurface = 0.f; m.subsurface_radius = 0.1f;  // DEFAULT_SSS_RADIUS
     return m;
 }
 NDD Mat mat_stop(const McStop& s) {
@@ -207,6 +212,7 @@ NDD Mat mat_stop(const McStop& s) {
     m.roughness = s.roughness; m.abbe = s.abbe; m.inner_max_steps = s.inner_max_steps;
     m.film_thickness = s.film_thickness; m.film_ior = s.film_ior; m.film_strength = s.film_strength;
     m.film_angle_scale = s.film_angle_scale; m.film_perturb = s.film_perturb; m.film_perturb_scale = s.film_perturb_scale;
+    m.subsurface = s.subsurface; m.subsurface_radius = s.subsurface_radius;
     return m;
 }
 // apply_part_mask (template.wgsl:394-402) for the parts this path reads
@@ -216,6 +222,7 @@ NDD Mat part_mask(Mat m, const McScene& S) {
     if (!S.refraction_on) m.transparency = 0.f;
     if (!S.dispersion_on) m.abbe = 0.f;
     if (!S.iridescence_on) m.film_strength = 0.f;
+    if (!S.subsurface_on) m.subsurface = 0.f;
     return m;
 }
 // mix_material(a, b, h) = mix(b, a, h) (template.wgsl:1980-2001)
@@ -235,6 +242,8 @@ NDD Mat mat_mix(const Mat& a, const Mat& b, float h) {
     r.film_angle_scale = b.film_angle_scale * (1.f - h) + a.film_angle_scale * h;
     r.film_perturb = b.film_perturb * (1.f - h) + a.film_perturb * h;
     r.film_perturb_scale = b.film_perturb_scale * (1.f - h) + a.film_perturb_scale * h;
+    r.subsurface = b.subsurface * (1.f - h) + a.subsurface * h;
+    r.subsurface_radius = b.subsurface_radius * (1.f - h) + a.subsurface_radius * h;
     return r;
 }
 // sample_gradient + orbit_trap_to_strip (template.wgsl:2003-2035)
@@ -774,6 +783,239 @@ NDD Sp8 shade_lights(const McArgs& a, const Basis& B, const V3<float>& pos, cons
     return r;
 }
 
+// ----------------------------------------------------------------------------- subsurface
+// Subsurface / subsurface_setup (template.wgsl:3088-3114): amount, radius (at least 0.0005),
+// each wavelength's reach = max((albedo / peak)^2, 0.05), and the blurred normal: the
+// tetrahedral gradient at pos + n radius/2 with taps radius/4.
+struct Sss {
+    float amount, radius;
+    Sp8 reach;
+    V3<float> blur;
+};
+template <bool EXT>
+NDD Sss sss_setup(const DevScene& sc, const V3<float>& pos, const V3<float>& n, const Mat& m, const Sp8& albedo,
+                  unsigned long long& de) {
+    Sss s;
+    s.amount = fclamp(m.subsurface, 0.f, 1.f);
+    s.radius = fmaxf(m.subsurface_radius, 0.0005f);  // SSS_MIN_RADIUS
+    s.blur = n;
+    s.reach = sp_splat(0.f);
+    if (s.amount <= 0.001f) {  // MATERIAL_EPSILON
+        s.amount = 0.f;
+        return s;
+    }
+    const float peak = fmaxf(sp_max(albedo), 1e-4f);  // EPSILON
+    for (int k = 0; k < 8; ++k) {
+        const float rel = albedo.v[k] / peak;
+        s.reach.v[k] = fmaxf(rel * rel, 0.05f);  // SSS_REACH_FLOOR
+    }
+    const float blur = s.radius * 0.5f;  // SSS_BLUR_PER_RADIUS
+    const float e = blur * 0.5f;         // SSS_BLUR_TAP_FRACTION
+    const V3<float> c = mk3(pos.x + n.x * blur, pos.y + n.y * blur, pos.z + n.z * blur);
+    const float K[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+    V3<float> g = mk3(0.f, 0.f, 0.f);
+    for (int i = 0; i < 4; ++i) {
+        const float d = scene_de<false, EXT>(sc, mk3(c.x + K[i][0] * e, c.y + K[i][1] * e, c.z + K[i][2] * e), nullptr);
+        g = mk3(g.x + K[i][0] * d, g.y + K[i][1] * d, g.z + K[i][2] * d);
+    }
+    de += 4;
+    const float len = sqrtf(dot3(g, g));
+    if (len > 1e-12f) s.blur = mk3(g.x / len, g.y / len, g.z / len);  // EPSILON_TINY
+    return s;
+}
+// diffuse_weight + subsurface_wrap (template.wgsl:3116-3134): n.l blended toward the blurred
+// normal's by the reach, wrapped by 0.75 reach (SSS_WRAP), plus light from behind
+NDD Sp8 sss_weight(const Sss& s, float ndl, const V3<float>& ld) {
+    const float nb = dot3(s.blur, ld);
+    Sp8 r;
+    for (int k = 0; k < 8; ++k) {
+        const float re = s.reach.v[k];
+        const float w = 0.75f * re;
+        const float wrap = fmaxf(ndl * (1.f - re) + nb * re + w, 0.f) / (1.f + w);
+        r.v[k] = (1.f - s.amount) * fmaxf(ndl, 0.f) + s.amount * (wrap + fmaxf(-ndl, 0.f));
+    }
+    return r;
+}
+// subsurface_transmit (template.wgsl:3136-3138)
+NDD Sp8 sss_transmit(const Sss& s, float depth) {
+    Sp8 r;
+    for (int k = 0; k < 8; ++k) r.v[k] = expf(-depth / (s.radius * s.reach.v[k]));
+    return r;
+}
+// shadow_factor + visibility_march for one ray, in line (template.wgsl:2282-2342, 2365-2375):
+// the same march as k_mc_shadow, for the rest of a subsurface light path past a surface
+// without subsurface
+template <bool EXT>
+NDD Sp8 vis_inline(const McArgs& a, const Basis& B, const V3<float>& o, const V3<float>& ld, float ldist, float k,
+                   unsigned long long& de) {
+    const DevScene& sc = *a.scene;
+    const McScene& S = *a.S;
+    const DevJob& J = *a.job;
+    const bool acc = J.accel != nullptr;
+    const AccelView av = accel_view(J);
+    float t = 0.004f, res = 1.f, flo = 0.004f;  // SHADOW_BIAS as t0, SHADOW_STEP_FLOOR
+    bool ins = false;
+    Sp8 tint = sp_splat(1.f);
+    for (int step = 0; step < S.shadow_steps; ++step) {
+        if (t >= ldist) break;
+        const V3<float> ps = mk3(o.x + ld.x * t, o.y + ld.y * t, o.z + ld.z * t);
+        if (!ins && acc) {
+            const float skip = accel_skip(av, ps);
+            if (shadow_skip_applies(skip, k, res, t)) {
+                t += skip;
+                continue;
+            }
+        }
+        const float d = scene_de<false, EXT>(sc, ps, nullptr);
+        de++;
+        if (ins) {
+            if (d > 0.001f * t + 0.0005f) ins = false;  // SHADOW_EXIT_SLOPE, _BIAS
+            t += fmaxf(fabsf(d), flo);
+            flo *= 1.15f;  // SHADOW_FLOOR_GROWTH
+            continue;
+        }
+        if (d < 0.0005f) {  // SHADOW_HIT_EPS
+            if (!S.transp) return sp_splat(0.f);
+            const Mat om = hit_material<EXT>(sc, S, ps);
+            de++;
+            const float al = fclamp(om.transparency, 0.f, 1.f);
+            if (al < 0.01f) return sp_splat(0.f);  // SHADOW_OPAQUE_ALPHA
+            const Sp8 tc = from_rgb(B, 1.f + (om.color[0] - 1.f) * al, 1.f + (om.color[1] - 1.f) * al,
+                                    1.f + (om.color[2] - 1.f) * al);
+            for (int q = 0; q < 8; ++q) tint.v[q] *= tc.v[q] * al;
+            if (sp_max(tint) < 0.004f) return sp_splat(0.f);  // SHADOW_TINT_CUT
+            ins = true;
+            t += flo;
+            continue;
+        }
+        res = fminf(res, k * d / t);
+        t += fmaxf(d, 0.001f);  // SHADOW_MIN_STEP
+    }
+    const float vis = fclamp(res, 0.f, 1.f);
+    for (int q = 0; q < 8; ++q) tint.v[q] *= vis;
+    return tint;
+}
+// subsurface_light_path (template.wgsl:3140-3218): the transmission toward a light through
+// subsurface material. Inside, steps of max(|DE|, floor) with the floor from
+// max(0.05 radius, 0.0005) growing by 1 + 6/64; the exit (DE > 0.0005 t + 0.00005 after the
+// warm-up) is refined by 7 bisections; more than 6 radii inside gives 0. Outside, steps of
+// max(DE, 0.001) with accelerator skips; a surface without subsurface hands over to the
+// soft shadow march. Without occlusion the march ends at the first exit.
+template <bool EXT>
+NDD Sp8 sss_light_path(const McArgs& a, const Basis& B, const V3<float>& pos, const V3<float>& n, bool start_inside,
+                       const Sss& s, const McLight& L, const V3<float>& ld, float ldist, bool occlusion,
+                       unsigned long long& de) {
+    if (!start_inside && !occlusion) return sp_splat(1.f);
+    const DevScene& sc = *a.scene;
+    const McScene& S = *a.S;
+    const DevJob& J = *a.job;
+    const bool acc = J.accel != nullptr;
+    const AccelView av = accel_view(J);
+    const V3<float> o = start_inside ? pos : mk3(pos.x + n.x * 0.004f, pos.y + n.y * 0.004f, pos.z + n.z * 0.004f);
+    const float cap = s.radius * 6.f;                          // SSS_PROBE_DEPTH_RADII
+    const float step_base = fmaxf(s.radius * 0.05f, 0.0005f);  // SSS_PROBE_STEP_PER_RADIUS, INSIDE_MIN_STEP
+    const float growth = 1.f + 6.f / 64.f;                     // INSIDE_STEP_GROWTH / SSS_MAX_STEPS
+    bool inside = start_inside;
+    float t = 0.f, prev = 0.f, entry = 0.f, depth = 0.f, flo = step_base;
+    int inner = 0;
+    for (int i = 0; i < S.sss_steps; ++i) {
+        if (t >= ldist) break;
+        const V3<float> p = mk3(o.x + ld.x * t, o.y + ld.y * t, o.z + ld.z * t);
+        if (!inside && acc) {
+            const float skip = accel_skip(av, p);
+            if (skip > 0.001f) {  // SHADOW_MIN_STEP
+                t += skip;
+                continue;
+            }
+        }
+        const float d = scene_de<false, EXT>(sc, p, nullptr);
+        de++;
+        if (inside) {
+            if (inner > 1 && d > 0.0005f * t + 0.00005f) {  // INSIDE_WARMUP_STEPS, inside_exited
+                float lo = prev, hi = t;                     // refine_inside_exit, 7 steps
+                for (int r = 0; r < 7; ++r) {
+                    const float mid = 0.5f * (lo + hi);
+                    const float dm = scene_de<false, EXT>(sc, mk3(o.x + ld.x * mid, o.y + ld.y * mid, o.z + ld.z * mid), nullptr);
+                    if (dm > 0.0005f * mid + 0.00005f) hi = mid;
+                    else lo = mid;
+                }
+                de += 7;
+                depth += hi - entry;
+                inside = false;
+                if (!occlusion) break;
+                t = hi + 0.004f;  // SHADOW_BIAS
+                continue;
+            }
+            prev = t;
+            t += fmaxf(fabsf(d), flo);
+            flo *= growth;
+            inner++;
+            if (depth + t - entry > cap) return sp_splat(0.f);
+            continue;
+        }
+        if (d < 0.0005f) {  // SHADOW_HIT_EPS
+            const Mat hm = part_mask(hit_material<EXT>(sc, S, p), S);
+            de++;
+            if (!(hm.subsurface > 0.001f)) {
+                Sp8 rest = sp_splat(1.f);
+                if (L.cast) {
+                    // shadow_factor(p - 2 bias dir, dir, light, dir, ldist - t + 2 bias)
+                    const V3<float> sp = mk3(p.x - ld.x * 0.008f, p.y - ld.y * 0.008f, p.z - ld.z * 0.008f);
+                    rest = vis_inline<EXT>(a, B, mk3(sp.x + ld.x * 0.004f, sp.y + ld.y * 0.004f, sp.z + ld.z * 0.004f),
+                                           ld, ldist - t + 0.008f, L.k, de);
+                }
+                const Sp8 tr = sss_transmit(s, depth);
+                for (int q = 0; q < 8; ++q) rest.v[q] *= tr.v[q];
+                return rest;
+            }
+            inside = true;
+            entry = t;
+            prev = t;
+            inner = 1;
+            flo = step_base;
+            t += flo;
+            continue;
+        }
+        t += fmaxf(d, 0.001f);
+    }
+    if (inside) depth += fminf(t, ldist) - entry;
+    return sss_transmit(s, depth);
+}
+// shade_local for a subsurface material (template.wgsl:3222-3331, the sss.amount > 0 branch):
+// every light is evaluated here through subsurface_light_path (nothing is queued and no
+// light is picked); diffuse weight from diffuse_weight, specular only where n.l > 0.
+template <bool EXT>
+NDD Sp8 shade_lights_sss(const McArgs& a, const Basis& B, const V3<float>& pos, const V3<float>& dir,
+                         const V3<float>& n, const Mat& m, const Sp8& film, const Sp8& ambient, bool shadows,
+                         unsigned long long& de) {
+    const McScene& S = *a.S;
+    const V3<float> view = mk3(-dir.x, -dir.y, -dir.z);
+    const float spec_power = 4.f + 124.f * fclamp(m.glossiness, 0.f, 1.f);
+    Sp8 albedo = from_rgb(B, m.color[0], m.color[1], m.color[2]);
+    for (int k = 0; k < 8; ++k) albedo.v[k] *= film.v[k];
+    const Sss s = sss_setup<EXT>(*a.scene, pos, n, m, albedo, de);
+    Sp8 acc = sp_splat(0.f);
+    for (int i = 0; i < S.light_count; ++i) {
+        const McLight& L = S.lights[i];
+        V3<float> ld, col;
+        float dist;
+        light_of(L, S.max_dist, pos, &ld, &col, &dist);
+        if (fmaxf(col.x, fmaxf(col.y, col.z)) < 1e-5f) continue;  // SURFACE_LIGHT_CUTOFF
+        const float ndl = dot3(n, ld);
+        const V3<float> hd = vnorm3(mk3(ld.x + view.x, ld.y + view.y, ld.z + view.z));
+        const float spec = S.specular_on ? powf(fmaxf(dot3(n, hd), 0.f), spec_power) * m.glossiness : 0.f;
+        const Sp8 w = sss_weight(s, ndl, ld);
+        const float sspec = ndl > 0.f ? spec : 0.f;
+        if (sp_max(w) <= 0.f && sspec <= 0.f) continue;
+        const Sp8 ls = from_rgb(B, col.x, col.y, col.z);
+        const Sp8 tr = sss_light_path<EXT>(a, B, pos, n, ndl <= 0.f, s, L, ld, dist, shadows && L.cast, de);
+        for (int k = 0; k < 8; ++k) acc.v[k] += ls.v[k] * tr.v[k] * (albedo.v[k] * w.v[k] + sspec * film.v[k]);
+    }
+    Sp8 r;
+    for (int k = 0; k < 8; ++k) r.v[k] = albedo.v[k] * ambient.v[k] + acc.v[k];
+    return r;
+}
+
 // refract (WGSL builtin) and refract_safe (template.wgsl:3387-3394): the normal is turned to
 // face the ray; total internal reflection returns the mirror direction.
 NDD V3<float> refract_safe(const V3<float>& d, const V3<float>& raw_n, float eta, bool* tir) {
@@ -798,8 +1040,9 @@ NDD float cauchy_ior(float ior_d, float abbe, float lambda) {
     return ior_d + b * (1.f / (lambda * lambda) - 2.89626f);
 }
 
-// --- shade (path_shade, template.wgsl:3902-4124). One path per lane.
-template <bool EXT>
+// --- shade (path_shade, template.wgsl:3902-4124). One path per lane. With SSS, a hit on a
+// subsurface material evaluates its lights in line through the material.
+template <bool EXT, bool SSS>
 __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
     const uint32_t* qin = ping ? a.P.live_b : a.P.live_a;
     uint32_t* qout = ping ? a.P.live_a : a.P.live_b;
@@ -935,7 +1178,9 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
                         de += 5;
                         ambient = sp_splat(0.15f * fclamp(1.f - 2.f * occ / radius, 0.f, 1.f));
                     }
-                    const Sp8 lit = shade_lights(a, B, hp, dir, nrm, mat, film, ambient, !reflected && S.shadows_on, wscale, p, sx, sy);
+                    const Sp8 lit = (SSS && mat.subsurface > 0.001f)
+                                        ? shade_lights_sss<EXT>(a, B, hp, dir, nrm, mat, film, ambient, !reflected && S.shadows_on, de)
+                                        : shade_lights(a, B, hp, dir, nrm, mat, film, ambient, !reflected && S.shadows_on, wscale, p, sx, sy);
                     for (int k = 0; k < 8; ++k) rad.v[k] += wscale.v[k] * lit.v[k];
                 }
                 // the continuation: r = hash * p_cont picks reflection below p_reflect, else
@@ -1285,6 +1530,7 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
             d.film_thickness = c.film_thickness; d.film_ior = c.film_ior; d.film_strength = c.film_strength;
             d.film_angle_scale = c.film_angle_scale; d.film_perturb = c.film_perturb;
             d.film_perturb_scale = c.film_perturb_scale;
+            d.subsurface = c.subsurface; d.subsurface_radius = c.subsurface_radius;
         }
     }
     S.light_count = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min((int)u.light_count, ND_MAX_LIGHTS);
@@ -1314,6 +1560,13 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     S.refraction_on = !(parts & ND_PART_REFRACTION);
     S.dispersion_on = !(parts & ND_PART_DISPERSION);
     S.iridescence_on = !(parts & ND_PART_IRIDESCENCE);
+    S.subsurface_on = !(parts & ND_PART_SUBSURFACE);
+    S.sss = 0;
+    if (S.subsurface_on)
+        for (int i = 0; i < ND_MAX_INSTANCES; ++i)
+            for (int c = 0; c < std::min((int)u.instances[i].color_count, 16); ++c)
+                if (u.instances[i].colors[c].subsurface > 0.001f) S.sss = 1;  // MATERIAL_EPSILON
+    S.sss_steps = std::min((int)u.max_steps, 128);
     S.transp = u.has_transparency > 0.5f && S.refraction_on;
     S.transmissive = 0;
     if (S.refraction_on)
@@ -1354,12 +1607,12 @@ static uint32_t mc_resident(K kernel) {
 
 void nd_resident_mc(uint32_t* r) {
     r[RES_MC_EXTEND] = mc_resident(nd::k_mc_extend<false, false>);
-    r[RES_MC_SHADE] = mc_resident(nd::k_mc_shade<false>);
+    r[RES_MC_SHADE] = mc_resident(nd::k_mc_shade<false, false>);
     r[RES_MC_SHADOW] = mc_resident(nd::k_mc_shadow<false, false>);
     r[RES_MC_IND_MARCH] = mc_resident(nd::k_mc_indirect_march<false>);
     r[RES_MC_IND_SHADE] = mc_resident(nd::k_mc_indirect_shade<false>);
     r[RES_MC_EXTEND_EXT] = mc_resident(nd::k_mc_extend<true, false>);
-    r[RES_MC_SHADE_EXT] = mc_resident(nd::k_mc_shade<true>);
+    r[RES_MC_SHADE_EXT] = mc_resident(nd::k_mc_shade<true, false>);
     r[RES_MC_SHADOW_EXT] = mc_resident(nd::k_mc_shadow<true, false>);
     r[RES_MC_IND_MARCH_EXT] = mc_resident(nd::k_mc_indirect_march<true>);
     r[RES_MC_IND_SHADE_EXT] = mc_resident(nd::k_mc_indirect_shade<true>);
@@ -1376,12 +1629,6 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field and approximations are not implemented");
     for (int i = 0; i < std::min((int)u.light_count, 8); ++i)
         if ((int)(u.lights[i].light_type + 0.5f) == 2) return nd_fail(c, ND_E_UNSUPPORTED, "ray lights are not implemented");
-    for (int i = 0; i < ND_MAX_INSTANCES; ++i)
-        for (int s = 0; s < (int)u.instances[i].color_count && s < 16; ++s) {
-            const nd_color_stop& cs = u.instances[i].colors[s];
-            if (cs.subsurface > 0.001f)
-                return nd_fail(c, ND_E_UNSUPPORTED, "instance %d stop %d: subsurface scattering is not implemented", i, s);
-        }
     if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
     nd_march_opts o{ND_NORMAL_TETRA, 0, 0, ND_KERNEL_PERSISTENT};
     DevJob job;
@@ -1468,6 +1715,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     const bool inside = S.transmissive != 0;  // paths can enter solids: extend runs march_inside
     const bool transp = S.transp != 0;        // shadow rays cross transparent occluders
     const bool adapt = S.adapt_on != 0;       // adaptive sampling: per-pixel state in c->mc_adapt
+    const bool sss = S.sss != 0;              // subsurface materials: the shade pass's SSS variant
     if (adapt) {
         if ((st = c->mc_adapt.ensure(c, 32 * (size_t)npix)) != ND_OK) return st;
         a.adapt = (float*)c->mc_adapt.p;
@@ -1491,13 +1739,15 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
             if (ext) {
                 if (inside) hipLaunchKernelGGL((nd::k_mc_extend<true, true>), dim3(g_ext), dim3(64), 0, st, a, ping);
                 else hipLaunchKernelGGL((nd::k_mc_extend<true, false>), dim3(g_ext), dim3(64), 0, st, a, ping);
-                hipLaunchKernelGGL(nd::k_mc_shade<true>, dim3(g_shade), dim3(64), 0, st, a, ping);
+                if (sss) hipLaunchKernelGGL((nd::k_mc_shade<true, true>), dim3(g_shade), dim3(64), 0, st, a, ping);
+                else hipLaunchKernelGGL((nd::k_mc_shade<true, false>), dim3(g_shade), dim3(64), 0, st, a, ping);
                 if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<true, true>), dim3(g_shadow), dim3(64), 0, st, a);
                 else hipLaunchKernelGGL((nd::k_mc_shadow<true, false>), dim3(g_shadow), dim3(64), 0, st, a);
             } else {
                 if (inside) hipLaunchKernelGGL((nd::k_mc_extend<false, true>), dim3(g_ext), dim3(64), 0, st, a, ping);
                 else hipLaunchKernelGGL((nd::k_mc_extend<false, false>), dim3(g_ext), dim3(64), 0, st, a, ping);
-                hipLaunchKernelGGL(nd::k_mc_shade<false>, dim3(g_shade), dim3(64), 0, st, a, ping);
+                if (sss) hipLaunchKernelGGL((nd::k_mc_shade<false, true>), dim3(g_shade), dim3(64), 0, st, a, ping);
+                else hipLaunchKernelGGL((nd::k_mc_shade<false, false>), dim3(g_shade), dim3(64), 0, st, a, ping);
                 if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<false, true>), dim3(g_shadow), dim3(64), 0, st, a);
                 else hipLaunchKernelGGL((nd::k_mc_shadow<false, false>), dim3(g_shadow), dim3(64), 0, st, a);
             }
@@ -1522,7 +1772,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         struct McGraphKey {
             nd::McArgs args;
             uint32_t grid[5];
-            int max_depth, indirect, ext, inside, transp, adapt;
+            int max_depth, indirect, ext, inside, transp, adapt, sss;
             uint64_t paths;
         } key;
         static_assert(sizeof(McGraphKey) <= sizeof(c->mc_graph_key), "graph key storage");
@@ -1533,6 +1783,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         key.inside = inside;
         key.transp = transp;
         key.adapt = adapt;
+        key.sss = sss;
         key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
         if (!c->mc_graph_exec || !c->graph_valid || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
             if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }

@@ -40,6 +40,7 @@ def lib():
         L.ndo_primary_dir.argtypes = [vp, i32, i32, vp]
         L.ndo_march_inside.argtypes = [vp, vp, vp, vp, i32, vp]
         L.ndo_visibility.argtypes = [vp, vp, C.c_double, vp, vp, C.c_double, C.c_double, i32, vp]
+        L.ndo_subsurface_path.argtypes = [vp, vp, C.c_double, vp, vp, i32, i32, i32, C.c_double, C.c_double, vp, vp]
         L.ndo_render_gbuffer.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp,
                                          i32, vp]
         L.ndo_flops_per_de.argtypes = [vp, vp, vp, i32, vp]
@@ -124,6 +125,19 @@ def visibility(u, hero, origin, direction, light_dist, k, steps, code_map=None):
     cm, cmp_ = _cm(code_map)
     lib().ndo_visibility(_u(u), cmp_, float(hero), _p(o), _p(d), float(light_dist), float(k), int(steps), _p(out))
     return out
+
+
+def subsurface_path(u, hero, pos, normal, start_inside, light, occlusion, amount, radius, albedo, code_map=None):
+    """subsurface_light_path toward light `light` from pos: (8-lane transmission, DE
+    evaluations, blurred normal, 8-lane reach) for a material of colour albedo."""
+    out = np.zeros(20)
+    p = np.ascontiguousarray(pos, np.float64)
+    n = np.ascontiguousarray(normal, np.float64)
+    a = np.ascontiguousarray(albedo, np.float64)
+    cm, cmp_ = _cm(code_map)
+    lib().ndo_subsurface_path(_u(u), cmp_, float(hero), _p(p), _p(n), int(start_inside), int(light), int(occlusion),
+                              float(amount), float(radius), _p(a), _p(out))
+    return out[:8].copy(), int(out[8]), out[9:12].copy(), out[12:20].copy()
 
 
 def primary_dir(u, px, py):

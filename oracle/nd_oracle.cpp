@@ -497,6 +497,19 @@ int ndo_helper(int op, const double* a, double* out) {
         case 31: { D3 r = film_noise(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
         case 32: { float h[3]; hash33(a[0], a[1], a[2], h); out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; return 0; }
         case 33: { D3 r = oct_quantize(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 34: { Subsurface<double> ss{};  // diffuse_weight: amount, n.l, blurred n.l, 8 reaches
+                   ss.amount = a[0]; ss.radius = 0.1; ss.blur_n = D3(1.0, 0.0, 0.0);
+                   for (int k = 0; k < 8; ++k) ss.reach.v[k] = a[3 + k];
+                   const double c = a[2], sn = std::sqrt(std::max(0.0, 1.0 - c * c));
+                   Spec<double> w = diffuse_weight(ss, a[1], D3(c, sn, 0.0));
+                   for (int k = 0; k < 8; ++k) out[k] = w.v[k];
+                   return 0; }
+        case 35: { Subsurface<double> ss{};  // subsurface_transmit: radius, depth, 8 reaches
+                   ss.amount = 1.0; ss.radius = a[0];
+                   for (int k = 0; k < 8; ++k) ss.reach.v[k] = a[2 + k];
+                   Spec<double> w = subsurface_transmit(ss, a[1]);
+                   for (int k = 0; k < 8; ++k) out[k] = w.v[k];
+                   return 0; }
         case 28: { Spectral<double> sp(a[0]); Mat m = default_mat(); m.ior = a[7]; m.abbe = a[8];
                    Spec<double> w = dispersion_weights(sp, D3(a[1], a[2], a[3]), D3(a[4], a[5], a[6]), m, a[9] > 0.5,
                                                        D3(a[10], a[11], a[12]), a[13]);
@@ -525,6 +538,33 @@ int ndo_visibility(const nd_uniforms* u, const int32_t* cm, double hero, const d
     Spec<double> v = visibility_march(*u, s, sp, V3<double>(origin[0], origin[1], origin[2]), V3<double>(dir[0], dir[1], dir[2]),
                                       light_dist, k, steps, (int*)nullptr);
     for (int i = 0; i < 8; ++i) out[i] = v.v[i];
+    return 0;
+}
+
+// subsurface_light_path from pos (surface normal n) toward light `light`, for a material of
+// colour albedo_rgb with the given subsurface amount and radius: out[0..7] = the spectral
+// transmission, out[8] = DE evaluations, out[9..11] = the blurred normal, out[12..19] = reach.
+int ndo_subsurface_path(const nd_uniforms* u, const int32_t* cm, double hero, const double* pos, const double* n,
+                        int start_inside, int light, int occlusion, double amount, double radius,
+                        const double* albedo_rgb, double* out) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    Spectral<double> sp(hero);
+    Mat m = default_mat();
+    for (int k = 0; k < 3; ++k) m.color[k] = albedo_rgb[k];
+    m.subsurface = amount;
+    m.subsurface_radius = radius;
+    const V3<double> p(pos[0], pos[1], pos[2]), nn(n[0], n[1], n[2]);
+    int ev = 0;
+    const Subsurface<double> ss = subsurface_setup(s, p, nn, m, sp.from_rgb(m.color[0], m.color[1], m.color[2]), &ev);
+    V3<double> ld, col;
+    double dist;
+    light_sample(u->lights[light], (double)u->max_dist, p, &ld, &col, &dist);
+    Spec<double> v = subsurface_light_path(*u, s, sp, p, nn, start_inside != 0, ss, u->lights[light], ld, dist,
+                                           occlusion != 0, &ev);
+    for (int k = 0; k < 8; ++k) out[k] = v.v[k];
+    out[8] = ev;
+    out[9] = ss.blur_n.x; out[10] = ss.blur_n.y; out[11] = ss.blur_n.z;
+    for (int k = 0; k < 8; ++k) out[12 + k] = ss.reach.v[k];
     return 0;
 }
 

@@ -145,19 +145,23 @@ struct Mat {
     double color[3];
     double glossiness, transparency, reflectiveness, ior, roughness, abbe, inner_max_steps;
     double film_thickness, film_ior, film_strength, film_angle_scale, film_perturb, film_perturb_scale;
+    double subsurface, subsurface_radius;
 };
 inline Mat stop_mat(const nd_color_stop& c) {
     return Mat{{c.color[0], c.color[1], c.color[2]}, c.glossiness, c.transparency, c.reflectiveness, c.ior, c.roughness,
                c.abbe, c.inner_max_steps, c.film_thickness, c.film_ior, c.film_strength, c.film_angle_scale,
-               c.film_perturb, c.film_perturb_scale};
+               c.film_perturb, c.film_perturb_scale, c.subsurface, c.subsurface_radius};
 }
-// default_material (template.wgsl:1965-1978)
-inline Mat default_mat() { return Mat{{0.8, 0.8, 0.85}, 0.3, 0.0, 0.0, 1.5, 0.0, 0.0, 64.0, 400.0, 1.8, 0.0, 1.0, 0.0, 0.2}; }
+// default_material (template.wgsl:1965-1978), DEFAULT_SSS_RADIUS = 0.1
+inline Mat default_mat() {
+    return Mat{{0.8, 0.8, 0.85}, 0.3, 0.0, 0.0, 1.5, 0.0, 0.0, 64.0, 400.0, 1.8, 0.0, 1.0, 0.0, 0.2, 0.0, 0.1};
+}
 // apply_part_mask (template.wgsl:394-402) for the parts this path reads
 inline Mat apply_part_mask(Mat m, uint32_t parts) {
     if (parts & ND_PART_COLOR_STRIPS) { m.color[0] = 0.72; m.color[1] = 0.72; m.color[2] = 0.73; }  // SIMPLE_ALBEDO
     if (parts & ND_PART_REFLECTIONS) m.reflectiveness = 0.0;
     if (parts & ND_PART_REFRACTION) m.transparency = 0.0;
+    if (parts & ND_PART_SUBSURFACE) m.subsurface = 0.0;
     if (parts & ND_PART_IRIDESCENCE) m.film_strength = 0.0;
     if (parts & ND_PART_DISPERSION) m.abbe = 0.0;
     return m;
@@ -179,6 +183,8 @@ inline Mat mix_mat(const Mat& a, const Mat& b, double h) {
     r.film_angle_scale = b.film_angle_scale * (1 - h) + a.film_angle_scale * h;
     r.film_perturb = b.film_perturb * (1 - h) + a.film_perturb * h;
     r.film_perturb_scale = b.film_perturb_scale * (1 - h) + a.film_perturb_scale * h;
+    r.subsurface = b.subsurface * (1 - h) + a.subsurface * h;
+    r.subsurface_radius = b.subsurface_radius * (1 - h) + a.subsurface_radius * h;
     return r;
 }
 // sample_gradient (template.wgsl:2003-2023)
@@ -382,10 +388,163 @@ template <class T> Spec<T> thin_film_tint(const Spectral<T>& sp, T cos_view, con
     return r;
 }
 
-// shade_local for subsurface-free materials (template.wgsl:3222-3331), with the
-// Monte Carlo light pick: when several shadowed lights reach the point and mc_light_pick is
-// set, one of them is chosen in proportion to its unshadowed weight and scaled by
-// total / weight.
+// ----------------------------------------------------------------------------- subsurface
+// Subsurface / subsurface_setup (template.wgsl:3088-3114): the amount, the radius (at least
+// SSS_MIN_RADIUS 0.0005), each wavelength's reach (its albedo relative to the peak, squared,
+// at least SSS_REACH_FLOOR 0.05) and a blurred normal, the tetrahedral gradient at
+// pos + n * radius/2 with taps radius/4 (SSS_BLUR_PER_RADIUS, SSS_BLUR_TAP_FRACTION)
+template <class T> struct Subsurface {
+    T amount, radius;
+    Spec<T> reach;
+    V3<T> blur_n;
+};
+template <class T, class P>
+Subsurface<T> subsurface_setup(const Scene<P>& s, V3<T> pos, V3<T> n, const Mat& m, const Spec<T>& albedo, int* evals) {
+    Subsurface<T> r;
+    r.amount = tclamp(T(m.subsurface), T(0), T(1));
+    r.radius = tmax(T(m.subsurface_radius), T(0.0005));
+    r.reach = splat(T(0));
+    r.blur_n = n;
+    if (r.amount <= T(0.001)) {  // MATERIAL_EPSILON
+        r.amount = T(0);
+        return r;
+    }
+    const T peak = tmax(smax(albedo), T(1e-4));  // EPSILON
+    for (int k = 0; k < 8; ++k) {
+        const T rel = albedo.v[k] / peak;
+        r.reach.v[k] = tmax(rel * rel, T(0.05));
+    }
+    const T blur = r.radius * T(0.5);
+    const T e = blur * T(0.5);
+    const V3<T> c = pos + n * blur;
+    const double K[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+    V3<T> g;
+    for (int i = 0; i < 4; ++i) {
+        V3<T> k(T(K[i][0]), T(K[i][1]), T(K[i][2]));
+        g = g + k * probed_de(s, c + k * e);
+    }
+    if (evals) *evals += 4;
+    const T len = length(g);
+    if (len > T(1e-12)) r.blur_n = g / len;  // EPSILON_TINY
+    return r;
+}
+// diffuse_weight with subsurface_wrap (template.wgsl:3116-3134): per wavelength, n.l is
+// blended toward the blurred normal's by the reach and wrapped by SSS_WRAP (0.75) * reach;
+// light from behind (max(-n.l, 0)) is added, and the profile is mixed in by the amount
+template <class T> Spec<T> diffuse_weight(const Subsurface<T>& s, T ndl, V3<T> light_dir) {
+    if (s.amount <= T(0)) return splat(tmax(ndl, T(0)));
+    const T ndl_blur = dot(s.blur_n, light_dir);
+    Spec<T> r;
+    for (int k = 0; k < 8; ++k) {
+        const T re = s.reach.v[k];
+        const T w = T(0.75) * re;
+        const T wrap = tmax(ndl + (ndl_blur - ndl) * re + w, T(0)) / (T(1) + w);
+        r.v[k] = (T(1) - s.amount) * tmax(ndl, T(0)) + s.amount * (wrap + tmax(-ndl, T(0)));
+    }
+    return r;
+}
+// subsurface_transmit (template.wgsl:3136-3138): exp(-depth / (radius * reach)) per wavelength
+template <class T> Spec<T> subsurface_transmit(const Subsurface<T>& s, T depth) {
+    Spec<T> r;
+    for (int k = 0; k < 8; ++k) r.v[k] = exp(-depth / (s.radius * s.reach.v[k]));
+    return r;
+}
+// refine_inside_exit (template.wgsl:3047-3059): bisection of [inside_t, outside_t] on the
+// inside-exit test DE > 0.0005 t + 0.00005, returning the outside end
+template <class T, class P>
+T refine_inside_exit(const Scene<P>& s, V3<T> origin, V3<T> dir, T lo, T hi, int iterations, int* evals) {
+    for (int i = 0; i < iterations; ++i) {
+        const T mid = T(0.5) * (lo + hi);
+        const T d = probed_de(s, origin + dir * mid);
+        if (evals) (*evals)++;
+        if (d > T(0.0005) * mid + T(0.00005)) hi = mid;  // INSIDE_EXIT_SLOPE, INSIDE_EXIT_BIAS
+        else lo = mid;
+    }
+    return hi;
+}
+// subsurface_light_path (template.wgsl:3140-3218): the transmission toward a light through
+// subsurface material. From outside (n.l > 0) without occlusion it is 1. Otherwise the ray
+// marches toward the light; inside a subsurface solid it takes steps of max(|DE|, floor),
+// the floor starting at max(0.05 radius, 0.0005) and growing by 1 + 6/64, leaves when the
+// inside-exit test holds after the warm-up (refined by 7 bisections), and returns 0 once
+// the depth inside exceeds 6 radii. Outside it steps by max(DE, 0.001) with accelerator
+// skips. A hit on a surface without subsurface hands the rest of the way to the soft
+// shadow march (shadow_factor from 2 biases before the hit); without occlusion the march
+// ends at the first exit. The result is exp(-depth / (radius * reach)) over the depth
+// travelled inside, times that shadow.
+template <class T, class P>
+Spec<T> subsurface_light_path(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& sp, V3<T> pos, V3<T> n,
+                              bool start_inside, const Subsurface<T>& ss, const nd_light& L, V3<T> ld, T ldist,
+                              bool occlusion, int* evals) {
+    if (!start_inside && !occlusion) return splat(T(1));
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    const V3<T> origin = start_inside ? pos : pos + n * T(0.004);  // SHADOW_BIAS
+    const T cap = ss.radius * T(6);                                // SSS_PROBE_DEPTH_RADII
+    const T step_base = tmax(ss.radius * T(0.05), T(0.0005));      // SSS_PROBE_STEP_PER_RADIUS, INSIDE_MIN_STEP
+    const T growth = T(1) + T(6) / T(64);                          // INSIDE_STEP_GROWTH / SSS_MAX_STEPS
+    const int steps = std::min(i32(u.max_steps), 128);             // SSS_PATH_MAX_STEPS
+    bool inside = start_inside;
+    T t = T(0), prev = T(0), entry = T(0), depth = T(0), step_floor = step_base;
+    int inner = 0;
+    for (int i = 0; i < steps; ++i) {
+        if (t >= ldist) break;
+        const V3<T> p = origin + ld * t;
+        if (!inside && s.accel) {
+            const T skip = accel_skip(s.accel.get(), p);
+            if (skip > T(0.001)) {  // SHADOW_MIN_STEP
+                t = t + skip;
+                continue;
+            }
+        }
+        const T d = probed_de(s, p);
+        if (evals) (*evals)++;
+        if (inside) {
+            if (inner > 1 && d > T(0.0005) * t + T(0.00005)) {  // INSIDE_WARMUP_STEPS, inside_exited
+                const T exit_t = refine_inside_exit(s, origin, ld, prev, t, 7, evals);  // SSS_EXIT_REFINE_STEPS
+                depth = depth + exit_t - entry;
+                inside = false;
+                if (!occlusion) break;
+                t = exit_t + T(0.004);
+                continue;
+            }
+            prev = t;
+            t = t + tmax(abs(d), step_floor);
+            step_floor = step_floor * growth;
+            inner += 1;
+            if (depth + t - entry > cap) return splat(T(0));
+            continue;
+        }
+        if (d < T(0.0005)) {  // SHADOW_HIT_EPS
+            const Mat hm = apply_part_mask(hit_material(u, s, p), parts);
+            if (evals) (*evals)++;
+            if (hm.subsurface <= 0.001) {
+                // shadow_factor(p - 2 bias dir, dir, light, dir, ldist - t + 2 bias)
+                Spec<T> rest = splat(T(1));
+                if (L.cast_shadows > 0.5f) {
+                    const double k = L.shadow_softness + (512.0 - L.shadow_softness) * L.hard_shadows;
+                    rest = visibility_march(u, s, sp, p - ld * T(0.008) + ld * T(0.004), ld, ldist - t + T(0.008), T(k),
+                                            std::min(i32(u.max_steps), 40), evals);
+                }
+                return subsurface_transmit(ss, depth) * rest;
+            }
+            inside = true;
+            entry = t;
+            prev = t;
+            inner = 1;
+            step_floor = step_base;
+            t = t + step_floor;
+            continue;
+        }
+        t = t + tmax(d, T(0.001));
+    }
+    if (inside) depth = depth + tmin(t, ldist) - entry;
+    return subsurface_transmit(ss, depth);
+}
+
+// shade_local (template.wgsl:3222-3331), with the Monte Carlo light pick: when several
+// shadowed lights reach the point and mc_light_pick is set, one of them is chosen in
+// proportion to its unshadowed weight and scaled by total / weight. A subsurface material
+// evaluates every light through subsurface_light_path with the wrapped diffuse weight.
 template <class T, class P>
 Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& sp, V3<T> pos, V3<T> dir, V3<T> n,
                     const Mat& m, const Spec<T>& ambient, bool use_shadows, Seed seed, int* evals,
@@ -399,6 +558,7 @@ Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& 
         albedo = albedo * film;
     }
     if (albedo_out) *albedo_out = albedo;  // g_shade_albedo
+    const Subsurface<T> sss = subsurface_setup(s, pos, n, m, albedo, evals);
     Spec<T> diffuse = splat(T(0)), specular = splat(T(0));
     const uint32_t parts = (uint32_t)u.debug_parts;
     int lc = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min(i32(u.light_count), ND_MAX_LIGHTS);
@@ -423,6 +583,19 @@ Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& 
         terms(L, &ld, &col, &dist, &ndotl, &spec);
         if (tmax(col.x, tmax(col.y, col.z)) < T(1e-5)) continue;
         Spec<T> ls = sp.from_rgb(col.x, col.y, col.z);
+        if (sss.amount > T(0)) {
+            // subsurface: every light is evaluated here, through the material (no light pick)
+            const T ndl = dot(n, ld);
+            const Spec<T> w = diffuse_weight(sss, ndl, ld);
+            const T sspec = ndl > T(0) ? spec : T(0);
+            if (smax(w) <= T(0) && sspec <= T(0)) continue;
+            const bool occlusion = shadows_on && L.cast_shadows > 0.5f;
+            const Spec<T> reaching =
+                ls * subsurface_light_path(u, s, sp, pos, n, ndl <= T(0), sss, L, ld, dist, occlusion, evals);
+            diffuse = diffuse + reaching * w;
+            specular = specular + reaching * sspec;
+            continue;
+        }
         if (ndotl <= T(0)) continue;
         if (pick_one && L.cast_shadows > 0.5f) {
             pick_w[i] = (col.x + col.y + col.z) * (ndotl + spec);
@@ -588,7 +761,7 @@ struct PathResult { double rgb[3]; double depth; double normal[3]; int evals; in
 // (:3819-4124): opaque and transmissive materials (Schlick reflection / refraction through
 // march_inside), Cauchy dispersion on the picked wavelength with the Gaussian angle
 // weights or the one-wavelength collapse, thin-film tints, the light pick and the primary
-// hit cache's packed normals. No fog, particles, photons, subsurface or approximations.
+// hit cache's packed normals and subsurface materials. No fog, particles, photons or approximations.
 template <class T, class P>
 PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, Seed seed) {
     PathResult out{};
@@ -681,6 +854,7 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
                     V3<T> bp = io + idir * b.dist;
                     V3<T> bn = normal_tetra(s, bp, T(mc.eps_c) * b.dist + T(mc.eps_f));
                     Mat bm = apply_part_mask(hit_material(u, s, bp), parts);
+                    bm.subsurface = 0.0;  // indirect hits are shaded without subsurface
                     out.evals += 5;
                     ambient = shade_local(u, s, sp, bp, idir, bn, bm, splat(T(0.15)), false, is, &out.evals);
                 }

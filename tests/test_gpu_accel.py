@@ -78,3 +78,29 @@ def test_path_trace_with_accelerator(renderer):
           flush=True)
     assert mse_gpu <= 2.0 * mse_floor + 1e-6
     assert np.allclose(g.reshape(-1, 3).mean(0), r64.reshape(-1, 3).mean(0), rtol=0.05, atol=1e-4)
+
+
+def test_grid_lookups_match_the_oracle(renderer):
+    """The device grid and its lookup (nd_accel_probe) against the oracle's FP32 grid on the
+    four-fractal scene: random points, points around the camera, the camera itself (the
+    cascades' common centre, where every camera ray starts) and the corners and centres of
+    every level: cell centres and exact cell boundaries, where the index must be that of a
+    correctly rounded division."""
+    from tests.test_oracle_accel import oracle_grid
+    u = nd.resize(nd.config_uniforms(3, 0), 64, 36)
+    nd.enable_accel(u, res=32, levels=5, safety=0.92)
+    rng = np.random.default_rng(1)
+    cam = np.array(list(u.camera_pos), np.float64)
+    special = [cam]
+    for lv in range(int(u.accel_levels)):
+        corner, cell = np.array(list(u.accel_params[lv])[:3]), float(u.accel_params[lv][3])
+        special += [corner + 0.5 * cell, corner + 16 * cell, corner + 16.5 * cell, corner + 7 * cell]
+    pts = np.concatenate([rng.uniform(-4, 4, (20000, 3)), cam + rng.uniform(-1.5, 1.5, (5000, 3)), np.array(special)])
+    pts = pts.astype(np.float32).astype(np.float64)
+    g = renderer.accel_probe(u, pts).astype(np.float64)
+    _, ref = oracle_grid(u, pts, prec=1)
+    rel = np.abs(g - ref) / np.maximum(ref, 1e-6)
+    print("\n[accel] lookups within 1e-5: %.5f, max rel %.3g" % ((rel <= 1e-5).mean(), rel.max()), flush=True)
+    assert ((g > 0) == (ref > 0)).all()
+    assert (rel <= 1e-5).mean() >= 0.999
+    assert np.all(rel[-len(special):] <= 1e-5), (g[-len(special):], ref[-len(special):])

@@ -108,15 +108,15 @@ def test_rect_matches_full_frame(renderer):
     assert np.array_equal(sub, full[5:18, 8:32])
 
 
-def test_unsupported_materials_are_rejected(renderer):
-    """Subsurface scattering is not implemented and is rejected by name; transmissive and
-    thin-film materials are accepted."""
+def test_material_features_are_accepted(renderer):
+    """Transmissive, thin-film and subsurface materials are accepted; fog is rejected by name."""
     u = c3(16, 9)
-    u.instances[1].colors[3].subsurface = 0.5
+    u.fog_count = 1.0
     with pytest.raises(nd.NdError) as e:
         renderer.path_trace(u, 1)
-    assert e.value.status == nd.abi.ND_E_UNSUPPORTED and "subsurface" in str(e.value)
+    assert e.value.status == nd.abi.ND_E_UNSUPPORTED and "fog" in str(e.value)
     u = c3(16, 9)
+    u.instances[1].colors[2].subsurface = 0.5
     u.instances[1].colors[3].transparency = 0.5
     u.instances[1].colors[4].film_strength = 0.5
     u.has_transparency = 1.0
