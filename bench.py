@@ -131,6 +131,29 @@ def gradient_leg(r, nd, torch, reps=3):
             "views": len(views), "pixels": total, "pixels_in_loss": used}
 
 
+def mesh_leg(r, nd, torch, flops, n=512, reps=3):
+    """The mesher's distance pass (cs_mesh_points) over an n^3 grid of the Mandelbox scene in
+    8^3 blocks, blocks and values in device memory: DE evaluations per second and FP32 rate."""
+    u = nd.config_uniforms(2, 0)
+    b = n // 8
+    z, y, x = torch.meshgrid(torch.arange(b), torch.arange(b), torch.arange(b), indexing="ij")
+    blocks = torch.stack([x.reshape(-1), y.reshape(-1), z.reshape(-1), torch.zeros(b ** 3, dtype=torch.int64)], 1)
+    blocks = blocks.to(torch.int32).contiguous().to("cuda:0")
+    vals = torch.empty(n ** 3, dtype=torch.float32, device="cuda:0")
+    origin, cell = (-4.0, -4.0, -4.0), 8.0 / n
+    r.mesh_points(u, origin, cell, 0.0, 8, 0, blocks, values=vals)
+    ms = []
+    for _ in range(reps):
+        ms.append(r.mesh_points(u, origin, cell, 0.0, 8, 0, blocks, values=vals)[1])
+    k = sum(ms) / len(ms)
+    # the Mandelbox DE runs all 15 iterations wherever it is evaluated: the march's mean
+    # FLOP per DE on this scene stands for the grid's
+    f = flops["workloads"]["2"]["march"]["flops_per_de"]
+    return {"grid": n, "points": n ** 3, "kernel_ms": round(k, 3), "de_evals_per_s": round(n ** 3 / (k * 1e-3), 1),
+            "tflops": round(f * n ** 3 / (k * 1e-3) / 1e12, 3),
+            "write_gb_per_s": round(4 * n ** 3 / (k * 1e-3) / 1e9, 2)}
+
+
 def accel_leg(r, nd, torch, abi, jobs, steps=2):
     """The same 16-camera batch with the empty-space accelerator as the app enables it by
     default (64^3 cells x 6 levels, safety 0.92; one grid per camera, built on the device in
@@ -244,9 +267,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the path-trace, 256-job and gradient legs")
-    ap.add_argument("--legs", default="c3,c4,c5,accel", help="comma list of extra legs: c3 path trace, c4 256 "
-                                                            "jobs, c5 gradient, accel the batch with the "
-                                                            "empty-space accelerator")
+    ap.add_argument("--legs", default="c3,c4,c5,accel,mesh", help="comma list of extra legs: c3 path trace, c4 256 "
+                                                                 "jobs, c5 gradient, accel the batch with the "
+                                                                 "empty-space accelerator, mesh the mesher's "
+                                                                 "512^3 distance pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run one elementwise copy of MIB MiB (reads MIB, writes MIB): "
                          "the reference kernel for scaling hardware byte counters")
@@ -323,6 +347,8 @@ def main():
         res["c3_path_trace"] = mc_leg(r, nd)
     if "accel" in legs:
         res["c2_accelerator"] = accel_leg(r, nd, torch, abi, jobs)
+    if "mesh" in legs:
+        res["mesh_points"] = mesh_leg(r, nd, torch, flops_table())
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(nd)

@@ -275,6 +275,31 @@ typedef struct nd_adjoint_io {
     uint32_t _pad;
 } nd_adjoint_io;
 
+/* A block grid for the mesher's distance pass (MeshParams, src/gpu/mesher.zig:16-27). Point
+ * (block * 8 + local + local_offset) of the grid lies at origin + index * cell; local runs
+ * over span^3 points per block, x fastest. */
+typedef struct nd_mesh_grid {
+    float origin[3];
+    float cell;
+    float iso;              /* subtracted from every DE value */
+    uint32_t span;          /* points per block edge, 1..8 (8: fine pass, 1: coarse pass) */
+    uint32_t local_offset;  /* added to every local index (block / 2 in the coarse pass) */
+    uint32_t device_ptrs;   /* 1: blocks and values are HIP device pointers */
+} nd_mesh_grid;
+
+/* cs_mesh_points (mesh.wgsl:49-61) as MeshGpu.run(.points) dispatches it
+ * (export_mesh.zig:134-160): values[b * span^3 + l] = scene_de(point l of block b) - iso for
+ * n_blocks blocks given as 4 u32 each (x, y, z, unused). kernel_ms (nullable) receives the
+ * kernel's time from HIP events on the context's stream. */
+nd_status nd_mesh_points(nd_ctx* ctx, const void* uniforms, size_t uniforms_size, const nd_mesh_grid* grid,
+                         const uint32_t* blocks, uint32_t n_blocks, float* values, float* kernel_ms);
+
+/* cs_mesh_probe (mesh.wgsl:63-92) as MeshGpu.run(.probe) dispatches it (play_mode.zig:389-402):
+ * per point (x, y, z, h) the DE at the point and the tetrahedral gradient with taps |h|,
+ * divided by 4 |h|: out = (d, gx, gy, gz). h < 0 evaluates the scene without its carves. */
+nd_status nd_mesh_probe(nd_ctx* ctx, const void* uniforms, size_t uniforms_size, const float* points, uint32_t n,
+                        float* out, uint32_t device_ptrs);
+
 /* The empty-space accelerator's skip distance (accel_skip, template.wgsl:426-447) at n
  * points (xyz triples, host memory) with the device grid the context builds or reuses for
  * the scene and cascades of `uniforms` (accel_enabled must be set). skip_out: n floats, host

@@ -60,6 +60,8 @@ def _load():
     lib.nd_adjoint.argtypes = [vp, P(Uniforms), u32, P(AdjointIO), P(BatchStats)]
     lib.nd_path_trace.argtypes = [vp, vp, C.c_size_t, u32, Rect, P(ImageOut), P(TraceStats)]
     lib.nd_accel_probe.argtypes = [vp, vp, C.c_size_t, vp, u32, vp]
+    lib.nd_mesh_points.argtypes = [vp, vp, C.c_size_t, P(abi.MeshGrid), vp, u32, vp, vp]
+    lib.nd_mesh_probe.argtypes = [vp, vp, C.c_size_t, vp, u32, vp, u32]
     lib.nd_config_job_count.argtypes = [u32]
     lib.nd_config_job_count.restype = u32
     lib.nd_uniforms_defaults.argtypes = [P(Uniforms)]
@@ -289,6 +291,32 @@ class Renderer:
         st = BatchStats()
         self._check(lib.nd_adjoint(self._h, arr, n, C.byref(io), C.byref(st)), "nd_adjoint")
         return loss.value, np.array(grad[:]), int(used.value), st
+
+    def mesh_points(self, u, origin, cell, iso, span, local_offset, blocks, values=None):
+        """cs_mesh_points: the DE minus iso on span^3 points of every block (blocks: (n, 4) u32
+        or a device tensor). values: float32 (n * span^3,) numpy array or device tensor,
+        allocated on the host when omitted. Returns (values, kernel ms)."""
+        dev = hasattr(blocks, "data_ptr")
+        if not dev:
+            blocks = np.ascontiguousarray(blocks, np.uint32).reshape(-1, 4)
+        n = int(blocks.shape[0])
+        if values is None:
+            values = np.zeros(n * span ** 3, np.float32)
+        g = abi.MeshGrid((C.c_float * 3)(*origin), cell, iso, span, local_offset, 1 if dev else 0)
+        ms = C.c_float(0.0)
+        ub = Uniforms.from_buffer_copy(bytes(u))
+        self._check(lib.nd_mesh_points(self._h, C.byref(ub), C.sizeof(Uniforms), C.byref(g), _ptr(blocks), n,
+                                       _ptr(values), C.byref(ms)), "nd_mesh_points")
+        return values, ms.value
+
+    def mesh_probe(self, u, points):
+        """cs_mesh_probe: (d, gradient) at points (n, 4) = (x, y, z, h); h < 0 without carves."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
+        out = np.zeros((len(p), 4), np.float32)
+        ub = Uniforms.from_buffer_copy(bytes(u))
+        self._check(lib.nd_mesh_probe(self._h, C.byref(ub), C.sizeof(Uniforms), p.ctypes.data, len(p),
+                                      out.ctypes.data, 0), "nd_mesh_probe")
+        return out
 
     def accel_probe(self, u, pts):
         """The accelerator's skip distance at points (n, 3) with the device grid of u's scene

@@ -38,17 +38,15 @@ NDD AccelView accel_view(const DevJob& J) {
 // still (the hardware log2 is accurate to an ulp), which finds the same level as a search
 // from level 0. The log is clamped at 0 before the conversion: at the centre itself (u = 0)
 // it is -inf, whose conversion saturates to INT_MIN and would wrap past every level.
-// Cells are indexed by floor((p - corner) / cell) with a correctly rounded division, as the
-// reference's floor((p - prm.xyz) / cell) (cell_index).
+// Cells are indexed by floor((p - corner) / cell), the quotient rounded as a correctly rounded
+// division rounds it (cell_index).
 NDD float cell_index(float x, float cell, float ic) {
-    // The product with the hardware reciprocal decides unless it lies within 1e-4 of a cell
-    // boundary, where it can round to the other side: the camera sits exactly on a boundary
-    // of every cascade (their common centre), so every camera ray's first lookup is one.
-    const float q = x * ic;
-    const float f = floorf(q);
-    const float fr = q - f;
-    if (fr < 1e-4f || fr > 0.9999f) return floorf(__fdiv_rn(x, cell));
-    return f;
+    // The product with the hardware reciprocal is corrected once by its exact residual
+    // (fma(-q, cell, x)), which lands exact quotients on their integer: the camera sits exactly
+    // on a cell boundary of every cascade (their common centre), so every camera ray's first
+    // lookup is such a quotient, and the uncorrected product can fall just below it.
+    const float q0 = x * ic;
+    return floorf(fmaf(fmaf(-q0, cell, x), ic, q0));
 }
 NDD float accel_skip(const AccelView& A, const V3<float>& p) {
     const float u = fmaxf(fabsf(p.x - A.cx), fmaxf(fabsf(p.y - A.cy), fabsf(p.z - A.cz)));
