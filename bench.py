@@ -131,6 +131,34 @@ def gradient_leg(r, nd, torch, reps=3):
             "views": len(views), "pixels": total, "pixels_in_loss": used}
 
 
+def c1_leg(r, nd, torch, abi, flops, reps=5):
+    """The Mandelbulb G-buffer workload: power 8, 8 iterations, 640x360, camera (0, 0, -2.5),
+    depth + analytic normal + trap + hit, no lighting; persistent and plain schedules."""
+    u = nd.config_uniforms(1, 0)
+    W, H = int(u.resolution[0]), int(u.resolution[1])
+    g = nd.GBuffer(W, H, device="cuda:0", fields=("depth", "normal", "trap", "hit"))
+    f = flops["workloads"]["1"]["march"]["flops_per_de"]
+    res = {"width": W, "height": H, "flops_per_de": round(f, 2)}
+    for name, kernel in (("persistent", abi.KERNEL_PERSISTENT), ("plain", abi.KERNEL_PLAIN)):
+        r.render_batch([(u, g)], shadows=False, kernel=kernel)
+        r.sync()
+        t0 = time.perf_counter()
+        de = ms = ms_march = de_march = 0.0
+        for _ in range(reps):
+            st = r.render_batch([(u, g)], shadows=False, kernel=kernel)
+            de += st.de_evals
+            ms += st.ms_total
+            ms_march += st.ms_march if kernel == abi.KERNEL_PERSISTENT else st.ms_total
+            de_march += st.de_evals_march
+        wall = (time.perf_counter() - t0) / reps
+        tfl = f * de_march / (ms_march * 1e-3) / 1e12 if ms_march > 0 else 0.0
+        res[name] = {"mrays_per_s": round(W * H / wall / 1e6, 2), "ms_per_frame": round(wall * 1e3, 3),
+                     "device_ms": round(ms / reps, 3), "de_evals_per_s": round(de / (ms * 1e-3), 1),
+                     "march_tflops": round(tfl, 3), "march_frac_fp32_peak": round(tfl / FP32_PEAK_TFLOPS, 4)}
+        log("c1 %s: %.3f ms per frame" % (name, wall * 1e3))
+    return res
+
+
 def mesh_leg(r, nd, torch, flops, n=512, reps=3):
     """The mesher's distance pass (cs_mesh_points) over an n^3 grid of the Mandelbox scene in
     8^3 blocks, blocks and values in device memory: DE evaluations per second and FP32 rate."""
@@ -238,7 +266,18 @@ def cpu_baseline(nd, target_s=40.0):
     """FP32 CPU restatement of the same per-ray math (the oracle instantiated for float),
     on host threads, over a band of rows of camera 0 at full 1920x1080."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    # every core this process may run on, up to the host share of one GPU (16 threads)
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, usable))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     u = nd.config_uniforms(2, 0)
     W, H = int(u.resolution[0]), int(u.resolution[1])
     rows = 4
@@ -253,6 +292,7 @@ def cpu_baseline(nd, target_s=40.0):
                           shadow_max_steps=128, threads=threads)
     dt = time.perf_counter() - t0
     return {"value": W * rows / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model,
             "sample": "camera 0 of the batch, rows %d..%d of 1920x1080 (%d rays), FP32 CPU restatement of "
                       "the same march + analytic normal + trap + 128-step shadow, %.1f s" % (y0, y0 + rows, W * rows, dt)}
 
@@ -267,7 +307,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the path-trace, 256-job and gradient legs")
-    ap.add_argument("--legs", default="c3,c4,c5,accel,mesh", help="comma list of extra legs: c3 path trace, c4 256 "
+    ap.add_argument("--legs", default="c1,c3,c4,c5,accel,mesh", help="comma list of extra legs: c1 Mandelbulb 640x360 "
+                                                                    "G-buffer, c3 path trace, c4 256 "
                                                                  "jobs, c5 gradient, accel the batch with the "
                                                                  "empty-space accelerator, mesh the mesher's "
                                                                  "512^3 distance pass")
@@ -337,6 +378,8 @@ def main():
         res["persistent_vs_plain"] = round(value / pv, 3) if pv > 0 else None
         res["plain_ms_per_step"] = round(plain["wall_s"] * 1e3 / steps, 3)
     legs = set() if args.no_legs else set(x.strip() for x in args.legs.split(",") if x.strip())
+    if "c1" in legs:
+        res["c1_mandelbulb"] = c1_leg(r, nd, torch, abi, flops_table())
     if "c4" in legs:
         res["c4_batch"] = batch_jobs_leg(r, nd, torch, abi)
     if "c5" in legs:

@@ -238,6 +238,9 @@ typedef struct nd_tuning {
     uint32_t chunk_rays;       /* rays a wave claims from the march queue per atomic; shadow claims are
                                   4x; 0 (default) = 96 when every job has the same scene, else 256 */
     uint32_t flags;            /* ND_TUNE_* */
+    uint32_t round_cap;        /* 0 = none; otherwise a persistent geometry pass stops each wave after this
+                                  many rounds and the call returns ND_E_TIMEOUT (every kernel still exits
+                                  normally; outputs of unfinished rays are undefined) */
 } nd_tuning;
 
 nd_status nd_get_tuning(nd_ctx* ctx, nd_tuning* out);
@@ -332,11 +335,13 @@ typedef struct nd_trace_stats {
 
 /* Path-traces rect with spp samples per pixel: sample i uses mc_sample = u.mc_sample + i,
  * the host's per-sample stamp (fr.zig:3761-3775), and trace_path's transport
- * (template.wgsl:3819-4137) for opaque scenes. out->color (required) receives the mean
- * over the samples after gamut_fit; depth and normal come from sample 0 (tetrahedral normal).
- * Transmissive, subsurface, film and dispersive materials, fog, photons, depth of field,
- * adaptive sampling, the hit cache and ray lights return ND_E_UNSUPPORTED. The cancel flag
- * is checked between chunks of samples. */
+ * (template.wgsl:3819-4137): opaque, transmissive (refraction with the inside march,
+ * Cauchy dispersion), thin-film and subsurface materials, the Monte Carlo light pick, the
+ * primary hit cache (primary_cache_key) and adaptive sampling (adapt_enabled). out->color
+ * (required) receives the mean over the samples after gamut_fit; depth and normal come from
+ * sample 0 (tetrahedral normal). Fog, photons, depth of field, the approximations and ray
+ * lights return ND_E_UNSUPPORTED, so the host can fall back to its WGSL pass. The cancel
+ * flag is checked between chunks of samples (ND_E_CANCELLED). */
 nd_status nd_path_trace(nd_ctx* ctx, const void* uniforms, size_t uniforms_size, uint32_t spp,
                         nd_rect rect, const nd_image_out* out, nd_trace_stats* stats);
 

@@ -48,6 +48,7 @@ def _load():
     lib.nd_last_error.argtypes = [vp]
     lib.nd_last_error.restype = C.c_char_p
     lib.nd_sync.argtypes = [vp]
+    lib.nd_set_cancel.argtypes = [vp, P(C.c_int)]
     lib.nd_ctx_stream.argtypes = [vp]
     lib.nd_ctx_stream.restype = vp
     lib.nd_bind_formulas.argtypes = [vp, P(C.c_int32), u32]
@@ -254,10 +255,21 @@ class Renderer:
         self._check(lib.nd_get_tuning(self._h, C.byref(t)), "nd_get_tuning")
         return t
 
-    def set_tuning(self, waves_per_cu=0, refill_min_idle=8, chunk_rays=0, track_latency=False, graphs=True):
+    def set_tuning(self, waves_per_cu=0, refill_min_idle=8, chunk_rays=0, track_latency=False, graphs=True,
+                   round_cap=0):
+        """round_cap > 0 stops every wave of a persistent geometry pass after that many rounds;
+        the call then raises NdError(ND_E_TIMEOUT)."""
         flags = (abi.TUNE_TRACK_LATENCY if track_latency else 0) | (0 if graphs else abi.TUNE_NO_GRAPH)
-        t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, flags)
+        t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, flags, round_cap)
         self._check(lib.nd_set_tuning(self._h, C.byref(t)), "nd_set_tuning")
+
+    def set_cancel(self, flag):
+        """Watches flag (a ctypes.c_int the caller keeps alive, or None to stop watching), like
+        the export cancel flag (src/app/export_image.zig:46-47): while it is non-zero, calls
+        return ND_E_CANCELLED; the path trace checks it between chunks of samples."""
+        self._cancel = flag
+        ptr = C.pointer(flag) if flag is not None else None
+        self._check(lib.nd_set_cancel(self._h, ptr), "nd_set_cancel")
 
     def autotune(self, jobs, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0):
         arr = (Job * len(jobs))()

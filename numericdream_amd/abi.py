@@ -133,7 +133,8 @@ class BatchStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("waves_per_cu", U32), ("refill_min_idle", U32), ("chunk_rays", U32), ("flags", U32)]
+    _fields_ = [("waves_per_cu", U32), ("refill_min_idle", U32), ("chunk_rays", U32), ("flags", U32),
+                ("round_cap", U32)]
 
 
 TUNE_TRACK_LATENCY = 1

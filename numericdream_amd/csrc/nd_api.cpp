@@ -447,6 +447,7 @@ nd_status nd_set_tuning(nd_ctx* c, const nd_tuning* t) {
     if (t->waves_per_cu > 64) return nd_fail(c, ND_E_INVALID, "waves_per_cu must be 0..64");
     if (t->flags & ~(ND_TUNE_TRACK_LATENCY | ND_TUNE_NO_GRAPH)) return nd_fail(c, ND_E_INVALID, "unknown tuning flags 0x%x", t->flags);
     c->tuning = *t;
+    c->round_cap = t->round_cap ? t->round_cap : 0xffffffffu;
     return ND_OK;
 }
 
@@ -756,7 +757,7 @@ nd_status nd_autotune(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_
     for (uint32_t w : wpcs)
         for (uint32_t mi : idles)
             for (uint32_t ch : chunks) {
-                nd_tuning t{w, mi, ch, saved.flags};
+                nd_tuning t{w, mi, ch, saved.flags, saved.round_cap};
                 c->tuning = t;
                 nd_batch_stats s;
                 nd_status st = nd_batch_impl(c, jobs, n, &o, &s);

@@ -35,7 +35,7 @@ def lib():
         L.ndo_abi_size.restype = i32
         L.ndo_scene_de.argtypes = [vp, vp, i32, vp, i32, vp, vp]
         L.ndo_march.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp, vp, vp]
-        L.ndo_normal.argtypes = [vp, vp, i32, vp, vp, i32, vp]
+        L.ndo_normal.argtypes = [vp, vp, i32, i32, vp, vp, i32, vp]
         L.ndo_shadow.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]
         L.ndo_primary_dir.argtypes = [vp, i32, i32, vp]
         L.ndo_march_inside.argtypes = [vp, vp, vp, vp, i32, vp]
@@ -85,12 +85,15 @@ def march(u, origins, dirs, prec=0, code_map=None):
     return hit.astype(bool), t, steps, ev
 
 
-def normal(u, pts, mode, e=None, code_map=None):
+def normal(u, pts, mode, e=None, code_map=None, prec=0):
+    """Normals at pts: mode 0 analytic (FP64 only), 1 tetrahedral, 2 central with tap e;
+    prec 1 gives the FP32 restatement of the stencils."""
     p = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
     e = np.zeros(len(p)) if e is None else np.ascontiguousarray(np.broadcast_to(e, (len(p),)), np.float64)
     out = np.zeros((len(p), 3))
     cm, cmp_ = _cm(code_map)
-    lib().ndo_normal(_u(u), cmp_, mode, _p(p), _p(e), len(p), _p(out))
+    if lib().ndo_normal(_u(u), cmp_, prec, mode, _p(p), _p(e), len(p), _p(out)) != 0:
+        raise ValueError("normal mode %d has no prec %d restatement" % (mode, prec))
     return out
 
 

@@ -165,12 +165,23 @@ int ndo_march(const nd_uniforms* u, const int32_t* cm, int prec, const double* o
 }
 
 // mode: 0 analytic (FP64 dual numbers), 1 tetra e = eps, 2 central h = eps / 2; e is given.
-int ndo_normal(const nd_uniforms* u, const int32_t* cm, int mode, const double* pos, const double* e, int n,
+// prec 1: the FP32 restatement of the finite-difference stencils (mode 1 tetrahedral, 2
+// central); the analytic normal (mode 0) exists in FP64 only.
+int ndo_normal(const nd_uniforms* u, const int32_t* cm, int prec, int mode, const double* pos, const double* e, int n,
                double* out) {
+    if (prec != 0 && mode != 1 && mode != 2) return 1;
     Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    Scene<float> sf = decode_scene<float>(*u, code_map(cm));
     for (int i = 0; i < n; ++i) {
         V3<double> p(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-        V3<double> r = mode == 1 ? normal_tetra(s, p, e[i]) : mode == 2 ? normal_central(s, p, e[i]) : normal_analytic(s, p);
+        V3<double> r;
+        if (prec != 0) {
+            V3<float> q((float)p.x, (float)p.y, (float)p.z);
+            V3<float> rf = mode == 1 ? normal_tetra(sf, q, (float)e[i]) : normal_central(sf, q, (float)e[i]);
+            r = V3<double>(rf.x, rf.y, rf.z);
+        } else {
+            r = mode == 1 ? normal_tetra(s, p, e[i]) : mode == 2 ? normal_central(s, p, e[i]) : normal_analytic(s, p);
+        }
         out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
     }
     return 0;

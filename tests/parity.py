@@ -53,7 +53,13 @@ def compare(gpu, ref):
     return st
 
 
-def pointwise(gpu, u, normal_mode, code_map=None):
+def _angle_deg(a, b):
+    c = np.clip(np.sum(a * b, axis=1) / np.maximum(np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1), 1e-30),
+                -1, 1)
+    return np.degrees(np.arccos(c))
+
+
+def pointwise(gpu, u, normal_mode, code_map=None, floor=False):
     """Normal and trap of the oracle (FP64) evaluated at the GPU's own hit points.
 
     At the workloads' epsilon (1e-3 of a pixel) the DE gradient of a fractal changes
@@ -78,4 +84,24 @@ def pointwise(gpu, u, normal_mode, code_map=None):
         st["pw_normal_deg_le_0.5"] = float((ang <= 0.5).mean())
         # discrete traps (iteration index) must match exactly; continuous ones to 1e-3
         st["pw_trap_equal"] = float(np.isclose(gpu["trap"][h], ref_trap, rtol=1e-3, atol=1e-5).mean())
+    if floor and h.any():
+        # What FP32 itself reaches at these points. The trap: the FP32 CPU restatement of
+        # the same iteration against FP64. The stencil normals: their FP32 restatement. The
+        # analytic normal has no FP32 restatement; its floor is how far the FP64 normal moves
+        # when each coordinate of the hit point moves by one FP32 ulp (seeded signs).
+        _, t32 = oracle.scene_de(u, pos, prec=1, trap=True, code_map=code_map)
+        st["floor_trap_equal"] = float(np.isclose(t32, ref_trap, rtol=1e-3, atol=1e-5).mean())
+        if normal_mode in (1, 2):
+            n32 = oracle.normal(u, pos, normal_mode, e, code_map=code_map, prec=1)
+        else:
+            rng = np.random.default_rng(20261203)
+            ulp = np.spacing(np.abs(pos).astype(np.float32)).astype(np.float64)
+            n32 = oracle.normal(u, pos + ulp * rng.choice([-1.0, 1.0], size=pos.shape), 0, code_map=code_map)
+        st["floor_normal_deg_le_0.5"] = float((_angle_deg(n32, ref_n) <= 0.5).mean())
     return st
+
+
+def meets(value, bar, floor, margin=0.03):
+    """value reaches the fixed bar, or comes within margin of the FP32 floor where the
+    floor itself lies below the bar (an ill-conditioned quantity)."""
+    return value >= bar or value >= floor - margin

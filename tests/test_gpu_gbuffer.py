@@ -85,9 +85,15 @@ def test_reference_shadow_rule_and_tetra(renderer):
     st = compare(g, ref)
     st.update(pointwise(g, u, 1))
     report("c2 tetra shadow40", st)
-    assert st["hit_agree_inner"] >= 0.999
-    assert st["depth_rel_le_1e-4"] >= 0.99
-    assert st["pw_trap_equal"] >= 0.995
+    assert st["hit_agree_inner"] >= 0.999, st
+    assert st["depth_rel_le_1e-4"] >= 0.99, st
+    assert st["pw_trap_equal"] >= 0.995, st
+    # the reference's stencil: 4 taps at e = eps(t) around the GPU's hit point
+    assert st["pw_normal_deg_median"] <= 0.1, st
+    assert st["pw_normal_deg_le_0.5"] >= 0.9, st
+    # the 40-step shadow of the reference rule, end to end
+    assert st["shadow_abs_le_0.02"] >= 0.97, st
+    assert st["shadow_abs_median"] <= 1e-3, st
 
 
 def single_instance(config_index, keep):
