@@ -215,6 +215,23 @@ nd_status nd_render_gbuffer(nd_ctx* ctx, const void* uniforms, size_t uniforms_s
                             const nd_march_opts* opts, nd_rect rect, const nd_gbuffer_out* out,
                             nd_batch_stats* stats);
 
+/* No counterpart in the reference. nd_render_gbuffer for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * It allocates nothing (no hipMalloc/hipFree, no allocating library routine): the job
+ * record, scene, pass counters and hit list live in one library-owned device array per GPU
+ * (about 35 MB, held from the library's load on that GPU until the process ends), whose device
+ * address nd_ctx_create looks up. Limits: out->device_ptrs must be 1 (ND_E_INVALID otherwise);
+ * at most ND_NOALLOC_MAX_RAYS ray slots, 64 per 8x8 tile of the rect, so 512x1024 pixels
+ * (ND_E_INVALID beyond); no empty-space accelerator (ND_E_UNSUPPORTED: its grids are
+ * allocated); persistent passes of at most ND_NOALLOC_MAX_GRID workgroups; no per-job latency.
+ * Results equal nd_render_gbuffer's. Every call on one GPU, from any context, shares that
+ * GPU's array and leaves the last call's contents in it: calls that overlap on one GPU race. */
+#define ND_NOALLOC_MAX_RAYS (1u << 19)
+#define ND_NOALLOC_MAX_GRID 4096u
+nd_status nd_render_gbuffer_noalloc(nd_ctx* ctx, const void* uniforms, size_t uniforms_size,
+                                    const nd_march_opts* opts, nd_rect rect, const nd_gbuffer_out* out,
+                                    nd_batch_stats* stats);
+
 /* Many jobs (frames of different cameras, formulas and resolutions). With
  * ND_KERNEL_PERSISTENT every pass of the whole batch is one launch; with ND_KERNEL_PLAIN
  * each job is its own launch. stats may be NULL. */

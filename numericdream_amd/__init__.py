@@ -53,6 +53,7 @@ def _load():
     lib.nd_ctx_stream.restype = vp
     lib.nd_bind_formulas.argtypes = [vp, P(C.c_int32), u32]
     lib.nd_render_gbuffer.argtypes = [vp, vp, C.c_size_t, P(MarchOpts), Rect, P(GBufferOut), P(BatchStats)]
+    lib.nd_render_gbuffer_noalloc.argtypes = [vp, vp, C.c_size_t, P(MarchOpts), Rect, P(GBufferOut), P(BatchStats)]
     lib.nd_render_batch.argtypes = [vp, P(Job), u32, P(MarchOpts), P(BatchStats)]
     lib.nd_batch_job_latency.argtypes = [vp, P(C.c_float), u32]
     lib.nd_get_tuning.argtypes = [vp, P(Tuning)]
@@ -226,6 +227,18 @@ class Renderer:
         out = gbuf.out()
         self._check(lib.nd_render_gbuffer(self._h, C.byref(u), abi.UNIFORMS_SIZE, C.byref(opts), r, C.byref(out),
                                           C.byref(st)), "nd_render_gbuffer")
+        return st
+
+    def render_gbuffer_noalloc(self, u, gbuf, rect=None, normal_mode=abi.NORMAL_ANALYTIC, shadows=True,
+                               shadow_max_steps=0, kernel=abi.KERNEL_PERSISTENT):
+        """render_gbuffer without any device allocation: gbuf must hold device tensors, the rect
+        at most abi.NOALLOC_MAX_RAYS ray slots (64 per 8x8 tile), no accelerator."""
+        opts = MarchOpts(normal_mode, shadow_max_steps, 1 if shadows else 0, kernel)
+        r = Rect(*(rect or (0, 0, 0, 0)))
+        st = BatchStats()
+        out = gbuf.out()
+        self._check(lib.nd_render_gbuffer_noalloc(self._h, C.byref(u), abi.UNIFORMS_SIZE, C.byref(opts), r,
+                                                  C.byref(out), C.byref(st)), "nd_render_gbuffer_noalloc")
         return st
 
     def render_batch(self, jobs, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0,

@@ -137,6 +137,9 @@ class Tuning(C.Structure):
                 ("round_cap", U32)]
 
 
+NOALLOC_MAX_RAYS = 1 << 19
+NOALLOC_MAX_GRID = 4096
+
 TUNE_TRACK_LATENCY = 1
 TUNE_NO_GRAPH = 2
 
@@ -194,7 +197,7 @@ class MeshGrid(C.Structure):
 
 
 EXPORTS = ["nd_ctx_create", "nd_ctx_destroy", "nd_last_error", "nd_sync", "nd_set_cancel", "nd_ctx_stream",
-           "nd_bind_formulas", "nd_render_gbuffer", "nd_render_batch", "nd_batch_job_latency",
+           "nd_bind_formulas", "nd_render_gbuffer", "nd_render_gbuffer_noalloc", "nd_render_batch", "nd_batch_job_latency",
            "nd_get_tuning", "nd_set_tuning", "nd_autotune", "nd_adjoint", "nd_path_trace", "nd_mesh_points", "nd_mesh_probe", "nd_accel_probe", "nd_accel_setup",
            "nd_config_job_count", "nd_uniforms_defaults", "nd_config_build", "nd_camera_look_at",
            "nd_set_pixel_epsilon"]

@@ -368,6 +368,7 @@ nd_status nd_ctx_create(int hip_device, nd_ctx** out) {
     if (hipSetDevice(hip_device) != hipSuccess) return ND_E_DEVICE;
     nd_ctx* c = new nd_ctx();
     c->device = hip_device;
+    c->noalloc_scratch = nd_noalloc_scratch(hip_device);  // device address of this GPU's scratch array
     for (int i = 0; i < ND_MAX_CODES; ++i) c->code_map[i] = i < 4 ? i : -1;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -499,8 +500,10 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     if (!c) return ND_E_INVALID;
     NdBatchExtra ex;
     if (extra) ex = *extra;
+    const NdFixedWorkspace* fx = ex.fixed;  // non-null: carve every buffer from it, allocate nothing
     if (!jobs || n == 0) return nd_fail(c, ND_E_INVALID, "empty batch");
     if (n > 65535) return nd_fail(c, ND_E_INVALID, "at most 65535 jobs per batch");
+    if (fx && n != 1) return nd_fail(c, ND_E_INVALID, "the fixed workspace holds one job");
     if (c->cancel && *c->cancel) return nd_fail(c, ND_E_CANCELLED, "cancelled before launch");
     nd_march_opts o{};
     if (opts) o = *opts;
@@ -508,12 +511,25 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     if (o.kernel > ND_KERNEL_PERSISTENT) return nd_fail(c, ND_E_INVALID, "kernel %u", o.kernel);
     if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
 
-    std::vector<DevJob>& hj = c->host_jobs;
-    hj.assign(n, DevJob{});
-    std::vector<uint32_t> base(n);
+    // host records: the context's vectors, or (fixed workspace, one job) the stack
+    DevJob fx_job;
+    uint32_t fx_base = 0;
+    std::vector<uint32_t> base_vec;
+    DevJob* hj;
+    uint32_t* base;
+    if (fx) {
+        std::memset(&fx_job, 0, sizeof(fx_job));
+        hj = &fx_job;
+        base = &fx_base;
+    } else {
+        c->host_jobs.assign(n, DevJob{});
+        base_vec.assign(n, 0u);
+        hj = c->host_jobs.data();
+        base = base_vec.data();
+    }
     uint64_t total = 0;
     size_t staging = 0;
-    bool any_shadow = false;
+    bool any_shadow = false, any_accel = false;
     for (uint32_t i = 0; i < n; ++i) {
         if (!jobs[i].uniforms) return nd_fail(c, ND_E_INVALID, "job %u: null uniforms", i);
         nd_status st = nd_pack_job(c, *jobs[i].uniforms, o, jobs[i].rect, hj[i]);
@@ -523,15 +539,24 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         base[i] = (uint32_t)total;
         total += hj[i].ray_count;
         any_shadow = any_shadow || hj[i].shadow_on;
+        any_accel = any_accel || hj[i].accel_levels > 0;
         const nd_gbuffer_out& go = jobs[i].out;
         size_t px = (size_t)hj[i].rect_w * hj[i].rect_h;
         if (!go.device_ptrs) staging += out_bytes(go, px) + 7 * 256;
     }
     if (total >= (1ull << 31)) return nd_fail(c, ND_E_INVALID, "batch of %llu ray slots exceeds 2^31", (unsigned long long)total);
+    if (fx) {
+        if (staging) return nd_fail(c, ND_E_INVALID, "without allocation the outputs must be device pointers (device_ptrs = 1)");
+        if (any_accel)
+            return nd_fail(c, ND_E_UNSUPPORTED, "without allocation the empty-space accelerator is not available (its grids are allocated)");
+        if (total > fx->max_rays)
+            return nd_fail(c, ND_E_INVALID, "%llu ray slots (64 per 8x8 tile of the rect) exceed the %u of the fixed workspace",
+                           (unsigned long long)total, fx->max_rays);
+    }
 
     // outputs: device pointers as given, host pointers through a staging buffer
-    nd_status st = c->staging.ensure(c, staging);
-    if (st != ND_OK) return st;
+    nd_status st = ND_OK;
+    if (staging && (st = c->staging.ensure(c, staging)) != ND_OK) return st;
     {
         char* sp = (char*)c->staging.p;
         auto carve = [&](size_t bytes) { char* r = sp; sp += (bytes + 255) & ~(size_t)255; return r; };
@@ -560,42 +585,69 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     for (uint32_t i = 0; i < n && !ext; ++i) ext = nd_scene_ext(hj[i].scene);
     c->last_shared = shared && !ext;  // the EXT kernels read each job's own scene
     c->last_ext = ext;
-    if ((st = nd_accel_prepare(c, hj.data(), n)) != ND_OK) return st;
+    if (!fx && (st = nd_accel_prepare(c, hj, n)) != ND_OK) return st;
 
-    if ((st = c->jobs.ensure(c, sizeof(DevJob) * n)) != ND_OK) return st;
-    if ((st = c->job_base.ensure(c, sizeof(uint32_t) * n)) != ND_OK) return st;
-    if ((st = c->ctr.ensure(c, sizeof(DevCounters))) != ND_OK) return st;
-    if ((st = c->shared.ensure(c, sizeof(DevScene))) != ND_OK) return st;
-    const bool track = (c->tuning.flags & ND_TUNE_TRACK_LATENCY) != 0;
-    // latency: per-job completion clocks, then (when tracking) one completion tick per ray slot
+    // device records: grown context buffers, or carved from the fixed workspace
+    const bool track = !fx && (c->tuning.flags & ND_TUNE_TRACK_LATENCY) != 0;
     const size_t lat_jobs = (sizeof(unsigned long long) * n + 255) & ~(size_t)255;
-    if ((st = c->latency.ensure(c, lat_jobs + (track ? sizeof(uint32_t) * (size_t)total : 0))) != ND_OK) return st;
+    char* fxp = fx ? fx->base : nullptr;
+    char* const fx_end = fx ? fx->base + fx->bytes : nullptr;
+    auto fcarve = [&](size_t bytes) -> char* {
+        char* r = fxp;
+        fxp += (bytes + 255) & ~(size_t)255;
+        return fxp <= fx_end ? r : nullptr;
+    };
+    DevJob* d_jobs;
+    uint32_t* d_job_base;
+    DevCounters* d_ctr;
+    DevScene* d_shared;
+    char* d_latency = nullptr;
+    if (fx) {
+        d_jobs = (DevJob*)fcarve(sizeof(DevJob) * n);
+        d_job_base = (uint32_t*)fcarve(sizeof(uint32_t) * n);
+        d_ctr = (DevCounters*)fcarve(sizeof(DevCounters));
+        d_shared = (DevScene*)fcarve(sizeof(DevScene));
+        if (!d_jobs || !d_job_base || !d_ctr || !d_shared) return nd_fail(c, ND_E_INVALID, "fixed workspace too small for the job records");
+    } else {
+        if ((st = c->jobs.ensure(c, sizeof(DevJob) * n)) != ND_OK) return st;
+        if ((st = c->job_base.ensure(c, sizeof(uint32_t) * n)) != ND_OK) return st;
+        if ((st = c->ctr.ensure(c, sizeof(DevCounters))) != ND_OK) return st;
+        if ((st = c->shared.ensure(c, sizeof(DevScene))) != ND_OK) return st;
+        // latency: per-job completion clocks, then (when tracking) one completion tick per ray slot
+        if ((st = c->latency.ensure(c, lat_jobs + (track ? sizeof(uint32_t) * (size_t)total : 0))) != ND_OK) return st;
+        d_jobs = (DevJob*)c->jobs.p;
+        d_job_base = (uint32_t*)c->job_base.p;
+        d_ctr = (DevCounters*)c->ctr.p;
+        d_shared = (DevScene*)c->shared.p;
+        d_latency = (char*)c->latency.p;
+    }
     hipError_t e;
-    if ((e = hipMemcpyAsync(c->jobs.p, hj.data(), sizeof(DevJob) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(d_jobs, hj, sizeof(DevJob) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return nd_hip_fail(c, e, "upload jobs");
-    if ((e = hipMemcpyAsync(c->job_base.p, base.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(d_job_base, base, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return nd_hip_fail(c, e, "upload job table");
-    if (shared && (e = hipMemcpyAsync(c->shared.p, &hj[0].scene, sizeof(DevScene), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    if (shared && (e = hipMemcpyAsync(d_shared, &hj[0].scene, sizeof(DevScene), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return nd_hip_fail(c, e, "upload scene");
 
+    // a failure after the uploads were queued waits for them: their source may be this frame
+    auto drained = [&](nd_status r) { (void)hipStreamSynchronize(c->stream); return r; };
     nd_batch_stats s{};
     float ms_total = 0, ms_march = 0, ms_surface = 0, ms_shadow = 0;
     const bool persistent = o.kernel != ND_KERNEL_PLAIN;
     if (!persistent) {
         // one launch per frame: the job record is read from device memory by one kernel each
-        if ((e = nd_launch_zero_counters((DevCounters*)c->ctr.p, c->stream)) != hipSuccess)
-            return nd_hip_fail(c, e, "zero counters");
+        if ((e = nd_launch_zero_counters(d_ctr, c->stream)) != hipSuccess)
+            return drained(nd_hip_fail(c, e, "zero counters"));
         (void)hipEventRecord(c->ev[0], c->stream);
         for (uint32_t i = 0; i < n; ++i) {
-            e = nd_launch_plain((const DevJob*)c->jobs.p + i, hj[i].tiles_x, hj[i].tiles_y, (DevCounters*)c->ctr.p,
-                                ext, c->stream);
-            if (e != hipSuccess) return nd_hip_fail(c, e, "launch plain march");
+            e = nd_launch_plain(d_jobs + i, hj[i].tiles_x, hj[i].tiles_y, d_ctr, ext, c->stream);
+            if (e != hipSuccess) return drained(nd_hip_fail(c, e, "launch plain march"));
         }
         (void)hipEventRecord(c->ev[3], c->stream);
         s.kernels = n;
     } else {
         // hit slots: every ray may hit, plus at most one partly used 64-slot chunk per wave
-        const bool acc = [&] { for (uint32_t i = 0; i < n; ++i) if (hj[i].accel) return true; return false; }();
+        const bool acc = any_accel;
         // Claim size: with one scene for every job (the 16-camera batch), 96-ray claims give the
         // march pass the finest tail (-4.5% against 256); a batch of different scenes changes
         // job and scene at more claims and is fastest at 256 (the 256-job batch: 62 ms at 256,
@@ -607,17 +659,26 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         const int res_shadow = ext ? (acc ? RES_SHADOW_ACC_EXT : RES_SHADOW_EXT)
                                    : acc ? (shared ? RES_SHADOW_ACC : RES_SHADOW_ACC_MIXED) : (shared ? RES_SHADOW : RES_SHADOW_MIXED);
         const int res_surface = ext ? RES_SURFACE_EXT : (shared ? RES_SURFACE : RES_SURFACE_MIXED);
-        const uint32_t grid_march = nd_pass_grid(c, res_march, claims);
+        const uint32_t grid_cap = fx ? fx->max_grid : 0xffffffffu;
+        const uint32_t grid_march = std::min(nd_pass_grid(c, res_march, claims), grid_cap);
+        const uint32_t grid_shadow = std::min(nd_pass_grid(c, res_shadow, claims), grid_cap);
+        const uint32_t grid_surface = std::min(nd_pass_grid(c, res_surface, ((uint64_t)total + 63) / 64), grid_cap);
         // the claim counters run past the end by at most one claim per wave (shadow claims
         // are 4 x chunk_rays); they must not wrap
-        const uint32_t grid_max = std::max(grid_march, nd_pass_grid(c, res_shadow, claims));
+        const uint32_t grid_max = std::max(grid_march, grid_shadow);
         if ((uint64_t)total + (uint64_t)grid_max * 4u * chunk >= (1ull << 32))
-            return nd_fail(c, ND_E_INVALID, "batch of %llu rays too large for one persistent launch",
-                           (unsigned long long)total);
+            return drained(nd_fail(c, ND_E_INVALID, "batch of %llu rays too large for one persistent launch",
+                                   (unsigned long long)total));
         size_t hit_cap = (size_t)total + (size_t)grid_march * 64;
         size_t hb_bytes = hit_cap * (4 + 2 + 4 + 4 + 2 + 7 * 4) + 16 * 256;
-        if ((st = c->hits.ensure(c, hb_bytes)) != ND_OK) return st;
-        char* hp = (char*)c->hits.p;
+        char* hp;
+        if (fx) {
+            if (!(hp = fcarve(hb_bytes)))
+                return drained(nd_fail(c, ND_E_INVALID, "fixed workspace too small for %zu hit slots", hit_cap));
+        } else {
+            if ((st = c->hits.ensure(c, hb_bytes)) != ND_OK) return drained(st);
+            hp = (char*)c->hits.p;
+        }
         auto carve = [&](size_t bytes) { char* r = hp; hp += (bytes + 255) & ~(size_t)255; return r; };
         NdPassLaunch L{};
         L.hb.ray = (uint32_t*)carve(4 * hit_cap);
@@ -633,15 +694,15 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.hb.sdz = (float*)carve(4 * hit_cap);
         L.hb.sdist = (float*)carve(4 * hit_cap);
         L.hb.capacity = (uint32_t)hit_cap;
-        L.jobs = (const DevJob*)c->jobs.p;
-        L.shared = shared ? (const DevScene*)c->shared.p : nullptr;
-        L.job_base = (const uint32_t*)c->job_base.p;
+        L.jobs = d_jobs;
+        L.shared = shared ? d_shared : nullptr;
+        L.job_base = d_job_base;
         L.n_jobs = (int)n;
         L.total_rays = (uint32_t)total;
-        L.ctr = (DevCounters*)c->ctr.p;
+        L.ctr = d_ctr;
         L.grid_march = grid_march;
-        L.grid_surface = nd_pass_grid(c, res_surface, ((uint64_t)total + 63) / 64);
-        L.grid_shadow = nd_pass_grid(c, res_shadow, claims);
+        L.grid_surface = grid_surface;
+        L.grid_shadow = grid_shadow;
         c->last_rays = (uint32_t)total;
         c->last_hb = L.hb;
         L.chunk = chunk;
@@ -651,44 +712,45 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.surface = !ex.skip_surface;
         L.accel = acc;
         L.ext = ext;
-        L.job_done_cycles = track ? (unsigned long long*)c->latency.p : nullptr;
-        L.ray_done = track ? (uint32_t*)((char*)c->latency.p + lat_jobs) : nullptr;
+        L.job_done_cycles = track ? (unsigned long long*)d_latency : nullptr;
+        L.ray_done = track ? (uint32_t*)(d_latency + lat_jobs) : nullptr;
         for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];
         e = nd_launch_persistent(L, c->stream);
-        if (e != hipSuccess) return nd_hip_fail(c, e, "launch persistent passes");
+        if (e != hipSuccess) return drained(nd_hip_fail(c, e, "launch persistent passes"));
         s.kernels = 3 + (any_shadow ? 1 : 0) + 1;
     }
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return nd_hip_fail(c, e, "batch execution");
     (void)hipEventElapsedTime(&ms_total, c->ev[0], c->ev[3]);
+    DevCounters dc;
+    if ((e = hipMemcpy(&dc, d_ctr, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
+        return nd_hip_fail(c, e, "read counters");
     if (persistent) {
         (void)hipEventElapsedTime(&ms_march, c->ev[0], c->ev[1]);
         (void)hipEventElapsedTime(&ms_surface, c->ev[1], c->ev[2]);
         (void)hipEventElapsedTime(&ms_shadow, c->ev[2], c->ev[3]);
-        DevCounters dc;
-        if ((e = hipMemcpy(&dc, c->ctr.p, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
-            return nd_hip_fail(c, e, "read counters");
         s.hits = dc.hits;
         s.de_evals_march = dc.de_march;
         s.march_wave_rounds = dc.march_rounds;
         s.de_evals_surface = dc.de_surface;
         s.de_evals_shadow = dc.de_shadow;
         s.de_evals = dc.de_march + dc.de_surface + dc.de_shadow;
-        std::vector<unsigned long long> done(n);
-        if ((e = hipMemcpy(done.data(), c->latency.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost)) != hipSuccess)
-            return nd_hip_fail(c, e, "read latency");
-        c->job_latency_ms.assign(n, 0.0f);
-        unsigned long long t0 = dc.clock_start;
-        for (uint32_t i = 0; i < n; ++i)
-            c->job_latency_ms[i] = done[i] > t0 ? (float)((double)(done[i] - t0) / (double)c->wall_khz) : 0.0f;
+        if (!fx) {
+            c->job_latency_ms.assign(n, 0.0f);
+            if (track) {
+                std::vector<unsigned long long> done(n);
+                if ((e = hipMemcpy(done.data(), d_latency, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost)) != hipSuccess)
+                    return nd_hip_fail(c, e, "read latency");
+                unsigned long long t0 = dc.clock_start;
+                for (uint32_t i = 0; i < n; ++i)
+                    c->job_latency_ms[i] = done[i] > t0 ? (float)((double)(done[i] - t0) / (double)c->wall_khz) : 0.0f;
+            }
+        }
         if (dc.rounds_capped) return nd_fail(c, ND_E_TIMEOUT, "%u waves reached the round cap", dc.rounds_capped);
     } else {
-        DevCounters dc;
-        if ((e = hipMemcpy(&dc, c->ctr.p, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
-            return nd_hip_fail(c, e, "read counters");
         s.de_evals = dc.de_march;
         s.de_evals_march = dc.de_march;
-        c->job_latency_ms.assign(n, ms_total);
+        if (!fx) c->job_latency_ms.assign(n, ms_total);
     }
     for (uint32_t i = 0; i < n; ++i) s.rays += (uint64_t)hj[i].rect_w * hj[i].rect_h;
     s.ms_total = ms_total;

@@ -35,6 +35,7 @@ struct nd_ctx {
     int wall_khz = 100000;
     uint32_t round_cap = 0xffffffffu;
     const volatile int* cancel = nullptr;
+    char* noalloc_scratch = nullptr;   // this device's no-allocation scratch array (device address)
     hipEvent_t ev[8] = {};
     bool graph_valid = false;          // false after any device buffer was reallocated
     hipGraphExec_t graph_exec = nullptr;
@@ -54,11 +55,26 @@ struct nd_ctx {
     std::vector<float> job_latency_ms;
 };
 
+// A fixed device region a batch carves all its buffers from instead of growing the context's
+// buffers (nd_render_gbuffer_noalloc): nothing is allocated, sizes beyond it are rejected.
+struct NdFixedWorkspace {
+    char* base;          // device address (from hipGetSymbolAddress), 256-byte aligned
+    size_t bytes;
+    uint32_t max_rays;   // ray slots (64 per 8x8 tile) of the batch
+    uint32_t max_grid;   // persistent workgroups per pass
+};
+
 // Internal options of a batch beyond the public nd_march_opts.
 struct NdBatchExtra {
     bool skip_surface = false;   // no normal / trap pass (the adjoint needs only t)
     bool adjoint_slots = false;  // attach adjoint tangents to the differentiable params
+    const NdFixedWorkspace* fixed = nullptr;
 };
+
+// Device address of the library's no-allocation scratch array on `device` (the current
+// device), looked up once per device; nullptr if the lookup failed.
+char* nd_noalloc_scratch(int device);
+size_t nd_noalloc_scratch_bytes();
 
 nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_opts* opts, nd_batch_stats* stats,
                         const NdBatchExtra* extra = nullptr);
