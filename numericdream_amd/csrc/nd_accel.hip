@@ -121,13 +121,19 @@ nd_status nd_accel_prepare(nd_ctx* c, DevJob* jobs, uint32_t n) {
 }
 
 namespace nd {
-// accel_skip at n points (xyz triples) with the grid of job J.
+// accel_skip at n points (xyz triples) with the grid of job J, read the way a ray reads it:
+// through the register cache of the cell of the previous point (point i - 1), so points in
+// ray order exercise the cache's box test and shuffled points the full lookup.
 __global__ void k_accel_probe(const DevJob* __restrict__ J, const float* __restrict__ pts, uint32_t n,
                               float* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const AccelView av = accel_view(*J);
-    out[i] = av.grid ? accel_skip(av, mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2])) : 0.f;
+    if (!av.grid) { out[i] = 0.f; return; }
+    AccelCache c;
+    accel_cache_reset(c);
+    if (i > 0) (void)accel_skip_cached(av, c, mk3(pts[3 * i - 3], pts[3 * i - 2], pts[3 * i - 1]));
+    out[i] = accel_skip_cached(av, c, mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
 }
 }  // namespace nd
 

@@ -48,7 +48,8 @@ NDD float cell_index(float x, float cell, float ic) {
     const float q0 = x * ic;
     return floorf(fmaf(fmaf(-q0, cell, x), ic, q0));
 }
-NDD float accel_skip(const AccelView& A, const V3<float>& p) {
+// The cell that holds p: its level and coordinates; false outside every grid.
+NDD bool accel_locate(const AccelView& A, const V3<float>& p, int* lvl, int* ix, int* iy, int* iz) {
     const float u = fmaxf(fabsf(p.x - A.cx), fmaxf(fabsf(p.y - A.cy), fabsf(p.z - A.cz)));
     int l = (int)ceilf(fmaxf(__builtin_amdgcn_logf(u * A.inv_half0), 0.f)) - 2;
     l = l < 0 ? 0 : l;
@@ -59,11 +60,41 @@ NDD float accel_skip(const AccelView& A, const V3<float>& p) {
         const float fx = cell_index(p.x - q[0], q[3], ic);
         const float fy = cell_index(p.y - q[1], q[3], ic);
         const float fz = cell_index(p.z - q[2], q[3], ic);
-        if (fx < 0.f || fy < 0.f || fz < 0.f || fx >= fres || fy >= fres || fz >= fres) continue;
-        const size_t idx = (((size_t)l * A.res + (size_t)fz) * A.res + (size_t)fy) * A.res + (size_t)fx;
-        return A.grid[idx];
+        // written so that a NaN coordinate (a ray whose DE overflowed) is outside every grid
+        if (!(fx >= 0.f && fy >= 0.f && fz >= 0.f && fx < fres && fy < fres && fz < fres)) continue;
+        *lvl = l; *ix = (int)fx; *iy = (int)fy; *iz = (int)fz;
+        return true;
     }
-    return 0.f;
+    return false;
+}
+// Index into the grid ([level][z][y][x]) of the cell that holds p, -1 outside every grid. A
+// grid of one scene holds fewer than 2^31 cells (checked where it is built).
+NDD int accel_index(const AccelView& A, const V3<float>& p) {
+    int l, ix, iy, iz;
+    if (!accel_locate(A, p, &l, &ix, &iy, &iz)) return -1;
+    return ((l * A.res + iz) * A.res + iy) * A.res + ix;
+}
+NDD float accel_skip(const AccelView& A, const V3<float>& p) {
+    const int idx = accel_index(A, p);
+    return idx < 0 ? 0.f : A.grid[idx];
+}
+
+// The last cell a ray read and its value. Consecutive probes of a ray near a surface step by
+// less than a cell, so most land in the cell of the previous probe: the value is then reused
+// from registers instead of gathered again. Reset whenever the lane takes a ray whose job may
+// have another grid.
+struct AccelCache {
+    int idx;  // -2: empty
+    float val;
+};
+NDD void accel_cache_reset(AccelCache& c) { c.idx = -2; c.val = 0.f; }
+NDD float accel_skip_cached(const AccelView& A, AccelCache& c, const V3<float>& p) {
+    const int idx = accel_index(A, p);
+    if (idx != c.idx) {
+        c.val = idx < 0 ? 0.f : A.grid[idx];
+        c.idx = idx;
+    }
+    return c.val;
 }
 
 // The accelerator test of a shadow-ray step (visibility_march, template.wgsl:2297-2303): a

@@ -88,16 +88,15 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
     int st;
     const bool acc = J.accel != nullptr;
     const AccelView av = accel_view(J);
+    AccelCache ac;
+    accel_cache_reset(ac);
     do {
-        // with the accelerator the grid value is read while the DE at the same point is
-        // evaluated; a skip then takes the step and the DE sample is not used
+        // march_to's loop: at a probe the accelerator's skip, when it applies, takes the step
+        // without a DE (template.wgsl:1890-1900); otherwise the DE decides
         float s = march_sample_t(m);
         const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
-        const bool probe = acc && m.probing;
-        const float skip = probe ? accel_skip(av, ps) : 0.f;
-        float d = scene_de<false, EXT>(sc, ps, nullptr);
-        st = probe ? march_skip_step(m, J, skip, &dist) : -1;
-        if (st < 0) st = march_update(m, J, s, d, &dist);
+        st = (acc && m.probing) ? march_skip_step(m, J, accel_skip_cached(av, ac, ps), &dist) : -1;
+        if (st < 0) st = march_update(m, J, s, scene_de<false, EXT>(sc, ps, nullptr), &dist);
     } while (st == MS_CONT);
     const size_t o = (size_t)y * J.rect_w + x;
     int evals = m.evals;
@@ -125,12 +124,15 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
         } else {
             const V3<float> so = mk3(hp.x + n.x * 0.004f, hp.y + n.y * 0.004f, hp.z + n.z * 0.004f);
             float t = 0.004f, res = 1.f;
+            accel_cache_reset(ac);
             for (int i = 0; i < J.shadow_steps; ++i) {
                 if (t >= ldist) break;
                 const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
-                const float skip = acc ? accel_skip(av, ps) : 0.f;
-                float d = scene_de<false, EXT>(sc, ps, nullptr);
-                if (acc && shadow_skip_applies(skip, J.light_k, res, t)) { t += skip; continue; }
+                if (acc) {
+                    const float skip = accel_skip_cached(av, ac, ps);
+                    if (shadow_skip_applies(skip, J.light_k, res, t)) { t += skip; continue; }
+                }
+                const float d = scene_de<false, EXT>(sc, ps, nullptr);
                 evals++;
                 if (d < 0.0005f) { res = 0.f; break; }
                 res = fminf(res, J.light_k * d * frcp(t));
@@ -215,11 +217,11 @@ NDD void write_miss(const DevJob& J, size_t o, int evals) {
 // Left to itself the compiler gives this kernel 84 VGPRs (93 without a shared scene), 5 waves
 // per SIMD. Forcing 7 waves with a shared scene (72 VGPRs, three spilled values) makes the
 // pass about 7% faster, and 6 waves without one about 4%: more waves hide the DE's dependent
-// latency better than the spills cost. The accelerated variant waits on grid gathers and
-// keeps the compiler's choice.
+// latency better than the spills cost. The accelerated variant (about 100 VGPRs, 4 waves left
+// to itself) is fastest at 6: 35.0 ms per 16-camera launch against 40.3 at 4 and 43.9 at 7.
 template <bool SHARED, bool ACC, bool EXT>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu((ACC || EXT) ? 1 : (SHARED ? 7 : 6), (ACC || EXT) ? 8 : (SHARED ? 7 : 6))))
+__attribute__((amdgpu_waves_per_eu(EXT ? 1 : 6 + (SHARED && !ACC), EXT ? 8 : 6 + (SHARED && !ACC))))
 k_march(PassArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     uint32_t cnext = 0, cend = 0;          // wave-uniform chunk of ray slots [cnext, cend)
@@ -239,6 +241,8 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
     MarchParams mp = {};
     bool acc = false;  // the ray's job uses the accelerator
     AccelView av = {};
+    AccelCache ac;
+    accel_cache_reset(ac);
     unsigned long long de = 0, nhits = 0;
     uint32_t rounds_used = 0;  // rounds that evaluated at least one DE
     for (uint32_t round = 0;; ++round) {
@@ -278,6 +282,7 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
                             mp = march_params(CJ);
                             acc = ACC && CJ.accel != nullptr;
                             if (acc) av = accel_view(CJ);
+                            if (ACC) accel_cache_reset(ac);
                             march_init(m);
                             active = true;
                         }
@@ -296,19 +301,16 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
         bool hit_now = false;
         float dist = 0.f;
         int st = -1;
-        bool used = false;  // the DE sample decided the step (not an accelerator skip)
+        bool used = false;  // the lane evaluated a DE this round (no accelerator skip applied)
         if (active) {
-            // with the accelerator the grid value is read while the DE at the same point is
-            // evaluated; a skip then takes the step and the DE sample is not used
+            // march_to's loop: at a probe the accelerator's skip, when it applies, takes the
+            // step without a DE (template.wgsl:1890-1900); a wave whose lanes all skip runs no DE
             const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             const float s = march_sample_t(m);
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
-            const bool probe = ACC && acc && m.probing;
-            const float skip = probe ? accel_skip(av, ps) : 0.f;
-            const float d = scene_de<false, EXT>(sc, ps, nullptr);
-            if (probe) st = march_skip_step(m, mp, skip, &dist);
+            if (ACC && acc && m.probing) st = march_skip_step(m, mp, accel_skip_cached(av, ac, ps), &dist);
             used = st < 0;
-            if (used) st = march_update(m, mp, s, d, &dist);
+            if (used) st = march_update(m, mp, s, scene_de<false, EXT>(sc, ps, nullptr), &dist);
         }
         de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(used));
         if (active) {
@@ -436,6 +438,8 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
     int job = 0, steps = 0, step = 0, sev = 0;  // step: march steps incl. accelerator skips; sev: DEs
     bool sacc = false;
     AccelView sav = {};
+    AccelCache sac;
+    accel_cache_reset(sac);
     V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
     float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
     uint32_t sray = 0;           // ray slot of the current shadow ray
@@ -474,6 +478,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                             k = jobs[job].light_k;
                             sacc = ACC && jobs[job].accel != nullptr;
                             if (sacc) sav = accel_view(jobs[job]);
+                            if (ACC) accel_cache_reset(sac);
                             t = 0.004f; res = 1.f; step = 0; sev = 0;
                             active = true;
                         }
@@ -492,14 +497,17 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
         if (active) {
             const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
-            const float skip = (ACC && sacc) ? accel_skip(sav, ps) : 0.f;
-            const float d = scene_de<false, EXT>(sc, ps, nullptr);
             step++;
-            skipped = ACC && sacc && shadow_skip_applies(skip, k, res, t);
+            float skip = 0.f;
+            if (ACC && sacc) {
+                skip = accel_skip_cached(sav, sac, ps);
+                skipped = shadow_skip_applies(skip, k, res, t);
+            }
             if (skipped) {
                 t += skip;
                 if (step >= steps || t >= ldist) done = true;
             } else {
+                const float d = scene_de<false, EXT>(sc, ps, nullptr);
                 sev++;
                 if (d < 0.0005f) { res = 0.f; done = true; }
                 else {

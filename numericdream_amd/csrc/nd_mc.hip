@@ -540,6 +540,8 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
     const MarchParams mp = march_params(J);
     const bool acc = J.accel != nullptr;
     const AccelView av = accel_view(J);
+    AccelCache ac;  // every ray of the call reads the same grid: valid across rays
+    accel_cache_reset(ac);
     uint32_t cnext = 0, cend = 0;
     bool work_left = true, active = false;
     uint32_t item = 0;
@@ -595,7 +597,7 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
         }
         float dist = 0.f;
         int st = -1;
-        bool used = false;  // the DE sample decided the step (not an accelerator skip)
+        bool used = false;  // the lane evaluated a DE this round (no accelerator skip applied)
         if (active && INSIDE && ins) {
             const V3<float> ps = mk3(org.x + dir.x * it, org.y + dir.y * it, org.z + dir.z * it);
             const float d = scene_de<false, EXT>(sc, ps, nullptr);
@@ -615,12 +617,10 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
         } else if (active) {
             const float s = march_sample_t(m);
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
-            const bool probe = acc && m.probing;
-            const float skip = probe ? accel_skip(av, ps) : 0.f;
-            const float d = scene_de<false, EXT>(sc, ps, nullptr);
-            if (probe) st = march_skip_step(m, mp, skip, &dist);
+            // an accelerator skip takes the step without a DE (template.wgsl:1890-1900)
+            if (acc && m.probing) st = march_skip_step(m, mp, accel_skip_cached(av, ac, ps), &dist);
             used = st < 0;
-            if (used) st = march_update(m, mp, s, d, &dist);
+            if (used) st = march_update(m, mp, s, scene_de<false, EXT>(sc, ps, nullptr), &dist);
             if (st != MS_CONT) {
                 R.finish(item, st == MS_HIT, dist);
                 active = false;
@@ -853,6 +853,8 @@ NDD Sp8 vis_inline(const McArgs& a, const Basis& B, const V3<float>& o, const V3
     const DevJob& J = *a.job;
     const bool acc = J.accel != nullptr;
     const AccelView av = accel_view(J);
+    AccelCache ac;
+    accel_cache_reset(ac);
     float t = 0.004f, res = 1.f, flo = 0.004f;  // SHADOW_BIAS as t0, SHADOW_STEP_FLOOR
     bool ins = false;
     Sp8 tint = sp_splat(1.f);
@@ -860,7 +862,7 @@ NDD Sp8 vis_inline(const McArgs& a, const Basis& B, const V3<float>& o, const V3
         if (t >= ldist) break;
         const V3<float> ps = mk3(o.x + ld.x * t, o.y + ld.y * t, o.z + ld.z * t);
         if (!ins && acc) {
-            const float skip = accel_skip(av, ps);
+            const float skip = accel_skip_cached(av, ac, ps);
             if (shadow_skip_applies(skip, k, res, t)) {
                 t += skip;
                 continue;
@@ -911,6 +913,8 @@ NDD Sp8 sss_light_path(const McArgs& a, const Basis& B, const V3<float>& pos, co
     const DevJob& J = *a.job;
     const bool acc = J.accel != nullptr;
     const AccelView av = accel_view(J);
+    AccelCache ac;
+    accel_cache_reset(ac);
     const V3<float> o = start_inside ? pos : mk3(pos.x + n.x * 0.004f, pos.y + n.y * 0.004f, pos.z + n.z * 0.004f);
     const float cap = s.radius * 6.f;                          // SSS_PROBE_DEPTH_RADII
     const float step_base = fmaxf(s.radius * 0.05f, 0.0005f);  // SSS_PROBE_STEP_PER_RADIUS, INSIDE_MIN_STEP
@@ -922,7 +926,7 @@ NDD Sp8 sss_light_path(const McArgs& a, const Basis& B, const V3<float>& pos, co
         if (t >= ldist) break;
         const V3<float> p = mk3(o.x + ld.x * t, o.y + ld.y * t, o.z + ld.z * t);
         if (!inside && acc) {
-            const float skip = accel_skip(av, p);
+            const float skip = accel_skip_cached(av, ac, p);
             if (skip > 0.001f) {  // SHADOW_MIN_STEP
                 t += skip;
                 continue;
@@ -1312,6 +1316,8 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
     const DevJob& J = *a.job;
     const bool acc = J.accel != nullptr;
     const AccelView av = accel_view(J);
+    AccelCache ac;  // one grid for every shadow ray of the call
+    accel_cache_reset(ac);
     const uint32_t n = a.P.ctr[C_SHADOW];
     const int max_steps = a.S->shadow_steps;
     uint32_t cnext = 0, cend = 0;
@@ -1369,10 +1375,14 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
         bool done = false, skipped = false;
         if (active) {
             const V3<float> ps = mk3(so.x + ld.x * t, so.y + ld.y * t, so.z + ld.z * t);
-            const float skip = (acc && !(TRANSP && ins)) ? accel_skip(av, ps) : 0.f;
-            const float d = scene_de<false, EXT>(sc, ps, nullptr);
             step++;
-            skipped = acc && !(TRANSP && ins) && shadow_skip_applies(skip, k, res, t);
+            float skip = 0.f;
+            if (acc && !(TRANSP && ins)) {
+                skip = accel_skip_cached(av, ac, ps);
+                skipped = shadow_skip_applies(skip, k, res, t);
+            }
+            // a skip takes the step without a DE (visibility_march, template.wgsl:2297-2303)
+            const float d = skipped ? 0.f : scene_de<false, EXT>(sc, ps, nullptr);
             if (skipped) {
                 t += skip;
             } else if (TRANSP && ins) {
