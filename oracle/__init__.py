@@ -143,6 +143,29 @@ def subsurface_path(u, hero, pos, normal, start_inside, light, occlusion, amount
     return out[:8].copy(), int(out[8]), out[9:12].copy(), out[12:20].copy()
 
 
+def sky(u, dirs):
+    """sky_color for directions (n, 3): RGB (n, 3)."""
+    L = lib()
+    L.ndo_sky.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    out = np.zeros_like(d)
+    L.ndo_sky(_u(u), _p(d), len(d), _p(out))
+    return out
+
+
+def calc_ao(u, pos, nrm, t, code_map=None):
+    """calc_ao at points (n, 3) with normals (n, 3) and ray depths t (n,)."""
+    L = lib()
+    L.ndo_calc_ao.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    nn = np.ascontiguousarray(nrm, np.float64).reshape(-1, 3)
+    tt = np.ascontiguousarray(np.broadcast_to(t, (len(p),)), np.float64)
+    out = np.zeros(len(p))
+    cm, cmp_ = _cm(code_map)
+    L.ndo_calc_ao(_u(u), cmp_, _p(p), _p(nn), _p(tt), len(p), _p(out))
+    return out
+
+
 def primary_dir(u, px, py):
     out = np.zeros(3)
     lib().ndo_primary_dir(_u(u), px, py, _p(out))

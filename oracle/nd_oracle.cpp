@@ -187,6 +187,25 @@ int ndo_normal(const nd_uniforms* u, const int32_t* cm, int prec, int mode, cons
     return 0;
 }
 
+// sky_color (template.wgsl:2050-2112) for n directions: RGB per direction.
+int ndo_sky(const nd_uniforms* u, const double* dirs, int n, double* out) {
+    for (int i = 0; i < n; ++i) {
+        V3<double> c = sky_color(*u, V3<double>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+    return 0;
+}
+
+// calc_ao (template.wgsl:1825-1841) of the scene at n points with their normals and ray depths.
+int ndo_calc_ao(const nd_uniforms* u, const int32_t* cm, const double* pos, const double* nrm, const double* t, int n,
+                double* out) {
+    Scene<double> s = decode_scene<double>(*u, code_map(cm));
+    for (int i = 0; i < n; ++i)
+        out[i] = calc_ao(s, V3<double>(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                         normalize(V3<double>(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2])), t[i]);
+    return 0;
+}
+
 int ndo_shadow(const nd_uniforms* u, const int32_t* cm, const double* origin, const double* dir, double light_dist,
                double k, int steps, double* vis, int32_t* evals) {
     Scene<double> s = decode_scene<double>(*u, code_map(cm));
@@ -514,6 +533,15 @@ int ndo_helper(int op, const double* a, double* out) {
                    const double c = a[2], sn = std::sqrt(std::max(0.0, 1.0 - c * c));
                    Spec<double> w = diffuse_weight(ss, a[1], D3(c, sn, 0.0));
                    for (int k = 0; k < 8; ++k) out[k] = w.v[k];
+                   return 0; }
+        case 36: {  // light_terms: l, n, view, glossiness (spec power mix(4, 128, glossiness)) -> n.l, specular
+                   const double g = a[9], power = 4.0 + (128.0 - 4.0) * std::min(std::max(g, 0.0), 1.0);
+                   light_terms(normalize(D3(a[0], a[1], a[2])), normalize(D3(a[3], a[4], a[5])),
+                               normalize(D3(a[6], a[7], a[8])), power, g, true, &out[0], &out[1]);
+                   return 0; }
+        case 37: { D3 r = perturb_normal(normalize(D3(a[0], a[1], a[2])), a[3], Seed{(float)a[4], (float)a[5]});
+                   out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 38: { double sc = 0; out[0] = roulette(a[0], Seed{(float)a[1], (float)a[2]}, &sc) ? 1.0 : 0.0; out[1] = sc;
                    return 0; }
         case 35: { Subsurface<double> ss{};  // subsurface_transmit: radius, depth, 8 reaches
                    ss.amount = 1.0; ss.radius = a[0];

@@ -248,6 +248,34 @@ template <class T> V3<T> cosine_sample(V3<T> n, Seed s) {
     return cosine_hemisphere(n, hash12w(s.x, s.y), hash12w(s.x + 91.73f, s.y + 31.13f));
 }
 
+// perturb_normal (template.wgsl:3396-3401): the normal bent toward a cosine-distributed
+// direction by alpha = roughness^2; unchanged at alpha <= MIN_ROUGHNESS (1e-4).
+template <class T> V3<T> perturb_normal(V3<T> n, T alpha, Seed s) {
+    if (alpha <= T(1e-4)) return n;
+    V3<T> h = cosine_sample(n, s);
+    return normalize(n + (h - n) * alpha);
+}
+
+// Russian roulette of path_shade (template.wgsl:4107-4115): survive with probability
+// clamp(max throughput, RR_MIN_SURVIVAL = 0.05, 1), decided by hash12w(seed + (59.3, 23.9));
+// a survivor's throughput is scaled by 1 / survive (*scale). Returns false when the path ends.
+template <class T> bool roulette(T thr_max, Seed sd, T* scale) {
+    T survive = tclamp(thr_max, T(0.05), T(1));
+    Seed rs = seed_add(sd, 59.3f, 23.9f);
+    if (T(hash12w(rs.x, rs.y)) > survive) { *scale = T(0); return false; }
+    *scale = T(1) / survive;
+    return true;
+}
+
+// light_terms (template.wgsl:3076-3087): n.l clamped at 0 and the Blinn-Phong specular
+// max(n.h, 0)^spec_power * glossiness with h = normalize(l + v), or 0 with the specular part off.
+template <class T> void light_terms(V3<T> ld, V3<T> n, V3<T> view, T spec_power, T glossiness, bool specular_on,
+                                    T* ndotl, T* spec) {
+    *ndotl = tmax(dot(n, ld), T(0));
+    V3<T> hd = normalize(ld + view);
+    *spec = specular_on ? pow(tmax(dot(n, hd), T(0)), spec_power) * glossiness : T(0);
+}
+
 struct LightSampleD { double dir[3], color[3], dist; };
 // light_sample for point and global lights (template.wgsl:2178-2190)
 template <class T> void light_sample(const nd_light& L, double max_dist, V3<T> pos, V3<T>* dir, V3<T>* color, T* dist) {
@@ -541,6 +569,21 @@ Spec<T> subsurface_light_path(const nd_uniforms& u, const Scene<P>& s, const Spe
     return subsurface_transmit(ss, depth);
 }
 
+// calc_ao (template.wgsl:1825-1841): 5 taps along the normal at radius (0.2 + 0.2 i), radius
+// = clamp(0.04 t, 0.003, 0.25); occlusion = sum of max(tap - DE, 0) weighted 0.6^i; the factor
+// clamp(1 - 2 occlusion / radius, 0, 1).
+template <class T, class P> T calc_ao(const Scene<P>& s, V3<T> pos, V3<T> n, T t) {
+    T radius = tclamp(t * T(0.04), T(0.003), T(0.25));
+    T occ = T(0), w = T(1);
+    for (int i = 0; i < 5; ++i) {
+        T step = radius * (T(0.2) + T(0.2) * T((double)i));
+        T dd = scene_de(s, pos + n * step);
+        occ = occ + tmax(step - dd, T(0)) * w;
+        w = w * T(0.6);
+    }
+    return tclamp(T(1) - T(2) * occ / radius, T(0), T(1));
+}
+
 // shade_local (template.wgsl:3222-3331), with the Monte Carlo light pick: when several
 // shadowed lights reach the point and mc_light_pick is set, one of them is chosen in
 // proportion to its unshadowed weight and scaled by total / weight. A subsurface material
@@ -568,9 +611,7 @@ Spec<T> shade_local(const nd_uniforms& u, const Scene<P>& s, const Spectral<T>& 
     T pick_total = T(0);
     auto terms = [&](const nd_light& L, V3<T>* ld, V3<T>* col, T* dist, T* ndotl, T* spec) {
         light_sample(L, u.max_dist, pos, ld, col, dist);
-        *ndotl = tmax(dot(n, *ld), T(0));
-        V3<T> hd = normalize(*ld + view);
-        *spec = (parts & ND_PART_SPECULAR) ? T(0) : pow(tmax(dot(n, hd), T(0)), spec_power) * T(m.glossiness);
+        light_terms(*ld, n, view, spec_power, T(m.glossiness), !(parts & ND_PART_SPECULAR), ndotl, spec);
     };
     auto shadow_of = [&](const nd_light& L, V3<T> ld, T dist) {
         double k = L.shadow_softness + (512.0 - L.shadow_softness) * L.hard_shadows;
@@ -822,11 +863,7 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
         const T eta = in_solid ? ior : T(1) / ior;
         T rough = tclamp(T(mat.roughness), T(0), T(1));
         T alpha = rough * rough;
-        V3<T> nr = nf;
-        if (alpha > T(1e-4)) {
-            V3<T> h = cosine_sample(nf, seed_add(sd, 5.19f, 12.73f));
-            nr = normalize(nf + (h - nf) * alpha);
-        }
+        V3<T> nr = perturb_normal(nf, alpha, seed_add(sd, 5.19f, 12.73f));
         T f = fresnel_schlick(dot(d, nr), ior);
         Spec<T> film = splat(T(1));
         if (!in_solid && mat.film_strength > 0.001) film = thin_film_tint(sp, dot(-d, film_normal(nr, hp, mat)), mat);
@@ -859,18 +896,9 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
                     ambient = shade_local(u, s, sp, bp, idir, bn, bm, splat(T(0.15)), false, is, &out.evals);
                 }
             } else if (!reflected) {
-                // calc_ao (template.wgsl:1825-1841)
                 if (!(parts & ND_PART_AMBIENT_OCCLUSION)) {
-                    T radius = tclamp(m.dist * T(0.04), T(0.003), T(0.25));
-                    T occ = T(0), w = T(1);
-                    for (int i = 0; i < 5; ++i) {
-                        T step = radius * (T(0.2) + T(0.2) * T((double)i));
-                        T dd = scene_de(s, hp + n * step);
-                        occ = occ + tmax(step - dd, T(0)) * w;
-                        w = w * T(0.6);
-                    }
                     out.evals += 5;
-                    ambient = ambient * tclamp(T(1) - T(2) * occ / radius, T(0), T(1));
+                    ambient = ambient * calc_ao(s, hp, n, m.dist);
                 }
             }
             Spec<T> lit = shade_local(u, s, sp, hp, d, n, mat, ambient, !reflected, sd, &out.evals);
@@ -907,11 +935,10 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
         T bias = tmax(T(0.003), T(2) * (T(mc.eps_c) * m.dist + T(mc.eps_f)));
         pos = hp + nd * bias;
         d = nd;
-        if (depth >= 4) {
-            T survive = tclamp(smax(throughput), T(0.05), T(1));
-            Seed rs = seed_add(sd, 59.3f, 23.9f);
-            if (T(hash12w(rs.x, rs.y)) > survive) break;
-            throughput = throughput * (T(1) / survive);
+        if (depth >= 4) {  // PATH_RR_START
+            T scale;
+            if (!roulette(smax(throughput), sd, &scale)) break;
+            throughput = throughput * scale;
         }
     }
     T rgb[3];

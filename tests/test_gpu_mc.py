@@ -146,3 +146,29 @@ def test_graph_replays_match_eager_launches(renderer):
                 assert float(x.sum()) == float(1 << 20)
     finally:
         renderer.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays)
+
+
+def test_diffuse_sphere_closed_form(renderer):
+    """The diffuse unit sphere under one point light with Monte Carlo indirect light off: the
+    path trace equals (1 - F) albedo (0.15 AO + L n.l shadow) per pixel, with AO = shadow = 1
+    on the convex sphere, the hit point and normal from the ray-sphere intersection, and
+    Schlick's F at index 1 (tests/test_oracle_mc_pins.py pins the oracle the same way)."""
+    from tests.test_oracle_mc_pins import diffuse_sphere_scene, sphere_closed_form
+    u, cm = diffuse_sphere_scene()
+    hit, depth, ndl, rad, cosi = sphere_closed_form(u)
+    renderer.bind_formulas(cm)
+    try:
+        g, gdepth, _, st = trace(renderer, u, 4)
+    finally:
+        renderer.bind_formulas([0, 1, 2, 3])
+    lit = hit & (ndl > 0.2) & (cosi > 0.3)
+    dark = hit & (ndl == 0.0) & (cosi > 0.3)
+    got = g.astype(np.float64)
+    err = max(np.abs(got[..., k][lit | dark] / rad[lit | dark] - 1).max() for k in range(3))
+    report("diffuse sphere", lit=lit.sum(), dark=dark.sum(), max_rel=err)
+    assert lit.sum() > 100 and dark.sum() > 20
+    eps = float(u.epsilon_coefficient) * depth[hit] + float(u.epsilon_floor)
+    assert np.all(np.abs(gdepth[hit] - depth[hit]) <= 2 * eps / cosi[hit] + 1e-5)
+    for k in range(3):
+        assert np.allclose(got[..., k][lit], rad[lit], rtol=2e-3), k
+        assert np.allclose(got[..., k][dark], rad[dark], rtol=2e-3), k
