@@ -21,13 +21,18 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3         # MI355X FP32 vector, packed (published)
-FP32_PEAK_UNPACKED_TFLOPS = 78.6  # without packed FP32 instructions
+FP32_PEAK_TFLOPS = 157.3         # MI355X FP32 vector (published)
 METRIC = "Mrays/s persistent-batched vs plain @1080p Mandelbox, %FP32 peak; grad rays/s"
 # HBM bytes per k_march launch on the 16-camera batch, from two rocprofv3 --pmc runs
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy; profiles/k_march_c2_traffic.md).
 # Hardware counters cannot be read inside the timed run, so the measured figure is carried here.
 K_MARCH_TRAFFIC_BYTES_16CAM = 1.462e9
+# Hardware counters of the same k_march launch (profiles/k_march_c2_valu_counters.md, two
+# rocprofv3 --pmc runs): active lanes per VALU instruction, the VALU lane-instruction rate as a
+# fraction of the 128 FP32 lanes/clk/CU the FP32 peak implies, and the FP32 FLOP rate the
+# hardware counts (FMA 2, add/mul/transcendental 1; folds and clamps are not FLOPs to it).
+K_MARCH_COUNTERS_16CAM = {"lane_util": 0.85, "valu_lane_rate_frac": 0.60, "hw_fp32_tflops": 43.6,
+                          "source": "profiles/k_march_c2_valu_counters.md"}
 
 
 def log(msg):
@@ -230,11 +235,11 @@ def mc_leg(r, nd, spp=64):
         r.path_trace(u, 1, color=col)
         r.sync()
         t0 = time.perf_counter()
-        for i in range(8):
+        for i in range(min(8, spp)):
             u.mc_sample = float(i)
             r.path_trace(u, 1, color=col)
         r.sync()
-        per_sample[name] = round((time.perf_counter() - t0) / 8 * 1e3, 3)
+        per_sample[name] = round((time.perf_counter() - t0) / min(8, spp) * 1e3, 3)
     r.set_tuning(t.waves_per_cu, t.refill_min_idle, t.chunk_rays)
     u.mc_sample = 0.0
     # the same frame with the primary hit cache on (primary_cache_key, as the reference's
@@ -312,6 +317,8 @@ def main():
                                                                  "jobs, c5 gradient, accel the batch with the "
                                                                  "empty-space accelerator, mesh the mesher's "
                                                                  "512^3 distance pass")
+    ap.add_argument("--c3-spp", type=int, default=64, help="samples per pixel of the path-trace leg's frames "
+                                                          "(64 is the workload; fewer for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run one elementwise copy of MIB MiB (reads MIB, writes MIB): "
                          "the reference kernel for scaling hardware byte counters")
@@ -364,7 +371,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": K_MARCH_TRAFFIC_BYTES_16CAM if len(jobs) == 16 else None,
-                     "frac_unpacked_peak": round(achieved / FP32_PEAK_UNPACKED_TFLOPS, 4),
+                     "counters": K_MARCH_COUNTERS_16CAM if len(jobs) == 16 else None,
                      "kernel": "k_march<true, false, false> (persistent march pass: shared scene, no accelerator, tuned formulas)",
                      "flops_per_de": round(f_march, 2), "de_evals_per_launch": pers["de_march"] // steps,
                      "ms_per_launch": round(pers["ms_march"] / steps, 3),
@@ -387,7 +394,7 @@ def main():
         res["grad_rays_per_s"] = g["grad_rays_per_s"]
         res["c5_gradient"] = g
     if "c3" in legs:
-        res["c3_path_trace"] = mc_leg(r, nd)
+        res["c3_path_trace"] = mc_leg(r, nd, spp=args.c3_spp)
     if "accel" in legs:
         res["c2_accelerator"] = accel_leg(r, nd, torch, abi, jobs)
     if "mesh" in legs:
