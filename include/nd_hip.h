@@ -158,7 +158,17 @@ typedef struct nd_rect {
  *           like the material (template.wgsl:2128-2155); 0 on a miss
  *   hit     1 at hits, 0 at misses (FragOut.normal.w)
  *   shadow  light-0 visibility in [0, 1]; 1 on a miss or when shadows are off
- *   evals   DE evaluations spent on the pixel (march + normal + trap + shadow) */
+ *   evals   DE evaluations spent on the pixel (march + normal + trap + shadow)
+ *   color   3 floats per pixel: fs_simple's colour attachment (FragOut.color.rgb,
+ *           template.wgsl:4492-4538): with PART_GEOMETRY_ONLY set in debug_parts the
+ *           simple_geometry view (a key light from the camera, ambient and depth cue), else
+ *           render_lite (shade_lite's lights, AO and Fresnel at the hit, the sky on a miss);
+ *           both with the reference's tetrahedral normal at the hit
+ *   albedo  3 floats per pixel: fs_aov's albedo (aov_albedo, template.wgsl:4853-4896):
+ *           the hit material's colour mixed toward white by max(reflectiveness,
+ *           transparency), the sky clamped to [0, 1] on a miss
+ * The colour and albedo outputs take one more pass after the geometry passes; the hotspot
+ * and overstep debug views (PART_HOTSPOTS, PART_OVERSTEP) return ND_E_UNSUPPORTED. */
 typedef struct nd_gbuffer_out {
     float* depth;
     float* normal;
@@ -167,6 +177,8 @@ typedef struct nd_gbuffer_out {
     uint8_t* hit;
     float* shadow;
     uint32_t* evals;
+    float* color;
+    float* albedo;
     uint32_t device_ptrs;
     uint32_t _pad;
 } nd_gbuffer_out;

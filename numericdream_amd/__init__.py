@@ -150,13 +150,16 @@ def _ptr(a):
 class GBuffer:
     """Host (numpy) or device (torch) G-buffer arrays for a w x h rect."""
 
-    FIELDS = ("depth", "normal", "position", "trap", "hit", "shadow", "evals")
+    FIELDS = ("depth", "normal", "position", "trap", "hit", "shadow", "evals", "color", "albedo")
 
     def __init__(self, w, h, device=None, fields=("depth", "normal", "position", "trap", "hit", "shadow", "evals")):
+        """fields: the outputs to allocate; color and albedo (fs_simple's colour attachment and
+        fs_aov's albedo, 3 floats per pixel each) are asked for by naming them."""
         self.w, self.h = w, h
         self.device = device
         shapes = {"depth": ((h, w), "f4"), "normal": ((h, w, 3), "f4"), "position": ((h, w, 3), "f4"),
-                  "trap": ((h, w), "f4"), "hit": ((h, w), "u1"), "shadow": ((h, w), "f4"), "evals": ((h, w), "u4")}
+                  "trap": ((h, w), "f4"), "hit": ((h, w), "u1"), "shadow": ((h, w), "f4"), "evals": ((h, w), "u4"),
+                  "color": ((h, w, 3), "f4"), "albedo": ((h, w, 3), "f4")}
         for k in self.FIELDS:
             if k not in fields:
                 setattr(self, k, None)
@@ -171,7 +174,8 @@ class GBuffer:
 
     def out(self):
         return GBufferOut(_ptr(self.depth), _ptr(self.normal), _ptr(self.position), _ptr(self.trap), _ptr(self.hit),
-                          _ptr(self.shadow), _ptr(self.evals), 0 if self.device is None else 1, 0)
+                          _ptr(self.shadow), _ptr(self.evals), _ptr(self.color), _ptr(self.albedo),
+                          0 if self.device is None else 1, 0)
 
     def numpy(self):
         r = {}

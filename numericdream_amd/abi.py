@@ -117,8 +117,8 @@ class Rect(C.Structure):
 
 class GBufferOut(C.Structure):
     _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("position", C.c_void_p), ("trap", C.c_void_p),
-                ("hit", C.c_void_p),
-                ("shadow", C.c_void_p), ("evals", C.c_void_p), ("device_ptrs", U32), ("_pad", U32)]
+                ("hit", C.c_void_p), ("shadow", C.c_void_p), ("evals", C.c_void_p), ("color", C.c_void_p),
+                ("albedo", C.c_void_p), ("device_ptrs", U32), ("_pad", U32)]
 
 
 class Job(C.Structure):

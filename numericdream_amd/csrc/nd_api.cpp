@@ -403,7 +403,7 @@ void nd_ctx_destroy(nd_ctx* c) {
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     c->jobs.release(); c->job_base.release(); c->ctr.release(); c->hits.release(); c->shared.release();
     c->staging.release(); c->latency.release(); c->adj.release(); c->mc.release();
-    c->accel.release(); c->accel_scenes.release(); c->mc_adapt.release();
+    c->accel.release(); c->accel_scenes.release(); c->mc_adapt.release(); c->color_scenes.release();
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -464,6 +464,11 @@ static size_t out_bytes(const nd_gbuffer_out& o, size_t px) {
     if (o.hit) b += px;
     if (o.shadow) b += 4 * px;
     if (o.evals) b += 4 * px;
+    if (o.color) b += 12 * px;
+    if (o.albedo) b += 12 * px;
+    // the colour pass reads the hit and depth of every pixel: internal buffers when not asked for
+    if ((o.color || o.albedo) && !o.depth) b += 4 * px;
+    if ((o.color || o.albedo) && !o.hit) b += px;
     return b;
 }
 
@@ -529,7 +534,7 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     }
     uint64_t total = 0;
     size_t staging = 0;
-    bool any_shadow = false, any_accel = false;
+    bool any_shadow = false, any_accel = false, any_color = false;
     for (uint32_t i = 0; i < n; ++i) {
         if (!jobs[i].uniforms) return nd_fail(c, ND_E_INVALID, "job %u: null uniforms", i);
         nd_status st = nd_pack_job(c, *jobs[i].uniforms, o, jobs[i].rect, hj[i]);
@@ -542,7 +547,18 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         any_accel = any_accel || hj[i].accel_levels > 0;
         const nd_gbuffer_out& go = jobs[i].out;
         size_t px = (size_t)hj[i].rect_w * hj[i].rect_h;
-        if (!go.device_ptrs) staging += out_bytes(go, px) + 7 * 256;
+        const bool color = go.color || go.albedo;
+        any_color = any_color || color;
+        if (color) {
+            const uint32_t parts = (uint32_t)jobs[i].uniforms->debug_parts;
+            if (parts & ((1u << 17) | (1u << 18)))  // PART_HOTSPOTS, PART_OVERSTEP
+                return nd_fail(c, ND_E_UNSUPPORTED, "job %u: the hotspot and overstep views are not implemented", i);
+        }
+        if (!go.device_ptrs) staging += out_bytes(go, px) + 9 * 256;
+        else if (color && (!go.depth || !go.hit)) {
+            if (fx) return nd_fail(c, ND_E_INVALID, "job %u: without allocation the colour outputs need the depth and hit outputs", i);
+            staging += ((!go.depth ? 4 * px : 0) + (!go.hit ? px : 0)) + 2 * 256;
+        }
     }
     if (total >= (1ull << 31)) return nd_fail(c, ND_E_INVALID, "batch of %llu ray slots exceeds 2^31", (unsigned long long)total);
     if (fx) {
@@ -564,9 +580,11 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
             const nd_gbuffer_out& go = jobs[i].out;
             DevJob& j = hj[i];
             size_t px = (size_t)j.rect_w * j.rect_h;
+            const bool color = go.color || go.albedo;
             if (go.device_ptrs) {
                 j.out_depth = go.depth; j.out_normal = go.normal; j.out_pos = go.position; j.out_trap = go.trap;
                 j.out_hit = go.hit; j.out_shadow = go.shadow; j.out_evals = go.evals;
+                j.out_color = go.color; j.out_albedo = go.albedo;
             } else {
                 j.out_depth = go.depth ? (float*)carve(4 * px) : nullptr;
                 j.out_normal = go.normal ? (float*)carve(12 * px) : nullptr;
@@ -575,7 +593,11 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
                 j.out_hit = go.hit ? (uint8_t*)carve(px) : nullptr;
                 j.out_shadow = go.shadow ? (float*)carve(4 * px) : nullptr;
                 j.out_evals = go.evals ? (uint32_t*)carve(4 * px) : nullptr;
+                j.out_color = go.color ? (float*)carve(12 * px) : nullptr;
+                j.out_albedo = go.albedo ? (float*)carve(12 * px) : nullptr;
             }
+            if (color && !j.out_depth) j.out_depth = (float*)carve(4 * px);
+            if (color && !j.out_hit) j.out_hit = (uint8_t*)carve(px);
         }
     }
 
@@ -621,9 +643,31 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         d_shared = (DevScene*)c->shared.p;
         d_latency = (char*)c->latency.p;
     }
+    // per-job material scenes of the colour pass
+    char* d_color_scenes = nullptr;
+    std::vector<unsigned char> color_scenes;
+    alignas(16) unsigned char fx_color_scene[ND_MC_SCENE_MAX_BYTES];
+    const size_t mcs = nd_mc_scene_bytes();
+    if (any_color) {
+        unsigned char* hs;
+        if (fx) {
+            if (!(d_color_scenes = fcarve(ND_MC_SCENE_MAX_BYTES))) return nd_fail(c, ND_E_INVALID, "fixed workspace too small for the colour pass");
+            hs = fx_color_scene;
+        } else {
+            if ((st = c->color_scenes.ensure(c, mcs * n)) != ND_OK) return st;
+            d_color_scenes = (char*)c->color_scenes.p;
+            color_scenes.resize(mcs * n);
+            hs = color_scenes.data();
+        }
+        for (uint32_t i = 0; i < n; ++i) nd_mc_scene_pack(*jobs[i].uniforms, hs + mcs * i);
+    }
     hipError_t e;
     if ((e = hipMemcpyAsync(d_jobs, hj, sizeof(DevJob) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return nd_hip_fail(c, e, "upload jobs");
+    if (any_color &&
+        (e = hipMemcpyAsync(d_color_scenes, fx ? fx_color_scene : color_scenes.data(), mcs * n, hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "upload material scenes");
     if ((e = hipMemcpyAsync(d_job_base, base, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return nd_hip_fail(c, e, "upload job table");
     if (shared && (e = hipMemcpyAsync(d_shared, &hj[0].scene, sizeof(DevScene), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
@@ -719,9 +763,16 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         if (e != hipSuccess) return drained(nd_hip_fail(c, e, "launch persistent passes"));
         s.kernels = 3 + (any_shadow ? 1 : 0) + 1;
     }
+    if (any_color) {
+        // fs_simple colour and fs_aov albedo from the hit and depth just written
+        if ((e = nd_launch_gbuffer_color(d_jobs, d_color_scenes, d_job_base, (int)n, (uint32_t)total, ext, c->stream)) != hipSuccess)
+            return drained(nd_hip_fail(c, e, "launch colour pass"));
+        (void)hipEventRecord(c->ev[4], c->stream);
+        s.kernels += 1;
+    }
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return nd_hip_fail(c, e, "batch execution");
-    (void)hipEventElapsedTime(&ms_total, c->ev[0], c->ev[3]);
+    (void)hipEventElapsedTime(&ms_total, c->ev[0], c->ev[any_color ? 4 : 3]);
     DevCounters dc;
     if ((e = hipMemcpy(&dc, d_ctr, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
         return nd_hip_fail(c, e, "read counters");
@@ -764,10 +815,11 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         if (go.device_ptrs) continue;
         const DevJob& j = hj[i];
         size_t px = (size_t)j.rect_w * j.rect_h;
-        struct { void* dst; const void* src; size_t b; } cp[7] = {
+        struct { void* dst; const void* src; size_t b; } cp[9] = {
             {go.depth, j.out_depth, 4 * px}, {go.normal, j.out_normal, 12 * px}, {go.position, j.out_pos, 12 * px},
             {go.trap, j.out_trap, 4 * px},
-            {go.hit, j.out_hit, px}, {go.shadow, j.out_shadow, 4 * px}, {go.evals, j.out_evals, 4 * px}};
+            {go.hit, j.out_hit, px}, {go.shadow, j.out_shadow, 4 * px}, {go.evals, j.out_evals, 4 * px},
+            {go.color, j.out_color, 12 * px}, {go.albedo, j.out_albedo, 12 * px}};
         for (auto& k : cp) {
             if (!k.dst) continue;
             if ((e = hipMemcpy(k.dst, k.src, k.b, hipMemcpyDeviceToHost)) != hipSuccess) return nd_hip_fail(c, e, "read outputs");

@@ -42,6 +42,7 @@ struct nd_ctx {
     hipGraphExec_t mc_graph_exec = nullptr;         // the path trace's per-chunk passes
     alignas(8) unsigned char mc_graph_key[1024] = {};  // what that graph baked in
     DevBuf jobs, job_base, ctr, hits, shared, staging, latency, adj, mc;
+    DevBuf color_scenes;             // per-job material scenes of the colour pass
     DevBuf mc_adapt;                 // adaptive sampling state per pixel of the last path-trace rect
     uint32_t mc_adapt_key[6] = {};   // that rect and target size (the state carries over to the next call)
     DevBuf accel, accel_scenes;              // accelerator grids of the last batch, their scenes

@@ -113,6 +113,8 @@ struct DevJob {
     uint8_t* out_hit;
     float* out_shadow;
     uint32_t* out_evals;
+    float* out_color;      // fs_simple colour (3 floats per pixel), or null
+    float* out_albedo;     // fs_aov albedo (3 floats per pixel), or null
 };
 
 // Device counters of one batch (zeroed by a kernel before every batch).

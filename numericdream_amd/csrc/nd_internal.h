@@ -31,6 +31,13 @@ struct NdPassLaunch {
 hipError_t nd_launch_plain(const DevJob* d_job, int tiles_x, int tiles_y, DevCounters* ctr, bool ext, hipStream_t s);
 hipError_t nd_launch_zero_counters(DevCounters* c, hipStream_t s);
 hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s);
+// fs_simple colour and fs_aov albedo of every pixel of a batch after its geometry passes
+// (scenes: one packed material scene per job, nd_mc_scene_bytes() each).
+constexpr size_t ND_MC_SCENE_MAX_BYTES = 8192;  // upper bound of nd_mc_scene_bytes()
+size_t nd_mc_scene_bytes();
+void nd_mc_scene_pack(const nd_uniforms& u, void* out);
+hipError_t nd_launch_gbuffer_color(const DevJob* jobs, const void* scenes, const uint32_t* job_base, int n_jobs,
+                                   uint32_t total, bool ext, hipStream_t s);
 
 // Resident wave64 workgroups per CU of each persistent kernel (blockDim 64, no dynamic LDS).
 // "_MIXED" variants take the scene from each job, the others share one scene; "_EXT"

@@ -71,6 +71,7 @@ struct McScene {
     float sky_yaw_c, sky_yaw_s, sky_inv_falloff, sky_sun_cos, sky_sun_intensity, sky_intensity;
     int sky_sun;
     float max_dist;
+    int simple_view;   // PART_GEOMETRY_ONLY set: fs_simple draws simple_geometry, else render_lite
 };
 
 // ----------------------------------------------------------------------------- random numbers
@@ -1571,6 +1572,7 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     S.dispersion_on = !(parts & ND_PART_DISPERSION);
     S.iridescence_on = !(parts & ND_PART_IRIDESCENCE);
     S.subsurface_on = !(parts & ND_PART_SUBSURFACE);
+    S.simple_view = (parts & ND_PART_GEOMETRY_ONLY) != 0;
     S.sss = 0;
     if (S.subsurface_on)
         for (int i = 0; i < ND_MAX_INSTANCES; ++i)
@@ -1607,6 +1609,111 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     S.sky_sun_intensity = u.sky.sun_intensity;
     S.sky_intensity = u.sky.intensity;
     S.max_dist = u.max_dist;
+}
+
+namespace nd {
+// ----------------------------------------------------------------------------- G-buffer colour
+// undo_tonemap (template.wgsl:4139-4142): the inverse of c / (1 + c), clamped
+NDD V3<float> undo_tonemap(V3<float> c) {
+    const float k = 1.f / 255.f;  // 1 / UNORM8_MAX
+    c = mk3(fclamp(c.x, 0.f, 1.f), fclamp(c.y, 0.f, 1.f), fclamp(c.z, 0.f, 1.f));
+    return mk3(c.x / fmaxf(1.f - c.x, k), c.y / fmaxf(1.f - c.y, k), c.z / fmaxf(1.f - c.z, k));
+}
+// fs_simple's colour attachment and fs_aov's albedo for one pixel of every job, from the
+// geometry passes' hit and depth (template.wgsl:4417-4538, 4853-4896). One thread per ray
+// slot of the batch (64 per 8x8 tile of each job's rect).
+template <bool EXT>
+__global__ void __launch_bounds__(256) k_gbuffer_color(const DevJob* __restrict__ jobs, const McScene* __restrict__ scenes,
+                                                      const uint32_t* __restrict__ job_base, int n_jobs, uint32_t total) {
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot >= total) return;
+    const int job = find_job(job_base, n_jobs, slot);
+    const DevJob& J = jobs[job];
+    int x, y;
+    if (!slot_pixel(J, slot - J.ray_base, &x, &y)) return;
+    const McScene& S = scenes[job];
+    const size_t o = (size_t)y * J.rect_w + x;
+    const V3<float> dir = primary_dir(J, J.rect_x + x, J.rect_y + y);
+    const V3<float> bg = mk3(0.10f, 0.11f, 0.13f);  // SIMPLE_BACKGROUND
+    V3<float> color, albedo;
+    if (!J.out_hit[o]) {
+        const V3<float> sky = sky_color(S, dir);
+        color = S.simple_view ? undo_tonemap(bg) : sky;
+        albedo = mk3(fclamp(sky.x, 0.f, 1.f), fclamp(sky.y, 0.f, 1.f), fclamp(sky.z, 0.f, 1.f));
+    } else {
+        const DevScene& sc = J.scene;
+        const float t = J.out_depth[o];
+        const V3<float> hp = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
+        const V3<float> n = normal_tetra<EXT>(sc, hp, J.eps_c * t + J.eps_f);  // estimate_normal
+        const Mat m = part_mask(hit_material<EXT>(sc, S, hp), S);
+        if (S.simple_view) {
+            // simple_geometry: a key light from the camera (SIMPLE_KEY_UP 0.45, _LEFT 0.35),
+            // ambient 0.18, albedo 0.72/0.72/0.73, depth cue 0.55 toward the background
+            const V3<float> kd = vnorm3(mk3(-dir.x + J.up[0] * 0.45f - J.right[0] * 0.35f,
+                                            -dir.y + J.up[1] * 0.45f - J.right[1] * 0.35f,
+                                            -dir.z + J.up[2] * 0.45f - J.right[2] * 0.35f));
+            const float shade = 0.18f + (1.f - 0.18f) * fmaxf(dot3(n, kd), 0.f);
+            const float cue = fclamp(t / fmaxf(J.max_dist, 1e-4f), 0.f, 1.f) * 0.55f;
+            color = undo_tonemap(mk3(0.72f * shade * (1.f - cue) + bg.x * cue, 0.72f * shade * (1.f - cue) + bg.y * cue,
+                                     0.73f * shade * (1.f - cue) + bg.z * cue));
+        } else {
+            // shade_lite (template.wgsl:4424-4444): ambient 0.15 x AO, every light without a
+            // shadow, Blinn-Phong with the half vector of l and -d, all times 1 - Fresnel
+            float ao = 1.f;
+            if (S.ao_on) {
+                const float radius = fclamp(t * 0.04f, 0.003f, 0.25f);
+                float occ = 0.f, w = 1.f;
+                for (int k = 0; k < 5; ++k) {
+                    const float step = radius * (0.2f + 0.2f * (float)k);
+                    const float dd = scene_de<false, EXT>(sc, mk3(hp.x + n.x * step, hp.y + n.y * step, hp.z + n.z * step), nullptr);
+                    occ += fmaxf(step - dd, 0.f) * w;
+                    w *= 0.6f;
+                }
+                ao = fclamp(1.f - 2.f * occ / radius, 0.f, 1.f);
+            }
+            const float ior = fmaxf(m.ior, 1.f);
+            const float r0r = (1.f - ior) / (1.f + ior), r0 = r0r * r0r;
+            const float cth = fclamp(1.f - fabsf(dot3(dir, n)), 0.f, 1.f), c2 = cth * cth;
+            const float fres = r0 + (1.f - r0) * c2 * c2 * cth;
+            const float power = 4.f + 124.f * fclamp(m.glossiness, 0.f, 1.f);
+            V3<float> dif = mk3(0.15f * ao, 0.15f * ao, 0.15f * ao), spc = mk3(0.f, 0.f, 0.f);
+            for (int i = 0; i < S.light_count; ++i) {
+                V3<float> ld, lc;
+                float ldist;
+                light_of(S.lights[i], S.max_dist, hp, &ld, &lc, &ldist);
+                const float ndotl = fmaxf(dot3(n, ld), 0.f);
+                if (ndotl <= 0.f) continue;
+                dif = mk3(dif.x + lc.x * ndotl, dif.y + lc.y * ndotl, dif.z + lc.z * ndotl);
+                if (S.specular_on) {
+                    const V3<float> h = vnorm3(mk3(ld.x - dir.x, ld.y - dir.y, ld.z - dir.z));
+                    const float sp = powf(fmaxf(dot3(n, h), 0.f), power) * m.glossiness;
+                    spc = mk3(spc.x + lc.x * sp, spc.y + lc.y * sp, spc.z + lc.z * sp);
+                }
+            }
+            const float f1 = 1.f - fres;
+            color = mk3(f1 * (m.color[0] * dif.x + spc.x), f1 * (m.color[1] * dif.y + spc.y), f1 * (m.color[2] * dif.z + spc.z));
+        }
+        // aov_albedo: the colour mixed toward white by max(reflectiveness, transparency)
+        const float sp = fclamp(fmaxf(m.reflectiveness, m.transparency), 0.f, 1.f);
+        albedo = mk3(fclamp(m.color[0] + (1.f - m.color[0]) * sp, 0.f, 1.f), fclamp(m.color[1] + (1.f - m.color[1]) * sp, 0.f, 1.f),
+                     fclamp(m.color[2] + (1.f - m.color[2]) * sp, 0.f, 1.f));
+    }
+    if (J.out_color) { J.out_color[3 * o] = color.x; J.out_color[3 * o + 1] = color.y; J.out_color[3 * o + 2] = color.z; }
+    if (J.out_albedo) { J.out_albedo[3 * o] = albedo.x; J.out_albedo[3 * o + 1] = albedo.y; J.out_albedo[3 * o + 2] = albedo.z; }
+}
+}  // namespace nd
+
+static_assert(sizeof(nd::McScene) <= ND_MC_SCENE_MAX_BYTES, "ND_MC_SCENE_MAX_BYTES");
+size_t nd_mc_scene_bytes() { return sizeof(nd::McScene); }
+void nd_mc_scene_pack(const nd_uniforms& u, void* out) { pack_mc_scene(u, *(nd::McScene*)out); }
+hipError_t nd_launch_gbuffer_color(const DevJob* jobs, const void* scenes, const uint32_t* job_base, int n_jobs,
+                                   uint32_t total, bool ext, hipStream_t s) {
+    const dim3 grid((total + 255) / 256), block(256);
+    if (ext)
+        hipLaunchKernelGGL(nd::k_gbuffer_color<true>, grid, block, 0, s, jobs, (const nd::McScene*)scenes, job_base, n_jobs, total);
+    else
+        hipLaunchKernelGGL(nd::k_gbuffer_color<false>, grid, block, 0, s, jobs, (const nd::McScene*)scenes, job_base, n_jobs, total);
+    return hipGetLastError();
 }
 
 template <typename K>

@@ -23,7 +23,8 @@ constexpr size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 constexpr size_t kHitSlotBytes = 4 + 2 + 4 + 4 + 2 + 7 * 4;
 constexpr size_t kHitSlots = (size_t)ND_NOALLOC_MAX_RAYS + (size_t)ND_NOALLOC_MAX_GRID * 64;
 constexpr size_t kScratchBytes = round256(sizeof(DevJob)) + round256(sizeof(uint32_t)) + round256(sizeof(DevCounters)) +
-                                 round256(sizeof(DevScene)) + kHitSlots * kHitSlotBytes + 16 * 256;
+                                 round256(sizeof(DevScene)) + round256(ND_MC_SCENE_MAX_BYTES) +
+                                 kHitSlots * kHitSlotBytes + 16 * 256;
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;

@@ -189,6 +189,21 @@ def render_gbuffer(u, rect=None, prec=0, normal_mode=0, shadows=True, shadow_max
     return out
 
 
+def gbuffer_color(u, rect=None, prec=0, threads=None, code_map=None):
+    """fs_simple's colour attachment and fs_aov's albedo of a rect: (colour, albedo), each
+    (h, w, 3) float32."""
+    L = lib()
+    vp, i32 = C.c_void_p, C.c_int
+    L.ndo_gbuffer_color.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, vp, i32]
+    W, H = int(u.resolution[0]), int(u.resolution[1])
+    x, y, w, h = rect if rect else (0, 0, W, H)
+    col = np.zeros((h, w, 3), np.float32)
+    alb = np.zeros((h, w, 3), np.float32)
+    cm, cmp_ = _cm(code_map)
+    L.ndo_gbuffer_color(_u(u), cmp_, prec, x, y, w, h, _p(col), _p(alb), threads or min(16, os.cpu_count() or 1))
+    return col, alb
+
+
 def flops_per_de(u, pts, code_map=None):
     p = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
     tops = np.zeros(1)

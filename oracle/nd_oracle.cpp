@@ -234,6 +234,42 @@ int ndo_render_gbuffer(const nd_uniforms* u, const int32_t* cm, int prec, int no
                                    cam_offset);
 }
 
+// fs_simple colour and fs_aov albedo of a rect (3 floats per pixel each; either may be null).
+int ndo_gbuffer_color(const nd_uniforms* u, const int32_t* cm, int prec, int rx, int ry, int rw, int rh, float* color,
+                      float* albedo, int nthreads) {
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        Scene<T> s = decode_scene<T>(*u, code_map(cm));
+        attach_accel(*u, s);
+        MarchCfg mc = march_cfg(*u);
+        std::atomic<int> next_row{0};
+        auto worker = [&]() {
+            for (;;) {
+                int y = next_row.fetch_add(1);
+                if (y >= rh) break;
+                for (int x = 0; x < rw; ++x) {
+                    V3<T> dir = primary_dir<T>(*u, rx + x, ry + y, (int)u->resolution[0], (int)u->resolution[1]);
+                    V3<T> org = V3<T>::from(u->camera_pos);
+                    MarchResult<T> m = march_to(s, mc, org, dir, T(u->max_dist));
+                    T rgb[3], alb[3];
+                    gbuffer_color(*u, s, mc, org, dir, m.hit, m.dist, rgb, alb);
+                    size_t i = (size_t)y * rw + x;
+                    for (int k = 0; k < 3; ++k) {
+                        if (color) color[3 * i + k] = (float)val(rgb[k]);
+                        if (albedo) albedo[3 * i + k] = (float)val(alb[k]);
+                    }
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int i = 1; i < std::max(1, nthreads); ++i) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
+    };
+    if (prec == 0) run(0.0); else run(0.0f);
+    return 0;
+}
+
 // Mean ND-FLOP v1 per scene_de over the given points (uniform-only subexpressions
 // hoisted); *tops_out receives the mean transcendental count.
 double ndo_flops_per_de(const nd_uniforms* u, const int32_t* cm, const double* p, int n, double* tops_out) {
@@ -543,6 +579,7 @@ int ndo_helper(int op, const double* a, double* out) {
                    out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
         case 38: { double sc = 0; out[0] = roulette(a[0], Seed{(float)a[1], (float)a[2]}, &sc) ? 1.0 : 0.0; out[1] = sc;
                    return 0; }
+        case 39: { D3 r = undo_tonemap(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
         case 35: { Subsurface<double> ss{};  // subsurface_transmit: radius, depth, 8 reaches
                    ss.amount = 1.0; ss.radius = a[0];
                    for (int k = 0; k < 8; ++k) ss.reach.v[k] = a[2 + k];

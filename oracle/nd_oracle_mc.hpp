@@ -682,6 +682,68 @@ template <class T> T fresnel_schlick(T cos_theta, T ior) {
     return r0 + (T(1) - r0) * c2 * c2 * c;
 }
 
+// undo_tonemap (template.wgsl:4139-4142): c / max(1 - c, 1 / 255) of c clamped to [0, 1]
+template <class T> V3<T> undo_tonemap(V3<T> c) {
+    const T k = T(1.0 / 255.0);
+    c = V3<T>(tclamp(c.x, T(0), T(1)), tclamp(c.y, T(0), T(1)), tclamp(c.z, T(0), T(1)));
+    return V3<T>(c.x / tmax(T(1) - c.x, k), c.y / tmax(T(1) - c.y, k), c.z / tmax(T(1) - c.z, k));
+}
+
+// fs_simple's colour attachment (template.wgsl:4492-4538) and fs_aov's albedo (:4853-4896)
+// for a primary ray that the march ended at (hit, t). With PART_GEOMETRY_ONLY set,
+// simple_geometry (:4492-4512): a key light normalize(-d + 0.45 up - 0.35 right) with
+// ambient 0.18 on albedo (0.72, 0.72, 0.73), mixed toward the background (0.10, 0.11, 0.13)
+// by 0.55 t / max_dist, through undo_tonemap; otherwise render_lite with shade_lite
+// (:4424-4444): (1 - F) (colour (0.15 AO + sum L n.l) + sum L (n.h)^p glossiness), no shadows.
+// The normal is the tetrahedral estimate_normal at e = eps(t). aov_albedo: the colour mixed
+// toward white by max(reflectiveness, transparency); a miss shows the sky, clamped for the albedo.
+template <class T, class P>
+void gbuffer_color(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, bool hit, T t,
+                   T* rgb, T* alb) {
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    const bool simple = (parts & ND_PART_GEOMETRY_ONLY) != 0u;
+    const V3<T> bg(T(0.10), T(0.11), T(0.13));
+    V3<T> color, albedo;
+    if (!hit) {
+        const V3<T> sky = sky_color(u, dir);
+        color = simple ? undo_tonemap(bg) : sky;
+        albedo = V3<T>(tclamp(sky.x, T(0), T(1)), tclamp(sky.y, T(0), T(1)), tclamp(sky.z, T(0), T(1)));
+    } else {
+        const V3<T> hp = org + dir * t;
+        const V3<T> n = normal_tetra(s, hp, T(mc.eps_c) * t + T(mc.eps_f));
+        const Mat m = apply_part_mask(hit_material(u, s, hp), parts);
+        if (simple) {
+            const V3<T> key = normalize(-dir + V3<T>::from(u.camera_up) * T(0.45) - V3<T>::from(u.camera_right) * T(0.35));
+            const T shade = T(0.18) + (T(1) - T(0.18)) * tmax(dot(n, key), T(0));
+            const T cue = tclamp(t / tmax(T(u.max_dist), T(1e-4)), T(0), T(1)) * T(0.55);
+            const V3<T> a = V3<T>(T(0.72), T(0.72), T(0.73)) * shade;
+            color = undo_tonemap(a + (bg - a) * cue);
+        } else {
+            const T ao = (parts & ND_PART_AMBIENT_OCCLUSION) ? T(1) : calc_ao(s, hp, n, t);
+            const T fres = fresnel_schlick(dot(dir, n), tmax(T(m.ior), T(1)));
+            const T power = T(4.0) + (T(128.0) - T(4.0)) * tclamp(T(m.glossiness), T(0), T(1));
+            V3<T> dif(T(0.15) * ao, T(0.15) * ao, T(0.15) * ao), spc;
+            const int lc = (parts & ND_PART_DIRECT_LIGHTS) ? 0 : std::min(i32(u.light_count), ND_MAX_LIGHTS);
+            for (int i = 0; i < lc; ++i) {
+                V3<T> ld, lcol; T dist;
+                light_sample(u.lights[i], u.max_dist, hp, &ld, &lcol, &dist);
+                const T ndotl = tmax(dot(n, ld), T(0));
+                if (ndotl <= T(0)) continue;
+                dif = dif + lcol * ndotl;
+                if (!(parts & ND_PART_SPECULAR)) spc = spc + lcol * (pow(tmax(dot(n, normalize(ld - dir)), T(0)), power) * T(m.glossiness));
+            }
+            const V3<T> mc3(T(m.color[0]), T(m.color[1]), T(m.color[2]));
+            color = (cmul(mc3, dif) + spc) * (T(1) - fres);
+        }
+        const T sp = tclamp(tmax(T(m.reflectiveness), T(m.transparency)), T(0), T(1));
+        albedo = V3<T>(tclamp(T(m.color[0]) + (T(1) - T(m.color[0])) * sp, T(0), T(1)),
+                       tclamp(T(m.color[1]) + (T(1) - T(m.color[1])) * sp, T(0), T(1)),
+                       tclamp(T(m.color[2]) + (T(1) - T(m.color[2])) * sp, T(0), T(1)));
+    }
+    rgb[0] = color.x; rgb[1] = color.y; rgb[2] = color.z;
+    alb[0] = albedo.x; alb[1] = albedo.y; alb[2] = albedo.z;
+}
+
 // cauchy_ior (template.wgsl:3403-3409): n(lambda) = n_d + B (1 / lambda^2 - 1 / lambda_d^2),
 // B = (n_d - 1) / (V (1 / lambda_F^2 - 1 / lambda_C^2)), lambda in micrometres
 template <class T> T cauchy_ior(T ior_d, T abbe, T lambda) {
