@@ -198,11 +198,13 @@ typedef struct nd_batch_stats {
     uint64_t de_evals_surface;
     uint64_t de_evals_shadow;
     float ms_total;         /* device time of the whole call (hipEvents) */
-    float ms_march;
-    float ms_surface;
-    float ms_shadow;
+    float ms_march;         /* with graph = 1: the march, surface and shadow passes together */
+    float ms_surface;       /* with graph = 1: 0 (the replay is timed as a whole) */
+    float ms_shadow;        /* with graph = 1: 0 */
     uint32_t kernels;       /* kernel launches issued */
-    uint32_t _pad;
+    uint32_t graph;         /* 1: the persistent passes ran as a replayed hipGraph (a launch record
+                               seen the batch before is captured; the first of a shape runs pass
+                               by pass) */
     uint64_t march_wave_rounds;  /* persistent march: wave rounds, each offering one DE slot to each
                                     of 64 lanes; de_evals_march / (64 x this) is the lane occupancy */
 } nd_batch_stats;
@@ -256,8 +258,11 @@ nd_status nd_batch_job_latency(nd_ctx* ctx, float* ms_out, uint32_t n_jobs);
 
 /* ---- autotuner ----------------------------------------------------------------- */
 #define ND_TUNE_TRACK_LATENCY 1u  /* record per-job completion times (nd_batch_job_latency) */
-#define ND_TUNE_NO_GRAPH 2u       /* nd_path_trace: launch each chunk's passes one by one instead of
-                                     replaying the captured hipGraph */
+#define ND_TUNE_NO_GRAPH 2u       /* nd_path_trace: launch each pass of a chunk one by one instead of
+                                     replaying the captured hipGraph of the chunk's passes */
+#define ND_TUNE_GBUFFER_GRAPH 4u  /* persistent G-buffer passes: capture them into a hipGraph when a batch
+                                     repeats the last launch record (same grids, claims, flags and buffers)
+                                     and replay it while it stays the same (off by default) */
 
 typedef struct nd_tuning {
     uint32_t waves_per_cu;     /* persistent grid = CUs x waves_per_cu wave64 workgroups for every pass;
@@ -270,11 +275,18 @@ typedef struct nd_tuning {
     uint32_t round_cap;        /* 0 = none; otherwise a persistent geometry pass stops each wave after this
                                   many rounds and the call returns ND_E_TIMEOUT (every kernel still exits
                                   normally; outputs of unfinished rays are undefined) */
+    uint32_t tile_width;       /* pixels across the tile a wave64 covers: 4, 8, 16 or 32 (tile height
+                                  64 / width); 0 = 8 (8x8). Results do not depend on it */
+    uint32_t refill_period;    /* the step budget of the persistent march and shadow passes: a wave takes
+                                  this many march steps between refills of its idle lanes (a wave whose
+                                  lanes are all idle refills at once); 0 = 1 */
 } nd_tuning;
 
 nd_status nd_get_tuning(nd_ctx* ctx, nd_tuning* out);
 nd_status nd_set_tuning(nd_ctx* ctx, const nd_tuning* t);
-/* Times candidate tunings on the given batch and keeps the fastest. */
+/* Times candidate tunings on the given batch and keeps the fastest: coordinate descent over
+ * waves per CU, refill threshold, claim size, tile width and refill period, from the current
+ * tuning, two sweeps, the best of two runs per candidate. Outputs do not depend on the tuning. */
 nd_status nd_autotune(nd_ctx* ctx, const nd_job* jobs, uint32_t n_jobs,
                       const nd_march_opts* opts, nd_tuning* best);
 

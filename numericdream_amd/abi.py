@@ -129,12 +129,12 @@ class BatchStats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("hits", C.c_uint64), ("de_evals", C.c_uint64),
                 ("de_evals_march", C.c_uint64), ("de_evals_surface", C.c_uint64),
                 ("de_evals_shadow", C.c_uint64), ("ms_total", F), ("ms_march", F), ("ms_surface", F),
-                ("ms_shadow", F), ("kernels", U32), ("_pad", U32), ("march_wave_rounds", C.c_uint64)]
+                ("ms_shadow", F), ("kernels", U32), ("graph", U32), ("march_wave_rounds", C.c_uint64)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("waves_per_cu", U32), ("refill_min_idle", U32), ("chunk_rays", U32), ("flags", U32),
-                ("round_cap", U32)]
+                ("round_cap", U32), ("tile_width", U32), ("refill_period", U32)]
 
 
 NOALLOC_MAX_RAYS = 1 << 19
@@ -142,6 +142,7 @@ NOALLOC_MAX_GRID = 4096
 
 TUNE_TRACK_LATENCY = 1
 TUNE_NO_GRAPH = 2
+TUNE_GBUFFER_GRAPH = 4
 
 
 class AdjointIO(C.Structure):

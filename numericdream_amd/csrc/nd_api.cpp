@@ -39,6 +39,8 @@ nd_status nd_hip_fail(nd_ctx* c, hipError_t e, const char* what) {
 
 nd_status DevBuf::ensure(nd_ctx* c, size_t bytes) {
     if (bytes <= cap) return ND_OK;
+    // captured graphs hold the old pointers: no graph outlives a buffer it names
+    nd_drop_graphs(c);
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -49,8 +51,14 @@ nd_status DevBuf::ensure(nd_ctx* c, size_t bytes) {
         return nd_hip_fail(c, e, "hipMalloc");
     }
     cap = want;
-    c->graph_valid = false;  // captured graphs hold the old pointers
     return ND_OK;
+}
+
+void nd_drop_graphs(nd_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->mc_graph_exec) (void)hipGraphExecDestroy(c->mc_graph_exec);
+    c->graph_exec = nullptr;
+    c->mc_graph_exec = nullptr;
 }
 void DevBuf::release() {
     if (p) (void)hipFree(p);
@@ -294,8 +302,12 @@ nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, n
     if ((uint64_t)rect.x + rect.w > (uint64_t)j.target_w || (uint64_t)rect.y + rect.h > (uint64_t)j.target_h)
         return nd_fail(c, ND_E_INVALID, "rect %ux%u+%u+%u outside the %dx%d target", rect.w, rect.h, rect.x, rect.y, j.target_w, j.target_h);
     j.rect_x = (int)rect.x; j.rect_y = (int)rect.y; j.rect_w = (int)rect.w; j.rect_h = (int)rect.h;
-    j.tiles_x = (j.rect_w + 7) / 8;
-    j.tiles_y = (j.rect_h + 7) / 8;
+    // tiles of 2^tile_lg x 64 / 2^tile_lg pixels, one wave64 each (8 x 8 unless tuned)
+    const uint32_t tw = c->tuning.tile_width ? c->tuning.tile_width : 8u;
+    j.tile_lg = tw == 4 ? 2 : tw == 16 ? 4 : tw == 32 ? 5 : 3;
+    const int th = 64 >> j.tile_lg;
+    j.tiles_x = (j.rect_w + (1 << j.tile_lg) - 1) >> j.tile_lg;
+    j.tiles_y = (j.rect_h + th - 1) / th;
     j.ray_count = (uint32_t)j.tiles_x * (uint32_t)j.tiles_y * 64u;
     for (int k = 0; k < 3; ++k) {
         j.cam[k] = u.camera_pos[k];
@@ -398,8 +410,7 @@ void nd_ctx_destroy(nd_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->mc_graph_exec) (void)hipGraphExecDestroy(c->mc_graph_exec);
+    nd_drop_graphs(c);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     c->jobs.release(); c->job_base.release(); c->ctr.release(); c->hits.release(); c->shared.release();
     c->staging.release(); c->latency.release(); c->adj.release(); c->mc.release();
@@ -446,7 +457,10 @@ nd_status nd_set_tuning(nd_ctx* c, const nd_tuning* t) {
     if (t->refill_min_idle < 1 || t->refill_min_idle > 64) return nd_fail(c, ND_E_INVALID, "refill_min_idle must be 1..64");
     if (t->chunk_rays > (1u << 14)) return nd_fail(c, ND_E_INVALID, "chunk_rays must be 0..2^14");
     if (t->waves_per_cu > 64) return nd_fail(c, ND_E_INVALID, "waves_per_cu must be 0..64");
-    if (t->flags & ~(ND_TUNE_TRACK_LATENCY | ND_TUNE_NO_GRAPH)) return nd_fail(c, ND_E_INVALID, "unknown tuning flags 0x%x", t->flags);
+    if (t->flags & ~(ND_TUNE_TRACK_LATENCY | ND_TUNE_NO_GRAPH | ND_TUNE_GBUFFER_GRAPH)) return nd_fail(c, ND_E_INVALID, "unknown tuning flags 0x%x", t->flags);
+    if (t->tile_width != 0 && t->tile_width != 4 && t->tile_width != 8 && t->tile_width != 16 && t->tile_width != 32)
+        return nd_fail(c, ND_E_INVALID, "tile_width must be 0, 4, 8, 16 or 32");
+    if (t->refill_period > 16) return nd_fail(c, ND_E_INVALID, "refill_period must be 0..16");
     c->tuning = *t;
     c->round_cap = t->round_cap ? t->round_cap : 0xffffffffu;
     return ND_OK;
@@ -515,6 +529,9 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     if (o.normal_mode > ND_NORMAL_CENTRAL) return nd_fail(c, ND_E_INVALID, "normal_mode %u", o.normal_mode);
     if (o.kernel > ND_KERNEL_PERSISTENT) return nd_fail(c, ND_E_INVALID, "kernel %u", o.kernel);
     if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
+    // launches below are checked with hipGetLastError: start from a clean slate, so an error
+    // left by an earlier call that was not checked is not reported as this batch's
+    (void)hipGetLastError();
 
     // host records: the context's vectors, or (fixed workspace, one job) the stack
     DevJob fx_job;
@@ -724,7 +741,8 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
             hp = (char*)c->hits.p;
         }
         auto carve = [&](size_t bytes) { char* r = hp; hp += (bytes + 255) & ~(size_t)255; return r; };
-        NdPassLaunch L{};
+        NdPassLaunch L;
+        std::memset(&L, 0, sizeof(L));  // compared bytewise as the graph key: no stray padding
         L.hb.ray = (uint32_t*)carve(4 * hit_cap);
         L.hb.job = (uint16_t*)carve(2 * hit_cap);
         L.hb.t = (float*)carve(4 * hit_cap);
@@ -752,15 +770,68 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
         L.chunk = chunk;
         L.min_idle = c->tuning.refill_min_idle;
         L.round_cap = c->round_cap;
+        L.refill_period = c->tuning.refill_period ? c->tuning.refill_period : 1u;
         L.any_shadow = any_shadow;
         L.surface = !ex.skip_surface;
         L.accel = acc;
         L.ext = ext;
         L.job_done_cycles = track ? (unsigned long long*)d_latency : nullptr;
         L.ray_done = track ? (uint32_t*)(d_latency + lat_jobs) : nullptr;
-        for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];
-        e = nd_launch_persistent(L, c->stream);
-        if (e != hipSuccess) return drained(nd_hip_fail(c, e, "launch persistent passes"));
+        // The passes of a batch shape are captured into a hipGraph when the same launch record -
+        // grids, claims, flags and every device pointer - comes a second time, and replayed
+        // while it stays the same. The first batch of a shape runs pass by pass, which also
+        // loads the kernels' code before any capture. The job records are uploaded before each
+        // replay. No memset node: the counters are cleared by a kernel. The timing events stay
+        // out of the graph (an event recorded only by a graph node has no time to read back):
+        // a replay is timed as a whole, between events recorded around the graph launch.
+        const bool graphs = !fx && !c->graph_broken && (c->tuning.flags & ND_TUNE_GBUFFER_GRAPH);
+        bool replayed = false;
+        if (graphs) {
+            const bool same = c->graph_exec && std::memcmp(&L, &c->graph_key, sizeof(L)) == 0;
+            const bool repeat = std::memcmp(&L, &c->graph_seen, sizeof(L)) == 0;
+            if (!same && !repeat) std::memcpy(&c->graph_seen, &L, sizeof(L));
+            if (!same && repeat) {
+                if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+                c->graph_exec = nullptr;
+                hipGraph_t g = nullptr;
+                e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    hipError_t el = nd_launch_persistent(L, c->stream);
+                    e = hipStreamEndCapture(c->stream, &g);
+                    if (e == hipSuccess) e = el;
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (e == hipSuccess) {
+                    std::memcpy(&c->graph_key, &L, sizeof(L));
+                } else {
+                    // capture unavailable: leave the stream out of capture mode and launch pass by pass
+                    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+                    if (hipStreamIsCapturing(c->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                        hipGraph_t g2 = nullptr;
+                        (void)hipStreamEndCapture(c->stream, &g2);
+                        if (g2) (void)hipGraphDestroy(g2);
+                    }
+                    (void)hipGetLastError();
+                    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+                    c->graph_exec = nullptr;
+                    c->graph_broken = true;
+                }
+            }
+            if (c->graph_exec) {
+                (void)hipEventRecord(c->ev[0], c->stream);
+                e = hipGraphLaunch(c->graph_exec, c->stream);
+                if (e != hipSuccess) return drained(nd_hip_fail(c, e, "replay persistent passes"));
+                (void)hipEventRecord(c->ev[3], c->stream);
+                replayed = true;
+            }
+        }
+        if (!replayed) {
+            for (int k = 0; k < 4; ++k) L.ev[k] = c->ev[k];
+            e = nd_launch_persistent(L, c->stream);
+            if (e != hipSuccess) return drained(nd_hip_fail(c, e, "launch persistent passes"));
+        }
+        s.graph = replayed ? 1u : 0u;
         s.kernels = 3 + (any_shadow ? 1 : 0) + 1;
     }
     if (any_color) {
@@ -772,14 +843,22 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
     }
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return nd_hip_fail(c, e, "batch execution");
-    (void)hipEventElapsedTime(&ms_total, c->ev[0], c->ev[any_color ? 4 : 3]);
+    // a failed time read leaves no error behind for the next call's launch checks
+    auto elapsed = [&](float* ms, hipEvent_t a, hipEvent_t b) {
+        if (hipEventElapsedTime(ms, a, b) != hipSuccess) { *ms = 0.0f; (void)hipGetLastError(); }
+    };
+    elapsed(&ms_total, c->ev[0], c->ev[any_color ? 4 : 3]);
     DevCounters dc;
     if ((e = hipMemcpy(&dc, d_ctr, sizeof(dc), hipMemcpyDeviceToHost)) != hipSuccess)
         return nd_hip_fail(c, e, "read counters");
     if (persistent) {
-        (void)hipEventElapsedTime(&ms_march, c->ev[0], c->ev[1]);
-        (void)hipEventElapsedTime(&ms_surface, c->ev[1], c->ev[2]);
-        (void)hipEventElapsedTime(&ms_shadow, c->ev[2], c->ev[3]);
+        if (s.graph) {
+            elapsed(&ms_march, c->ev[0], c->ev[3]);  // the replayed passes as one
+        } else {
+            elapsed(&ms_march, c->ev[0], c->ev[1]);
+            elapsed(&ms_surface, c->ev[1], c->ev[2]);
+            elapsed(&ms_shadow, c->ev[2], c->ev[3]);
+        }
         s.hits = dc.hits;
         s.de_evals_march = dc.de_march;
         s.march_wave_rounds = dc.march_rounds;
@@ -863,20 +942,45 @@ nd_status nd_autotune(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_march_
     o.kernel = ND_KERNEL_PERSISTENT;
     const nd_tuning saved = c->tuning;
     const uint32_t occ = c->resident[RES_MARCH] ? c->resident[RES_MARCH] : 8;
-    const uint32_t wpcs[] = {occ / 2 ? occ / 2 : 1, occ, occ * 2};
-    const uint32_t idles[] = {1, 8, 32};
-    const uint32_t chunks[] = {96, 256, 1024};
+    // candidates per parameter; 0 is each parameter's default rule
+    const std::vector<uint32_t> cand[5] = {{0, std::max(occ / 2, 1u), occ, occ * 2},  // waves per CU
+                                           {1, 8, 32},                                  // refill threshold
+                                           {0, 96, 256, 1024},                          // claim size
+                                           {8, 16, 32, 4},                              // tile width
+                                           {1, 2, 4}};                                  // refill period (step budget)
+    auto field = [](nd_tuning& t, int k) -> uint32_t& {
+        switch (k) {
+            case 0: return t.waves_per_cu;
+            case 1: return t.refill_min_idle;
+            case 2: return t.chunk_rays;
+            case 3: return t.tile_width;
+            default: return t.refill_period;
+        }
+    };
+    auto time_of = [&](const nd_tuning& t, float* ms) -> nd_status {
+        c->tuning = t;
+        *ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {  // the best of two runs
+            nd_batch_stats s;
+            nd_status st = nd_batch_impl(c, jobs, n, &o, &s);
+            if (st != ND_OK) return st;
+            *ms = std::min(*ms, s.ms_total);
+        }
+        return ND_OK;
+    };
     nd_tuning bt = saved;
-    float bms = 1e30f;
-    for (uint32_t w : wpcs)
-        for (uint32_t mi : idles)
-            for (uint32_t ch : chunks) {
-                nd_tuning t{w, mi, ch, saved.flags, saved.round_cap};
-                c->tuning = t;
-                nd_batch_stats s;
-                nd_status st = nd_batch_impl(c, jobs, n, &o, &s);
-                if (st != ND_OK) { c->tuning = saved; return st; }
-                if (s.ms_total < bms) { bms = s.ms_total; bt = t; }
+    float bms;
+    nd_status st = time_of(bt, &bms);
+    if (st != ND_OK) { c->tuning = saved; return st; }
+    for (int sweep = 0; sweep < 2; ++sweep)
+        for (int k = 0; k < 5; ++k)
+            for (uint32_t v : cand[k]) {
+                nd_tuning t = bt;
+                if (field(t, k) == v) continue;
+                field(t, k) = v;
+                float ms;
+                if ((st = time_of(t, &ms)) != ND_OK) { c->tuning = saved; return st; }
+                if (ms < bms) { bms = ms; bt = t; }
             }
     c->tuning = bt;
     if (best) *best = bt;

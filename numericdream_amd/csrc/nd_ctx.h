@@ -12,9 +12,12 @@
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
-    nd_status ensure(nd_ctx* c, size_t bytes);
+    nd_status ensure(nd_ctx* c, size_t bytes);  // grows; a reallocation destroys c's captured graphs
     void release();
 };
+
+// Destroys the context's captured graphs (G-buffer passes and path-trace chunk passes).
+void nd_drop_graphs(nd_ctx* c);
 
 // A cached accelerator grid: its key (scene + cascades) and place in nd_ctx::accel.
 struct NdAccelEntry {
@@ -37,8 +40,12 @@ struct nd_ctx {
     const volatile int* cancel = nullptr;
     char* noalloc_scratch = nullptr;   // this device's no-allocation scratch array (device address)
     hipEvent_t ev[8] = {};
-    bool graph_valid = false;          // false after any device buffer was reallocated
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec = nullptr;  // the persistent G-buffer passes of the last batch shape
+                                          // (destroyed, with the path trace's, when a device
+                                          // buffer is reallocated: both hold the old pointers)
+    NdPassLaunch graph_key{};             // what that graph baked in
+    NdPassLaunch graph_seen{};            // the launch record of the last eager batch: a repeat is captured
+    bool graph_broken = false;            // capture failed once: batches launch pass by pass
     hipGraphExec_t mc_graph_exec = nullptr;         // the path trace's per-chunk passes
     alignas(8) unsigned char mc_graph_key[1024] = {};  // what that graph baked in
     DevBuf jobs, job_base, ctr, hits, shared, staging, latency, adj, mc;

@@ -83,6 +83,7 @@ struct DevJob {
     int target_w, target_h;
     int rect_x, rect_y, rect_w, rect_h;
     int tiles_x, tiles_y;
+    int tile_lg;           // log2 of the tile width (tile = 2^tile_lg x 64 / 2^tile_lg pixels)
     uint32_t ray_base;     // first ray slot of this job in the batch (64 slots per 8x8 tile)
     uint32_t ray_count;    // tiles_x * tiles_y * 64
     int max_steps;

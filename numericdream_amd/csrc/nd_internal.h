@@ -19,6 +19,7 @@ struct NdPassLaunch {
     uint32_t chunk;
     uint32_t min_idle;
     uint32_t round_cap;
+    uint32_t refill_period;
     int any_shadow;
     int surface;                 // run the normal / trap pass
     unsigned long long* job_done_cycles;  // per-job completion clock, or null

@@ -152,14 +152,14 @@ NDD int plain_pixel(const DevJob& J, const DevScene& sc, int x, int y) {
 }
 
 // ============================================================================ plain kernel
-// One lane per pixel of one job; blockDim 64 = one 8x8 tile per wave64. 6 waves per SIMD
+// One lane per pixel of one job; blockDim 64 = one tile (8x8 by default) per wave64. 6 waves per SIMD
 // (80 VGPRs) is this kernel's fastest setting, about 7% ahead of the compiler's 4.
 template <bool EXT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 6, EXT ? 8 : 6))) k_plain(const DevJob* __restrict__ jobp, DevCounters* ctr) {
     const DevJob& J = *jobp;
     const DevScene& sc = J.scene;
-    const int x = blockIdx.x * 8 + (threadIdx.x & 7);
-    const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const int x = (blockIdx.x << J.tile_lg) + (int)(threadIdx.x & ((1u << J.tile_lg) - 1u));
+    const int y = blockIdx.y * (64 >> J.tile_lg) + (int)(threadIdx.x >> J.tile_lg);
     const bool live = x < J.rect_w && y < J.rect_h;
     const int ev = live ? plain_pixel<EXT>(J, sc, x, y) : 0;
     unsigned long long v = (unsigned long long)ev;
@@ -197,6 +197,7 @@ struct PassArgs {
     uint32_t shadow_chunk;   // hits per claim of the shadow pass
     uint32_t min_idle;
     uint32_t round_cap;
+    uint32_t refill_period;  // march steps between refills (a wave with every lane idle refills at once)
     uint32_t* ray_done;  // per ray slot: completion time in wall-clock ticks after clock_start, or null
 };
 
@@ -248,7 +249,7 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
     for (uint32_t round = 0;; ++round) {
         if (work_left) {
             uint64_t need = __builtin_amdgcn_ballot_w64(!active);
-            if ((uint32_t)__popcll(need) >= a.min_idle || need == ~0ull) {
+            if (((uint32_t)__popcll(need) >= a.min_idle && round % a.refill_period == 0u) || need == ~0ull) {
                 // latency tracking reads the clock once per refill: a ray that retired since the
                 // last refill finished no later than now
                 if (a.ray_done && lat_ray != ~0u) {
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
     for (uint32_t round = 0;; ++round) {
         if (work_left) {
             uint64_t need = __builtin_amdgcn_ballot_w64(!active);
-            if ((uint32_t)__popcll(need) >= a.min_idle || need == ~0ull) {
+            if (((uint32_t)__popcll(need) >= a.min_idle && round % a.refill_period == 0u) || need == ~0ull) {
                 if (a.ray_done && lat_ray != ~0u) {
                     a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
                     lat_ray = ~0u;
@@ -597,6 +598,7 @@ hipError_t nd_launch_persistent(const NdPassLaunch& L, hipStream_t s) {
     a.shadow_chunk = 4 * L.chunk;
     a.min_idle = L.min_idle;
     a.round_cap = L.round_cap;
+    a.refill_period = L.refill_period ? L.refill_period : 1u;
     a.ray_done = L.job_done_cycles ? L.ray_done : nullptr;
     hipLaunchKernelGGL(nd::k_zero_counters, dim3(1), dim3(64), 0, s, L.ctr);
     if (a.ray_done) {

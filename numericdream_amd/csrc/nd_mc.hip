@@ -1902,7 +1902,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         key.adapt = adapt;
         key.sss = sss;
         key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
-        if (!c->mc_graph_exec || !c->graph_valid || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
+        if (!c->mc_graph_exec || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
             if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }
             hipGraph_t g = nullptr;
             if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return nd_hip_fail(c, e, "begin graph capture");
@@ -1914,7 +1914,6 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) { c->mc_graph_exec = nullptr; return nd_hip_fail(c, e, "instantiate path-trace graph"); }
             std::memcpy(c->mc_graph_key, &key, sizeof(key));
-            c->graph_valid = true;
         }
     }
     // chunk records, uploaded MC_MAX_CHUNKS at a time. With the primary hit cache, sample 0

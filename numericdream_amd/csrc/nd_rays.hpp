@@ -20,7 +20,8 @@ NDD V3<float> primary_dir(const DevJob& J, int px, int py) {
     return mk3(dx * il, dy * il, dz * il);
 }
 
-// Ray slot -> (job, x, y inside the rect). Slots enumerate 8x8 tiles of each job.
+// Ray slot -> (job, x, y inside the rect). Slots enumerate the tiles of each job, 64 per tile
+// (2^tile_lg wide, 64 / 2^tile_lg high), row by row within a tile.
 NDD int find_job(const uint32_t* __restrict__ job_base, int n_jobs, uint32_t r) {
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
@@ -32,8 +33,8 @@ NDD int find_job(const uint32_t* __restrict__ job_base, int n_jobs, uint32_t r) 
 NDD bool slot_pixel(const DevJob& J, uint32_t local, int* x, int* y) {
     uint32_t tile = local >> 6, l = local & 63u;
     int tx = (int)(tile % (uint32_t)J.tiles_x), ty = (int)(tile / (uint32_t)J.tiles_x);
-    *x = tx * 8 + (int)(l & 7u);
-    *y = ty * 8 + (int)(l >> 3);
+    *x = (tx << J.tile_lg) + (int)(l & ((1u << J.tile_lg) - 1u));
+    *y = ty * (64 >> J.tile_lg) + (int)(l >> J.tile_lg);
     return *x < J.rect_w && *y < J.rect_h;
 }
 
