@@ -267,6 +267,51 @@ def mc_leg(r, nd, spp=64):
             "ms_per_1spp_call": per_sample, "primary_cache": cached}
 
 
+def dream_leg(r, nd, spp=64, reps=2):
+    """The reference's benchmark scenes (bench/wavefront_opaque.dream, bench/wavefront_glass.dream,
+    kept under tests/golden/scenes) loaded by nd_scene_from_dream at their 960x540 export size and
+    path traced at spp samples per pixel, best of reps, as the reference's -bench harness renders
+    them (Monte Carlo on, the export's empty-space accelerator at res 64, 6 levels, safety 0.92).
+    Not rendered here: the photon map an export traces first, and the glass scene's fog emitter
+    (this library's path trace has neither). Also without the accelerator."""
+    import torch
+    here = os.path.dirname(os.path.abspath(__file__))
+    out = {}
+    for name in ("opaque", "glass"):
+        u, info = nd.load_dream(os.path.join(here, "tests", "golden", "scenes", "wavefront_%s.dream" % name))
+        r.bind_formulas(list(info.code_to_builtin[:info.n_codes]))
+        u.mc_enabled = 1.0
+        u.primary_cache_key = 1.0
+        had_fog = u.fog_count > 0
+        u.fog_count = 0.0
+        W, H = int(u.resolution[0]), int(u.resolution[1])
+        col = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+        res = {"width": W, "height": H, "spp": spp, "fog_removed": bool(had_fog),
+               "photon_map": False}
+        for label, accel in (("accel", True), ("no_accel", False)):
+            v = nd.Uniforms.from_buffer_copy(u)
+            if accel:
+                nd.enable_accel(v, res=64, levels=6, safety=0.92)
+            r.path_trace(v, 1, color=col)  # warmup
+            r.sync()
+            best, st = None, None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                _, _, _, s_ = r.path_trace(v, spp, color=col)
+                r.sync()
+                wall = time.perf_counter() - t0
+                if best is None or wall < best:
+                    best, st = wall, s_
+            log("scene %s (%s): %.1f ms for %d spp, %.2f ms per sample" % (name, label, best * 1e3, spp, best * 1e3 / spp))
+            res[label] = {"ms_per_frame": round(best * 1e3, 2), "ms_per_sample": round(best * 1e3 / spp, 3),
+                          "samples_per_s": round(st.paths / best, 1), "msamples_per_s": round(st.paths / best / 1e6, 3),
+                          "segments_per_path": round(st.segments / st.paths, 3),
+                          "de_evals_per_path": round(st.de_evals / st.paths, 1)}
+        out[name] = res
+    r.bind_formulas([0, 1, 2, 3])
+    return out
+
+
 def cpu_baseline(nd, target_s=40.0):
     """FP32 CPU restatement of the same per-ray math (the oracle instantiated for float),
     on host threads, over a band of rows of camera 0 at full 1920x1080."""
@@ -312,11 +357,13 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the path-trace, 256-job and gradient legs")
-    ap.add_argument("--legs", default="c1,c3,c4,c5,accel,mesh", help="comma list of extra legs: c1 Mandelbulb 640x360 "
+    ap.add_argument("--dream-spp", type=int, default=64, help="samples per pixel of the scene-file leg")
+    ap.add_argument("--legs", default="c1,c3,c4,c5,accel,mesh,dream", help="comma list of extra legs: c1 Mandelbulb 640x360 "
                                                                     "G-buffer, c3 path trace, c4 256 "
                                                                  "jobs, c5 gradient, accel the batch with the "
                                                                  "empty-space accelerator, mesh the mesher's "
-                                                                 "512^3 distance pass")
+                                                                 "512^3 distance pass, dream the reference's two "
+                                                                 "benchmark scene files")
     ap.add_argument("--c3-spp", type=int, default=64, help="samples per pixel of the path-trace leg's frames "
                                                           "(64 is the workload; fewer for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -397,6 +444,8 @@ def main():
         res["c3_path_trace"] = mc_leg(r, nd, spp=args.c3_spp)
     if "accel" in legs:
         res["c2_accelerator"] = accel_leg(r, nd, torch, abi, jobs)
+    if "dream" in legs:
+        res["dream_scenes"] = dream_leg(r, nd, spp=args.dream_spp)
     if "mesh" in legs:
         res["mesh_points"] = mesh_leg(r, nd, torch, flops_table())
     if not args.no_cpu_baseline:

@@ -56,6 +56,44 @@ void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg, float fraction);
  * res 64, levels 6, safety 0.92 (accel.zig:13-14, accel_state.zig:19). */
 void nd_accel_setup(nd_uniforms* u, uint32_t res, uint32_t levels, float safety);
 
+/* What a .dream scene file asks of an export beyond its Uniforms block. */
+typedef struct nd_dream_info {
+    uint32_t version;            /* the file's format version (at most 12) */
+    int32_t export_width;        /* the saved export size */
+    int32_t export_height;
+    uint32_t width;              /* the size the block was built for */
+    uint32_t height;
+    uint32_t mc_enabled;         /* the Monte Carlo renderer is on in the file */
+    float mc_export_samples;     /* samples per pixel of an export (mc.export_samples) */
+    float mc_max_samples;        /* the interactive view's sample cap (mc.max_samples) */
+    uint32_t photons_wanted;     /* an export would trace a photon map first (photon.enabled, not
+                                    replaced by approximations, 3D view); the block's photon_enabled
+                                    stays 0 (no map is built here) */
+    uint32_t sky_image;          /* the sky names an image: no image is loaded, the procedural sky
+                                    is used, as when the app cannot load it */
+    uint32_t stereo;             /* the file's stereo export is on: the block is the mono view */
+    uint32_t screen_shaders;     /* post effects in the file (not part of the block) */
+    uint32_t keyframes;          /* animation keys in the file (a still export does not apply them) */
+    uint32_t n_codes;            /* formulas in the scene: bind them with nd_bind_formulas */
+    int32_t code_to_builtin[ND_MAX_CODES];  /* code c -> nd_builtin_formula (-1 past n_codes) */
+} nd_dream_info;
+
+/* Builds the Uniforms block an export of a .dream scene file (JSON text, len bytes) renders:
+ * the file is read as the app reads it (parseScene / applyScene, src/app/scene_file.zig:
+ * 447-755, 811-855: unknown fields ignored, fields without a default required, at most 4
+ * instances, 3 mix-ins, 16 colour stops, 8 lights, 4 fog emitters kept), then the block is
+ * filled as buildUniforms does for an export (src/app/export_image.zig:97-167) with the
+ * file's render settings, plus the renderer's formula codes, approximation and sky stamps.
+ * width / height 0 take the file's export size. Formulas are named by file: "" is the built-in
+ * Mandelbulb, otherwise the file's stem must be one of nd_builtin_formula's (with its @param
+ * table compiled in; saved values override by name). Errors: ND_E_INVALID for malformed JSON,
+ * a missing required field or a wrong type; ND_E_UNSUPPORTED for a file from a newer version;
+ * ND_E_INVALID naming the field for warps, particle systems, image formulas, the FFT view or an
+ * unknown formula file, which this library does not render. err (err_cap bytes, may be null)
+ * receives the message. */
+nd_status nd_scene_from_dream(const char* text, size_t len, uint32_t width, uint32_t height, nd_uniforms* out,
+                              nd_dream_info* info, char* err, size_t err_cap);
+
 #ifdef __cplusplus
 }
 #endif

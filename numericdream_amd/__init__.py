@@ -75,6 +75,8 @@ def _load():
     lib.nd_set_pixel_epsilon.restype = None
     lib.nd_accel_setup.argtypes = [P(Uniforms), u32, u32, C.c_float]
     lib.nd_accel_setup.restype = None
+    lib.nd_scene_from_dream.argtypes = [C.c_char_p, C.c_size_t, u32, u32, P(Uniforms), P(abi.DreamInfo), C.c_char_p,
+                                        C.c_size_t]
     return lib
 
 
@@ -93,6 +95,24 @@ def config_uniforms(config, index=0, seed=0):
     if st != 0:
         raise NdError(st, "nd_config_build(%d, %d)" % (config, index))
     return u
+
+
+def load_dream(scene, width=0, height=0):
+    """The Uniforms block an export of a .dream scene renders, and what else the file asks of
+    the export (abi.DreamInfo), through nd_scene_from_dream. scene: a path, or the file's
+    text (str or bytes). width / height 0 take the file's export size. Bind the scene's
+    formulas with Renderer.bind_formulas(info.code_to_builtin[:info.n_codes])."""
+    if isinstance(scene, str) and not scene.lstrip().startswith("{"):
+        with open(scene, "rb") as fh:
+            scene = fh.read()
+    text = scene.encode() if isinstance(scene, str) else bytes(scene)
+    u = Uniforms()
+    info = abi.DreamInfo()
+    err = C.create_string_buffer(512)
+    st = lib.nd_scene_from_dream(text, len(text), width, height, C.byref(u), C.byref(info), err, len(err))
+    if st != 0:
+        raise NdError(st, "nd_scene_from_dream: %s" % err.value.decode(errors="replace"))
+    return u, info
 
 
 def default_uniforms():

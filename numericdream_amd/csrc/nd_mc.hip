@@ -44,11 +44,14 @@ struct McInst {
     McStop stops[16];
 };
 struct McLight {
-    int type;  // 0 point, 1 global
+    int type;  // 0 point, 1 global, 2 ray (a beam from v along axis)
     int cast;
     float color[3];  // colour * brightness
     float v[3];      // position or normalised direction
     float k;         // penumbra
+    float axis[3];   // ray light: the beam's unit axis
+    float waist;     // ray light: beam radius at the source
+    float tan_spread;  // ray light: tan of the half-angle of its widening
 };
 struct McScene {
     McInst inst[ND_DEV_MAX_INST];
@@ -405,20 +408,32 @@ NDD V3<float> cosine_sample(const V3<float>& n, float sx, float sy) {
                       tg.z * local.x + bt.z * local.y + n.z * local.z));
 }
 
+// light_sample (template.wgsl:2178-2229) at a surface point: point, global, and ray lights
+// without a segment to integrate over (a Gaussian beam of radius w = waist + t tan(spread) at
+// depth t along its axis, lighting only what lies in front of the source)
 NDD void light_of(const McLight& L, float max_dist, const V3<float>& p, V3<float>* dir, V3<float>* col, float* dist) {
     if (L.type == 1) {
         *dir = mk3(L.v[0], L.v[1], L.v[2]);
         *col = mk3(L.color[0], L.color[1], L.color[2]);
         *dist = max_dist * 2.f;
-    } else {
-        V3<float> to = mk3(L.v[0] - p.x, L.v[1] - p.y, L.v[2] - p.z);
-        float d = len3(to);
-        float inv = 1.f / fmaxf(d, 1e-4f);
-        *dir = mk3(to.x * inv, to.y * inv, to.z * inv);
-        float a = 1.f / fmaxf(d * d, 0.01f);
-        *col = mk3(L.color[0] * a, L.color[1] * a, L.color[2] * a);
-        *dist = d;
+        return;
     }
+    V3<float> to = mk3(L.v[0] - p.x, L.v[1] - p.y, L.v[2] - p.z);
+    float d = len3(to);
+    float inv = 1.f / fmaxf(d, 1e-4f);
+    *dir = mk3(to.x * inv, to.y * inv, to.z * inv);
+    *dist = d;
+    float a;
+    if (L.type == 0) {
+        a = 1.f / fmaxf(d * d, 0.01f);
+    } else {
+        const float t0 = -(to.x * L.axis[0] + to.y * L.axis[1] + to.z * L.axis[2]);
+        const float qx = -to.x - L.axis[0] * t0, qy = -to.y - L.axis[1] * t0, qz = -to.z - L.axis[2] * t0;
+        const float w = L.waist + fmaxf(t0, 0.f) * L.tan_spread;
+        const float profile = expf(-(qx * qx + qy * qy + qz * qz) / (w * w));
+        a = t0 > 0.f ? profile / (w * w) : 0.f;
+    }
+    *col = mk3(L.color[0] * a, L.color[1] * a, L.color[2] * a);
 }
 
 // ----------------------------------------------------------------------------- path state
@@ -1548,7 +1563,8 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     for (int i = 0; i < S.light_count; ++i) {
         const nd_light& L = u.lights[i];
         nd::McLight& d = S.lights[i];
-        d.type = (int)(L.light_type + 0.5f) == 1 ? 1 : 0;
+        const int lt = (int)(L.light_type + 0.5f);
+        d.type = lt == 1 ? 1 : lt == 2 ? 2 : 0;
         d.cast = L.cast_shadows > 0.5f;
         for (int k = 0; k < 3; ++k) d.color[k] = L.color[k] * L.brightness;
         if (d.type == 1) {
@@ -1560,6 +1576,13 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
             for (int k = 0; k < 3; ++k) d.v[k] = L.position_or_direction[k];
         }
         d.k = L.shadow_softness + (512.0f - L.shadow_softness) * L.hard_shadows;
+        // ray light: axis = ray_direction / max(|ray_direction|, EPSILON_FINE), spread <= 1.5 rad
+        const double al = std::sqrt((double)L.ray_direction[0] * L.ray_direction[0] +
+                                    (double)L.ray_direction[1] * L.ray_direction[1] +
+                                    (double)L.ray_direction[2] * L.ray_direction[2]);
+        for (int k = 0; k < 3; ++k) d.axis[k] = (float)(L.ray_direction[k] / std::max(al, 1e-6));
+        d.waist = L.waist;
+        d.tan_spread = std::tan(std::min(std::max(L.spread, 0.0f), 1.5f));
     }
     S.shadows_on = !(parts & ND_PART_SHADOWS);
     S.specular_on = !(parts & ND_PART_SPECULAR);
@@ -1744,8 +1767,6 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     const nd_uniforms& u = *(const nd_uniforms*)uniforms;
     if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.dof_enabled > 0.5f || u.approx_flags != 0.0f)
         return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field and approximations are not implemented");
-    for (int i = 0; i < std::min((int)u.light_count, 8); ++i)
-        if ((int)(u.lights[i].light_type + 0.5f) == 2) return nd_fail(c, ND_E_UNSUPPORTED, "ray lights are not implemented");
     if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
     nd_march_opts o{ND_NORMAL_TETRA, 0, 0, ND_KERNEL_PERSISTENT};
     DevJob job;

@@ -212,6 +212,14 @@ def flops_per_de(u, pts, code_map=None):
     return f, float(tops[0])
 
 
+def light_sample(light_type, pos_or_dir, color, brightness, pos, ray_direction=(0.0, -1.0, 0.0), spread=0.0,
+                 waist=0.05, max_dist=10.0):
+    """light_sample at a surface point (FP64): (direction, colour, distance)."""
+    a = [light_type, *pos_or_dir, *color, brightness, *ray_direction, spread, waist, max_dist, *pos]
+    o = helper(40, a, 8)
+    return o[0:3], o[3:6], o[6]
+
+
 def helper(op, args, nout=4):
     a = np.zeros(64); a[:len(args)] = args
     out = np.zeros(nout)

@@ -580,6 +580,25 @@ int ndo_helper(int op, const double* a, double* out) {
         case 38: { double sc = 0; out[0] = roulette(a[0], Seed{(float)a[1], (float)a[2]}, &sc) ? 1.0 : 0.0; out[1] = sc;
                    return 0; }
         case 39: { D3 r = undo_tonemap(D3(a[0], a[1], a[2])); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+        case 40: {  // light_sample: type, position_or_direction, colour, brightness, ray_direction, spread,
+                    // waist, max_dist, surface point -> direction, colour, distance
+                   nd_light L{};
+                   L.light_type = (float)a[0];
+                   for (int k = 0; k < 3; ++k) {
+                       L.position_or_direction[k] = (float)a[1 + k];
+                       L.color[k] = (float)a[4 + k];
+                       L.ray_direction[k] = (float)a[8 + k];
+                   }
+                   L.brightness = (float)a[7];
+                   L.spread = (float)a[11];
+                   L.waist = (float)a[12];
+                   D3 dir, col;
+                   double dist = 0;
+                   light_sample(L, a[13], D3(a[14], a[15], a[16]), &dir, &col, &dist);
+                   out[0] = dir.x; out[1] = dir.y; out[2] = dir.z;
+                   out[3] = col.x; out[4] = col.y; out[5] = col.z;
+                   out[6] = dist;
+                   return 0; }
         case 35: { Subsurface<double> ss{};  // subsurface_transmit: radius, depth, 8 reaches
                    ss.amount = 1.0; ss.radius = a[0];
                    for (int k = 0; k < 8; ++k) ss.reach.v[k] = a[2 + k];

@@ -277,7 +277,11 @@ template <class T> void light_terms(V3<T> ld, V3<T> n, V3<T> view, T spec_power,
 }
 
 struct LightSampleD { double dir[3], color[3], dist; };
-// light_sample for point and global lights (template.wgsl:2178-2190)
+// light_sample (template.wgsl:2178-2229) at a surface point, i.e. light_sample_ex with a
+// zero-length segment: point, global, and ray lights. A ray light is a Gaussian beam from its
+// position along axis = ray_direction / max(|ray_direction|, 1e-6): at depth t0 along the axis
+// and distance r from it, colour * brightness * exp(-r^2 / w^2) / w^2 with w = waist +
+// max(t0, 0) tan(clamp(spread, 0, 1.5)), and nothing behind the source (t0 <= 0).
 template <class T> void light_sample(const nd_light& L, double max_dist, V3<T> pos, V3<T>* dir, V3<T>* color, T* dist) {
     int type = i32(L.light_type + 0.5f);
     V3<T> lc = V3<T>::from(L.color) * T(L.brightness);
@@ -285,13 +289,24 @@ template <class T> void light_sample(const nd_light& L, double max_dist, V3<T> p
         *dir = normalize(V3<T>::from(L.position_or_direction));
         *color = lc;
         *dist = T(max_dist * 2.0);
-    } else {
-        V3<T> to = V3<T>::from(L.position_or_direction) - pos;
-        T d = length(to);
-        *dir = to / tmax(d, T(1e-4));
-        *color = lc / tmax(d * d, T(0.01));
-        *dist = d;
+        return;
     }
+    V3<T> to = V3<T>::from(L.position_or_direction) - pos;
+    T d = length(to);
+    *dir = to / tmax(d, T(1e-4));
+    *dist = d;
+    if (type != 2) {
+        *color = lc / tmax(d * d, T(0.01));
+        return;
+    }
+    const V3<T> rd = V3<T>::from(L.ray_direction);
+    const V3<T> axis = rd / tmax(length(rd), T(1e-6));
+    const V3<T> u0 = -to;
+    const T t0 = dot(u0, axis);
+    const V3<T> perp0 = u0 - axis * t0;
+    const T w = T(L.waist) + tmax(t0, T(0)) * T(std::tan(std::min(std::max((double)L.spread, 0.0), 1.5)));
+    const T profile = exp(-dot(perp0, perp0) / (w * w));
+    *color = lc * (t0 > T(0) ? profile / (w * w) : T(0));
 }
 
 // visibility_march (template.wgsl:2282-2342) with shadow_factor's parameters (:2343-2375):
