@@ -355,6 +355,34 @@ nd_status nd_accel_probe(nd_ctx* ctx, const void* uniforms, size_t uniforms_size
 nd_status nd_adjoint(nd_ctx* ctx, const nd_uniforms* views, uint32_t n_views,
                      const nd_adjoint_io* io, nd_batch_stats* stats);
 
+/* No counterpart in the reference (it renders, it does not differentiate). The gradient of a
+ * colour loss: each view's direct-lighting image - nd_path_trace's colour at the primary hit
+ * with Monte Carlo off and the reflection continuation cut (mc_enabled = 0, PART_REFLECTIONS
+ * set): ambient 0.15 x AO and every light (point, global, ray) with its soft shadow, Schlick's
+ * local share, the hero wavelength's spectrum to RGB and gamut_fit - and its derivative with
+ * respect to the same six parameters as nd_adjoint: the hit depth's implicit-function tangent,
+ * then dual numbers through the normal, AO, lights and shadows. loss = sum over hit pixels whose
+ * target red is >= 0 of |rgb - target|^2. Outputs, all optional: rgb_out (3 floats per pixel,
+ * 0 at misses), drgb_out (18 per pixel: [c * 6 + k] = d rgb_c / d theta_k). The scene must keep
+ * its material constant over the surface (every visible instance one colour stop or none, the
+ * same for all), roughness <= 0.01, no transparency, thin film or subsurface unless their parts
+ * are off, no accelerator: ND_E_UNSUPPORTED otherwise;
+ * mc_enabled != 0 or reflections on: ND_E_INVALID. With the primary hit cache on the colour is
+ * the unquantised normal's. */
+typedef struct nd_radiance_io {
+    const float* target_rgb;
+    float* rgb_out;
+    float* drgb_out;
+    double* grad_out;
+    double* loss_out;
+    uint64_t* pixels_used;
+    uint32_t device_ptrs;
+    uint32_t _pad;
+} nd_radiance_io;
+
+nd_status nd_radiance_adjoint(nd_ctx* ctx, const nd_uniforms* views, uint32_t n_views,
+                              const nd_radiance_io* io, nd_batch_stats* stats);
+
 /* ---- wavefront Monte Carlo path trace -------------------------------------------- */
 typedef struct nd_image_out {
     float* color;      /* RGB float per pixel: the average over spp samples */

@@ -60,6 +60,7 @@ def _load():
     lib.nd_set_tuning.argtypes = [vp, P(Tuning)]
     lib.nd_autotune.argtypes = [vp, P(Job), u32, P(MarchOpts), P(Tuning)]
     lib.nd_adjoint.argtypes = [vp, P(Uniforms), u32, P(AdjointIO), P(BatchStats)]
+    lib.nd_radiance_adjoint.argtypes = [vp, P(Uniforms), u32, P(abi.RadianceIO), P(BatchStats)]
     lib.nd_path_trace.argtypes = [vp, vp, C.c_size_t, u32, Rect, P(ImageOut), P(TraceStats)]
     lib.nd_accel_probe.argtypes = [vp, vp, C.c_size_t, vp, u32, vp]
     lib.nd_mesh_points.argtypes = [vp, vp, C.c_size_t, P(abi.MeshGrid), vp, u32, vp, vp]
@@ -343,6 +344,25 @@ class Renderer:
                        C.addressof(used), 1 if dev else 0, 0)
         st = BatchStats()
         self._check(lib.nd_adjoint(self._h, arr, n, C.byref(io), C.byref(st)), "nd_adjoint")
+        return loss.value, np.array(grad[:]), int(used.value), st
+
+    def radiance_adjoint(self, views, target=None, rgb_out=None, drgb_out=None):
+        """Colour-loss gradient of the direct-lighting images of views (nd_radiance_adjoint).
+
+        target / rgb_out / drgb_out: per-view maps concatenated in view order (W*H*3 floats;
+        drgb has 18 per pixel, [c * 6 + k]), all numpy (host) or all torch tensors on the
+        device; a pixel whose target red is negative is left out of the loss. Returns (loss,
+        grad[6], pixels_used, stats) with the parameter order of adjoint()."""
+        n = len(views)
+        arr = (Uniforms * n)(*views)
+        dev = any(x is not None and hasattr(x, "data_ptr") for x in (target, rgb_out, drgb_out))
+        grad = (C.c_double * 6)()
+        loss = C.c_double()
+        used = C.c_uint64()
+        io = abi.RadianceIO(_ptr(target), _ptr(rgb_out), _ptr(drgb_out), C.addressof(grad), C.addressof(loss),
+                            C.addressof(used), 1 if dev else 0, 0)
+        st = BatchStats()
+        self._check(lib.nd_radiance_adjoint(self._h, arr, n, C.byref(io), C.byref(st)), "nd_radiance_adjoint")
         return loss.value, np.array(grad[:]), int(used.value), st
 
     def mesh_points(self, u, origin, cell, iso, span, local_offset, blocks, values=None):

@@ -192,6 +192,12 @@ FORMULA_COUNT = len(FORMULA_FILES)
 PARAM_COUNT = 6
 
 # every symbol the headers declare (include/nd_hip.h, include/nd_config.h)
+class RadianceIO(C.Structure):
+    _fields_ = [("target_rgb", C.c_void_p), ("rgb_out", C.c_void_p), ("drgb_out", C.c_void_p),
+                ("grad_out", C.c_void_p), ("loss_out", C.c_void_p), ("pixels_used", C.c_void_p),
+                ("device_ptrs", U32), ("_pad", U32)]
+
+
 class MeshGrid(C.Structure):
     _fields_ = [("origin", C.c_float * 3), ("cell", C.c_float), ("iso", C.c_float), ("span", C.c_uint32),
                 ("local_offset", C.c_uint32), ("device_ptrs", C.c_uint32)]
@@ -201,7 +207,7 @@ EXPORTS = ["nd_ctx_create", "nd_ctx_destroy", "nd_last_error", "nd_sync", "nd_se
            "nd_bind_formulas", "nd_render_gbuffer", "nd_render_gbuffer_noalloc", "nd_render_batch", "nd_batch_job_latency",
            "nd_get_tuning", "nd_set_tuning", "nd_autotune", "nd_adjoint", "nd_path_trace", "nd_mesh_points", "nd_mesh_probe", "nd_accel_probe", "nd_accel_setup",
            "nd_config_job_count", "nd_uniforms_defaults", "nd_config_build", "nd_camera_look_at",
-           "nd_set_pixel_epsilon", "nd_scene_from_dream"]
+           "nd_set_pixel_epsilon", "nd_scene_from_dream", "nd_radiance_adjoint"]
 
 
 class DreamInfo(C.Structure):

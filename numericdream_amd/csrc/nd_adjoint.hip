@@ -11,9 +11,13 @@
 // L = sum over pixels hit in both renders of (t - t_target)^2; each wave reduces its part in
 // FP64 and adds it with one atomic per component.
 //
+// nd_radiance_adjoint (at the end) does the same for the direct-lighting colour; its kernel
+// lives with the path trace's shading (nd_mc.hip).
+//
 // There is no counterpart in the reference (it renders, it does not differentiate).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -212,6 +216,150 @@ extern "C" nd_status nd_adjoint(nd_ctx* c, const nd_uniforms* views, uint32_t n,
         stats->ms_total = fs.ms_total + ms_adj;
         stats->de_evals_surface = fs.hits;
         stats->de_evals += fs.hits;
+        stats->kernels += 1;
+    }
+    return ND_OK;
+}
+
+// ----------------------------------------------------------------------------- radiance adjoint
+// The scenes the radiance adjoint differentiates exactly: the material does not vary over the
+// surface, and nothing but the local shading reaches the pixel.
+static nd_status radiance_scope(nd_ctx* c, const nd_uniforms& u, uint32_t v) {
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    if (u.mc_enabled > 0.5f) return nd_fail(c, ND_E_INVALID, "view %u: the radiance adjoint needs mc_enabled = 0", v);
+    if (!(parts & ND_PART_REFLECTIONS))
+        return nd_fail(c, ND_E_INVALID, "view %u: the radiance adjoint needs the reflection part off (PART_REFLECTIONS)", v);
+    if (u.accel_enabled > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "view %u: the radiance adjoint runs without the accelerator", v);
+    const nd_color_stop* first = nullptr;
+    int first_count = -1;
+    const int ni = std::min(std::max((int)u.instance_count, 0), ND_MAX_INSTANCES);
+    for (int i = 0; i < ni; ++i) {
+        const nd_fractal_instance& in = u.instances[i];
+        if (in.hidden > 0.5f) continue;
+        const int cc = (int)in.color_count;
+        if (cc > 1) return nd_fail(c, ND_E_UNSUPPORTED, "view %u instance %d: colour strips vary the material over the surface", v, i);
+        if (first_count < 0) { first_count = cc; first = cc ? &in.colors[0] : nullptr; }
+        else if (cc != first_count || (cc && std::memcmp(first, &in.colors[0], sizeof(nd_color_stop)) != 0))
+            return nd_fail(c, ND_E_UNSUPPORTED, "view %u: instances of different materials", v);
+        if (cc) {
+            const nd_color_stop& st = in.colors[0];
+            if (st.roughness > 0.01f) return nd_fail(c, ND_E_UNSUPPORTED, "view %u instance %d: roughness above 0.01", v, i);
+            if (st.transparency > 0.f && !(parts & ND_PART_REFRACTION))
+                return nd_fail(c, ND_E_UNSUPPORTED, "view %u instance %d: transparency", v, i);
+            if (st.film_strength > 0.001f && !(parts & ND_PART_IRIDESCENCE))
+                return nd_fail(c, ND_E_UNSUPPORTED, "view %u instance %d: thin film", v, i);
+            if (st.subsurface > 0.001f && !(parts & ND_PART_SUBSURFACE))
+                return nd_fail(c, ND_E_UNSUPPORTED, "view %u instance %d: subsurface", v, i);
+        }
+    }
+    return ND_OK;
+}
+
+extern "C" nd_status nd_radiance_adjoint(nd_ctx* c, const nd_uniforms* views, uint32_t n, const nd_radiance_io* io,
+                                         nd_batch_stats* stats) {
+    if (!c) return ND_E_INVALID;
+    if (!views || n == 0 || !io) return nd_fail(c, ND_E_INVALID, "radiance adjoint needs views and an io record");
+    if (n > 65535) return nd_fail(c, ND_E_INVALID, "at most 65535 views");
+    nd_status st;
+    for (uint32_t v = 0; v < n; ++v)
+        if ((st = radiance_scope(c, views[v], v)) != ND_OK) return st;
+    hipError_t e;
+    std::vector<uint64_t> base(n);
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        base[v] = total;
+        uint64_t w = (uint64_t)views[v].resolution[0], h = (uint64_t)views[v].resolution[1];
+        if (w == 0 || h == 0) return nd_fail(c, ND_E_INVALID, "view %u has an empty resolution", v);
+        total += w * h;
+    }
+    const bool dev = io->device_ptrs != 0;
+    const size_t mcs = nd_mc_scene_bytes();
+    size_t need = 256 + 8 * 8 + 8 * (size_t)n + 256 + mcs * n + 256 + 4 * total + 256;
+    if (io->target_rgb && !dev) need += 12 * total + 256;
+    if (io->rgb_out && !dev) need += 12 * total + 256;
+    if (io->drgb_out && !dev) need += 72 * total + 256;
+    if ((st = c->adj.ensure(c, need)) != ND_OK) return st;
+    char* p = (char*)c->adj.p;
+    auto carve = [&](size_t b) { char* r = p; p += (b + 255) & ~(size_t)255; return r; };
+    double* acc = (double*)carve(8 * 8);
+    uint64_t* d_base = (uint64_t*)carve(8 * (size_t)n);
+    char* d_scenes = carve(mcs * n);
+    float* d_depth = (float*)carve(4 * total);
+    const float* d_target = nullptr;
+    if (io->target_rgb) {
+        if (dev) d_target = io->target_rgb;
+        else {
+            float* t = (float*)carve(12 * total);
+            if ((e = hipMemcpyAsync(t, io->target_rgb, 12 * total, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+                return nd_hip_fail(c, e, "upload target colour");
+            d_target = t;
+        }
+    }
+    float* d_rgb = io->rgb_out ? (dev ? io->rgb_out : (float*)carve(12 * total)) : nullptr;
+    float* d_drgb = io->drgb_out ? (dev ? io->drgb_out : (float*)carve(72 * total)) : nullptr;
+    std::vector<unsigned char> hs(mcs * n);
+    for (uint32_t v = 0; v < n; ++v) nd_mc_scene_pack(views[v], hs.data() + mcs * v);
+    if ((e = hipMemcpyAsync(d_base, base.data(), 8 * (size_t)n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "upload view table");
+    if ((e = hipMemcpyAsync(d_scenes, hs.data(), mcs * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "upload material scenes");
+    hipLaunchKernelGGL(nd::k_fill_f64, dim3(1), dim3(64), 0, c->stream, acc, 8);
+    if (d_rgb) hipLaunchKernelGGL(nd::k_fill_f32, dim3(1024), dim3(256), 0, c->stream, d_rgb, 3 * total, 0.0f);
+    if (d_drgb) hipLaunchKernelGGL(nd::k_fill_f32, dim3(1024), dim3(256), 0, c->stream, d_drgb, 18 * total, 0.0f);
+
+    // forward: the persistent march over all views (the hit list), parameters as adjoint slots
+    std::vector<nd_job> jobs(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        jobs[v].uniforms = &views[v];
+        jobs[v].rect = nd_rect{0, 0, 0, 0};
+        std::memset(&jobs[v].out, 0, sizeof(nd_gbuffer_out));
+        jobs[v].out.depth = d_depth + base[v];
+        jobs[v].out.device_ptrs = 1;
+    }
+    nd_march_opts o{ND_NORMAL_ANALYTIC, 0, 0, ND_KERNEL_PERSISTENT};
+    NdBatchExtra ex;
+    ex.skip_surface = true;
+    ex.adjoint_slots = true;
+    nd_batch_stats fs{};
+    if ((st = nd_batch_impl(c, jobs.data(), n, &o, &fs, &ex)) != ND_OK) return st;
+
+    NdRadianceLaunch L;
+    L.jobs = (const DevJob*)c->jobs.p;
+    L.shared = c->last_shared ? (const DevScene*)c->shared.p : nullptr;
+    L.ctr = (const DevCounters*)c->ctr.p;
+    L.hb = c->last_hb;
+    L.scenes = d_scenes;
+    L.target = d_target;
+    L.rgb = d_rgb;
+    L.drgb = d_drgb;
+    L.pix_base = d_base;
+    L.acc = acc;
+    (void)hipEventRecord(c->ev[4], c->stream);
+    const uint32_t res = nd_resident_radiance(c->last_ext);
+    const uint64_t waves = ((uint64_t)c->last_rays + 63) / 64;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(waves, (uint64_t)c->cu_count * res));
+    if ((e = nd_launch_radiance_adjoint(L, grid, c->last_ext, c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "launch radiance adjoint");
+    (void)hipEventRecord(c->ev[5], c->stream);
+    double h_acc[8];
+    if ((e = hipMemcpyAsync(h_acc, acc, sizeof(h_acc), hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+        return nd_hip_fail(c, e, "read gradient");
+    if (!dev) {
+        if (io->rgb_out && (e = hipMemcpyAsync(io->rgb_out, d_rgb, 12 * total, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+            return nd_hip_fail(c, e, "read colour");
+        if (io->drgb_out && (e = hipMemcpyAsync(io->drgb_out, d_drgb, 72 * total, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+            return nd_hip_fail(c, e, "read d colour");
+    }
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return nd_hip_fail(c, e, "radiance adjoint execution");
+    float ms_adj = 0.f;
+    if (hipEventElapsedTime(&ms_adj, c->ev[4], c->ev[5]) != hipSuccess) { ms_adj = 0.f; (void)hipGetLastError(); }
+    if (io->loss_out) *io->loss_out = h_acc[0];
+    if (io->grad_out) for (int k = 0; k < 6; ++k) io->grad_out[k] = h_acc[1 + k];
+    if (io->pixels_used) *io->pixels_used = (uint64_t)h_acc[7];
+    if (stats) {
+        *stats = fs;
+        stats->ms_surface = ms_adj;  // the adjoint pass takes the surface pass's slot
+        stats->ms_total = fs.ms_total + ms_adj;
         stats->kernels += 1;
     }
     return ND_OK;

@@ -40,6 +40,23 @@ void nd_mc_scene_pack(const nd_uniforms& u, void* out);
 hipError_t nd_launch_gbuffer_color(const DevJob* jobs, const void* scenes, const uint32_t* job_base, int n_jobs,
                                    uint32_t total, bool ext, hipStream_t s);
 
+// The radiance adjoint over the compacted hits of the batch just run (nd_mc.hip).
+struct NdRadianceLaunch {
+    const DevJob* jobs;
+    const DevScene* shared;  // non-null when every job has the same scene
+    const DevCounters* ctr;
+    DevHitBuffers hb;
+    const void* scenes;      // one packed material scene per job
+    const float* target;     // 3 floats per pixel, or null (no loss)
+    float* rgb;              // 3 floats per pixel, or null
+    float* drgb;             // 18 floats per pixel ([c * 6 + k]), or null
+    const uint64_t* pix_base;  // first pixel of each job
+    double* acc;             // loss, 6 gradient components, pixels used
+};
+// ext: some job uses a formula beyond the four tuned ones (the kernel then reads each job's scene)
+hipError_t nd_launch_radiance_adjoint(const NdRadianceLaunch& L, uint32_t grid, bool ext, hipStream_t s);
+uint32_t nd_resident_radiance(bool ext);  // resident wave64 workgroups per CU of the radiance adjoint
+
 // Resident wave64 workgroups per CU of each persistent kernel (blockDim 64, no dynamic LDS).
 // "_MIXED" variants take the scene from each job, the others share one scene; "_EXT"
 // variants also run the formulas beyond the four tuned ones (and take each job's scene).

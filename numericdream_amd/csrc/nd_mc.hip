@@ -1724,11 +1724,261 @@ __global__ void __launch_bounds__(256) k_gbuffer_color(const DevJob* __restrict_
     if (J.out_color) { J.out_color[3 * o] = color.x; J.out_color[3 * o + 1] = color.y; J.out_color[3 * o + 2] = color.z; }
     if (J.out_albedo) { J.out_albedo[3 * o] = albedo.x; J.out_albedo[3 * o + 1] = albedo.y; J.out_albedo[3 * o + 2] = albedo.z; }
 }
+
+// ----------------------------------------------------------------------------- radiance adjoint
+// d(direct-lighting colour)/d(theta), theta = (Mandelbulb power, Mandelbox scale, Mandelbox min
+// radius, camera x, y, z), for the hits of a geometry batch. The hit is first polished to the
+// surface DE(o + t d) = eps(t) by Newton steps from the march's hit (which its bisection leaves
+// up to eps(t) past it): the point the implicit-function tangent describes, and the one the
+// march converges to as its bisection window shrinks. The colour there is the path trace's at
+// depth 0 with Monte Carlo off and the reflection continuation cut: the scene's constant
+// material, Schlick's local share, ambient 0.15 x AO and every light with its soft shadow,
+// through the hero wavelength's spectrum to RGB and gamut_fit (template.wgsl:3819-4137).
+// Tangent slots: 0..2 camera (and space, for the hit depth's implicit-function tangent),
+// 3..5 the formula parameters the batch's jobs carry as adjoint slots.
+using D6 = DualN<6>;
+NDD V3<D6> d3(const V3<float>& v) { return mk3(dconst<6>(v.x), dconst<6>(v.y), dconst<6>(v.z)); }
+NDD V3<D6> dadd(const V3<D6>& a, const V3<D6>& b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+NDD V3<D6> dscale(const V3<D6>& a, const D6& s) { return mk3(a.x * s, a.y * s, a.z * s); }
+NDD V3<D6> dnorm(const V3<D6>& g) {
+    const D6 l = ssqrt(dot3(g, g));
+    return mk3(g.x / l, g.y / l, g.z / l);
+}
+NDD D6 dclamp(const D6& x, float lo, float hi) { return sclamp(x, dconst<6>(lo), dconst<6>(hi)); }
+NDD D6 dmaxf(const D6& x, float b) { return smax_(x, dconst<6>(b)); }
+
+// light_of with dual positions: direction, colour scale a (colour = L.color * a) and distance
+NDD void light_of_d(const McLight& L, float max_dist, const V3<D6>& p, V3<D6>* dir, D6* a, D6* dist) {
+    if (L.type == 1) {
+        *dir = d3(mk3(L.v[0], L.v[1], L.v[2]));
+        *a = dconst<6>(1.f);
+        *dist = dconst<6>(max_dist * 2.f);
+        return;
+    }
+    const V3<D6> to = mk3(dconst<6>(L.v[0]) - p.x, dconst<6>(L.v[1]) - p.y, dconst<6>(L.v[2]) - p.z);
+    const D6 d = ssqrt(dot3(to, to));
+    const D6 inv = dconst<6>(1.f) / dmaxf(d, 1e-4f);
+    *dir = dscale(to, inv);
+    *dist = d;
+    if (L.type == 0) {
+        *a = dconst<6>(1.f) / dmaxf(d * d, 0.01f);
+    } else {
+        const V3<D6> ax = d3(mk3(L.axis[0], L.axis[1], L.axis[2]));
+        const D6 t0 = -dot3(to, ax);
+        const V3<D6> q = mk3(-to.x - ax.x * t0, -to.y - ax.y * t0, -to.z - ax.z * t0);
+        const D6 w = dmaxf(t0, 0.f) * L.tan_spread + L.waist;
+        const D6 w2 = w * w;
+        const D6 e = -dot3(q, q) / w2;
+        const D6 profile = dchain(e, expf(e.v), expf(e.v));
+        *a = t0.v > 0.f ? profile / w2 : dconst<6>(0.f);
+    }
+}
+// the opaque soft shadow of k_mc_shadow with dual origin, direction and distance
+template <bool EXT>
+NDD D6 vis_d(const DevScene& sc, const V3<D6>& o, const V3<D6>& ld, const D6& ldist, float k, int steps) {
+    D6 t = dconst<6>(0.004f), res = dconst<6>(1.f);
+    if (!(t.v < ldist.v)) return res;
+    for (int step = 1; step <= steps; ++step) {
+        const V3<D6> p = dadd(o, dscale(ld, t));
+        const D6 d = scene_de_d<D6, EXT>(sc, p);
+        if (d.v < 0.0005f) return dconst<6>(0.f);  // SHADOW_HIT_EPS
+        res = smin_(res, d * k / t);
+        t = t + dmaxf(d, 0.001f);
+        if (!(t.v < ldist.v)) break;
+    }
+    return dclamp(res, 0.f, 1.f);
+}
+// the spectrum -> RGB weights of to_rgb_fit (before gamut_fit)
+NDD void rgb_weights(float hero, float* wr, float* wg, float* wb) {
+    float l[8];
+    lambdas_of(hero, l);
+    float sr = 0.f, sg = 0.f, sb = 0.f;
+    for (int k = 0; k < 8; ++k) {
+        float x = 1.056f * cie_lobe(l[k], 599.8f, 37.9f, 31.0f) + 0.362f * cie_lobe(l[k], 442.0f, 16.0f, 26.7f) -
+                  0.065f * cie_lobe(l[k], 501.1f, 20.4f, 26.2f);
+        float y = 0.821f * cie_lobe(l[k], 568.8f, 46.9f, 40.5f) + 0.286f * cie_lobe(l[k], 530.9f, 16.3f, 31.1f);
+        float z = 1.217f * cie_lobe(l[k], 437.0f, 11.8f, 36.0f) + 0.681f * cie_lobe(l[k], 459.0f, 26.0f, 13.8f);
+        wr[k] = 3.2404542f * x - 1.5371385f * y - 0.4985314f * z;
+        wg[k] = -0.9692660f * x + 1.8760108f * y + 0.0415560f * z;
+        wb[k] = 0.0556434f * x - 0.2040259f * y + 1.0572252f * z;
+        sr += wr[k]; sg += wg[k]; sb += wb[k];
+    }
+    for (int k = 0; k < 8; ++k) { wr[k] /= sr; wg[k] /= sg; wb[k] /= sb; }
+}
+NDD V3<float> lin_rgb(const float* wr, const float* wg, const float* wb, const Sp8& s) {
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    for (int k = 0; k < 8; ++k) { c0 += s.v[k] * wr[k]; c1 += s.v[k] * wg[k]; c2 += s.v[k] * wb[k]; }
+    return mk3(c0, c1, c2);
+}
+
+template <bool EXT>
+__global__ void __launch_bounds__(64) k_radiance_adjoint(NdRadianceLaunch a) {
+    const uint32_t n_hits = a.ctr->hit_count;
+    const int slot_of[6] = {3, 4, 5, 0, 1, 2};  // theta order -> tangent slot
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i - threadIdx.x < n_hits; i += gridDim.x * 64) {
+        if (i >= n_hits) continue;
+        const uint32_t r = a.hb.ray[i];
+        if (r == ~0u) continue;
+        const int job = a.hb.job[i];
+        const DevJob& J = a.jobs[job];
+        const DevScene& sc = a.shared ? *a.shared : J.scene;
+        const McScene& S = ((const McScene*)a.scenes)[job];
+        int x, y;
+        slot_pixel(J, r - J.ray_base, &x, &y);
+        const size_t o = (size_t)y * J.rect_w + x;
+        const int gx = J.rect_x + x, gy = J.rect_y + y;
+        const V3<float> dir = primary_dir(J, gx, gy);
+        // F(t, theta) = DE(o + t d; theta) - eps_c t - eps_f: Newton from the march's hit
+        // (F <= 0 there, F' < 0 going in), kept inside [t / 2, t + eps(t)], then the hit depth's
+        // tangent dt/dtheta = -dF/dtheta / F' from the last evaluation
+        const float tm = a.hb.t[i];
+        float t = tm;
+        D6 de;
+        float den = 0.f;
+        for (int it = 0; it < 4; ++it) {
+            const V3<float> q = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
+            de = scene_de_d<D6, EXT>(sc, mk3(dvar<6>(q.x, 0), dvar<6>(q.y, 1), dvar<6>(q.z, 2)));
+            den = de.d[0] * dir.x + de.d[1] * dir.y + de.d[2] * dir.z - J.eps_c;
+            if (it == 3 || !(den < 0.f)) break;
+            const float F = de.v - (J.eps_c * t + J.eps_f);
+            t = fminf(fmaxf(t - F / den, 0.5f * tm), tm + J.eps_c * tm + J.eps_f);
+        }
+        if (den == 0.f) continue;
+        const V3<float> hp = mk3(J.cam[0] + dir.x * t, J.cam[1] + dir.y * t, J.cam[2] + dir.z * t);
+        D6 td = dconst<6>(t);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) td.d[k] = -de.d[k] / den;
+        const V3<D6> od = mk3(dvar<6>(J.cam[0], 0), dvar<6>(J.cam[1], 1), dvar<6>(J.cam[2], 2));
+        const V3<D6> dd = d3(dir);
+        const V3<D6> hpd = dadd(od, dscale(dd, td));
+        // estimate_normal at e = eps(t)
+        const D6 e = td * J.eps_c + J.eps_f;
+        const float K[4][3] = {{1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}};
+        V3<D6> g = d3(mk3(0.f, 0.f, 0.f));
+        for (int q = 0; q < 4; ++q) {
+            const V3<D6> pq = mk3(hpd.x + e * K[q][0], hpd.y + e * K[q][1], hpd.z + e * K[q][2]);
+            const D6 dq = scene_de_d<D6, EXT>(sc, pq);
+            g = mk3(g.x + dq * K[q][0], g.y + dq * K[q][1], g.z + dq * K[q][2]);
+        }
+        const V3<D6> nd = dnorm(g);
+        const Mat m = part_mask(hit_material<EXT>(sc, S, hp), S);  // constant over the surface
+        const V3<D6> nf = dot3(dd, nd).v > 0.f ? mk3(-nd.x, -nd.y, -nd.z) : nd;
+        const float ior = fmaxf(m.ior, 1.f);
+        const float r0r = (1.f - ior) / (1.f + ior), r0 = r0r * r0r;
+        const D6 cth = dclamp(dconst<6>(1.f) - sabs(dot3(dd, nf)), 0.f, 1.f);
+        const D6 c2 = cth * cth;
+        const D6 f = c2 * c2 * cth * (1.f - r0) + r0;
+        const float transparency = fclamp(m.transparency, 0.f, 1.f);
+        const float refl = fclamp(m.reflectiveness, 0.f, 1.f);
+        const D6 p_reflect = dclamp(f * (1.f - refl) + refl, 0.f, 1.f);
+        const D6 share = (dconst<6>(1.f) - p_reflect) * (1.f - transparency);
+        V3<D6> c = d3(mk3(0.f, 0.f, 0.f));
+        if (share.v > 0.001f) {
+            const float bx = (float)gx + 0.5f, by = (float)gy + 0.5f;  // mc_sample 0
+            const float hero = hash12w(add_rn(bx, 19.31f), add_rn(by, 47.11f));
+            const Basis B = basis_of(hero);
+            float wr[8], wg[8], wb[8];
+            rgb_weights(hero, wr, wg, wb);
+            const Sp8 albedo = from_rgb(B, m.color[0], m.color[1], m.color[2]);
+            D6 ao = dconst<6>(1.f);
+            if (S.ao_on) {
+                const D6 radius = dclamp(td * 0.04f, 0.003f, 0.25f);
+                D6 occ = dconst<6>(0.f);
+                float w = 1.f;
+                for (int q = 0; q < 5; ++q) {
+                    const D6 step = radius * (0.2f + 0.2f * (float)q);
+                    const D6 dq = scene_de_d<D6, EXT>(sc, dadd(hpd, dscale(nd, step)));
+                    occ = occ + dmaxf(step - dq, 0.f) * w;
+                    w *= 0.6f;
+                }
+                ao = dclamp(dconst<6>(1.f) - occ * 2.f / radius, 0.f, 1.f);
+            }
+            const V3<float> ra = lin_rgb(wr, wg, wb, albedo);
+            c = mk3(ao * (0.15f * ra.x), ao * (0.15f * ra.y), ao * (0.15f * ra.z));
+            const float spec_power = 4.f + 124.f * fclamp(m.glossiness, 0.f, 1.f);
+            for (int l = 0; l < S.light_count; ++l) {
+                const McLight& L = S.lights[l];
+                V3<D6> ld;
+                D6 la, ldist;
+                light_of_d(L, S.max_dist, hpd, &ld, &la, &ldist);
+                if (fmaxf(L.color[0], fmaxf(L.color[1], L.color[2])) * la.v < 1e-5f) continue;
+                const D6 ndotl = dmaxf(dot3(nd, ld), 0.f);
+                if (!(ndotl.v > 0.f)) continue;
+                D6 spec = dconst<6>(0.f);
+                if (S.specular_on) {
+                    const V3<D6> hd = dnorm(mk3(ld.x - dir.x, ld.y - dir.y, ld.z - dir.z));
+                    const D6 nh = dmaxf(dot3(nd, hd), 0.f);
+                    spec = spow(nh, dconst<6>(spec_power)) * m.glossiness;
+                }
+                D6 vis = dconst<6>(1.f);
+                if (S.shadows_on && L.cast && S.shadow_steps > 0) {
+                    const V3<D6> so = dadd(hpd, dscale(nd, dconst<6>(0.004f)));
+                    vis = vis_d<EXT>(sc, so, ld, ldist, L.k, S.shadow_steps);
+                    if (!(vis.v > 0.f)) continue;
+                }
+                const Sp8 ls = from_rgb(B, L.color[0], L.color[1], L.color[2]);
+                Sp8 al;
+                for (int k = 0; k < 8; ++k) al.v[k] = albedo.v[k] * ls.v[k];
+                const V3<float> rd = lin_rgb(wr, wg, wb, al), rs = lin_rgb(wr, wg, wb, ls);
+                const D6 w = la * vis;
+                const D6 wd = w * ndotl, wsp = w * spec;
+                c = mk3(c.x + wd * rd.x + wsp * rs.x, c.y + wd * rd.y + wsp * rs.y, c.z + wd * rd.z + wsp * rs.z);
+            }
+            c = dscale(c, share);
+            // gamut_fit
+            const D6 yl = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+            if (!(yl.v > 0.f)) {
+                c = d3(mk3(0.f, 0.f, 0.f));
+            } else {
+                const D6 mn = smin_(c.x, smin_(c.y, c.z));
+                if (mn.v < 0.f) {
+                    const D6 fit = yl / (yl - mn);
+                    c = mk3(yl + (c.x - yl) * fit, yl + (c.y - yl) * fit, yl + (c.z - yl) * fit);
+                }
+            }
+        }
+        const uint64_t pix = a.pix_base[job] + o;
+        const D6 cc[3] = {c.x, c.y, c.z};
+        if (a.rgb)
+            for (int q = 0; q < 3; ++q) a.rgb[pix * 3 + q] = cc[q].v;
+        if (a.drgb)
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) a.drgb[pix * 18 + q * 6 + k] = cc[q].d[slot_of[k]];
+        if (a.target && a.target[pix * 3] >= 0.f) {
+            for (int q = 0; q < 3; ++q) {
+                const double res = (double)cc[q].v - (double)a.target[pix * 3 + q];
+                acc[0] += res * res;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[1 + k] += 2.0 * res * (double)cc[q].d[slot_of[k]];
+            }
+            acc[7] += 1.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (threadIdx.x == 0 && v != 0.0) atomicAdd(&a.acc[k], v);
+    }
+}
 }  // namespace nd
 
 static_assert(sizeof(nd::McScene) <= ND_MC_SCENE_MAX_BYTES, "ND_MC_SCENE_MAX_BYTES");
 size_t nd_mc_scene_bytes() { return sizeof(nd::McScene); }
 void nd_mc_scene_pack(const nd_uniforms& u, void* out) { pack_mc_scene(u, *(nd::McScene*)out); }
+hipError_t nd_launch_radiance_adjoint(const NdRadianceLaunch& L, uint32_t grid, bool ext, hipStream_t s) {
+    if (ext) hipLaunchKernelGGL(nd::k_radiance_adjoint<true>, dim3(grid), dim3(64), 0, s, L);
+    else hipLaunchKernelGGL(nd::k_radiance_adjoint<false>, dim3(grid), dim3(64), 0, s, L);
+    return hipGetLastError();
+}
+uint32_t nd_resident_radiance(bool ext) {
+    int b = 0;
+    const hipError_t e = ext ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_radiance_adjoint<true>, 64, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, nd::k_radiance_adjoint<false>, 64, 0);
+    return e == hipSuccess && b > 0 ? (uint32_t)b : 4u;
+}
 hipError_t nd_launch_gbuffer_color(const DevJob* jobs, const void* scenes, const uint32_t* job_base, int n_jobs,
                                    uint32_t total, bool ext, hipStream_t s) {
     const dim3 grid((total + 255) / 256), block(256);

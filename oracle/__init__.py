@@ -264,6 +264,26 @@ def adjoint_pixels(u, pixels, h=1e-3, code_map=None):
     return t, gi, gf, ok
 
 
+def radiance_pixels(u, pixels, t=None, h=1e-6, code_map=None):
+    """FP64 direct-lighting colour of pixels [(x, y), ...] (Monte Carlo off, the reflection
+    continuation cut) and its derivative with respect to (bulb power, box scale, box min radius,
+    camera x/y/z): dual numbers at the hit, and central differences with step h at the
+    perturbed scenes' own polished roots. t: per-pixel hit depths to take the dual at (no
+    differences then). Returns dict t, rgb (n, 3), dual and fd (n, 3, 6), ok (bit k: FD valid)."""
+    L = lib()
+    L.ndo_radiance_pixels.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    pix = np.ascontiguousarray(pixels, np.int32).reshape(-1, 2)
+    n = len(pix)
+    tin = None if t is None else np.ascontiguousarray(t, np.float64)
+    out = {"t": np.zeros(n), "rgb": np.zeros((n, 3)), "dual": np.zeros((n, 3, 6)), "fd": np.zeros((n, 3, 6)),
+           "ok": np.zeros(n, np.int32)}
+    cm, cmp_ = _cm(code_map)
+    L.ndo_radiance_pixels(_u(u), cmp_, _p(pix), _p(tin) if tin is not None else None, n, float(h), _p(out["t"]),
+                          _p(out["rgb"]), _p(out["dual"]), _p(out["fd"]), _p(out["ok"]))
+    return out
+
+
 def render_mc(u, spp=1, rect=None, prec=0, threads=None, code_map=None, per_sample=False):
     """Monte Carlo path trace (opaque scenes) of a rect: mean colour over spp samples
     (sample i uses mc_sample = u.mc_sample + i), depth and normal of sample 0, DE evaluations,

@@ -116,6 +116,46 @@ int render_gbuffer_t(const nd_uniforms* u, const int32_t* cm, const GbufOpts& o,
 
 }  // namespace
 
+// The direct-lighting colour of a primary hit at depth t along d from o, as trace_path
+// computes it at depth 0 with Monte Carlo off and the reflection continuation cut
+// (PART_REFLECTIONS): the material of the hit point (constant over the surface in the scenes
+// this is used on), the tetrahedral normal at eps(t), Schlick's split local_share =
+// (1 - p_reflect)(1 - transparency), ambient 0.15 * AO, shade_local with shadows, then the
+// spectrum to RGB and gamut_fit (template.wgsl:3819-4137). T is double, or Dual<6> with the
+// scene's parameter tangents and o's camera tangents, for d(colour)/d(theta).
+template <class T>
+static void direct_rgb(const nd_uniforms& u, const Scene<T>& s, const Scene<double>& s64, const MarchCfg& mc, V3<T> o,
+                       V3<double> d, T t, int px, int py, T* rgb) {
+    const uint32_t parts = (uint32_t)u.debug_parts;
+    const float ms = u.mc_sample;
+    const Seed seed{((float)px + 0.5f) + ms * 91.73f, ((float)py + 0.5f) + ms * 57.31f};
+    const Seed st_seed = seed_add(seed, ms * 27.31f, ms * 45.17f);
+    const float hero_f = hash12w(seed.x + 19.31f, seed.y + 47.11f) + ms * 0.61803399f;
+    const Seed sd = seed_add(st_seed, 3.1f, 7.7f);  // depth 0
+    Spectral<T> sp{T(hero_f)};
+    const V3<T> dir(T(d.x), T(d.y), T(d.z));
+    const V3<T> hp = o + dir * t;
+    const V3<T> n = normal_tetra(s, hp, T(mc.eps_c) * t + T(mc.eps_f));
+    const V3<double> hp64(val(hp.x), val(hp.y), val(hp.z));
+    const Mat mat = apply_part_mask(hit_material(u, s64, hp64), parts);
+    const V3<T> nf = dot(dir, n) > T(0) ? -n : n;
+    const T ior = tmax(T(mat.ior), T(1));
+    const T f = fresnel_schlick(dot(dir, nf), ior);
+    const T transparency = tclamp(T(mat.transparency), T(0), T(1));
+    const T refl = tclamp(T(mat.reflectiveness), T(0), T(1));
+    const T p_reflect = tclamp(refl + (T(1) - refl) * f, T(0), T(1));
+    const T local_share = (T(1) - p_reflect) * (T(1) - transparency);
+    Spec<T> radiance = splat(T(0));
+    if (local_share > T(0.001)) {
+        Spec<T> ambient = splat(T(0.15));
+        if (!(parts & ND_PART_AMBIENT_OCCLUSION)) ambient = ambient * calc_ao(s, hp, n, t);
+        int evals = 0;
+        radiance = shade_local(u, s, sp, hp, dir, n, mat, ambient, true, sd, &evals) * local_share;
+    }
+    sp.to_rgb(radiance, rgb);
+    gamut_fit(rgb);
+}
+
 extern "C" {
 
 int ndo_abi_size(void) { return (int)sizeof(nd_uniforms); }
@@ -455,6 +495,87 @@ int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, in
         else pool.emplace_back([&] { run(0.0f); });
     }
     for (auto& th : pool) th.join();
+    return 0;
+}
+
+// The direct-lighting colour (direct_rgb) of pixels and its derivative with respect to theta =
+// (Mandelbulb power, Mandelbox scale, Mandelbox min radius, camera x, y, z): the hit depth's
+// implicit-function tangent, then dual numbers through the normal, AO, lights and shadows.
+// Without t_in the hit is the polished root of DE(o + d t) = eps(t) and g_fd gets central
+// differences with step h (the perturbed scene's own polished root each time; ok bit k set
+// where both roots exist); with t_in the dual is taken at the given depths (the GPU's). rgb
+// gets the colour, g_dual and g_fd 18 values per pixel: [c * 6 + k] = d rgb_c / d theta_k.
+int ndo_radiance_pixels(const nd_uniforms* u, const int32_t* cm, const int32_t* pix, const double* t_in, int n,
+                        double h, double* t_root, double* rgb, double* g_dual, double* g_fd, int32_t* ok) {
+    using D6 = Dual<6>;
+    const CodeMap map = code_map(cm);
+    Scene<double> s = decode_scene<double>(*u, map);
+    MarchCfg mc = march_cfg(*u);
+    int bulb = -1, box = -1;
+    for (int i = 0; i < s.count; ++i) {
+        if (s.inst[i].hidden) continue;
+        if (s.inst[i].formula == F_MANDELBULB && bulb < 0) bulb = i;
+        if (s.inst[i].formula == F_MANDELBOX && box < 0) box = i;
+    }
+    // dual slots: 0..2 camera x, y, z (and space, for the implicit-function tangent), 3 power,
+    // 4 box scale, 5 min radius; outputs in theta order (power, scale, min radius, camera)
+    Scene<D6> sd = decode_scene<D6>(*u, map);
+    if (bulb >= 0) sd.inst[bulb].p.p[0] = D6::var(s.inst[bulb].p.p[0], 3);
+    if (box >= 0) {
+        sd.inst[box].p.p[0] = D6::var(s.inst[box].p.p[0], 4);
+        sd.inst[box].p.p[1] = D6::var(s.inst[box].p.p[1], 5);
+    }
+    const int slot_of[6] = {3, 4, 5, 0, 1, 2};
+    const V3<double> o0 = V3<double>::from(u->camera_pos);
+    const int W = (int)u->resolution[0], H = (int)u->resolution[1];
+    for (int i = 0; i < n; ++i) {
+        const int px = pix[2 * i], py = pix[2 * i + 1];
+        const V3<double> d = primary_dir<double>(*u, px, py, W, H);
+        for (int k = 0; k < 18; ++k) g_dual[18 * i + k] = g_fd[18 * i + k] = 0;
+        for (int c = 0; c < 3; ++c) rgb[3 * i + c] = 0;
+        ok[i] = 0;
+        double t;
+        if (t_in) t = t_in[i];
+        else if (!polished_root(s, mc, o0, d, &t)) t = 0;
+        t_root[i] = t;
+        if (!(t > 0)) continue;
+        // dt/dtheta from one dual evaluation with spatial seeds (slots 0..2) and the parameters
+        const V3<double> p = o0 + d * t;
+        const D6 de = scene_de(sd, V3<D6>(D6::var(p.x, 0), D6::var(p.y, 1), D6::var(p.z, 2)));
+        const double den = de.d[0] * d.x + de.d[1] * d.y + de.d[2] * d.z - mc.eps_c;
+        if (den == 0) continue;
+        D6 td(t);
+        for (int k = 0; k < 6; ++k) td.d[k] = -de.d[k] / den;
+        const V3<D6> od(D6::var(o0.x, 0), D6::var(o0.y, 1), D6::var(o0.z, 2));
+        D6 c6[3];
+        direct_rgb(*u, sd, s, mc, od, d, td, px, py, c6);
+        for (int c = 0; c < 3; ++c) {
+            rgb[3 * i + c] = c6[c].v;
+            for (int k = 0; k < 6; ++k) g_dual[18 * i + 6 * c + k] = c6[c].d[slot_of[k]];
+        }
+        if (t_in) continue;
+        for (int k = 0; k < 6; ++k) {
+            double cp[3], cmn[3];
+            bool both = true;
+            for (int sgn = 0; sgn < 2 && both; ++sgn) {
+                const double delta = sgn == 0 ? h : -h;
+                Scene<double> q = s;
+                V3<double> o = o0;
+                if (k == 0 && bulb >= 0) q.inst[bulb].p.p[0] += delta;
+                if (k == 1 && box >= 0) q.inst[box].p.p[0] += delta;
+                if (k == 2 && box >= 0) q.inst[box].p.p[1] += delta;
+                if (k == 3) o.x += delta;
+                if (k == 4) o.y += delta;
+                if (k == 5) o.z += delta;
+                double tk;
+                if (!polished_root(q, mc, o, d, &tk)) { both = false; break; }
+                direct_rgb(*u, q, q, mc, o, d, tk, px, py, sgn == 0 ? cp : cmn);
+            }
+            if (!both) continue;
+            ok[i] |= 1 << k;
+            for (int c = 0; c < 3; ++c) g_fd[18 * i + 6 * c + k] = (cp[c] - cmn[c]) / (2 * h);
+        }
+    }
     return 0;
 }
 
