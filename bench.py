@@ -131,9 +131,28 @@ def gradient_leg(r, nd, torch, reps=3):
     for _ in range(reps):
         loss, grad, used, st = r.adjoint(views, target=target)
     wall = (time.perf_counter() - t0) / reps
-    return {"grad_rays_per_s": round(total / wall, 1), "ms_per_call": round(wall * 1e3, 3),
-            "device_ms": round(st.ms_total, 3), "adjoint_pass_ms": round(st.ms_surface, 3),
-            "views": len(views), "pixels": total, "pixels_in_loss": used}
+    res = {"grad_rays_per_s": round(total / wall, 1), "ms_per_call": round(wall * 1e3, 3),
+           "device_ms": round(st.ms_total, 3), "adjoint_pass_ms": round(st.ms_surface, 3),
+           "views": len(views), "pixels": total, "pixels_in_loss": used}
+    # the colour loss of the same views lit by a point, a global and a ray light with soft
+    # shadows (nd_radiance_adjoint: Monte Carlo off, reflection continuation cut)
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from tests.radiance_scene import radiance_scene
+    cviews = [radiance_scene(view=i, size=(512, 512)) for i in range(len(views))]
+    ctarget = torch.zeros(3 * total, dtype=torch.float32, device="cuda:0")
+    r.radiance_adjoint(cviews, rgb_out=ctarget)
+    for v in cviews:
+        v.instances[0].params[0] = 8.2
+    r.radiance_adjoint(cviews, target=ctarget)
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        closs, cgrad, cused, cst = r.radiance_adjoint(cviews, target=ctarget)
+    cwall = (time.perf_counter() - t0) / reps
+    res["radiance"] = {"grad_rays_per_s": round(total / cwall, 1), "ms_per_call": round(cwall * 1e3, 3),
+                       "device_ms": round(cst.ms_total, 3), "adjoint_pass_ms": round(cst.ms_surface, 3),
+                       "pixels_in_loss": cused, "lights": 3}
+    return res
 
 
 def c1_leg(r, nd, torch, abi, flops, reps=5):
