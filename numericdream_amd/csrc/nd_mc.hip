@@ -507,6 +507,16 @@ NDD uint32_t append(uint32_t* ctr, bool want, uint32_t* slot) {
 }
 
 // --- generate: camera paths of samples [sample_base, sample_base + spp_chunk)
+// The scene records as the kernels read them: through kernel arguments declared __restrict__,
+// so the compiler knows no store of the kernel reaches them and keeps their loads scalar
+// (s_load into SGPRs) instead of per-lane global loads inside the DE loops.
+NDD McArgs restricted(const McArgs& a0, const DevScene* __restrict__ scene, const McScene* __restrict__ mcs) {
+    McArgs a = a0;
+    a.scene = scene;
+    a.S = mcs;
+    return a;
+}
+
 __global__ void k_mc_generate(McArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = a.npix * a.ch->spp_chunk;
@@ -685,7 +695,8 @@ struct IndirectRays {
 // kernel's INSIDE variant (scenes with transmissive materials) keeps the compiler's choice.
 template <bool EXT, bool INSIDE>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu((EXT || INSIDE) ? 1 : 7, (EXT || INSIDE) ? 8 : 7))) k_mc_extend(McArgs a, int ping) {
+__attribute__((amdgpu_waves_per_eu((EXT || INSIDE) ? 1 : 7, (EXT || INSIDE) ? 8 : 7))) k_mc_extend(McArgs a0, int ping, const DevScene* __restrict__ scene, const McScene* __restrict__ mcs) {
+    McArgs a = restricted(a0, scene, mcs);
     const uint32_t* q = ping ? a.P.live_b : a.P.live_a;
     const uint32_t n = a.P.ctr[C_LIVE_IN];
     ExtendRays R{a, q};
@@ -693,7 +704,8 @@ __attribute__((amdgpu_waves_per_eu((EXT || INSIDE) ? 1 : 7, (EXT || INSIDE) ? 8 
 }
 
 template <bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 7, EXT ? 8 : 7))) k_mc_indirect_march(McArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 7, EXT ? 8 : 7))) k_mc_indirect_march(McArgs a0, const DevScene* __restrict__ scene, const McScene* __restrict__ mcs) {
+    McArgs a = restricted(a0, scene, mcs);
     IndirectRays R{a};
     persistent_march<EXT, false>(a, R, a.P.ctr[C_IND], a.P.ctr + C_HEAD_IND);
 }
@@ -1063,7 +1075,8 @@ NDD float cauchy_ior(float ior_d, float abbe, float lambda) {
 // --- shade (path_shade, template.wgsl:3902-4124). One path per lane. With SSS, a hit on a
 // subsurface material evaluates its lights in line through the material.
 template <bool EXT, bool SSS>
-__global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
+__global__ void __launch_bounds__(64) k_mc_shade(McArgs a0, int ping, const DevScene* __restrict__ scene, const McScene* __restrict__ mcs) {
+    McArgs a = restricted(a0, scene, mcs);
     const uint32_t* qin = ping ? a.P.live_b : a.P.live_a;
     uint32_t* qout = ping ? a.P.live_a : a.P.live_b;
     const uint32_t n = a.P.ctr[C_LIVE_IN];
@@ -1292,7 +1305,8 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a, int ping) {
 // --- indirect shading: shade_local at the indirect hit, no shadows, ambient 0.15
 // (indirect_light_traced, template.wgsl:3359-3377)
 template <bool EXT>
-__global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
+__global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a0, const DevScene* __restrict__ scene, const McScene* __restrict__ mcs) {
+    McArgs a = restricted(a0, scene, mcs);
     const uint32_t n = a.P.ctr[C_IND];
     const DevJob& J = *a.job;
     const DevScene& sc = *a.scene;
@@ -1327,7 +1341,8 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a) {
 // from_rgb(mix(1, colour, a)) * a in the path's spectral basis and steps through the solid
 // (steps of max(|DE|, floor), floor 0.004 growing by 1.15) until DE > 0.001 t + 0.0005.
 template <bool EXT, bool TRANSP>
-__global__ void __launch_bounds__(64) k_mc_shadow(McArgs a) {
+__global__ void __launch_bounds__(64) k_mc_shadow(McArgs a0, const DevScene* __restrict__ scene, const McScene* __restrict__ mcs) {
+    McArgs a = restricted(a0, scene, mcs);
     const DevScene& sc = *a.scene;
     const DevJob& J = *a.job;
     const bool acc = J.accel != nullptr;
@@ -2125,27 +2140,27 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         for (int depth = 0; depth < max_depth; ++depth) {
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 1);
             if (ext) {
-                if (inside) hipLaunchKernelGGL((nd::k_mc_extend<true, true>), dim3(g_ext), dim3(64), 0, st, a, ping);
-                else hipLaunchKernelGGL((nd::k_mc_extend<true, false>), dim3(g_ext), dim3(64), 0, st, a, ping);
-                if (sss) hipLaunchKernelGGL((nd::k_mc_shade<true, true>), dim3(g_shade), dim3(64), 0, st, a, ping);
-                else hipLaunchKernelGGL((nd::k_mc_shade<true, false>), dim3(g_shade), dim3(64), 0, st, a, ping);
-                if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<true, true>), dim3(g_shadow), dim3(64), 0, st, a);
-                else hipLaunchKernelGGL((nd::k_mc_shadow<true, false>), dim3(g_shadow), dim3(64), 0, st, a);
+                if (inside) hipLaunchKernelGGL((nd::k_mc_extend<true, true>), dim3(g_ext), dim3(64), 0, st, a, ping, a.scene, a.S);
+                else hipLaunchKernelGGL((nd::k_mc_extend<true, false>), dim3(g_ext), dim3(64), 0, st, a, ping, a.scene, a.S);
+                if (sss) hipLaunchKernelGGL((nd::k_mc_shade<true, true>), dim3(g_shade), dim3(64), 0, st, a, ping, a.scene, a.S);
+                else hipLaunchKernelGGL((nd::k_mc_shade<true, false>), dim3(g_shade), dim3(64), 0, st, a, ping, a.scene, a.S);
+                if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<true, true>), dim3(g_shadow), dim3(64), 0, st, a, a.scene, a.S);
+                else hipLaunchKernelGGL((nd::k_mc_shadow<true, false>), dim3(g_shadow), dim3(64), 0, st, a, a.scene, a.S);
             } else {
-                if (inside) hipLaunchKernelGGL((nd::k_mc_extend<false, true>), dim3(g_ext), dim3(64), 0, st, a, ping);
-                else hipLaunchKernelGGL((nd::k_mc_extend<false, false>), dim3(g_ext), dim3(64), 0, st, a, ping);
-                if (sss) hipLaunchKernelGGL((nd::k_mc_shade<false, true>), dim3(g_shade), dim3(64), 0, st, a, ping);
-                else hipLaunchKernelGGL((nd::k_mc_shade<false, false>), dim3(g_shade), dim3(64), 0, st, a, ping);
-                if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<false, true>), dim3(g_shadow), dim3(64), 0, st, a);
-                else hipLaunchKernelGGL((nd::k_mc_shadow<false, false>), dim3(g_shadow), dim3(64), 0, st, a);
+                if (inside) hipLaunchKernelGGL((nd::k_mc_extend<false, true>), dim3(g_ext), dim3(64), 0, st, a, ping, a.scene, a.S);
+                else hipLaunchKernelGGL((nd::k_mc_extend<false, false>), dim3(g_ext), dim3(64), 0, st, a, ping, a.scene, a.S);
+                if (sss) hipLaunchKernelGGL((nd::k_mc_shade<false, true>), dim3(g_shade), dim3(64), 0, st, a, ping, a.scene, a.S);
+                else hipLaunchKernelGGL((nd::k_mc_shade<false, false>), dim3(g_shade), dim3(64), 0, st, a, ping, a.scene, a.S);
+                if (transp) hipLaunchKernelGGL((nd::k_mc_shadow<false, true>), dim3(g_shadow), dim3(64), 0, st, a, a.scene, a.S);
+                else hipLaunchKernelGGL((nd::k_mc_shadow<false, false>), dim3(g_shadow), dim3(64), 0, st, a, a.scene, a.S);
             }
             if (depth == 0 && indirect) {
                 if (ext) {
-                    hipLaunchKernelGGL(nd::k_mc_indirect_march<true>, dim3(g_im), dim3(64), 0, st, a);
-                    hipLaunchKernelGGL(nd::k_mc_indirect_shade<true>, dim3(g_is), dim3(64), 0, st, a);
+                    hipLaunchKernelGGL(nd::k_mc_indirect_march<true>, dim3(g_im), dim3(64), 0, st, a, a.scene, a.S);
+                    hipLaunchKernelGGL(nd::k_mc_indirect_shade<true>, dim3(g_is), dim3(64), 0, st, a, a.scene, a.S);
                 } else {
-                    hipLaunchKernelGGL(nd::k_mc_indirect_march<false>, dim3(g_im), dim3(64), 0, st, a);
-                    hipLaunchKernelGGL(nd::k_mc_indirect_shade<false>, dim3(g_is), dim3(64), 0, st, a);
+                    hipLaunchKernelGGL(nd::k_mc_indirect_march<false>, dim3(g_im), dim3(64), 0, st, a, a.scene, a.S);
+                    hipLaunchKernelGGL(nd::k_mc_indirect_shade<false>, dim3(g_is), dim3(64), 0, st, a, a.scene, a.S);
                 }
             }
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 2);
