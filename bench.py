@@ -275,6 +275,20 @@ def mc_leg(r, nd, spp=64):
     log("c3 mc with the primary hit cache: %.1f ms for %d spp" % (wall_c * 1e3, spp))
     cached = {"ms_per_frame": round(wall_c * 1e3, 2), "msamples_per_s": round(stc.paths / wall_c / 1e6, 3),
               "de_evals_per_path": round(stc.de_evals / stc.paths, 1)}
+    # the same frame with adaptive sampling (threshold 0.02 from 16 samples, the app's
+    # defaults): converged pixels start no more paths
+    ua = nd.config_uniforms(3, 0)
+    ua.adapt_enabled = 1.0
+    ua.adapt_row_width = float(W)
+    r.path_trace(ua, 1, color=col)
+    r.sync()
+    t0 = time.perf_counter()
+    _, _, _, sta = r.path_trace(ua, spp, color=col)
+    r.sync()
+    wall_a = time.perf_counter() - t0
+    log("c3 mc adaptive: %.1f ms for %d spp, %d segments (%d without)" % (wall_a * 1e3, spp, sta.segments, st.segments))
+    adaptive = {"ms_per_frame": round(wall_a * 1e3, 2), "segments_per_path": round(sta.segments / sta.paths, 3),
+                "threshold": 0.02, "min_samples": 16}
     f_de = flops_table()["workloads"]["3"]["march"]["flops_per_de"]
     tflops = f_de * st.de_evals / (st.ms_total / 1e3) / 1e12 if st.ms_total > 0 else 0.0
     return {"tflops": round(tflops, 3), "frac_fp32_peak": round(tflops / FP32_PEAK_TFLOPS, 4),
@@ -283,7 +297,7 @@ def mc_leg(r, nd, spp=64):
             "msamples_per_s": round(st.paths / wall / 1e6, 3), "segments_per_s": round(st.segments / wall, 1),
             "segments_per_path": round(st.segments / st.paths, 3), "shadow_rays": int(st.shadow_rays),
             "de_evals_per_s": round(st.de_evals / wall, 1), "de_evals_per_path": round(st.de_evals / st.paths, 1),
-            "ms_per_1spp_call": per_sample, "primary_cache": cached}
+            "ms_per_1spp_call": per_sample, "primary_cache": cached, "adaptive": adaptive}
 
 
 def dream_leg(r, nd, spp=64, reps=2):

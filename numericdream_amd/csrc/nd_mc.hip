@@ -506,7 +506,6 @@ NDD uint32_t append(uint32_t* ctr, bool want, uint32_t* slot) {
     return (uint32_t)__popcll(m);
 }
 
-// --- generate: camera paths of samples [sample_base, sample_base + spp_chunk)
 // The scene records as the kernels read them: through kernel arguments declared __restrict__,
 // so the compiler knows no store of the kernel reaches them and keeps their loads scalar
 // (s_load into SGPRs) instead of per-lane global loads inside the DE loops.
@@ -517,6 +516,11 @@ NDD McArgs restricted(const McArgs& a0, const DevScene* __restrict__ scene, cons
     return a;
 }
 
+// --- generate: camera paths of samples [sample_base, sample_base + spp_chunk). With adaptive
+// sampling, a sample of a pixel that had converged before this chunk starts no path (the
+// sample skip of cs_wf_generate, template.wgsl:5134-5146) and the live queue holds only the
+// paths started; the resolve drops the same samples, and those of pixels that converge
+// within the chunk.
 __global__ void k_mc_generate(McArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = a.npix * a.ch->spp_chunk;
@@ -525,8 +529,18 @@ __global__ void k_mc_generate(McArgs a) {
     const uint32_t pix = p % a.npix, s = a.ch->sample_base + p / a.npix;
     const int x = (int)(pix % (uint32_t)J.rect_w), y = (int)(pix / (uint32_t)J.rect_w);
     const int gx = J.rect_x + x, gy = J.rect_y + y;
-    const V3<float> d = primary_dir(J, gx, gy);
     const float ms = add_rn(a.ch->mc_sample0, (float)s);
+    if (a.S->adapt_on) {
+        const bool on = (float)gx < a.S->adapt_row_width;
+        const bool skip = on && ms > 0.5f && a.adapt[8 * (size_t)pix + 3] > 0.5f;
+        uint32_t slot = 0;
+        append(a.P.ctr + C_LIVE_IN, !skip, &slot);
+        if (skip) return;
+        a.P.live_a[slot] = p;
+    } else {
+        a.P.live_a[p] = p;
+    }
+    const V3<float> d = primary_dir(J, gx, gy);
     // base seed = frag + mc_sample * (91.73, 57.31) (fs_main, template.wgsl:4818)
     const float bx = mad_rn(ms, 91.73f, (float)gx + 0.5f);
     const float by = mad_rn(ms, 57.31f, (float)gy + 0.5f);
@@ -550,7 +564,6 @@ __global__ void k_mc_generate(McArgs a) {
     a.P.flags[p] = fl;
     a.P.lambda[p] = 0.f;
     a.P.inner[p] = 64;  // PATH_DEFAULT_INNER_STEPS
-    a.P.live_a[p] = p;
 }
 
 // --- persistent march over a queue of rays. RAY provides setup(i, org, dir), finish(i,
@@ -1537,8 +1550,9 @@ __global__ void k_mc_counters(uint32_t* ctr, int op) {
         ctr[C_LIVE_IN] = ctr[C_LIVE_OUT];
     }
 }
-__global__ void k_mc_set_live(uint32_t* ctr, const McChunk* ch, uint32_t npix) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[C_LIVE_IN] = npix * ch->spp_chunk;
+// the live queue of a chunk: every path, or (adaptive sampling) none yet, generate appends
+__global__ void k_mc_set_live(uint32_t* ctr, const McChunk* ch, uint32_t npix, int adapt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[C_LIVE_IN] = adapt ? 0u : npix * ch->spp_chunk;
 }
 // Copies chunk record k into the current-chunk slot that the captured passes read.
 __global__ void k_mc_select_chunk(McChunk* recs, uint32_t k) {
@@ -2134,7 +2148,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     // replayed for every chunk (the chunk's sample range is read from a.ch).
     auto enqueue_chunk = [&](hipStream_t st) {
         hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 0);
-        hipLaunchKernelGGL(nd::k_mc_set_live, dim3(1), dim3(64), 0, st, Pp.ctr, (const nd::McChunk*)a.ch, npix);
+        hipLaunchKernelGGL(nd::k_mc_set_live, dim3(1), dim3(64), 0, st, Pp.ctr, (const nd::McChunk*)a.ch, npix, adapt ? 1 : 0);
         hipLaunchKernelGGL(nd::k_mc_generate, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, st, a);
         int ping = 0;
         for (int depth = 0; depth < max_depth; ++depth) {
