@@ -280,6 +280,13 @@ typedef struct nd_tuning {
     uint32_t refill_period;    /* the step budget of the persistent march and shadow passes: a wave takes
                                   this many march steps between refills of its idle lanes (a wave whose
                                   lanes are all idle refills at once); 0 = 1 */
+    uint32_t mc_pool_log2;     /* the path trace's path pool: a chunk of samples holds up to 2^k paths
+                                  (pixels x samples), k 16..27; 0 = the largest k in 24..27 whose pool
+                                  fits in a quarter of the device's memory (27 on a 288 GB device). A
+                                  larger pool means fewer chunks, each with one tail of bounce passes,
+                                  for about 200 B + 68 B per light of memory a path. Adaptive sampling
+                                  caps a chunk at half its minimum sample count. Results do not depend
+                                  on it beyond float summation order */
 } nd_tuning;
 
 nd_status nd_get_tuning(nd_ctx* ctx, nd_tuning* out);

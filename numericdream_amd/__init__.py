@@ -294,15 +294,16 @@ class Renderer:
         return t
 
     def set_tuning(self, waves_per_cu=0, refill_min_idle=8, chunk_rays=0, track_latency=False, graphs=True,
-                   round_cap=0, tile_width=0, refill_period=0, gbuffer_graphs=False):
+                   round_cap=0, tile_width=0, refill_period=0, gbuffer_graphs=False, mc_pool_log2=0):
         """round_cap > 0 stops every wave of a persistent geometry pass after that many rounds;
         the call then raises NdError(ND_E_TIMEOUT). tile_width: 4, 8, 16 or 32 pixels per wave64
         tile row (0 = 8); refill_period: march steps between refills (0 = 1). graphs: the path
         trace replays a captured hipGraph of each chunk's passes; gbuffer_graphs: the persistent
-        G-buffer passes are captured when a batch repeats the last launch record."""
+        G-buffer passes are captured when a batch repeats the last launch record; mc_pool_log2: the
+        path trace's pool of 2^k paths per chunk of samples (0 = sized by device memory)."""
         flags = ((abi.TUNE_TRACK_LATENCY if track_latency else 0) | (0 if graphs else abi.TUNE_NO_GRAPH)
                  | (abi.TUNE_GBUFFER_GRAPH if gbuffer_graphs else 0))
-        t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, flags, round_cap, tile_width, refill_period)
+        t = Tuning(waves_per_cu, refill_min_idle, chunk_rays, flags, round_cap, tile_width, refill_period, mc_pool_log2)
         self._check(lib.nd_set_tuning(self._h, C.byref(t)), "nd_set_tuning")
 
     def set_cancel(self, flag):

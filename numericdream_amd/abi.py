@@ -134,7 +134,8 @@ class BatchStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("waves_per_cu", U32), ("refill_min_idle", U32), ("chunk_rays", U32), ("flags", U32),
-                ("round_cap", U32), ("tile_width", U32), ("refill_period", U32)]
+                ("round_cap", U32), ("tile_width", U32), ("refill_period", U32),
+                ("mc_pool_log2", U32)]
 
 
 NOALLOC_MAX_RAYS = 1 << 19

@@ -389,6 +389,7 @@ nd_status nd_ctx_create(int hip_device, nd_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) {
         c->cu_count = prop.multiProcessorCount;
+        c->mem_total = prop.totalGlobalMem;
         std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     }
     int khz = 0;
@@ -461,6 +462,8 @@ nd_status nd_set_tuning(nd_ctx* c, const nd_tuning* t) {
     if (t->tile_width != 0 && t->tile_width != 4 && t->tile_width != 8 && t->tile_width != 16 && t->tile_width != 32)
         return nd_fail(c, ND_E_INVALID, "tile_width must be 0, 4, 8, 16 or 32");
     if (t->refill_period > 16) return nd_fail(c, ND_E_INVALID, "refill_period must be 0..16");
+    if (t->mc_pool_log2 != 0 && (t->mc_pool_log2 < 16 || t->mc_pool_log2 > 27))
+        return nd_fail(c, ND_E_INVALID, "mc_pool_log2 must be 0 or 16..27");
     c->tuning = *t;
     c->round_cap = t->round_cap ? t->round_cap : 0xffffffffu;
     return ND_OK;

@@ -33,6 +33,7 @@ struct nd_ctx {
     int code_map[ND_MAX_CODES];
     nd_tuning tuning{};
     int cu_count = 256;
+    size_t mem_total = 0;  // device memory in bytes (hipDeviceProp_t::totalGlobalMem)
     uint32_t resident[RES_COUNT] = {};  // resident waves per CU of each persistent kernel
     char arch[64] = {0};
     int wall_khz = 100000;

@@ -229,7 +229,7 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
     int cjob = 0;                          // wave-uniform job of the chunk position
     uint32_t cjstart = 1, cjend = 0;       // ray slots of cjob
     uint32_t hnext = 0, hend = 0;          // wave-uniform chunk of hit-list slots
-    bool work_left = true;
+    bool work_left = true, first_claim = true;
     bool active = false;
     uint32_t ray = 0;
     int job = 0, px = 0, py = 0;
@@ -258,9 +258,8 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
                 }
                 while (need != 0ull) {
                     if (cnext >= cend) {
-                        uint32_t base = 0;
-                        if (__lane_id() == 0) base = atomicAdd(&a.ctr->ray_next, a.chunk);
-                        base = uniform_u32(base);
+                        const uint32_t base = claim_chunk(&a.ctr->ray_next, a.total_rays, a.chunk, first_claim);
+                        first_claim = false;
                         if (base >= a.total_rays) { work_left = false; break; }
                         cnext = base;
                         cend = uniform_u32(min(base + a.chunk, a.total_rays));
@@ -364,10 +363,10 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
     // unused slots of the last hit chunk are marked empty for the later passes
     for (uint32_t i = hnext + __lane_id(); i < hend; i += 64) a.hb.ray[i] = ~0u;
     if (a.ray_done && lat_ray != ~0u) a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
-    if (__lane_id() == 0) {
+    if (__lane_id() == 0 && rounds_used) {  // a wave that marched nothing adds nothing
         atomicAdd(&a.ctr->march_rounds, (unsigned long long)rounds_used);
         atomicAdd(&a.ctr->de_march, de);
-        atomicAdd(&a.ctr->hits, nhits);
+        if (nhits) atomicAdd(&a.ctr->hits, nhits);
     }
 }
 
@@ -434,7 +433,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                                                  const DevScene* __restrict__ shared) {
     const uint32_t n_hits = a.ctr->hit_count;
     uint32_t cnext = 0, cend = 0;
-    bool work_left = true, active = false;
+    bool work_left = true, active = false, first_claim = true;
     uint32_t hit = 0;
     int job = 0, steps = 0, step = 0, sev = 0;  // step: march steps incl. accelerator skips; sev: DEs
     bool sacc = false;
@@ -457,9 +456,8 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
                 }
                 while (need != 0ull) {
                     if (cnext >= cend) {
-                        uint32_t base = 0;
-                        if (__lane_id() == 0) base = atomicAdd(&a.ctr->shadow_next, a.shadow_chunk);
-                        base = uniform_u32(base);
+                        const uint32_t base = claim_chunk(&a.ctr->shadow_next, n_hits, a.shadow_chunk, first_claim);
+                        first_claim = false;
                         if (base >= n_hits) { work_left = false; break; }
                         cnext = base;
                         cend = uniform_u32(min(base + a.shadow_chunk, n_hits));
@@ -536,7 +534,7 @@ __global__ void __launch_bounds__(64) k_shadow(PassArgs a, const DevJob* __restr
         }
     }
     if (a.ray_done && lat_ray != ~0u) a.ray_done[lat_ray] = (uint32_t)((unsigned long long)wall_clock64() - c0);
-    if (__lane_id() == 0) atomicAdd(&a.ctr->de_shadow, de);
+    if (__lane_id() == 0 && de) atomicAdd(&a.ctr->de_shadow, de);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {

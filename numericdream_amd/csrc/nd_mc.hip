@@ -582,7 +582,7 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
     AccelCache ac;  // every ray of the call reads the same grid: valid across rays
     accel_cache_reset(ac);
     uint32_t cnext = 0, cend = 0;
-    bool work_left = true, active = false;
+    bool work_left = true, active = false, first_claim = true;
     uint32_t item = 0;
     V3<float> org = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 1.f);
     March m;
@@ -598,9 +598,8 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
             if (__popcll(need) >= 8 || need == ~0ull) {
                 while (need != 0ull) {
                     if (cnext >= cend) {
-                        uint32_t base = 0;
-                        if (__lane_id() == 0) base = atomicAdd(head, a.chunk);
-                        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+                        const uint32_t base = claim_chunk(head, n, a.chunk, first_claim);
+                        first_claim = false;
                         if (base >= n) { work_left = false; break; }
                         cnext = base;
                         cend = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(base + a.chunk, n));
@@ -667,7 +666,7 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
         }
         de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(used));
     }
-    if (__lane_id() == 0) atomicAdd(de_ctr(a), de);
+    if (__lane_id() == 0 && de) atomicAdd(de_ctr(a), de);
 }
 
 struct ExtendRays {
@@ -1312,7 +1311,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a0, int ping, const DevS
         if (cont) qout[slot] = p;
     }
     for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off);
-    if (threadIdx.x == 0) atomicAdd(de_ctr(a), de);
+    if (threadIdx.x == 0 && de) atomicAdd(de_ctr(a), de);
 }
 
 // --- indirect shading: shade_local at the indirect hit, no shadows, ambient 0.15
@@ -1345,7 +1344,7 @@ __global__ void __launch_bounds__(64) k_mc_indirect_shade(McArgs a0, const DevSc
         for (int k = 0; k < 8; ++k) atomicAdd(&a.P.rad[8 * p + k], a.P.iw[8 * i + k] * L.v[k]);
     }
     for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off);
-    if (threadIdx.x == 0) atomicAdd(de_ctr(a), de);
+    if (threadIdx.x == 0 && de) atomicAdd(de_ctr(a), de);
 }
 
 // --- shadow: visibility_march (template.wgsl:2282-2342) with shadow_factor's parameters,
@@ -1365,7 +1364,7 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a0, const DevScene* __r
     const uint32_t n = a.P.ctr[C_SHADOW];
     const int max_steps = a.S->shadow_steps;
     uint32_t cnext = 0, cend = 0;
-    bool work_left = true, active = false;
+    bool work_left = true, active = false, first_claim = true;
     uint32_t item = 0;
     V3<float> so = mk3(0.f, 0.f, 0.f), ld = mk3(0.f, 0.f, 1.f);
     float t = 0.f, res = 1.f, ldist = 0.f, k = 0.f;
@@ -1380,9 +1379,8 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a0, const DevScene* __r
             if (__popcll(need) >= 8 || need == ~0ull) {
                 while (need != 0ull) {
                     if (cnext >= cend) {
-                        uint32_t base = 0;
-                        if (__lane_id() == 0) base = atomicAdd(a.P.ctr + C_HEAD_SH, a.chunk);
-                        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+                        const uint32_t base = claim_chunk(a.P.ctr + C_HEAD_SH, n, a.chunk, first_claim);
+                        first_claim = false;
                         if (base >= n) { work_left = false; break; }
                         cnext = base;
                         cend = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(base + a.chunk, n));
@@ -1469,7 +1467,7 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a0, const DevScene* __r
         }
         de += (unsigned long long)__popcll(act & ~__builtin_amdgcn_ballot_w64(skipped));
     }
-    if (__lane_id() == 0) atomicAdd(de_ctr(a), de);
+    if (__lane_id() == 0 && de) atomicAdd(de_ctr(a), de);
 }
 
 // --- resolve: per pixel, the average over this chunk's samples is added into out_rgb
@@ -2056,27 +2054,45 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     nd::McScene S;
     pack_mc_scene(u, S);
     const uint32_t npix = (uint32_t)job.rect_w * (uint32_t)job.rect_h;
-    // sample chunk: bound the path pool to about 2^24 paths
-    uint32_t chunk = std::max(1u, std::min(spp, (uint32_t)((1u << 24) / std::max(npix, 1u))));
-    const size_t P = (size_t)npix * chunk;
-    // device layout
-    size_t bytes = 0;
-    auto sz = [&](size_t b) { bytes += (b + 255) & ~(size_t)255; };
-    sz(sizeof(DevJob)); sz(sizeof(DevScene)); sz(sizeof(nd::McScene));
-    for (int i = 0; i < 6; ++i) sz(4 * P);  // pos, dir
-    sz(32 * P); sz(32 * P);                 // thr, rad
-    for (int i = 0; i < 4; ++i) sz(4 * P);  // hero, seeds, hit_t
-    sz(P); sz(P);                           // depth, flags
-    sz(4 * P); sz(2 * P);                   // lambda, inner
-    sz(4 * (size_t)npix); sz(4 * (size_t)npix); sz(4 * (size_t)npix);  // primary hit cache
-    sz(4 * P); sz(4 * P);                   // live queues
-    const size_t SQ = P * (size_t)std::max(S.light_count, 1);
-    sz(4 * SQ); for (int i = 0; i < 8; ++i) sz(4 * SQ); sz(32 * SQ);
-    sz(4 * P); for (int i = 0; i < 7; ++i) sz(4 * P); sz(P); sz(32 * P);
-    sz(256);
-    sz(sizeof(nd::McChunk) * (nd::MC_MAX_CHUNKS + 1));
+    // Sample chunk: the path pool holds up to 2^k paths (pixels x samples), k = mc_pool_log2, by
+    // default the largest k in 24..27 whose pool fits in a quarter of the device's memory: on a
+    // 288 GB device a 1080p frame of 64 samples is one chunk, with one tail of bounce passes. The
+    // shadow queue (paths x lights entries of 8 floats) keeps 32-bit float indexes, so paths x
+    // lights stays within 2^29. Adaptive sampling takes chunks of at most half its minimum sample
+    // count, so that pixels converged in one chunk start no paths in the next.
+    const size_t lights = (size_t)std::max(S.light_count, 1);
     const bool dev_out = out->device_ptrs != 0;
-    if (!dev_out) { sz(12 * (size_t)npix); sz(4 * (size_t)npix); sz(12 * (size_t)npix); }
+    auto layout_bytes = [&](size_t P) {
+        size_t bytes = 0;
+        auto sz = [&](size_t b) { bytes += (b + 255) & ~(size_t)255; };
+        sz(sizeof(DevJob)); sz(sizeof(DevScene)); sz(sizeof(nd::McScene));
+        for (int i = 0; i < 6; ++i) sz(4 * P);  // pos, dir
+        sz(32 * P); sz(32 * P);                 // thr, rad
+        for (int i = 0; i < 4; ++i) sz(4 * P);  // hero, seeds, hit_t
+        sz(P); sz(P);                           // depth, flags
+        sz(4 * P); sz(2 * P);                   // lambda, inner
+        sz(4 * (size_t)npix); sz(4 * (size_t)npix); sz(4 * (size_t)npix);  // primary hit cache
+        sz(4 * P); sz(4 * P);                   // live queues
+        const size_t SQ = P * lights;
+        sz(4 * SQ); for (int i = 0; i < 8; ++i) sz(4 * SQ); sz(32 * SQ);
+        sz(4 * P); for (int i = 0; i < 7; ++i) sz(4 * P); sz(P); sz(32 * P);
+        sz(256);
+        sz(sizeof(nd::McChunk) * (nd::MC_MAX_CHUNKS + 1));
+        if (!dev_out) { sz(12 * (size_t)npix); sz(4 * (size_t)npix); sz(12 * (size_t)npix); }
+        return bytes;
+    };
+    uint32_t pool_log2 = c->tuning.mc_pool_log2;
+    if (!pool_log2) {
+        pool_log2 = 24;
+        while (pool_log2 < 27 && layout_bytes((size_t)1 << (pool_log2 + 1)) <= c->mem_total / 4) ++pool_log2;
+    }
+    const size_t pool = std::min((size_t)1 << pool_log2, ((size_t)1 << 29) / lights);
+    uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp, pool / std::max(npix, 1u)));
+    if (S.adapt_on) chunk = std::min(chunk, std::max(1u, (uint32_t)(S.adapt_min_samples * 0.5f)));
+    const size_t P = (size_t)npix * chunk;
+    if (P * lights > ((size_t)1 << 29)) return nd_fail(c, ND_E_INVALID, "%u pixels x %zu lights exceed the shadow queue's 2^29 entries", npix, lights);
+    const size_t bytes = layout_bytes(P);
+    const size_t SQ = P * lights;
     if ((st = c->mc.ensure(c, bytes)) != ND_OK) return st;
     char* base = (char*)c->mc.p;
     auto carve = [&](size_t b) { char* r = base; base += (b + 255) & ~(size_t)255; return r; };

@@ -50,6 +50,10 @@ def test_argument_checks(renderer):
     with pytest.raises(nd.NdError) as e:
         renderer.set_tuning(refill_min_idle=0)
     assert e.value.status == abi.ND_E_INVALID
+    for k in (8, 28):
+        with pytest.raises(nd.NdError) as e:
+            renderer.set_tuning(mc_pool_log2=k)
+        assert e.value.status == abi.ND_E_INVALID
     renderer.set_tuning()
     # after every rejected call the context still renders
     st = renderer.render_gbuffer(u, g)
