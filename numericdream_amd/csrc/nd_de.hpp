@@ -277,7 +277,9 @@ template <class S> struct BoxT { S Sc_, m2, F2, L; };
 // Branch-free: the fold factor is selected, the reciprocal always computed (a zero r2 is
 // covered by the r2 < m2 arm), the rotation is a compile-time choice and the fold factor
 // is folded into the scale: z' = z (t S) + posc, dr' = dr (t |S|) + max(|cs|, 1).
-template <bool ROT, bool CLAMP>
+// F1: the fixed radius is exactly 1, so F2 / r2 is the reciprocal itself (1 x is exact: the
+// same results with one multiply less per iteration).
+template <bool ROT, bool CLAMP, bool F1 = false>
 NDD Carry<float> box_step(const Carry<float>& c, const V3<float>& posc, const BoxF& b, const float* rot_cs) {
     V3<float> z;
     z.x = fclamp(c.z.x, -b.L, b.L) * 2.f - c.z.x;
@@ -287,7 +289,7 @@ NDD Carry<float> box_step(const Carry<float>& c, const V3<float>& posc, const Bo
     // sphere_fold's three arms (template.wgsl:959-969) as one clamp: with m2 <= F2,
     // F2 / r2 > F2 / m2 exactly when r2 < m2 and F2 / r2 <= 1 exactly when r2 >= F2, so
     // t = clamp(F2 / r2, 1, F2 / m2). The host keeps m2 > F2 off this path (kF < 1).
-    float t = b.F2 * frcp(r2);
+    float t = F1 ? frcp(r2) : b.F2 * frcp(r2);
     if (CLAMP) {
         t = __builtin_amdgcn_fmed3f(t, 1.f, b.kF);
     } else {
@@ -298,11 +300,13 @@ NDD Carry<float> box_step(const Carry<float>& c, const V3<float>& posc, const Bo
     if (ROT) {
         z = rotate_cs(mk3(z.x * t, z.y * t, z.z * t), rot_cs);
         o.z = mk3(z.x * b.S + posc.x, z.y * b.S + posc.y, z.z * b.S + posc.z);
+        o.dr = c.dr * (t * b.absS) + b.dr_add;
     } else {
         const float ts = t * b.S;
         o.z = mk3(z.x * ts + posc.x, z.y * ts + posc.y, z.z * ts + posc.z);
+        // t > 0, so |t S| = t |S| bit for bit: the FMA takes it through its abs modifier
+        o.dr = c.dr * fabsf(ts) + b.dr_add;
     }
-    o.dr = c.dr * (t * b.absS) + b.dr_add;
     return o;
 }
 template <class S, class I> NDD Carry<S> box_step_t(const Carry<S>& c, const V3<S>& posc, const S& scale, const S& m2,
@@ -426,12 +430,12 @@ NDD float trap_finish(const TrapAcc& a, const DevInst& in, int iters) {
 
 // The Mandelbox iteration with the uniform choices (rotation, clamp form) taken outside
 // the loop.
-template <bool ROT, bool CLAMP, bool TRAP>
+template <bool ROT, bool CLAMP, bool TRAP, bool F1 = false>
 NDD Carry<float> box_loop(Carry<float> c, const V3<float>& posc, const BoxF& b, const DevInst& in, int iters,
                           TrapAcc& acc) {
     for (int it = 0; it < iters; ++it) {
         V3<float> zp = c.z;
-        c = box_step<ROT, CLAMP>(c, posc, b, in.rot_cs);
+        c = box_step<ROT, CLAMP, F1>(c, posc, b, in.rot_cs);
         if (TRAP) trap_update(acc, in, c.z, zp, it);
     }
     return c;
@@ -490,9 +494,11 @@ NDD float formula_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
             if (in.rot_on) {
                 if (clamp) c = box_loop<true, true, TRAP>(c, posc, b, in, iters, acc);
                 else c = box_loop<true, false, TRAP>(c, posc, b, in, iters, acc);
+            } else if (clamp) {
+                if (b.F2 == 1.f) c = box_loop<false, true, TRAP, true>(c, posc, b, in, iters, acc);
+                else c = box_loop<false, true, TRAP>(c, posc, b, in, iters, acc);
             } else {
-                if (clamp) c = box_loop<false, true, TRAP>(c, posc, b, in, iters, acc);
-                else c = box_loop<false, false, TRAP>(c, posc, b, in, iters, acc);
+                c = box_loop<false, false, TRAP>(c, posc, b, in, iters, acc);
             }
             if (TRAP) *trap = trap_finish(acc, in, iters);
             return fdiv(len3(c.z), fabsf(c.dr));
