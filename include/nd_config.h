@@ -56,6 +56,8 @@ void nd_set_pixel_epsilon(nd_uniforms* u, float fov_deg, float fraction);
  * res 64, levels 6, safety 0.92 (accel.zig:13-14, accel_state.zig:19). */
 void nd_accel_setup(nd_uniforms* u, uint32_t res, uint32_t levels, float safety);
 
+#define ND_DREAM_PATH_MAX 256
+
 /* What a .dream scene file asks of an export beyond its Uniforms block. */
 typedef struct nd_dream_info {
     uint32_t version;            /* the file's format version (at most 12) */
@@ -76,6 +78,10 @@ typedef struct nd_dream_info {
     uint32_t keyframes;          /* animation keys in the file (a still export does not apply them) */
     uint32_t n_codes;            /* formulas in the scene: bind them with nd_bind_formulas */
     int32_t code_to_builtin[ND_MAX_CODES];  /* code c -> nd_builtin_formula (-1 past n_codes) */
+    /* the image each formula names ("image" of a formula, scene_file.zig FormulaDto), by image
+     * slot: instance * 4 for its own formula, + 1 + j for mix-in j; "" for none. The host decodes
+     * each file and binds it with nd_bind_image, as the app's HeightMaps.sync does. */
+    char image_path[ND_IMAGE_SLOTS][ND_DREAM_PATH_MAX];
 } nd_dream_info;
 
 /* Builds the Uniforms block an export of a .dream scene file (JSON text, len bytes) renders:
@@ -88,8 +94,9 @@ typedef struct nd_dream_info {
  * Mandelbulb, otherwise the file's stem must be one of nd_builtin_formula's (with its @param
  * table compiled in; saved values override by name). Errors: ND_E_INVALID for malformed JSON,
  * a missing required field or a wrong type; ND_E_UNSUPPORTED for a file from a newer version;
- * ND_E_INVALID naming the field for warps, particle systems, image formulas, the FFT view or an
- * unknown formula file, which this library does not render. err (err_cap bytes, may be null)
+ * ND_E_INVALID naming the field for warps, particle systems, the FFT view or an unknown formula
+ * file, which this library does not render. Image formulas load; their image paths come back in
+ * info->image_path. err (err_cap bytes, may be null)
  * receives the message. */
 nd_status nd_scene_from_dream(const char* text, size_t len, uint32_t width, uint32_t height, nd_uniforms* out,
                               nd_dream_info* info, char* err, size_t err_cap);

@@ -115,9 +115,10 @@ typedef enum nd_builtin_formula {
     ND_FORMULA_FBMSDF = 50,            /* formulas/fbmsdf_primitive_carve.wgsl */
     ND_FORMULA_SEASCAPE = 51,          /* formulas/seascape_primitive.wgsl */
     ND_FORMULA_DRAGON = 52,            /* formulas/dragon_ifs_twist.wgsl */
-    ND_FORMULA_ROMANESCO = 53          /* formulas/romanesco_ifs.wgsl */
+    ND_FORMULA_ROMANESCO = 53,         /* formulas/romanesco_ifs.wgsl */
+    ND_FORMULA_HEIGHTMAP = 54          /* formulas/heightmap_primitive.wgsl */
 } nd_builtin_formula;
-#define ND_FORMULA_COUNT 54
+#define ND_FORMULA_COUNT 55
 
 #define ND_MAX_CODES 16
 
@@ -222,6 +223,35 @@ void* nd_ctx_stream(nd_ctx* ctx);
 /* code_to_builtin[c] is the nd_builtin_formula for FractalInstance.code == c (c < n <= 16).
  * The default binding is identity: code c -> builtin c for c < 4. */
 nd_status nd_bind_formulas(nd_ctx* ctx, const int32_t* code_to_builtin, uint32_t n);
+
+/* ---- height-map images (the @image formula, formulas/heightmap_primitive.wgsl) ----- */
+/* Replaces the reference's HeightMaps (src/gpu/height_maps.zig: sync / decode / upload) and
+ * the image_data binding its shader reads (template.wgsl:1110-1193). The host decodes the
+ * image file as the reference does with stb_image and passes the pixels; the library does
+ * everything after the pixel read. A slot is instance * 4 for an instance's own formula and
+ * instance * 4 + 1 + j for its mix-in j (IMAGE_SLOTS_PER_FRACTAL, template.wgsl:1341-1356). */
+#define ND_IMAGE_SLOTS 16
+#define ND_IMAGE_MAX_EDGE 2048u   /* larger images are box-downsampled by an integer factor */
+typedef enum nd_image_format {
+    ND_IMAGE_RGBA16 = 0,   /* 4 x uint16 per pixel, straight alpha (stbi_load_16 with 4 channels) */
+    ND_IMAGE_RGBA32F = 1   /* 4 x float per pixel (stbi_loadf with 4 channels, an HDR file): RGB is
+                              normalised by its finite minimum and maximum, alpha is taken as 1 */
+} nd_image_format;
+typedef struct nd_image_desc {
+    uint32_t width, height;   /* after downsampling */
+    uint32_t words;           /* packed words: 4 per texel */
+    float slope[4];           /* per channel: the largest change between neighbouring texels, x 1.001 */
+} nd_image_desc;
+/* Binds the image to a slot (pixels null or w / h 0 unbinds it). A heightmap formula in a
+ * slot without an image renders its flat base, as the reference does with no image loaded.
+ * ND_E_INVALID for a slot out of range or an empty or oversized (> 2^28 pixels) image. */
+nd_status nd_bind_image(nd_ctx* ctx, uint32_t slot, nd_image_format fmt, const void* pixels, uint32_t w,
+                        uint32_t h);
+/* The texel words a bound image becomes (host only, no context): premultiplied RGBA16 in two
+ * words, then the slopes dilated over 8 + 2 and 64 + 2 texels as 8-bit codes per channel
+ * (height_maps.zig: decode). words null: only desc is filled (a size query). */
+nd_status nd_image_pack(nd_image_format fmt, const void* pixels, uint32_t w, uint32_t h, uint32_t* words,
+                        size_t words_cap, nd_image_desc* desc);
 
 /* ---- geometry passes ----------------------------------------------------------- */
 /* One frame (or the rect of it) of depth / normal / trap / hit / shadow. */

@@ -52,6 +52,8 @@ def _load():
     lib.nd_ctx_stream.argtypes = [vp]
     lib.nd_ctx_stream.restype = vp
     lib.nd_bind_formulas.argtypes = [vp, P(C.c_int32), u32]
+    lib.nd_bind_image.argtypes = [vp, u32, i32, vp, u32, u32]
+    lib.nd_image_pack.argtypes = [i32, vp, u32, u32, vp, C.c_size_t, P(abi.ImageDesc)]
     lib.nd_render_gbuffer.argtypes = [vp, vp, C.c_size_t, P(MarchOpts), Rect, P(GBufferOut), P(BatchStats)]
     lib.nd_render_gbuffer_noalloc.argtypes = [vp, vp, C.c_size_t, P(MarchOpts), Rect, P(GBufferOut), P(BatchStats)]
     lib.nd_render_batch.argtypes = [vp, P(Job), u32, P(MarchOpts), P(BatchStats)]
@@ -168,6 +170,32 @@ def _ptr(a):
     return a.ctypes.data
 
 
+def _image_array(pixels):
+    a = np.ascontiguousarray(pixels)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("an image is H x W x 4")
+    if a.dtype == np.uint16:
+        return a, abi.IMAGE_RGBA16
+    if a.dtype == np.float32:
+        return a, abi.IMAGE_RGBA32F
+    raise ValueError("image pixels are uint16 or float32")
+
+
+def image_pack(pixels):
+    """The texel words nd_bind_image makes of an image (host only): (words as uint32 array,
+    width, height, per-channel slope bound)."""
+    a, fmt = _image_array(pixels)
+    d = abi.ImageDesc()
+    st = lib.nd_image_pack(fmt, a.ctypes.data, a.shape[1], a.shape[0], None, 0, C.byref(d))
+    if st != abi.ND_OK:
+        raise NdError(st, "nd_image_pack")
+    words = np.zeros(d.words, dtype=np.uint32)
+    st = lib.nd_image_pack(fmt, a.ctypes.data, a.shape[1], a.shape[0], words.ctypes.data, words.size, C.byref(d))
+    if st != abi.ND_OK:
+        raise NdError(st, "nd_image_pack")
+    return words, d.width, d.height, np.array(d.slope[:], dtype=np.float32)
+
+
 class GBuffer:
     """Host (numpy) or device (torch) G-buffer arrays for a w x h rect."""
 
@@ -243,6 +271,15 @@ class Renderer:
     def bind_formulas(self, code_to_builtin):
         arr = (C.c_int32 * len(code_to_builtin))(*code_to_builtin)
         self._check(lib.nd_bind_formulas(self._h, arr, len(code_to_builtin)), "nd_bind_formulas")
+
+    def bind_image(self, slot, pixels):
+        """Binds a decoded image (H x W x 4: uint16 straight-alpha RGBA, or float32 HDR RGBA) to
+        image slot `slot` (instance * 4, + 1 + j for mix-in j); None unbinds the slot."""
+        if pixels is None:
+            self._check(lib.nd_bind_image(self._h, slot, 0, None, 0, 0), "nd_bind_image")
+            return
+        a, fmt = _image_array(pixels)
+        self._check(lib.nd_bind_image(self._h, slot, fmt, a.ctypes.data, a.shape[1], a.shape[0]), "nd_bind_image")
 
     def render_gbuffer(self, u, gbuf, rect=None, normal_mode=abi.NORMAL_ANALYTIC, shadows=True, shadow_max_steps=0,
                        kernel=abi.KERNEL_PERSISTENT):

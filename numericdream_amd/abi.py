@@ -188,7 +188,7 @@ FORMULA_FILES = [
         "mandelbulbdual_hybrid_bulb", "quatjulia_bulb_julia", "bulbbox_hybrid_bulb_box", "mandalay_box_fold",
         "wobblekifs_ifs_twist_julia", "radioactive_ifs_carve_twist", "recursivesphere_ifs_carve_twist", "tree_ifs",
         "kleinianlimit_kleinian", "fbmsdf_primitive_carve", "seascape_primitive", "dragon_ifs_twist",
-        "romanesco_ifs")]
+        "romanesco_ifs", "heightmap_primitive")]
 FORMULA_COUNT = len(FORMULA_FILES)
 PARAM_COUNT = 6
 
@@ -208,11 +208,22 @@ EXPORTS = ["nd_ctx_create", "nd_ctx_destroy", "nd_last_error", "nd_sync", "nd_se
            "nd_bind_formulas", "nd_render_gbuffer", "nd_render_gbuffer_noalloc", "nd_render_batch", "nd_batch_job_latency",
            "nd_get_tuning", "nd_set_tuning", "nd_autotune", "nd_adjoint", "nd_path_trace", "nd_mesh_points", "nd_mesh_probe", "nd_accel_probe", "nd_accel_setup",
            "nd_config_job_count", "nd_uniforms_defaults", "nd_config_build", "nd_camera_look_at",
-           "nd_set_pixel_epsilon", "nd_scene_from_dream", "nd_radiance_adjoint"]
+           "nd_set_pixel_epsilon", "nd_scene_from_dream", "nd_radiance_adjoint", "nd_bind_image", "nd_image_pack"]
+
+# height-map images (nd_bind_image / nd_image_pack)
+IMAGE_RGBA16 = 0
+IMAGE_RGBA32F = 1
+IMAGE_SLOTS = 16
+IMAGE_MAX_EDGE = 2048
+
+
+class ImageDesc(C.Structure):
+    _fields_ = [("width", U32), ("height", U32), ("words", U32), ("slope", F * 4)]
 
 
 class DreamInfo(C.Structure):
     _fields_ = [("version", U32), ("export_width", C.c_int32), ("export_height", C.c_int32), ("width", U32),
                 ("height", U32), ("mc_enabled", U32), ("mc_export_samples", F), ("mc_max_samples", F),
                 ("photons_wanted", U32), ("sky_image", U32), ("stereo", U32), ("screen_shaders", U32),
-                ("keyframes", U32), ("n_codes", U32), ("code_to_builtin", C.c_int32 * 16)]
+                ("keyframes", U32), ("n_codes", U32), ("code_to_builtin", C.c_int32 * 16),
+                ("image_path", (C.c_char * 256) * 16)]

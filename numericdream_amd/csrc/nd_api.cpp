@@ -129,6 +129,7 @@ static const FormulaSlots kSlots[ND_FORMULA_COUNT] = {
     {-1, -1, -1, -1, -1},  // seascape_primitive.wgsl:22-24
     {-1, -1, -1, -1, -1},  // dragon_ifs_twist.wgsl:13-15
     {0, -1, -1, -1, -1},   // romanesco_ifs.wgsl:24-26
+    {-1, -1, -1, -1, -1},  // heightmap_primitive.wgsl:10-12 (primitive_de_iterations: 1)
 };
 
 // I is DevInst or DevMixin (the same formula fields).
@@ -251,6 +252,8 @@ nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
         in.trap_radius = fi.trap_radius;
         in.trap_tube = fi.trap_tube;
         hoist(in);
+        // image slots: instance * 4 for its own formula, + 1 + j for mix-in j (template.wgsl:1341-1356)
+        if (in.formula == ND_FORMULA_HEIGHTMAP) nd_image_hoist(c, 4 * i, in.h);
         // hybrid mix-ins (eval_mixed, template.wgsl:1361-1389; hybrid_cycle_len, :1332-1338)
         in.hybrid = fi.mixin_count > 0.5f;
         if (in.hybrid) {
@@ -269,6 +272,7 @@ nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
             base.iters = in.hyb_base_n;
             for (int k = 0; k < 16; ++k) { base.p[k] = in.p[k]; base.dparam[k] = -1; }
             hoist(base);
+            if (base.formula == ND_FORMULA_HEIGHTMAP) nd_image_hoist(c, 4 * i, base.h);
             for (int j = 0; j < in.mix_count; ++j) {
                 const nd_mixin_params& mp = fi.mixins[j];
                 DevMixin& m = in.mix[1 + j];
@@ -280,6 +284,7 @@ nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s) {
                 cycle += m.iters;
                 for (int k = 0; k < 16; ++k) { m.p[k] = mp.params[k]; m.dparam[k] = -1; }
                 hoist(m);
+                if (m.formula == ND_FORMULA_HEIGHTMAP) nd_image_hoist(c, 4 * i + 1 + j, m.h);
             }
             in.hyb_cycle = std::max(cycle, 1);
         }
@@ -416,6 +421,7 @@ void nd_ctx_destroy(nd_ctx* c) {
     c->jobs.release(); c->job_base.release(); c->ctr.release(); c->hits.release(); c->shared.release();
     c->staging.release(); c->latency.release(); c->adj.release(); c->mc.release();
     c->accel.release(); c->accel_scenes.release(); c->mc_adapt.release(); c->color_scenes.release();
+    c->image_buf.release();
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -26,6 +26,12 @@ struct NdAccelEntry {
     size_t count;
 };
 
+// A height-map image bound to a slot: its packed texel words (nd_image.cpp).
+struct NdImage {
+    std::vector<uint32_t> words;
+    nd_image_desc desc{};
+};
+
 struct nd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -61,6 +67,9 @@ struct nd_ctx {
     uint32_t last_rays = 0;         // rays of the last persistent batch
     DevHitBuffers last_hb{};        // hit list of the last persistent batch
     std::vector<DevJob> host_jobs;
+    NdImage images[ND_IMAGE_SLOTS];  // bound height-map images
+    DevBuf image_buf;                // their device layout: 16 slot headers of 8 words, then texels
+    std::vector<uint32_t> image_words;  // host copy of image_buf
     std::vector<float> job_latency_ms;
 };
 
@@ -90,6 +99,8 @@ nd_status nd_batch_impl(nd_ctx* c, const nd_job* jobs, uint32_t n, const nd_marc
 nd_status nd_fail(nd_ctx* c, nd_status st, const char* fmt, ...);
 nd_status nd_hip_fail(nd_ctx* c, hipError_t e, const char* what);
 nd_status nd_pack_scene(nd_ctx* c, const nd_uniforms& u, DevScene& s);
+// A heightmap formula's image fields (h[0..8]) for image slot `slot`.
+void nd_image_hoist(const nd_ctx* c, int slot, float* h);
 // Some visible instance uses a formula beyond the four tuned ones.
 bool nd_scene_ext(const DevScene& s);
 nd_status nd_pack_job(nd_ctx* c, const nd_uniforms& u, const nd_march_opts& o, nd_rect rect, DevJob& j);

@@ -1215,6 +1215,117 @@ struct FRomanesco {
     ND_FIN { return with_value(c.dr, __uint_as_float(sbits(c.dr) & ~16383u)); }
 };
 
+// ============================================================================= image
+// formulas/heightmap_primitive.wgsl:10-64 with image_info / image_texel / image_sample
+// (template.wgsl:1110-1193). The host packer puts the slot's image into h[]: h[0] texel
+// offset (word index), h[1] width (0: no image bound), h[2] height, h[3..6] the per-channel
+// slope bound, h[7..8] the image buffer's device address (nd_image_hoist).
+struct ImgInfo {
+    const uint32_t* buf;
+    uint32_t base, w, h;
+    float slope[4];
+};
+template <class I> NDD ImgInfo img_info(const I& in) {
+    ImgInfo f;
+    f.base = __float_as_uint(in.h[0]);
+    f.w = __float_as_uint(in.h[1]);
+    f.h = __float_as_uint(in.h[2]);
+    for (int c = 0; c < 4; ++c) f.slope[c] = in.h[3 + c];
+    const uint64_t lo = __float_as_uint(in.h[7]), hi = __float_as_uint(in.h[8]);
+    f.buf = (const uint32_t*)(uintptr_t)(lo | (hi << 32));
+    if (!f.buf) f.w = 0;
+    return f;
+}
+// one texel's channels and its two dilated slope bounds (8-bit codes, 16 steps per octave
+// below 2^1; code 0 is 0)
+struct ImgTexel { float v[4], nr[4], fr[4]; };
+NDD float img_slope(uint32_t code) {
+    return code == 0u ? 0.f : fexp2(((float)code - 255.f) * (1.f / 16.f) + 1.f);
+}
+NDD ImgTexel img_texel(const ImgInfo& f, uint32_t x, uint32_t y) {
+    const uint32_t* t = f.buf + f.base + (y * f.w + x) * 4u;
+    const uint32_t rg = t[0], ba = t[1], nw = t[2], fw = t[3];
+    ImgTexel o;
+    const uint32_t q[4] = {rg & 0xffffu, rg >> 16, ba & 0xffffu, ba >> 16};
+    for (int c = 0; c < 4; ++c) {
+        o.v[c] = (float)q[c] * (1.f / 65535.f);
+        o.nr[c] = img_slope((nw >> (8 * c)) & 0xffu);
+        o.fr[c] = img_slope((fw >> (8 * c)) & 0xffu);
+    }
+    return o;
+}
+template <class S> struct ImgSample { S v[4], nr[4], fr[4]; };
+// bilinear at uv in [0, 1]^2, clamped at the edges: the weights carry uv's tangents
+template <class S> NDD ImgSample<S> img_sample(const ImgInfo& f, const S& u, const S& v) {
+    ImgSample<S> o;
+    for (int c = 0; c < 4; ++c) { o.v[c] = C<S>(0.f); o.nr[c] = C<S>(0.f); o.fr[c] = C<S>(0.f); }
+    if (f.w == 0u || f.h == 0u) return o;
+    const uint32_t lx = f.w - 1u, ly = f.h - 1u;
+    const S tx = sclamp(u * (float)f.w - 0.5f, C<S>(0.f), C<S>((float)lx));
+    const S ty = sclamp(v * (float)f.h - 0.5f, C<S>(0.f), C<S>((float)ly));
+    const uint32_t x0 = min((uint32_t)vv(tx), lx), y0 = min((uint32_t)vv(ty), ly);
+    const uint32_t x1 = min(x0 + 1u, lx), y1 = min(y0 + 1u, ly);
+    const S fx = tx - (float)x0, fy = ty - (float)y0;
+    const S gx = C<S>(1.f) - fx, gy = C<S>(1.f) - fy;
+    const ImgTexel a = img_texel(f, x0, y0), b = img_texel(f, x1, y0), cc = img_texel(f, x0, y1), d = img_texel(f, x1, y1);
+    // mix(mix(a, b, fx), mix(c, d, fx), fy), mix(x, y, t) = x (1 - t) + y t
+    for (int c = 0; c < 4; ++c) {
+        o.v[c] = (gx * a.v[c] + fx * b.v[c]) * gy + (gx * cc.v[c] + fx * d.v[c]) * fy;
+        o.nr[c] = (gx * a.nr[c] + fx * b.nr[c]) * gy + (gx * cc.nr[c] + fx * d.nr[c]) * fy;
+        o.fr[c] = (gx * a.fr[c] + fx * b.fr[c]) * gy + (gx * cc.fr[c] + fx * d.fr[c]) * fy;
+    }
+    return o;
+}
+struct FHeightmap {
+    ND_STEP {
+        (void)pos;
+        const float size = fmaxf(in.p[0], K_EPS);
+        const V3<S> q = c.z * sdiv(C<S>(1.f), smax_(sabs(c.dr), C<S>(K_EPS_FINE)) * size);
+        const float lift = in.p[1], base = fmaxf(in.p[2], 0.f);
+        const bool up = lift >= 0.f;
+        float k[4] = {in.p[3], in.p[4], in.p[5], in.p[6]};
+        if (in.p[7] > 0.5f) {  // invert: (-w.xyz, dot(w, 1))
+            const float sum = k[0] + k[1] + k[2] + k[3];
+            k[0] = -k[0]; k[1] = -k[1]; k[2] = -k[2]; k[3] = sum;
+        }
+        const ImgInfo info = img_info(in);
+        float half_x = 1.f, half_y = 1.f, texel = 0.f, h_lo = 0.f, h_hi = 0.f;
+        ImgSample<S> sm;
+        for (int ch = 0; ch < 4; ++ch) { sm.v[ch] = C<S>(0.f); sm.nr[ch] = C<S>(0.f); sm.fr[ch] = C<S>(0.f); }
+        if (info.w > 0u) {
+            const float mx = fmaxf((float)info.w, (float)info.h);
+            half_x = (float)info.w / mx;
+            half_y = (float)info.h / mx;
+            texel = 2.f / mx;
+            sm = img_sample(info, q.x * (1.f / half_x) * 0.5f + 0.5f, q.z * (1.f / half_y) * 0.5f + 0.5f);
+            h_lo = fminf(0.f, k[3] + fminf(k[0], 0.f) + fminf(k[1], 0.f) + fminf(k[2], 0.f));
+            h_hi = fmaxf(0.f, k[3] + fmaxf(k[0], 0.f) + fmaxf(k[1], 0.f) + fmaxf(k[2], 0.f));
+        }
+        const S top = (sm.v[0] * k[0] + sm.v[1] * k[1] + sm.v[2] * k[2] + sm.v[3] * k[3]) * lift;
+        const S dy = up ? q.y - top : top - q.y;
+        const S flat_side = up ? C<S>(-base) - q.y : q.y - base;
+        const float ak[4] = {fabsf(k[0]), fabsf(k[1]), fabsf(k[2]), fabsf(k[3])};
+        const float per_texel = fabsf(lift) / fmaxf(texel, K_EPS_FINE);
+        const float gx = per_texel * (ak[0] * info.slope[0] + ak[1] * info.slope[1] + ak[2] * info.slope[2] + ak[3] * info.slope[3]);
+        const S gn = (sm.nr[0] * ak[0] + sm.nr[1] * ak[1] + sm.nr[2] * ak[2] + sm.nr[3] * ak[3]) * per_texel;
+        const S gf = (sm.fr[0] * ak[0] + sm.fr[1] * ak[1] + sm.fr[2] * ak[2] + sm.fr[3] * ak[3]) * per_texel;
+        const S cx = dy * (1.f / sqrtf(1.f + gx * gx));
+        const S cn = sdiv(dy, ssqrt(gn * gn + 1.f)), cf = sdiv(dy, ssqrt(gf * gf + 1.f));
+        const float nearb = 8.f * texel, farb = 64.f * texel;
+        const S c_near = sclamp(cn, C<S>(-nearb), C<S>(nearb)), c_far = sclamp(cf, C<S>(-farb), C<S>(farb));
+        const S curved = vv(dy) >= 0.f ? smax_(cx, smax_(c_near, c_far)) : smin_(cx, smin_(c_near, c_far));
+        const float t_min = fminf(lift * h_lo, lift * h_hi), t_max = fmaxf(lift * h_lo, lift * h_hi);
+        const float lo = up ? -base : fminf(t_min, base);
+        const float hi = up ? fmaxf(t_max, -base) : base;
+        const V3<S> e = mk3(sabs(q.x) - half_x, smax_(C<S>(lo) - q.y, q.y - hi), sabs(q.z) - half_y);
+        const S bounds = len3(vmax0(e)) + smin_(smax_(e.x, smax_(e.y, e.z)), C<S>(0.f));
+        const S d = smax_(smax_(curved, flat_side), bounds);
+        const S elev = sclamp((q.y - lo) * (1.f / fmaxf(hi - lo, K_EPS)), C<S>(0.f), C<S>(1.f));
+        ND_RET(mk3(C<S>(0.f), elev, C<S>(0.f)), d * size);
+    }
+    ND_FIN { return c.dr; }
+};
+
 #undef ND_STEP
 #undef ND_FIN
 #undef ND_RET
@@ -1301,7 +1412,8 @@ NDD S ext_run_d(const DevInst& in, const V3<S>& pos) {
     X(ND_FORMULA_FBMSDF, FFbmSdf)                                          \
     X(ND_FORMULA_SEASCAPE, FSeascape)                                      \
     X(ND_FORMULA_DRAGON, FDragon)                                          \
-    X(ND_FORMULA_ROMANESCO, FRomanesco)
+    X(ND_FORMULA_ROMANESCO, FRomanesco)                                    \
+    X(ND_FORMULA_HEIGHTMAP, FHeightmap)
 
 // ----------------------------------------------------------------------------- hybrids
 // One de_step of any formula (eval_step, fr.zig:515-539), parameters from an instance or

@@ -382,7 +382,10 @@ void color_stop(Reader& R, const JVal& c, const std::string& p, nd_color_stop* s
 
 // InstanceDto (scene_file.zig:151-193) -> FractalInstanceState -> FractalInstance (toGpu,
 // scene_state.zig:278-327), formula codes stamped as the renderer does.
-void instance(Reader& R, const JVal& in, const std::string& p, Codes& codes, nd_fractal_instance* g) {
+// images[0] receives the path of the instance formula's image (its "image" field), images[1 + j]
+// mix-in j's: the host loads them into image slots instance * 4 and instance * 4 + 1 + j.
+void instance(Reader& R, const JVal& in, const std::string& p, Codes& codes, nd_fractal_instance* g,
+              std::string images[1 + ND_MAX_MIXINS]) {
     std::memset(g, 0, sizeof(*g));
     float offset[3] = {0, 0, 0}, scale[3] = {1, 1, 1}, rot[3] = {0, 0, 0};
     R.vec3(in, p, "offset", true, offset);
@@ -419,7 +422,7 @@ void instance(Reader& R, const JVal& in, const std::string& p, Codes& codes, nd_
         const std::string fp = p + ".formula";
         const std::string path = R.string(*fm, fp, "path", true, "");
         const JVal* params = R.array(*fm, fp, "params", true);
-        if (!R.string(*fm, fp, "image", false, "").empty()) R.fail(fp + ".image", "image-driven formulas are not implemented");
+        images[0] = R.string(*fm, fp, "image", false, "");
         const FormulaRow* f = formula_of(path);
         if (!f) { R.fail(fp + ".path", ("no bound formula for '" + path + "'").c_str()); return; }
         formula_params(R, *f, params, fp, g->params);
@@ -443,7 +446,7 @@ void instance(Reader& R, const JVal& in, const std::string& p, Codes& codes, nd_
             const std::string mfp = mp + ".formula";
             const std::string mpath = R.string(*mf, mfp, "path", true, "");
             const JVal* mparams = R.array(*mf, mfp, "params", true);
-            if (!R.string(*mf, mfp, "image", false, "").empty()) R.fail(mfp + ".image", "image-driven formulas are not implemented");
+            images[1 + j] = R.string(*mf, mfp, "image", false, "");
             const FormulaRow* mfr = formula_of(mpath);
             if (!mfr) { R.fail(mfp + ".path", ("no bound formula for '" + mpath + "'").c_str()); return; }
             formula_params(R, *mfr, mparams, mfp, g->mixins[j].params);
@@ -791,10 +794,18 @@ extern "C" nd_status nd_scene_from_dream(const char* text, size_t len, uint32_t 
         if (i >= ni) {  // parsed like the others, then dropped (applyScene keeps max_instances)
             Codes spare;
             nd_fractal_instance g;
-            instance(R, in, idx(S, "instances", i), spare, &g);
+            std::string spare_images[1 + ND_MAX_MIXINS];
+            instance(R, in, idx(S, "instances", i), spare, &g, spare_images);
             continue;
         }
-        instance(R, in, idx(S, "instances", i), codes, &u->instances[i]);
+        std::string images[1 + ND_MAX_MIXINS];
+        instance(R, in, idx(S, "instances", i), codes, &u->instances[i], images);
+        for (int k = 0; k < 1 + ND_MAX_MIXINS; ++k) {
+            if (images[k].size() >= ND_DREAM_PATH_MAX)
+                R.fail(idx(S, "instances", i) + ".formula.image", "image path longer than ND_DREAM_PATH_MAX - 1 bytes");
+            else
+                std::memcpy(di.image_path[4 * i + k], images[k].c_str(), images[k].size() + 1);
+        }
         const nd_fractal_instance& g = u->instances[i];
         if (g.hidden <= 0.5f)
             for (int k = 0; k < (int)g.color_count; ++k)

@@ -47,6 +47,8 @@ def lib():
         L.ndo_flops_per_de.restype = C.c_double
         L.ndo_helper.argtypes = [i32, vp, vp]
         L.ndo_formula_step.argtypes = [i32, vp, C.c_double, vp, vp, vp]
+        L.ndo_bind_image.argtypes = [i32, vp, i32, i32, i32, vp]
+        L.ndo_image_sample.argtypes = [i32, vp, i32, vp]
         _lib = L
     return _lib
 
@@ -303,3 +305,23 @@ def render_mc(u, spp=1, rect=None, prec=0, threads=None, code_map=None, per_samp
     if per_sample:
         out["samples"] = samp
     return out
+
+
+def bind_image(slot, words=None, w=0, h=0, slope=(0.0, 0.0, 0.0, 0.0)):
+    """The oracle's copy of a height-map image slot: the packed texel words (4 per texel),
+    size and per-channel slope bound; words None empties the slot."""
+    L = lib()
+    if words is None:
+        assert L.ndo_bind_image(slot, None, 0, 0, 0, None) == 0
+        return
+    wd = np.ascontiguousarray(words, dtype=np.uint32)
+    sl = np.ascontiguousarray(slope, dtype=np.float64)
+    assert L.ndo_bind_image(slot, wd.ctypes.data, wd.size, int(w), int(h), sl.ctypes.data) == 0
+
+
+def image_sample(slot, uv):
+    """Bilinear image_sample of slot at points uv (n x 2): (value, near, far), each n x 4."""
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((len(uv), 12))
+    assert lib().ndo_image_sample(slot, uv.ctypes.data, len(uv), out.ctypes.data) == 0
+    return out[:, 0:4], out[:, 4:8], out[:, 8:12]

@@ -160,6 +160,37 @@ extern "C" {
 
 int ndo_abi_size(void) { return (int)sizeof(nd_uniforms); }
 
+// image_sample (template.wgsl:1180-1193) of the image in `slot` at n points (u, v): 12
+// doubles per point (value, near, far, 4 channels each).
+int ndo_image_sample(int slot, const double* uv, int n, double* out) {
+    if (slot < 0 || slot >= 16) return 1;
+    const ndo::OImage& im = ndo::oracle_image(slot);
+    for (int i = 0; i < n; ++i) {
+        const ndo::OImageSample<double> s = ndo::image_sample(im, uv[2 * i], uv[2 * i + 1]);
+        for (int c = 0; c < 4; ++c) {
+            out[12 * i + c] = s.v[c];
+            out[12 * i + 4 + c] = s.nr[c];
+            out[12 * i + 8 + c] = s.fr[c];
+        }
+    }
+    return 0;
+}
+
+// A height-map image for the heightmap formula in slot `slot` (0..15): n_words packed texel
+// words (4 per texel, as nd_image_pack makes them), its size and per-channel slope bound.
+// words null or w 0: the slot holds no image.
+int ndo_bind_image(int slot, const uint32_t* words, int n_words, int w, int h, const double* slope) {
+    if (slot < 0 || slot >= 16) return 1;
+    ndo::OImage& im = ndo::oracle_image(slot);
+    if (!words || w <= 0 || h <= 0) { im = ndo::OImage(); return 0; }
+    if ((long)n_words != 4L * w * h) return 1;
+    im.words.assign(words, words + n_words);
+    im.w = (uint32_t)w;
+    im.h = (uint32_t)h;
+    for (int c = 0; c < 4; ++c) im.slope[c] = slope[c];
+    return 0;
+}
+
 int ndo_scene_de(const nd_uniforms* u, const int32_t* cm, int prec, const double* p, int n, double* d_out,
                  double* trap_out) {
     if (prec == 0) {

@@ -291,7 +291,9 @@ template <class T, class P> V3<T> rotate_xyz_deg(V3<T> z, P dx, P dy, P dz) {
 
 // ----------------------------------------------------------------------------- formulas
 // Parameters of one formula instance (p = the 16 @param slots, in P).
-template <class P> struct Params { P p[16]; };
+// p: the 16 @param slots; image: the image slot the formula samples (instance * 4 for its
+// own formula, + 1 + j for mix-in j: IMAGE_SLOTS_PER_FRACTAL, template.wgsl:1341-1356)
+template <class P> struct Params { P p[16]; int image = -1; };
 
 // mandelbrot.wgsl:1-13
 template <class P> int bulb_iterations(const Params<P>& p) { return i32(val(p.p[1])); }
@@ -491,11 +493,13 @@ template <class Q, class P> Scene<Q> recast_scene(const Scene<P>& s) {
         b.step_safety = a.step_safety;
         b.trap = a.trap;
         for (int k = 0; k < 16; ++k) b.p.p[k] = Q((double)val(a.p.p[k]));
+        b.p.image = a.p.image;
         b.hybrid = a.hybrid; b.mix_count = a.mix_count; b.base_n = a.base_n; b.total = a.total;
         for (int j = 0; j < ND_MAX_MIXINS; ++j) {
             b.mix_formula[j] = a.mix_formula[j];
             b.mix_iters[j] = a.mix_iters[j];
             for (int k = 0; k < 16; ++k) b.mix_p[j].p[k] = Q((double)val(a.mix_p[j].p[k]));
+            b.mix_p[j].image = a.mix_p[j].image;
         }
     }
     return r;
@@ -519,6 +523,7 @@ template <class P> Scene<P> decode_scene(const nd_uniforms& u, const CodeMap& cm
         }
         in.step_safety = fi.step_safety;
         for (int k = 0; k < 16; ++k) in.p.p[k] = P(fi.params[k]);
+        in.p.image = 4 * i;  // g_image_slot = slot * IMAGE_SLOTS_PER_FRACTAL (fr.zig:494)
         for (int k = 0; k < 3; ++k) { in.trap.center[k] = fi.trap_center[k]; in.trap.box[k] = fi.trap_box[k]; }
         in.trap.shape = i32(fi.trap_shape + 0.5f);  // orbit_trap (template.wgsl:1216-1225)
         in.trap.mode = i32(fi.trap_mode + 0.5f);
@@ -532,6 +537,7 @@ template <class P> Scene<P> decode_scene(const nd_uniforms& u, const CodeMap& cm
             in.mix_formula[j] = cm.lookup(i32(fi.mixins[j].code));  // mixin_code (:1321-1323)
             in.mix_iters[j] = std::max(i32(fi.mixins[j].iterations), 0);
             for (int k = 0; k < 16; ++k) in.mix_p[j].p[k] = P(fi.mixins[j].params[k]);
+            in.mix_p[j].image = 4 * i + 1 + j;  // st.image_slot += 1 + j (template.wgsl:1352)
         }
     }
     s.carve_count = std::min(i32(u.carve_count), ND_MAX_CARVES);

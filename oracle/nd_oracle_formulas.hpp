@@ -24,7 +24,7 @@ enum {
     F_MANDELBOXTWIST, F_PRISMKIFS, F_BUFFALO, F_BUFFALOJULIA, F_BUFFALOTWIST, F_APOLLONIAN, F_APOLLONIANJULIA,
     F_KLEINIANJULIA, F_PSEUDOKLEINIAN, F_JULIABULB, F_MANDELBULBMIRROR, F_MANDELBULBTWIST, F_MANDELBULBDUAL,
     F_QUATJULIA, F_BULBBOX, F_MANDALAY, F_WOBBLEKIFS, F_RADIOACTIVE, F_RECURSIVESPHERE, F_TREE,
-    F_KLEINIANLIMIT, F_FBMSDF, F_SEASCAPE, F_DRAGON, F_ROMANESCO, F_COUNT
+    F_KLEINIANLIMIT, F_FBMSDF, F_SEASCAPE, F_DRAGON, F_ROMANESCO, F_HEIGHTMAP, F_COUNT
 };
 
 constexpr double TAU = 6.28318530718;   // template.wgsl:60
@@ -934,6 +934,107 @@ struct OSeascape {
     O_FIN { return c.dr; }
 };
 
+// formulas/heightmap_primitive.wgsl:10-64 with image_info / image_texel / image_sample
+// (template.wgsl:1110-1193), reading the slot's packed texel words (the same words the
+// library's nd_image_pack makes; bound with ndo_bind_image).
+struct OImage {
+    std::vector<uint32_t> words;  // 4 per texel: RG, BA (16-bit pairs), near / far slope codes
+    uint32_t w = 0, h = 0;
+    double slope[4] = {0, 0, 0, 0};
+};
+inline OImage& oracle_image(int slot) {
+    static OImage images[16];
+    return images[slot < 0 || slot >= 16 ? 0 : slot];
+}
+inline double image_slope_decode(uint32_t code) {
+    return code == 0 ? 0.0 : std::exp2(((double)code - 255.0) / 16.0 + 1.0);
+}
+struct OImageTexel { double v[4], nr[4], fr[4]; };
+inline OImageTexel image_texel(const OImage& im, uint32_t x, uint32_t y) {
+    const uint32_t* t = &im.words[4 * ((size_t)y * im.w + x)];
+    OImageTexel o;
+    const uint32_t q[4] = {t[0] & 0xffffu, t[0] >> 16, t[1] & 0xffffu, t[1] >> 16};
+    for (int c = 0; c < 4; ++c) {
+        o.v[c] = (double)q[c] / 65535.0;
+        o.nr[c] = image_slope_decode((t[2] >> (8 * c)) & 0xffu);
+        o.fr[c] = image_slope_decode((t[3] >> (8 * c)) & 0xffu);
+    }
+    return o;
+}
+template <class T> struct OImageSample { T v[4], nr[4], fr[4]; };
+template <class T> OImageSample<T> image_sample(const OImage& im, T u, T v) {
+    OImageSample<T> o;
+    for (int c = 0; c < 4; ++c) { o.v[c] = T(0); o.nr[c] = T(0); o.fr[c] = T(0); }
+    if (im.w == 0 || im.h == 0) return o;
+    const uint32_t lx = im.w - 1, ly = im.h - 1;
+    T tx = tclamp(u * T((double)im.w) - T(0.5), T(0), T((double)lx));
+    T ty = tclamp(v * T((double)im.h) - T(0.5), T(0), T((double)ly));
+    const uint32_t x0 = std::min((uint32_t)val(tx), lx), y0 = std::min((uint32_t)val(ty), ly);
+    const uint32_t x1 = std::min(x0 + 1, lx), y1 = std::min(y0 + 1, ly);
+    T fx = tx - T((double)x0), fy = ty - T((double)y0);
+    const OImageTexel a = image_texel(im, x0, y0), b = image_texel(im, x1, y0), c = image_texel(im, x0, y1),
+                      d = image_texel(im, x1, y1);
+    for (int ch = 0; ch < 4; ++ch) {
+        o.v[ch] = wmix(wmix(T(a.v[ch]), T(b.v[ch]), fx), wmix(T(c.v[ch]), T(d.v[ch]), fx), fy);
+        o.nr[ch] = wmix(wmix(T(a.nr[ch]), T(b.nr[ch]), fx), wmix(T(c.nr[ch]), T(d.nr[ch]), fx), fy);
+        o.fr[ch] = wmix(wmix(T(a.fr[ch]), T(b.fr[ch]), fx), wmix(T(c.fr[ch]), T(d.fr[ch]), fx), fy);
+    }
+    return o;
+}
+struct OHeightmap {
+    O_ONE
+    O_STEP {
+        (void)pos;
+        auto v = [&](int k) { return (double)val(pp.p[k]); };
+        const double size = std::max(v(0), EPSILON);
+        V3<T> q = c.z / (T(size) * tmax(abs(c.dr), T(EPSILON_FINE)));
+        const double lift = v(1), base = std::max(v(2), 0.0);
+        const bool up = lift >= 0.0;
+        double k[4] = {v(3), v(4), v(5), v(6)};
+        if (v(7) > 0.5) {  // select(w, vec4f(-w.xyz, dot(w, vec4f(1.0))), p[7] > 0.5)
+            const double sum = k[0] + k[1] + k[2] + k[3];
+            k[0] = -k[0]; k[1] = -k[1]; k[2] = -k[2]; k[3] = sum;
+        }
+        const OImage& im = oracle_image(pp.image);
+        double half_x = 1, half_y = 1, texel = 0, h_lo = 0, h_hi = 0;
+        OImageSample<T> s;
+        for (int ch = 0; ch < 4; ++ch) { s.v[ch] = T(0); s.nr[ch] = T(0); s.fr[ch] = T(0); }
+        const double slope[4] = {im.w ? im.slope[0] : 0.0, im.w ? im.slope[1] : 0.0, im.w ? im.slope[2] : 0.0,
+                                 im.w ? im.slope[3] : 0.0};
+        if (im.w > 0) {
+            const double mx = (double)std::max(im.w, im.h);
+            half_x = im.w / mx;
+            half_y = im.h / mx;
+            texel = 2.0 / mx;
+            s = image_sample(im, q.x / T(half_x) * T(0.5) + T(0.5), q.z / T(half_y) * T(0.5) + T(0.5));
+            h_lo = std::min(0.0, k[3] + std::min(k[0], 0.0) + std::min(k[1], 0.0) + std::min(k[2], 0.0));
+            h_hi = std::max(0.0, k[3] + std::max(k[0], 0.0) + std::max(k[1], 0.0) + std::max(k[2], 0.0));
+        }
+        T top = T(lift) * (T(k[0]) * s.v[0] + T(k[1]) * s.v[1] + T(k[2]) * s.v[2] + T(k[3]) * s.v[3]);
+        T dy = up ? q.y - top : top - q.y;
+        T flat_side = up ? -T(base) - q.y : q.y - T(base);
+        const double ak[4] = {std::fabs(k[0]), std::fabs(k[1]), std::fabs(k[2]), std::fabs(k[3])};
+        const double per_texel = std::fabs(lift) / std::max(texel, EPSILON_FINE);
+        const double gx = per_texel * (ak[0] * slope[0] + ak[1] * slope[1] + ak[2] * slope[2] + ak[3] * slope[3]);
+        T gn = T(per_texel) * (T(ak[0]) * s.nr[0] + T(ak[1]) * s.nr[1] + T(ak[2]) * s.nr[2] + T(ak[3]) * s.nr[3]);
+        T gf = T(per_texel) * (T(ak[0]) * s.fr[0] + T(ak[1]) * s.fr[1] + T(ak[2]) * s.fr[2] + T(ak[3]) * s.fr[3]);
+        T cx = dy / T(std::sqrt(1.0 + gx * gx));
+        T cn = dy / sqrt(T(1) + gn * gn), cf = dy / sqrt(T(1) + gf * gf);
+        const double nearb = 8.0 * texel, farb = 64.0 * texel;  // IMAGE_SLOPE_RADIUS_NEAR / _FAR
+        T c_near = tclamp(cn, T(-nearb), T(nearb)), c_far = tclamp(cf, T(-farb), T(farb));
+        T curved = dy >= T(0) ? tmax(cx, tmax(c_near, c_far)) : tmin(cx, tmin(c_near, c_far));
+        const double t_min = std::min(lift * h_lo, lift * h_hi), t_max = std::max(lift * h_lo, lift * h_hi);
+        const double lo = up ? -base : std::min(t_min, base);
+        const double hi = up ? std::max(t_max, -base) : base;
+        V3<T> e(abs(q.x) - T(half_x), tmax(T(lo) - q.y, q.y - T(hi)), abs(q.z) - T(half_y));
+        T bounds = length(vmax(e, V3<T>())) + tmin(tmax(e.x, tmax(e.y, e.z)), T(0));
+        T d = tmax(tmax(curved, flat_side), bounds);
+        T elevation = tclamp((q.y - T(lo)) / T(std::max(hi - lo, EPSILON)), T(0), T(1));
+        return {V3<T>(T(0), elevation, T(0)), d * T(size)};
+    }
+    O_FIN { return c.dr; }
+};
+
 // formulas/dragon_ifs_twist.wgsl:7-123 (matrices as column arrays, as WGSL mat3x3f)
 struct DMat { double c[3][3]; };  // c[col][row]
 inline DMat dm_rz(double a) { return DMat{{{std::cos(a), std::sin(a), 0}, {-std::sin(a), std::cos(a), 0}, {0, 0, 1}}}; }
@@ -1159,7 +1260,7 @@ struct ORomanesco {
     X(F_MANDELBULBTWIST, OMandelbulbTwist) X(F_MANDELBULBDUAL, OMandelbulbDual) X(F_QUATJULIA, OQuatJulia)          \
     X(F_BULBBOX, OBulbBox) X(F_MANDALAY, OMandalay) X(F_WOBBLEKIFS, OWobbleKifs) X(F_RADIOACTIVE, ORadioactive)     \
     X(F_RECURSIVESPHERE, ORecursiveSphere) X(F_TREE, OTree) X(F_KLEINIANLIMIT, OKleinianLimit) X(F_FBMSDF, OFbmSdf) \
-    X(F_SEASCAPE, OSeascape) X(F_DRAGON, ODragon) X(F_ROMANESCO, ORomanesco)
+    X(F_SEASCAPE, OSeascape) X(F_DRAGON, ODragon) X(F_ROMANESCO, ORomanesco) X(F_HEIGHTMAP, OHeightmap)
 
 template <class P> int ext_iterations(int f, const Params<P>& p) {
     switch (f) {

@@ -26,6 +26,8 @@ ID = {f: 4 + i for i, f in enumerate(FILES)}
 BUILTIN_FILES = {0: "mandelbrot", 1: "amazingbox_box", 2: "menger3_ifs", 3: "kleinian_kleinian"}
 for k, v in BUILTIN_FILES.items():
     ID[v] = k
+# the image formula: its tests bind an image (tests/test_image.py, tests/test_gpu_heightmap.py)
+ID["heightmap_primitive"] = 4 + len(FILES)
 
 _PARAMS = None
 
