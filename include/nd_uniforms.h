@@ -306,6 +306,7 @@ typedef struct nd_uniforms {
 #define ND_PART_IRIDESCENCE (1u << 10)
 #define ND_PART_DISPERSION (1u << 11)
 #define ND_PART_SKY (1u << 12)
+#define ND_PART_DEPTH_OF_FIELD (1u << 13)
 #define ND_PART_MC_INDIRECT (1u << 14)
 #define ND_PART_GEOMETRY_ONLY (1u << 16)
 

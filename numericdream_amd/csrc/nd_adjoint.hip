@@ -230,6 +230,8 @@ static nd_status radiance_scope(nd_ctx* c, const nd_uniforms& u, uint32_t v) {
     if (!(parts & ND_PART_REFLECTIONS))
         return nd_fail(c, ND_E_INVALID, "view %u: the radiance adjoint needs the reflection part off (PART_REFLECTIONS)", v);
     if (u.accel_enabled > 0.5f) return nd_fail(c, ND_E_UNSUPPORTED, "view %u: the radiance adjoint runs without the accelerator", v);
+    if (u.dof_enabled > 0.5f && u.aperture > 0.0001f)
+        return nd_fail(c, ND_E_UNSUPPORTED, "view %u: the radiance adjoint is the pinhole colour (depth of field off)", v);
     const nd_color_stop* first = nullptr;
     int first_count = -1;
     const int ni = std::min(std::max((int)u.instance_count, 0), ND_MAX_INSTANCES);

@@ -75,6 +75,12 @@ struct McScene {
     int sky_sun;
     float max_dist;
     int simple_view;   // PART_GEOMETRY_ONLY set: fs_simple draws simple_geometry, else render_lite
+    // depth of field (cs_wf_generate, template.wgsl:5150-5182): a pixel whose centre ray's hit
+    // lies outside the focus range takes dof_n lens samples per sample, averaged by the resolve
+    int dof_on;        // dof_enabled, aperture > DOF_MIN_APERTURE, PART_DEPTH_OF_FIELD on
+    int slots;         // path slots per pixel sample: the wavefront's inner count with DOF, else 1
+    int dof_n;         // lens samples of an out-of-focus pixel: min(mc_inner_count(32), slots)
+    float aperture, focus_distance, focus_range;
 };
 
 // ----------------------------------------------------------------------------- random numbers
@@ -94,6 +100,12 @@ NDD uint32_t hash_u32(float px, float py) {
     return x ^ y;
 }
 NDD float hash12w(float px, float py) { return (float)hash_u32(px, py) * 2.3283064365386963e-10f; }
+NDD float ffract(float x) { return x - floorf(x); }
+// interleaved gradient noise (hash12, template.wgsl:2413-2415):
+// fract(52.9829189 fract(dot(p, (0.06711056, 0.00583715)))), one rounding per operation
+NDD float hash12(float px, float py) {
+    return ffract(mul_rn(52.9829189f, ffract(add_rn(mul_rn(px, 0.06711056f), mul_rn(py, 0.00583715f)))));
+}
 // seed arithmetic without contraction, as the shader's separate multiply and add
 NDD float mad_rn(float a, float b, float c) { return add_rn(mul_rn(a, b), c); }
 
@@ -453,6 +465,7 @@ struct McPaths {
     // valid, bit 1 hit; distance; octahedral-packed normal
     float* ct;
     uint32_t *cf, *cn;
+    float* dof_ap;  // per pixel: the effective aperture of its centre ray (depth of field)
     // queues (path indices) and counters
     uint32_t *live_a, *live_b;
     uint32_t* shadow_path;
@@ -493,6 +506,17 @@ struct McArgs {
 
 NDD unsigned long long* de_ctr(const McArgs& a) { return (unsigned long long*)(a.P.ctr + C_DE); }
 
+// Path index p = (sample in chunk * npix + pixel) * slots + lens slot.
+NDD uint32_t path_pixel(const McArgs& a, uint32_t p) { return (p / (uint32_t)a.S->slots) % a.npix; }
+NDD uint32_t path_sample(const McArgs& a, uint32_t p) { return a.ch->sample_base + (p / (uint32_t)a.S->slots) / a.npix; }
+NDD uint32_t path_slot(const McArgs& a, uint32_t p) { return p % (uint32_t)a.S->slots; }
+
+// The lens samples a pixel takes per sample (n = mc_inner_count(DOF_SAMPLES_HQ) when its
+// effective aperture exceeds DOF_MIN_APERTURE, else 1).
+NDD uint32_t pixel_lens_samples(const McArgs& a, uint32_t pix) {
+    return (a.S->dof_on && a.P.dof_ap[pix] > 0.0001f) ? (uint32_t)a.S->dof_n : 1u;
+}
+
 NDD uint32_t append(uint32_t* ctr, bool want, uint32_t* slot) {
     const uint64_t m = __builtin_amdgcn_ballot_w64(want);
     if (!m) return 0;
@@ -522,49 +546,124 @@ NDD McArgs restricted(const McArgs& a0, const DevScene* __restrict__ scene, cons
 // paths started; the resolve drops the same samples, and those of pixels that converge
 // within the chunk.
 __global__ void k_mc_generate(McArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;  // one pixel sample
     const uint32_t n = a.npix * a.ch->spp_chunk;
-    if (p >= n) return;
+    if (q >= n) return;
     const DevJob& J = *a.job;
-    const uint32_t pix = p % a.npix, s = a.ch->sample_base + p / a.npix;
+    const McScene& S = *a.S;
+    const uint32_t pix = q % a.npix, s = a.ch->sample_base + q / a.npix;
     const int x = (int)(pix % (uint32_t)J.rect_w), y = (int)(pix / (uint32_t)J.rect_w);
     const int gx = J.rect_x + x, gy = J.rect_y + y;
     const float ms = add_rn(a.ch->mc_sample0, (float)s);
-    if (a.S->adapt_on) {
-        const bool on = (float)gx < a.S->adapt_row_width;
-        const bool skip = on && ms > 0.5f && a.adapt[8 * (size_t)pix + 3] > 0.5f;
-        uint32_t slot = 0;
-        append(a.P.ctr + C_LIVE_IN, !skip, &slot);
-        if (skip) return;
-        a.P.live_a[slot] = p;
-    } else {
-        a.P.live_a[p] = p;
+    const uint32_t slots = (uint32_t)S.slots;
+    const uint32_t lens = pixel_lens_samples(a, pix);
+    bool skip = false;
+    if (S.adapt_on) {
+        const bool on = (float)gx < S.adapt_row_width;
+        skip = on && ms > 0.5f && a.adapt[8 * (size_t)pix + 3] > 0.5f;
     }
+    const uint32_t n_paths = skip ? 0u : lens;
+    if (S.adapt_on || S.dof_on) {
+        // the live queue holds only the paths started
+        uint32_t slot = 0;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(true);
+        const int leader = __builtin_ctzll(m);
+        // wave-wide exclusive prefix of n_paths
+        uint32_t incl = n_paths;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)incl, off);
+            if ((int)__lane_id() >= off) incl += o;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)incl, 63 - __builtin_clzll(m));
+        uint32_t base = 0;
+        if ((int)__lane_id() == leader && total) base = atomicAdd(a.P.ctr + C_LIVE_IN, total);
+        base = (uint32_t)__shfl((int)base, leader);
+        slot = base + incl - n_paths;
+        for (uint32_t j = 0; j < n_paths; ++j) a.P.live_a[slot + j] = q * slots + j;
+    } else {
+        a.P.live_a[q] = q;
+    }
+    if (n_paths == 0) return;
     const V3<float> d = primary_dir(J, gx, gy);
     // base seed = frag + mc_sample * (91.73, 57.31) (fs_main, template.wgsl:4818)
     const float bx = mad_rn(ms, 91.73f, (float)gx + 0.5f);
     const float by = mad_rn(ms, 57.31f, (float)gy + 0.5f);
-    a.P.px[p] = J.cam[0]; a.P.py[p] = J.cam[1]; a.P.pz[p] = J.cam[2];
-    a.P.dx[p] = d.x; a.P.dy[p] = d.y; a.P.dz[p] = d.z;
-    for (int k = 0; k < 8; ++k) { a.P.thr[8 * p + k] = 1.f; a.P.rad[8 * p + k] = 0.f; }
-    // path_begin (template.wgsl:3819-3839)
-    a.P.seedx[p] = mad_rn(ms, 27.31f, bx);
-    a.P.seedy[p] = mad_rn(ms, 45.17f, by);
-    a.P.hero[p] = mad_rn(ms, 0.61803399f, hash12w(add_rn(bx, 19.31f), add_rn(by, 47.11f)));
-    a.P.depth[p] = 0;
-    uint8_t fl = 0;
-    if (a.S->cache_on) {
+    V3<float> focus = mk3(0.f, 0.f, 0.f);
+    float ap = 0.f;
+    if (lens > 1u) {
+        // focus_t = focus_distance / max(dot(dir, forward), EPSILON); focus point on the ray
+        const float ft = S.focus_distance / fmaxf(d.x * J.fwd[0] + d.y * J.fwd[1] + d.z * J.fwd[2], 1e-4f);
+        focus = mk3(J.cam[0] + d.x * ft, J.cam[1] + d.y * ft, J.cam[2] + d.z * ft);
+        ap = a.P.dof_ap[pix];
+    }
+    uint8_t cache_fl = 0;
+    float cache_t = 0.f;
+    if (S.cache_on && lens == 1u) {
         // a primary segment stored by sample 0 of this call (an earlier chunk) is reused
         const uint32_t c = a.P.cf[pix];
         if (c & 1u) {
-            fl = (uint8_t)(8u | ((c & 2u) ? 2u : 0u));
-            a.P.hit_t[p] = a.P.ct[pix];
+            cache_fl = (uint8_t)(8u | ((c & 2u) ? 2u : 0u));
+            cache_t = a.P.ct[pix];
         }
     }
-    a.P.flags[p] = fl;
-    a.P.lambda[p] = 0.f;
-    a.P.inner[p] = 64;  // PATH_DEFAULT_INNER_STEPS
+    for (uint32_t j = 0; j < n_paths; ++j) {
+        const uint32_t p = q * slots + j;
+        float sx0 = bx, sy0 = by;
+        V3<float> o = mk3(J.cam[0], J.cam[1], J.cam[2]), dir = d;
+        uint8_t fl = cache_fl;
+        if (lens > 1u) {
+            // lens sample: seed + s (91.37, 51.91), sample_unit_disk * aperture on the camera's
+            // right / up, aimed at the focus point (template.wgsl:5171-5179, 4709-4713)
+            sx0 = add_rn(bx, mul_rn((float)j, 91.37f));
+            sy0 = add_rn(by, mul_rn((float)j, 51.91f));
+            const float r = sqrtf(hash12(sx0, sy0));
+            const float th = hash12w(add_rn(sx0, 17.17f), add_rn(sy0, 71.71f)) * 6.28318530718f;
+            const float dxl = r * cosf(th) * ap, dyl = r * sinf(th) * ap;
+            o = mk3(J.cam[0] + J.right[0] * dxl + J.up[0] * dyl, J.cam[1] + J.right[1] * dxl + J.up[1] * dyl,
+                    J.cam[2] + J.right[2] * dxl + J.up[2] * dyl);
+            dir = vnorm3(mk3(focus.x - o.x, focus.y - o.y, focus.z - o.z));
+            fl = 32u;  // a lens path: no primary hit cache
+        }
+        a.P.px[p] = o.x; a.P.py[p] = o.y; a.P.pz[p] = o.z;
+        a.P.dx[p] = dir.x; a.P.dy[p] = dir.y; a.P.dz[p] = dir.z;
+        for (int k = 0; k < 8; ++k) { a.P.thr[8 * p + k] = 1.f; a.P.rad[8 * p + k] = 0.f; }
+        // path_begin (template.wgsl:3819-3839)
+        a.P.seedx[p] = mad_rn(ms, 27.31f, sx0);
+        a.P.seedy[p] = mad_rn(ms, 45.17f, sy0);
+        a.P.hero[p] = mad_rn(ms, 0.61803399f, hash12w(add_rn(sx0, 19.31f), add_rn(sy0, 47.11f)));
+        a.P.depth[p] = 0;
+        if (fl & 8u) a.P.hit_t[p] = cache_t;
+        a.P.flags[p] = fl;
+        a.P.lambda[p] = 0.f;
+        a.P.inner[p] = 64;  // PATH_DEFAULT_INNER_STEPS
+    }
 }
+
+// --- depth-of-field probe (cs_wf_generate, template.wgsl:5151-5163): once per call, each
+// pixel's centre ray is marched and the effective aperture of its hit distance stored:
+// aperture * smoothstep(range / 2, range / 2 + falloff, |t - focus_t|) with falloff
+// max(4 aperture, 0.05) (dof_aperture, :4726-4730); a miss counts as max_dist.
+struct DofProbeRays {
+    const McArgs& a;
+    NDD void setup(uint32_t i, V3<float>* o, V3<float>* d) const {
+        const DevJob& J = *a.job;
+        *o = mk3(J.cam[0], J.cam[1], J.cam[2]);
+        *d = primary_dir(J, J.rect_x + (int)(i % (uint32_t)J.rect_w), J.rect_y + (int)(i / (uint32_t)J.rect_w));
+    }
+    NDD void finish(uint32_t i, bool hit, float t) const {
+        const DevJob& J = *a.job;
+        const McScene& S = *a.S;
+        const V3<float> d = primary_dir(J, J.rect_x + (int)(i % (uint32_t)J.rect_w), J.rect_y + (int)(i / (uint32_t)J.rect_w));
+        const float ft = S.focus_distance / fmaxf(d.x * J.fwd[0] + d.y * J.fwd[1] + d.z * J.fwd[2], 1e-4f);
+        const float probe_t = hit ? t : S.max_dist;
+        const float half = fmaxf(S.focus_range, 0.f) * 0.5f;
+        const float falloff = fmaxf(S.aperture * 4.f, 0.05f);
+        const float x = fclamp((fabsf(probe_t - ft) - half) / falloff, 0.f, 1.f);
+        a.P.dof_ap[i] = S.aperture * (x * x * (3.f - 2.f * x));
+    }
+    NDD int inside(uint32_t) const { return 0; }
+    NDD bool skip(uint32_t) const { return false; }
+};
 
 // --- persistent march over a queue of rays. RAY provides setup(i, org, dir), finish(i,
 // hit, t) and inside(i): the inside-march step budget of a ray that starts in a solid, 0
@@ -680,7 +779,7 @@ struct ExtendRays {
     NDD void finish(uint32_t i, bool hit, float t) const {
         const uint32_t p = q[i];
         a.P.hit_t[p] = t;
-        a.P.flags[p] = (uint8_t)((a.P.flags[p] & 21u) | (hit ? 2u : 0u));
+        a.P.flags[p] = (uint8_t)((a.P.flags[p] & 53u) | (hit ? 2u : 0u));
     }
     NDD bool skip(uint32_t i) const { return (a.P.flags[q[i]] & 8u) != 0; }
     NDD int inside(uint32_t i) const {
@@ -713,6 +812,13 @@ __attribute__((amdgpu_waves_per_eu((EXT || INSIDE) ? 1 : 7, (EXT || INSIDE) ? 8 
     const uint32_t n = a.P.ctr[C_LIVE_IN];
     ExtendRays R{a, q};
     persistent_march<EXT, INSIDE>(a, R, n, a.P.ctr + C_HEAD_EXT);
+}
+
+template <bool EXT>
+__global__ void __launch_bounds__(64) k_mc_dof_probe(McArgs a0, const DevScene* __restrict__ scene, const McScene* __restrict__ mcs) {
+    McArgs a = restricted(a0, scene, mcs);
+    DofProbeRays R{a};
+    persistent_march<EXT, false>(a, R, a.npix, a.P.ctr + C_HEAD_EXT);
 }
 
 template <bool EXT>
@@ -1114,15 +1220,18 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a0, int ping, const DevS
             // path_seed (template.wgsl:3845-3847)
             const float sx = add_rn(a.P.seedx[p], mad_rn((float)depth, 17.77f, 3.1f));
             const float sy = add_rn(a.P.seedy[p], mad_rn((float)depth, 41.13f, 7.7f));
-            const uint32_t pix = p % a.npix;
+            const uint32_t pix = path_pixel(a, p);
+            // the pixel's own outputs and cache entry come from sample 0 (its first lens slot)
+            const bool first = path_sample(a, p) == 0 && path_slot(a, p) == 0;
+            const bool lens = (fl & 32u) != 0;
             if (!(fl & 2u)) {
                 const V3<float> sky = sky_color(S, dir);
                 const Sp8 sk = from_rgb(B, sky.x, sky.y, sky.z);
                 for (int k = 0; k < 8; ++k) rad.v[k] += thr.v[k] * sk.v[k];
-                if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
+                if (depth == 0 && first) {
                     if (a.out_depth) a.out_depth[pix] = 0.f;
                     if (a.out_normal) { a.out_normal[3 * pix] = 0.f; a.out_normal[3 * pix + 1] = 0.f; a.out_normal[3 * pix + 2] = 0.f; }
-                    if (S.cache_on) { a.P.ct[pix] = a.P.hit_t[p]; a.P.cf[pix] = 1u; }  // a cached miss
+                    if (S.cache_on && !lens) { a.P.ct[pix] = a.P.hit_t[p]; a.P.cf[pix] = 1u; }  // a cached miss
                 }
             } else if (sp_max(thr) >= 0.002f) {
                 const float t = a.P.hit_t[p];
@@ -1137,10 +1246,10 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a0, int ping, const DevS
                 } else {
                     nrm = normal_tetra<EXT>(sc, hp, eps);
                     de += 4;
-                    if (S.cache_on && !in_solid && (depth == 0 || (depth == 1 && (fl & 16u)))) {
+                    if (S.cache_on && !lens && !in_solid && (depth == 0 || (depth == 1 && (fl & 16u)))) {
                         const uint32_t packed = oct_pack(nrm);
                         nrm = oct_unpack(packed);
-                        if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
+                        if (depth == 0 && first) {
                             a.P.cn[pix] = packed;
                             a.P.ct[pix] = t;
                             a.P.cf[pix] = 3u;  // valid, hit
@@ -1149,7 +1258,7 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a0, int ping, const DevS
                 }
                 const Mat mat = part_mask(hit_material<EXT>(sc, S, hp), S);
                 de += 1;
-                if (depth == 0 && a.ch->sample_base + p / a.npix == 0) {
+                if (depth == 0 && first) {
                     if (a.out_depth) a.out_depth[pix] = t;
                     if (a.out_normal) { a.out_normal[3 * pix] = nrm.x; a.out_normal[3 * pix + 1] = nrm.y; a.out_normal[3 * pix + 2] = nrm.z; }
                 }
@@ -1296,7 +1405,8 @@ __global__ void __launch_bounds__(64) k_mc_shade(McArgs a0, int ping, const DevS
                         a.P.depth[p] = (uint8_t)(depth + 1);
                         // a mirror reflection at depth 0 (template.wgsl:4117-4118)
                         const bool mirror = S.cache_on && depth == 0 && !in_solid && kind == 1 && rough * rough <= 1e-4f;
-                        a.P.flags[p] = (uint8_t)(((reflected || kind == 1) ? 1u : 0u) | (solid ? 4u : 0u) | (mirror ? 16u : 0u));
+                        a.P.flags[p] = (uint8_t)(((reflected || kind == 1) ? 1u : 0u) | (solid ? 4u : 0u) | (mirror ? 16u : 0u) |
+                                                 (lens ? 32u : 0u));
                         a.P.inner[p] = (uint16_t)inner;
                     } else {
                         cont = false;
@@ -1470,6 +1580,26 @@ __global__ void __launch_bounds__(64) k_mc_shadow(McArgs a0, const DevScene* __r
     if (__lane_id() == 0 && de) atomicAdd(de_ctr(a), de);
 }
 
+// One sample's colour of a pixel: the gamut-fitted RGB of its path, or with depth of field
+// the mean over its lens paths (fs_wf_resolve, template.wgsl:5373-5394).
+NDD void sample_rgb(const McArgs& a, uint32_t s, uint32_t pix, float rgb[3]) {
+    const uint32_t lens = pixel_lens_samples(a, pix);
+    const uint32_t p0 = (s * a.npix + pix) * (uint32_t)a.S->slots;
+    rgb[0] = rgb[1] = rgb[2] = 0.f;
+    for (uint32_t j = 0; j < lens; ++j) {
+        const uint32_t p = p0 + j;
+        Sp8 r;
+        for (int k = 0; k < 8; ++k) r.v[k] = a.P.rad[8 * p + k];
+        float c[3];
+        to_rgb_fit(a.P.hero[p], r, c);
+        rgb[0] += c[0]; rgb[1] += c[1]; rgb[2] += c[2];
+    }
+    if (lens > 1u) {
+        const float inv = 1.f / (float)lens;
+        rgb[0] *= inv; rgb[1] *= inv; rgb[2] *= inv;
+    }
+}
+
 // --- resolve: per pixel, the average over this chunk's samples is added into out_rgb
 // (scaled by 1 / spp_total)
 __global__ void k_mc_resolve(McArgs a) {
@@ -1478,11 +1608,8 @@ __global__ void k_mc_resolve(McArgs a) {
     float acc[3] = {0.f, 0.f, 0.f};
     const uint32_t cs = a.ch->spp_chunk;
     for (uint32_t s = 0; s < cs; ++s) {
-        const uint32_t p = s * a.npix + pix;
-        Sp8 r;
-        for (int k = 0; k < 8; ++k) r.v[k] = a.P.rad[8 * p + k];
         float rgb[3];
-        to_rgb_fit(a.P.hero[p], r, rgb);
+        sample_rgb(a, s, pix, rgb);
         acc[0] += rgb[0]; acc[1] += rgb[1]; acc[2] += rgb[2];
     }
     const float inv = 1.f / (float)a.ch->spp_total;
@@ -1507,11 +1634,8 @@ __global__ void k_mc_resolve_adapt(McArgs a) {
     for (uint32_t s = 0; s < a.ch->spp_chunk; ++s) {
         const float ms = add_rn(a.ch->mc_sample0, (float)(a.ch->sample_base + s));
         if (on && ms > 0.5f && s4[3] > 0.5f) continue;
-        const uint32_t p = s * a.npix + pix;
-        Sp8 r;
-        for (int k = 0; k < 8; ++k) r.v[k] = a.P.rad[8 * p + k];
         float rgb[3];
-        to_rgb_fit(a.P.hero[p], r, rgb);
+        sample_rgb(a, s, pix, rgb);
         for (int k = 0; k < 3; ++k) sum[k] += rgb[k];
         taken += 1.f;
         if (on) {
@@ -1659,6 +1783,16 @@ static void pack_mc_scene(const nd_uniforms& u, nd::McScene& S) {
     S.sky_sun_intensity = u.sky.sun_intensity;
     S.sky_intensity = u.sky.intensity;
     S.max_dist = u.max_dist;
+    // depth of field (cs_wf_generate, template.wgsl:5150-5182): the wavefront's inner slots
+    // per pixel (wavefrontInner, fractal_renderer.zig:4335-4337) and mc_inner_count(32)
+    S.dof_on = u.dof_enabled > 0.5f && u.aperture > 0.0001f && !(parts & ND_PART_DEPTH_OF_FIELD);
+    const int inner = std::min(std::max((int)std::max(u.mc_inner_samples, 1.0f), 1), 32);
+    const int n_dof = u.mc_enabled > 0.5f ? std::min(std::max((int)u.mc_inner_samples, 1), 32) : 32;
+    S.slots = S.dof_on ? inner : 1;
+    S.dof_n = std::min(n_dof, inner);
+    S.aperture = u.aperture;
+    S.focus_distance = u.focus_distance;
+    S.focus_range = u.focus_range;
 }
 
 namespace nd {
@@ -2042,8 +2176,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     if (!out || !out->color) return nd_fail(c, ND_E_INVALID, "path trace needs a colour output");
     if (spp == 0 || spp > 65536) return nd_fail(c, ND_E_INVALID, "spp %u", spp);
     const nd_uniforms& u = *(const nd_uniforms*)uniforms;
-    if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.dof_enabled > 0.5f || u.approx_flags != 0.0f)
-        return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog, depth of field and approximations are not implemented");
+    if (u.photon_enabled > 0.5f || u.fog_count > 0.5f || u.approx_flags != 0.0f)
+        return nd_fail(c, ND_E_UNSUPPORTED, "photons, fog and approximations are not implemented");
     if (hipSetDevice(c->device) != hipSuccess) return nd_fail(c, ND_E_DEVICE, "hipSetDevice");
     nd_march_opts o{ND_NORMAL_TETRA, 0, 0, ND_KERNEL_PERSISTENT};
     DevJob job;
@@ -2061,6 +2195,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     // lights stays within 2^29. Adaptive sampling takes chunks of at most half its minimum sample
     // count, so that pixels converged in one chunk start no paths in the next.
     const size_t lights = (size_t)std::max(S.light_count, 1);
+    const size_t slots = (size_t)S.slots;  // path slots per pixel sample (lens samples with depth of field)
     const bool dev_out = out->device_ptrs != 0;
     auto layout_bytes = [&](size_t P) {
         size_t bytes = 0;
@@ -2072,6 +2207,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         sz(P); sz(P);                           // depth, flags
         sz(4 * P); sz(2 * P);                   // lambda, inner
         sz(4 * (size_t)npix); sz(4 * (size_t)npix); sz(4 * (size_t)npix);  // primary hit cache
+        sz(4 * (size_t)npix);                                               // depth-of-field aperture
         sz(4 * P); sz(4 * P);                   // live queues
         const size_t SQ = P * lights;
         sz(4 * SQ); for (int i = 0; i < 8; ++i) sz(4 * SQ); sz(32 * SQ);
@@ -2087,9 +2223,10 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         while (pool_log2 < 27 && layout_bytes((size_t)1 << (pool_log2 + 1)) <= c->mem_total / 4) ++pool_log2;
     }
     const size_t pool = std::min((size_t)1 << pool_log2, ((size_t)1 << 29) / lights);
-    uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp, pool / std::max(npix, 1u)));
+    uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp, pool / std::max<size_t>((size_t)npix * slots, 1)));
     if (S.adapt_on) chunk = std::min(chunk, std::max(1u, (uint32_t)(S.adapt_min_samples * 0.5f)));
-    const size_t P = (size_t)npix * chunk;
+    const size_t P = (size_t)npix * chunk * slots;
+    if (P >= ((size_t)1 << 32)) return nd_fail(c, ND_E_INVALID, "%u pixels x %zu lens slots exceed 2^32 paths", npix, slots);
     if (P * lights > ((size_t)1 << 29)) return nd_fail(c, ND_E_INVALID, "%u pixels x %zu lights exceed the shadow queue's 2^29 entries", npix, lights);
     const size_t bytes = layout_bytes(P);
     const size_t SQ = P * lights;
@@ -2110,6 +2247,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     Pp.depth = (uint8_t*)carve(P); Pp.flags = (uint8_t*)carve(P);
     Pp.lambda = (float*)carve(4 * P); Pp.inner = (uint16_t*)carve(2 * P);
     Pp.ct = (float*)carve(4 * (size_t)npix); Pp.cf = (uint32_t*)carve(4 * (size_t)npix); Pp.cn = (uint32_t*)carve(4 * (size_t)npix);
+    Pp.dof_ap = (float*)carve(4 * (size_t)npix);
     Pp.live_a = (uint32_t*)carve(4 * P); Pp.live_b = (uint32_t*)carve(4 * P);
     Pp.shadow_path = (uint32_t*)carve(4 * SQ);
     Pp.sox = (float*)carve(4 * SQ); Pp.soy = (float*)carve(4 * SQ); Pp.soz = (float*)carve(4 * SQ);
@@ -2149,6 +2287,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     const bool transp = S.transp != 0;        // shadow rays cross transparent occluders
     const bool adapt = S.adapt_on != 0;       // adaptive sampling: per-pixel state in c->mc_adapt
     const bool sss = S.sss != 0;              // subsurface materials: the shade pass's SSS variant
+    const bool dof = S.dof_on != 0;           // depth of field: lens samples per out-of-focus pixel
     if (adapt) {
         if ((st = c->mc_adapt.ensure(c, 32 * (size_t)npix)) != ND_OK) return st;
         a.adapt = (float*)c->mc_adapt.p;
@@ -2164,8 +2303,10 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     // replayed for every chunk (the chunk's sample range is read from a.ch).
     auto enqueue_chunk = [&](hipStream_t st) {
         hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 0);
-        hipLaunchKernelGGL(nd::k_mc_set_live, dim3(1), dim3(64), 0, st, Pp.ctr, (const nd::McChunk*)a.ch, npix, adapt ? 1 : 0);
-        hipLaunchKernelGGL(nd::k_mc_generate, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, st, a);
+        // with adaptive sampling or depth of field the generate pass appends the paths it starts
+        hipLaunchKernelGGL(nd::k_mc_set_live, dim3(1), dim3(64), 0, st, Pp.ctr, (const nd::McChunk*)a.ch, npix,
+                           (adapt || dof) ? 1 : 0);
+        hipLaunchKernelGGL(nd::k_mc_generate, dim3((uint32_t)(((size_t)npix * chunk + 255) / 256)), dim3(256), 0, st, a);
         int ping = 0;
         for (int depth = 0; depth < max_depth; ++depth) {
             hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, st, Pp.ctr, 1);
@@ -2205,7 +2346,7 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         struct McGraphKey {
             nd::McArgs args;
             uint32_t grid[5];
-            int max_depth, indirect, ext, inside, transp, adapt, sss;
+            int max_depth, indirect, ext, inside, transp, adapt, sss, dof, slots;
             uint64_t paths;
         } key;
         static_assert(sizeof(McGraphKey) <= sizeof(c->mc_graph_key), "graph key storage");
@@ -2217,6 +2358,8 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
         key.transp = transp;
         key.adapt = adapt;
         key.sss = sss;
+        key.dof = dof;
+        key.slots = (int)slots;
         key.max_depth = max_depth; key.indirect = indirect; key.paths = (uint64_t)P;
         if (!c->mc_graph_exec || std::memcmp(&key, c->mc_graph_key, sizeof(key)) != 0) {
             if (c->mc_graph_exec) { (void)hipGraphExecDestroy(c->mc_graph_exec); c->mc_graph_exec = nullptr; }
@@ -2245,6 +2388,12 @@ extern "C" nd_status nd_path_trace(nd_ctx* c, const void* uniforms, size_t size,
     hipLaunchKernelGGL(nd::k_mc_zero, dim3(256), dim3(256), 0, s, (float*)Pp.cf, (uint64_t)npix);  // empty hit cache
     hipLaunchKernelGGL(nd::k_mc_counters, dim3(1), dim3(64), 0, s, Pp.ctr, 3);
     (void)hipEventRecord(c->ev[6], s);
+    if (dof) {
+        // each pixel's centre ray, once per call: the effective aperture of its hit
+        const uint32_t g_probe = nd_pass_grid(c, RES_MC_EXTEND + rx, ((uint64_t)npix + 63) / 64);
+        if (ext) hipLaunchKernelGGL(nd::k_mc_dof_probe<true>, dim3(g_probe), dim3(64), 0, s, a, a.scene, a.S);
+        else hipLaunchKernelGGL(nd::k_mc_dof_probe<false>, dim3(g_probe), dim3(64), 0, s, a, a.scene, a.S);
+    }
     for (uint32_t k0 = 0; k0 < n_chunks; k0 += nd::MC_MAX_CHUNKS) {
         const uint32_t nk = std::min(nd::MC_MAX_CHUNKS, n_chunks - k0);
         if (k0 > 0 && (e = hipStreamSynchronize(s)) != hipSuccess) return nd_hip_fail(c, e, "path trace execution");

@@ -468,9 +468,23 @@ int ndo_adjoint_pixels(const nd_uniforms* u, const int32_t* cm, const int32_t* p
 // Monte Carlo path trace of a rect, spp samples per pixel with mc_sample = u.mc_sample + s
 // (the host's per-sample stamp, fractal_renderer.zig:3761-3775), averaged. Per-sample colors
 // (spp x h x w x 3) are written too when sample_rgb is non-null.
+// dof_aperture (template.wgsl:4726-4730): aperture * smoothstep(range / 2, range / 2 +
+// max(4 aperture, 0.05), |hit_t - focus_t|)
+static double dof_aperture(const nd_uniforms& u, double hit_t, double focus_t) {
+    const double half = std::max((double)u.focus_range, 0.0) * 0.5;
+    const double falloff = std::max((double)u.aperture * 4.0, 0.05);
+    const double x = std::min(std::max((std::fabs(hit_t - focus_t) - half) / falloff, 0.0), 1.0);
+    return (double)u.aperture * x * x * (3.0 - 2.0 * x);
+}
+
 int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, int ry, int rw, int rh, int spp,
                   float* rgb, float* depth, float* normal, float* sample_rgb, uint32_t* evals, int nthreads) {
     std::atomic<int> next_row{0};
+    // depth of field: on with dof_enabled, aperture > DOF_MIN_APERTURE and PART_DEPTH_OF_FIELD;
+    // lens samples min(mc_inner_count(32), the wavefront's inner slots)
+    const bool dof = u0->dof_enabled > 0.5f && u0->aperture > 0.0001f && !((uint32_t)u0->debug_parts & ND_PART_DEPTH_OF_FIELD);
+    const int inner = std::min(std::max((int)std::max(u0->mc_inner_samples, 1.0f), 1), 32);
+    const int dof_n = std::min(u0->mc_enabled > 0.5f ? std::min(std::max((int)u0->mc_inner_samples, 1), 32) : 32, inner);
     Scene<double> scene64;
     Scene<float> scene32;
     if (prec == 0) { scene64 = decode_scene<double>(*u0, code_map(cm)); attach_accel(*u0, scene64); }
@@ -491,6 +505,21 @@ int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, in
                 int px = rx + x, py = ry + y;
                 V3<T> dir = primary_dir<T>(u, px, py, (int)u.resolution[0], (int)u.resolution[1]);
                 V3<T> org = V3<T>::from(u.camera_pos);
+                // depth of field (cs_wf_generate, template.wgsl:5150-5182): the centre ray's
+                // hit sets the effective aperture; an out-of-focus pixel takes dof_n lens
+                // samples per sample, whose gamut-fitted colours are averaged (fs_wf_resolve)
+                int lens = 1;
+                double ap = 0.0;
+                V3<T> focus = org;
+                if (dof) {
+                    const V3<T> fw = V3<T>::from(u.camera_forward);
+                    const T ft = T((double)u.focus_distance) / tmax(dot(dir, fw), T(1e-4));
+                    focus = org + dir * ft;
+                    MarchResult<T> pr = march_to(s, mc, org, dir, T(mc.max_dist));
+                    const double probe_t = pr.hit ? (double)pr.dist : mc.max_dist;
+                    ap = dof_aperture(*u0, probe_t, (double)ft);
+                    if (ap > 0.0001) lens = dof_n;
+                }
                 // adaptive sampling (adapt_update, template.wgsl:4749-4760; the sample skip of
                 // fs_main / cs_wf_generate): a converged pixel takes no more samples and
                 // keeps the mean of those it took
@@ -502,7 +531,28 @@ int ndo_render_mc(const nd_uniforms* u0, const int32_t* cm, int prec, int rx, in
                     const float ms = u.mc_sample;
                     if (adapt && ms > 0.5f && st4[3] > 0.5f) continue;
                     Seed base{((float)px + 0.5f) + ms * 91.73f, ((float)py + 0.5f) + ms * 57.31f};
-                    PathResult r = trace_path<T>(u, s, mc, org, dir, base);
+                    PathResult r;
+                    if (lens == 1) {
+                        r = trace_path<T>(u, s, mc, org, dir, base);
+                    } else {
+                        double sum[3] = {0, 0, 0};
+                        for (int j = 0; j < lens; ++j) {
+                            // seed + s (91.37, 51.91); sample_unit_disk (template.wgsl:4709-4713)
+                            const Seed sj{base.x + (float)j * 91.37f, base.y + (float)j * 51.91f};
+                            const double rr = std::sqrt((double)hash12(sj.x, sj.y));
+                            const double th = (double)hash12w(sj.x + 17.17f, sj.y + 71.71f) * 6.28318530718;
+                            const double dx = rr * std::cos(th) * ap, dy = rr * std::sin(th) * ap;
+                            V3<T> o2(T((double)u.camera_pos[0] + u.camera_right[0] * dx + u.camera_up[0] * dy),
+                                     T((double)u.camera_pos[1] + u.camera_right[1] * dx + u.camera_up[1] * dy),
+                                     T((double)u.camera_pos[2] + u.camera_right[2] * dx + u.camera_up[2] * dy));
+                            const V3<T> d2 = normalize(focus - o2);
+                            const PathResult rj = trace_path<T>(u, s, mc, o2, d2, sj, true);
+                            for (int k = 0; k < 3; ++k) sum[k] += rj.rgb[k];
+                            if (j == 0) r = rj;
+                            else r.evals += rj.evals;
+                        }
+                        for (int k = 0; k < 3; ++k) r.rgb[k] = sum[k] / lens;
+                    }
                     ++taken;
                     if (adapt) adapt_update(st4, r.rgb, u0->adapt_min_samples, u0->adapt_threshold);
                     for (int k = 0; k < 3; ++k) acc[k] += r.rgb[k];
@@ -762,6 +812,14 @@ int ndo_helper(int op, const double* a, double* out) {
                                                        D3(a[10], a[11], a[12]), a[13]);
                    for (int k = 0; k < 8; ++k) out[k] = w.v[k];
                    return 0; }
+        // depth of field: dof_aperture(hit_t, focus_t) with aperture a[2], focus_range a[3]
+        case 60: { nd_uniforms u{}; u.aperture = (float)a[2]; u.focus_range = (float)a[3];
+                   out[0] = dof_aperture(u, a[0], a[1]); return 0; }
+        // hash12 (IGN) and the lens disk sample of the float seed (a[0], a[1])
+        case 61: { const float sx = (float)a[0], sy = (float)a[1];
+                   const double r = std::sqrt((double)hash12(sx, sy));
+                   const double th = (double)hash12w(sx + 17.17f, sy + 71.71f) * 6.28318530718;
+                   out[0] = hash12(sx, sy); out[1] = r * std::cos(th); out[2] = r * std::sin(th); return 0; }
     }
     return -1;
 }

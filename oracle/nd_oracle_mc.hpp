@@ -34,6 +34,12 @@ inline uint32_t hash_u32(float px, float py) {
     return x ^ y;
 }
 inline float hash12w(float px, float py) { return (float)hash_u32(px, py) * 2.3283064365386963e-10f; }
+// interleaved gradient noise (hash12, template.wgsl:2413-2415), in FP32
+inline float ffract(float x) { return x - std::floor(x); }
+inline float hash12(float px, float py) {
+    const float d = px * 0.06711056f + py * 0.00583715f;
+    return ffract(52.9829189f * ffract(d));
+}
 struct Seed { float x, y; };
 inline Seed seed_add(Seed s, float ax, float ay) { return {s.x + ax, s.y + ay}; }
 
@@ -880,8 +886,11 @@ struct PathResult { double rgb[3]; double depth; double normal[3]; int evals; in
 // march_inside), Cauchy dispersion on the picked wavelength with the Gaussian angle
 // weights or the one-wavelength collapse, thin-film tints, the light pick and the primary
 // hit cache's packed normals and subsurface materials. No fog, particles, photons or approximations.
+// lens: a depth-of-field lens sample, which runs without the primary hit cache
+// (g_primary_slot = -1, template.wgsl:5172).
 template <class T, class P>
-PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, Seed seed) {
+PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& mc, V3<T> org, V3<T> dir, Seed seed,
+                      bool lens = false) {
     PathResult out{};
     const float ms = u.mc_sample;
     Seed st_seed = seed_add(seed, ms * 27.31f, ms * 45.17f);
@@ -893,7 +902,7 @@ PathResult trace_path(const nd_uniforms& u, const Scene<P>& s, const MarchCfg& m
     // the primary hit cache (hit_cache_load/store, template.wgsl:3720-3766): on when the host
     // sets primary_cache_key; the primary hit and the hit after a mirror reflection at depth 0
     // keep their normals octahedral-packed
-    const bool cache = u.primary_cache_key > 0.5f;
+    const bool cache = u.primary_cache_key > 0.5f && !lens;
     bool second = false;
     int inner_steps = 64;  // PATH_DEFAULT_INNER_STEPS
     T lambda = T(0);        // picked wavelength in micrometres, 0 before the first dispersive hit
