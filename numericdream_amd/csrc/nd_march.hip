@@ -330,15 +330,29 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
         float dist = 0.f;
         int st = -1;
         bool used = false;  // the lane evaluated a DE this round (no accelerator skip applied)
-        if (active) {
-            // march_to's loop: at a probe the accelerator's skip, when it applies, takes the
-            // step without a DE (template.wgsl:1890-1900); a wave whose lanes all skip runs no DE
+        if (ACC || EXT) {
+            if (active) {
+                // march_to's loop: at a probe the accelerator's skip, when it applies, takes the
+                // step without a DE (template.wgsl:1890-1900); a wave whose lanes all skip runs
+                // no DE
+                const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
+                const float s = march_sample_t(m);
+                const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
+                if (ACC && acc && m.probing) st = march_skip_step(m, mp, accel_skip_cached(av, ac, ps), &dist);
+                used = st < 0;
+                if (used) st = march_update(m, mp, s, scene_de<false, EXT>(sc, ps, nullptr), &dist);
+            }
+        } else {
+            // Idle lanes step too, on the ray they last held (or the initial one): the wave
+            // issues the same instructions either way, and without a branch around the step
+            // the ray state is updated in place instead of copied in and out of the branch
+            // every round. An idle lane's result is discarded, and its state is initialised
+            // again when it takes a ray. The tuned DEs read no memory by position.
             const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             const float s = march_sample_t(m);
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
-            if (ACC && acc && m.probing) st = march_skip_step(m, mp, accel_skip_cached(av, ac, ps), &dist);
-            used = st < 0;
-            if (used) st = march_update(m, mp, s, scene_de<false, EXT>(sc, ps, nullptr), &dist);
+            const int st_any = march_update(m, mp, s, scene_de<false, EXT>(sc, ps, nullptr), &dist);
+            if (active) { st = st_any; used = true; }
         }
         de += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(used));
         if (active) {
