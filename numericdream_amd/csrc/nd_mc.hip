@@ -751,6 +751,21 @@ NDD void persistent_march(const McArgs& a, RAY& R, uint32_t n, uint32_t* head) {
                     active = false;
                 }
             }
+        } else if (!EXT && !INSIDE && !acc) {
+            // Idle lanes step too, on the ray they last held (or the initial one), as in the
+            // G-buffer march: the ray state is updated in place instead of copied in and out
+            // of a branch every round. An idle lane's result is discarded and its state set
+            // again when it takes a ray; the tuned DEs read no memory by position.
+            const float s = march_sample_t(m);
+            const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
+            const int st_any = march_update(m, mp, s, scene_de<false, EXT>(sc, ps, nullptr), &dist);
+            if (active) {
+                used = true;
+                if (st_any != MS_CONT) {
+                    R.finish(item, st_any == MS_HIT, dist);
+                    active = false;
+                }
+            }
         } else if (active) {
             const float s = march_sample_t(m);
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
