@@ -330,7 +330,7 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
         float dist = 0.f;
         int st = -1;
         bool used = false;  // the lane evaluated a DE this round (no accelerator skip applied)
-        if (ACC || EXT) {
+        if (ACC || EXT || !SHARED) {
             if (active) {
                 // march_to's loop: at a probe the accelerator's skip, when it applies, takes the
                 // step without a DE (template.wgsl:1890-1900); a wave whose lanes all skip runs
@@ -347,7 +347,9 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
             // issues the same instructions either way, and without a branch around the step
             // the ray state is updated in place instead of copied in and out of the branch
             // every round. An idle lane's result is discarded, and its state is initialised
-            // again when it takes a ray. The tuned DEs read no memory by position.
+            // again when it takes a ray. The tuned DEs read no memory by position. Only with
+            // one scene for every job: with several, an idle lane's stale job could hold
+            // another formula than the active lanes', a divergent DE of its own.
             const DevScene& sc = scene_of<SHARED>(jobs, shared, job);
             const float s = march_sample_t(m);
             const V3<float> ps = mk3(org.x + dir.x * s, org.y + dir.y * s, org.z + dir.z * s);
