@@ -26,14 +26,14 @@ METRIC = "Mrays/s persistent-batched vs plain @1080p Mandelbox, %FP32 peak; grad
 # HBM bytes per k_march launch on the 16-camera batch, from two rocprofv3 --pmc runs
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB copy; profiles/k_march_c2_traffic.md).
 # Hardware counters cannot be read inside the timed run, so the measured figure is carried here.
-K_MARCH_TRAFFIC_BYTES_16CAM = 2.092e9
+K_MARCH_TRAFFIC_BYTES_16CAM = 2.477e9
 # Hardware counters of the same k_march launch (profiles/k_march_c2_valu_counters.md, two
 # rocprofv3 --pmc runs): active lanes per VALU instruction, the VALU lane-instruction rate as a
 # fraction of the 128 FP32 lanes/clk/CU the FP32 peak implies, the FP32 FLOP rate the
 # hardware counts (FMA 2, add/mul/transcendental 1; folds and clamps are not FLOPs to it), and
 # the share of quad-cycles in which a SIMD issued two VALU instructions.
-K_MARCH_COUNTERS_16CAM = {"lane_util": 0.849, "valu_lane_rate_frac": 0.598, "hw_fp32_tflops": 43.1,
-                          "dual_issue_quad_cycles": 0.548, "source": "profiles/k_march_c2_valu_counters.md"}
+K_MARCH_COUNTERS_16CAM = {"lane_util": 0.902, "valu_lane_rate_frac": 0.631, "hw_fp32_tflops": 46.9,
+                          "dual_issue_quad_cycles": 0.533, "source": "profiles/k_march_c2_valu_counters.md"}
 
 
 def log(msg):
