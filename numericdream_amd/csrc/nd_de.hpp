@@ -529,8 +529,33 @@ NDD float formula_de_f(const DevInst& in, const V3<float>& pos, float* trap) {
     }
 }
 
-// Dual-number version (normals, adjoint). The trap is not needed on these paths.
-template <class S, bool EXT = false>
+// The Mandelbox step of the dual path with its parameters as plain floats (CONSTP below):
+// the same value arithmetic as box_step_t, without the zero parameter tangents that a dual
+// constant carries through every product.
+template <class S, class I> NDD Carry<S> box_step_fc(const Carry<S>& c, const V3<S>& posc, float scale, float t_in,
+                                                     float m2, float F2, float L, float dr_add, const I& in) {
+    const S nL = Sc<S>::c(-L), pL = Sc<S>::c(L);
+    V3<S> z = mk3(sclamp(c.z.x, nL, pL) * 2.f - c.z.x, sclamp(c.z.y, nL, pL) * 2.f - c.z.y,
+                  sclamp(c.z.z, nL, pL) * 2.f - c.z.z);
+    S r2 = dot3(z, z);
+    S dr = c.dr;
+    if (vv(r2) < m2) {
+        z = mk3(z.x * t_in, z.y * t_in, z.z * t_in); dr = dr * t_in;
+    } else if (vv(r2) < F2) {
+        S t = sdiv(Sc<S>::c(F2), r2);
+        z = z * t; dr = dr * t;
+    }
+    if (in.rot_on) z = rotate_cs(z, in.rot_cs);
+    Carry<S> o;
+    o.z = mk3(z.x * scale + posc.x, z.y * scale + posc.y, z.z * scale + posc.z);
+    o.dr = dr * fabsf(scale) + dr_add;
+    return o;
+}
+
+// Dual-number version (normals, adjoint). The trap is not needed on these paths. CONSTP:
+// the tangents are of the position only (analytic normals), so the formula's parameters
+// are plain constants; without it they carry their adjoint tangent slots (dparam).
+template <class S, bool EXT = false, bool CONSTP = false>
 NDD S formula_de_d(const DevInst& in, const V3<S>& pos) {
     if constexpr (EXT) {
         if (in.hybrid) return ext_de_d<S>(in, pos);
@@ -545,6 +570,14 @@ NDD S formula_de_d(const DevInst& in, const V3<S>& pos) {
             return bulb_finalize<S>(c);
         }
         case NDF_BOX: {
+            if constexpr (CONSTP) {
+                const float scale = in.p[0], m = in.p[1], F = in.p[2], L = in.p[3], cs = in.p[8];
+                const float m2 = m * m, F2 = F * F;
+                const float t_in = vv(sdiv(Sc<S>::c(F2), Sc<S>::c(m2)));
+                V3<S> posc = mk3(pos.x * cs + in.p[9], pos.y * cs + in.p[10], pos.z * cs + in.p[11]);
+                for (int it = 0; it < iters; ++it) c = box_step_fc<S>(c, posc, scale, t_in, m2, F2, L, in.h[10], in);
+                return sdiv(len3(c.z), sabs(c.dr));
+            }
             S scale = prm<S>(in, 0);
             S m = prm<S>(in, 1), F = prm<S>(in, 2);
             S m2 = m * m, F2 = F * F;
@@ -658,14 +691,14 @@ NDD float scene_de(const DevScene& sc, const V3<float>& p, float* trap) {
 }
 
 // Dual-number scene DE (union/smooth combines differentiated through their active branch).
-template <class S, bool EXT = false>
+template <class S, bool EXT = false, bool CONSTP = false>
 NDD S scene_de_d(const DevScene& sc, const V3<S>& p) {
     S d = Sc<S>::c(sc.empty_de);
     bool first = true;
     for (int i = 0; i < sc.count; ++i) {
         const DevInst& in = sc.inst[i];
         if (in.hidden) continue;
-        S di = formula_de_d<S, EXT>(in, instance_local<S>(in, p)) * in.de_factor;
+        S di = formula_de_d<S, EXT, CONSTP>(in, instance_local<S>(in, p)) * in.de_factor;
         if (first) { d = di; first = false; continue; }
         const float k = in.blend_k;
         const int mode = in.combine;

@@ -50,7 +50,7 @@ NDD V3<float> surface_normal(const DevScene& sc, int mode, const V3<float>& p, f
     // analytic: forward-mode derivative of the DE
     using D3 = DualN<3>;
     V3<D3> q = mk3(dvar<3>(p.x, 0), dvar<3>(p.y, 1), dvar<3>(p.z, 2));
-    D3 d = scene_de_d<D3, EXT>(sc, q);
+    D3 d = scene_de_d<D3, EXT, true>(sc, q);
     *evals += 1;
     return vnorm(mk3(d.d[0], d.d[1], d.d[2]));
 }
