@@ -48,9 +48,11 @@ void trampoline() {
     swapcontext(&g_lane->ctx, &g_sched);
 }
 void on_alarm(int) {
-    fprintf(stderr, "\n[emu] WATCHDOG: kernel %s still running; lane %d of block %u thread %u (swaps %llu)\n",
-            g_kernel, g_lane ? g_lane->lane : -1, g_lane ? g_lane->bid.x : 0, g_lane ? g_lane->tid.x : 0,
-            (unsigned long long)g_swaps);
+    // async-signal-safe: write(2) of fixed text and the kernel name, then abort
+    static const char msg[] = "\n[emu] WATCHDOG: a launch ran past EMU_WATCHDOG seconds: ";
+    (void)!write(2, msg, sizeof(msg) - 1);
+    (void)!write(2, g_kernel, strlen(g_kernel));
+    (void)!write(2, "\n", 1);
     abort();
 }
 int file_rank(const char* f) {
