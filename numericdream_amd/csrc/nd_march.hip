@@ -407,10 +407,11 @@ k_march(PassArgs a, const DevJob* __restrict__ jobs,
 }
 
 // --- pass 2: surface. Normal + trap of every hit (fixed cost per hit), and the shadow
-// ray of the hit, stored SoA for pass 3. 5 waves per SIMD (the compiler alone gives 4) is
-// about 3% faster on the 16-camera batch.
+// ray of the hit, stored SoA for pass 3. 4 waves per SIMD: with the Mandelbox normal's
+// parameters as plain floats, 4 waves are 2% faster than 5 on the 16-camera batch (5 were
+// 3% ahead of 4 before that change).
 template <bool SHARED, bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 5, EXT ? 8 : 5)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 4, EXT ? 8 : 4)))
 k_surface(PassArgs a, const DevJob* __restrict__ jobs,
                                                  const DevScene* __restrict__ shared) {
     const uint32_t n_hits = a.ctr->hit_count;
